@@ -1,0 +1,70 @@
+"""Pin the oracle against the reference's golden vectors and against the
+reference's own kernel built from its sources (oracle/_ref)."""
+import json
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN
+from minibude_amd import deck as deckmod
+from minibude_amd import metrics
+
+
+@pytest.fixture(scope="module")
+def bm1():
+    return deckmod.read_deck(GOLDEN / "bm1")
+
+
+@pytest.fixture(scope="module")
+def bm2():
+    return deckmod.read_deck(GOLDEN / "bm2")
+
+
+def test_bm1_golden_fp32(bm1):
+    # full 65,536 poses against data/bm1/ref_energies.out
+    e = O.fasten_f32(bm1)
+    max_diff, failed = metrics.validate(bm1.ref_energies, e)
+    assert metrics.is_valid(max_diff) and failed.size == 0
+    assert metrics.symmetric_max_rel_pct(bm1.ref_energies, e) < 0.005
+
+
+def test_bm1_golden_fp64(bm1):
+    e = O.fasten_f64(bm1, 0, 4096)
+    assert metrics.symmetric_max_rel_pct(bm1.ref_energies[:4096], e) < 0.006
+
+
+def test_bm2_golden(bm2):
+    # bm2 is 2,672 x 2,672 atoms: check a 96-pose slice from each end
+    for first in (0, bm2.nposes - 96):
+        e32 = O.fasten_f32(bm2, first, 96)
+        e64 = O.fasten_f64(bm2, first, 96)
+        ref = bm2.ref_energies[first:first + 96]
+        assert metrics.is_valid(metrics.validate(ref, e32)[0])
+        assert metrics.symmetric_max_rel_pct(ref, e32) < 0.005
+        assert metrics.symmetric_max_rel_pct(ref, e64) < 0.005
+
+
+@pytest.mark.skipif(not O.have_ref_kernel(), reason="oracle/_ref not built (needs the reference sources)")
+def test_fp32_oracle_is_bitwise_the_reference_kernel(bm1):
+    a = O.fasten_f32(bm1, 1000, 512)
+    b = O.ref_kernel(bm1, 1000, 512)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_synthetic_fixture_matches_oracle(name):
+    fx = GOLDEN / name
+    if not (fx / "ref_energies.f32").exists():
+        pytest.skip("fixture not generated")
+    meta = json.loads((fx / "manifest.json").read_text())
+    d = deckmod.workload_deck(name)
+    with tempfile.TemporaryDirectory() as tmp:
+        assert deckmod.write_deck(tmp, d) == meta["deck_sha256"]
+    ref = np.fromfile(fx / "ref_energies.f32", "<f4")
+    assert ref.size == d.nposes
+    idx = np.linspace(0, d.nposes - 1, 64).astype(int)
+    for i in idx[:16]:
+        got = O.fasten_f32(d, int(i), 1)
+        assert got.view(np.uint32)[0] == ref[i:i + 1].view(np.uint32)[0]
