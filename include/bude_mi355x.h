@@ -97,6 +97,45 @@ int bude_mi355x_context_ms(const bude_ctx *ctx, double *ms);
 /* Frees every device buffer of the context (src/hip/fasten.hpp:253-264). */
 int bude_mi355x_destroy(bude_ctx *ctx);
 
+/* ---- batched ragged fasten_main: one protein, many ligands with their own
+ *      atom and pose counts, in ONE persistent launch.  No counterpart in the
+ *      reference, whose driver runs one (ligand, pose set) per
+ *      Bude<PPWI>::fasten call (src/main.cpp:517-541) and skips pose counts
+ *      that do not divide ppwi*wgsize (src/main.cpp:525-530); every pose of
+ *      every ligand gets the energy fasten_main defines
+ *      (src/serial/fasten.hpp:18-132). ---- */
+typedef struct bude_batch bude_batch;
+
+typedef struct BudeBatchConfig {
+  int32_t ppwi;           /* poses per work-item; a tile is 64*ppwi poses of one ligand: 1, 2, 4, 8 */
+  int32_t wgsize;         /* work-items per work-group: 64, 128, 256, 512 */
+  int32_t waves_per_simd; /* persistent grid depth: CUs * 4 * waves_per_simd waves, 1..16 */
+  int32_t order;          /* 0: tiles of the heaviest ligands first, 1: ligand order */
+  int32_t use_graph;      /* 1: replay a captured hipGraph (counter reset + kernel) */
+} BudeBatchConfig;
+
+/* ligands[i] has natlig[i] atoms; poses[i] holds 6 rows of nposes[i] floats
+ * (row k at poses[i] + k*nposes[i], rows as in the deck's poses.in). */
+int bude_mi355x_batch_create(int device, const BudeAtom *protein, size_t natpro, const BudeFFParams *forcefield,
+                             size_t ntypes, size_t nligands, const BudeAtom *const *ligands, const size_t *natlig,
+                             const float *const *poses, const size_t *nposes, bude_batch **out);
+/* warmup + iterations persistent launches; device_ms[i] = hipEvent time of
+ * launch i (counter reset included), may be NULL. */
+int bude_mi355x_batch_run(bude_batch *b, const BudeBatchConfig *cfg, size_t warmup, size_t iterations,
+                          double *device_ms);
+/* The baseline the batch replaces: one plain launch per ligand, in ligand
+ * order on one stream; device_ms[i] covers all nligands launches of pass i. */
+int bude_mi355x_batch_run_separate(bude_batch *b, size_t ppwi, size_t wgsize, size_t warmup, size_t iterations,
+                                   double *device_ms);
+/* Times candidate configurations (tile width, work-group size, grid depth,
+ * tile order; graph replay) on this batch and returns the fastest. */
+int bude_mi355x_batch_autotune(bude_batch *b, BudeBatchConfig *best, double *best_ms);
+/* Energies of ligand `ligand` (nposes[ligand] floats) from the last run. */
+int bude_mi355x_batch_energies(bude_batch *b, size_t ligand, float *out);
+/* Sets every energy to NaN (so a later run can be checked to write all). */
+int bude_mi355x_batch_clear(bude_batch *b);
+int bude_mi355x_batch_destroy(bude_batch *b);
+
 /* ---- deck tooling: the synthetic generator behind the benchmark decks
  *      (the role makedeck/main.cpp:49-234 plays for bm1/bm2).  Every value
  *      comes from std::mt19937 words through fixed integer/IEEE mappings, so

@@ -40,6 +40,8 @@ LIB_SOURCES = [
     "deck_synth.cpp",
     "fasten_plain.hip",
     "capi_plain.hip",
+    "fasten_batch.hip",
+    "capi_batch.hip",
 ]
 
 

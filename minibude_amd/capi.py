@@ -38,11 +38,27 @@ EXPORTED = (
     "bude_mi355x_fasten",
     "bude_mi355x_context_ms",
     "bude_mi355x_destroy",
+    "bude_mi355x_batch_create",
+    "bude_mi355x_batch_run",
+    "bude_mi355x_batch_run_separate",
+    "bude_mi355x_batch_autotune",
+    "bude_mi355x_batch_energies",
+    "bude_mi355x_batch_clear",
+    "bude_mi355x_batch_destroy",
     "bude_synth_forcefield",
     "bude_synth_protein",
     "bude_synth_ligand",
     "bude_synth_poses",
 )
+
+
+class BatchConfig(ctypes.Structure):
+    """BudeBatchConfig (include/bude_mi355x.h)."""
+    _fields_ = [("ppwi", ctypes.c_int32), ("wgsize", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
+                ("order", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class BudeError(RuntimeError):
@@ -84,6 +100,14 @@ def _declare(lib: ctypes.CDLL) -> None:
         "bude_mi355x_fasten": (_I, [_P, _S, _S, _S, _S, _P, _P, _P]),
         "bude_mi355x_context_ms": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
         "bude_mi355x_destroy": (_I, [_P]),
+        "bude_mi355x_batch_create": (_I, [_I, _P, _S, _P, _S, _S, ctypes.POINTER(_P), ctypes.POINTER(_S),
+                                          ctypes.POINTER(_P), ctypes.POINTER(_S), ctypes.POINTER(_P)]),
+        "bude_mi355x_batch_run": (_I, [_P, _P, _S, _S, _P]),
+        "bude_mi355x_batch_run_separate": (_I, [_P, _S, _S, _S, _S, _P]),
+        "bude_mi355x_batch_autotune": (_I, [_P, _P, ctypes.POINTER(ctypes.c_double)]),
+        "bude_mi355x_batch_energies": (_I, [_P, _S, _P]),
+        "bude_mi355x_batch_clear": (_I, [_P]),
+        "bude_mi355x_batch_destroy": (_I, [_P]),
         "bude_synth_forcefield": (_I, [ctypes.c_uint64, _S, _P]),
         "bude_synth_protein": (_I, [ctypes.c_uint64, _S, _S, ctypes.c_float, _P]),
         "bude_synth_ligand": (_I, [ctypes.c_uint64, _S, _S, ctypes.c_float, _P]),

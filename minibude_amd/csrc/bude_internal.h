@@ -42,7 +42,10 @@ constexpr float kNpPDist = 1.0f;
 //    (zone1 ? 1 : 1 - b/distdslv) = wd * sat(1 - b/distdslv), wd = 0 when
 //    hphb_p == 0 or distdslv = -FLT_MAX (b < -FLT_MAX never holds).
 // Every hinge is continuous at its kinks, so evaluating the zone tests as
-// clamps changes results by O(ulp) only.
+// clamps changes results by O(ulp) only.  The charge and desolvation hinges
+// can equally be written in d with the offsets c0 / e0 (saves the b = d -
+// radij subtraction, at the cost of one rounding of the offset per type
+// pair).
 struct alignas(32) PairCoef {
   float radij;     // rp + rl
   float neg_rr;    // -1/radij        (FP32 reciprocal, as the reference)
@@ -50,7 +53,8 @@ struct alignas(32) PairCoef {
   float wq;        // 45 * (type_E ? -|chrg_init| : chrg_init)
   float neg_rd;    // -1/distdslv     (0 when the desolvation term is off)
   float wd;        // dslv_init       (0 when the desolvation term is off)
-  float pad0, pad1;
+  float c0;        // 1 + radij*elcdst1: charge hinge as sat(c0 + neg_e1 * d)
+  float e0;        // 1 + radij/distdslv (one rounding): desolvation hinge as sat(e0 + neg_rd * d)
 };
 static_assert(sizeof(PairCoef) == 32, "PairCoef is 32 bytes");
 

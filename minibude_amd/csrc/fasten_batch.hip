@@ -1,0 +1,115 @@
+// fasten_batch.hip -- batched ragged fasten_main: many ligands, each with its
+// own atom count and pose count, scored against one protein in ONE
+// persistent launch.
+//
+// The reference scores one (ligand, pose set) per fasten() call and its
+// driver never runs a partial work-group (src/main.cpp:525-530).  Here the
+// unit of work is a tile: 64 x RB consecutive poses of one ligand (one
+// wavefront's register-blocked share, exactly the plain kernel's per-wave
+// work, src/hip/fasten.hpp:21,48).  Tiles are listed on the host in
+// decreasing cost order (cost ~ natlig of the tile's ligand) and every wave
+// of a fixed-size grid pulls the next tile from a device counter until the
+// list is exhausted, so ligands of 8 and 96 atoms share the machine without
+// a tail of idle CUs and pose counts need not divide anything (the last tile
+// of a ligand is masked).
+#include <hip/hip_runtime.h>
+
+#include "bude_internal.h"
+#include "fasten_batch.h"
+#include "fasten_core.h"
+
+namespace bude {
+
+namespace {
+
+using namespace core;
+
+__global__ void reset_counter_kernel(unsigned int *__restrict__ counter) {
+  if (threadIdx.x == 0) counter[0] = 0u;
+}
+
+template <int RB, int WG, int FORM>
+__global__ __launch_bounds__(WG) void fasten_batch_kernel(
+    const float4 *__restrict__ protein, const int32_t *__restrict__ group_begin, const PairCoef *__restrict__ coef,
+    const float4 *__restrict__ ligands, const LigandMeta *__restrict__ meta, const int2 *__restrict__ tiles,
+    const float *__restrict__ poses, float *__restrict__ energies, unsigned int *__restrict__ counter, size_t stride,
+    int ngroups, int ntiles) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    // One atomic per wave; every wave leaves once the list is exhausted.
+    unsigned int t = 0;
+    if (lane == 0) t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+    if (t >= unsigned(ntiles)) break;
+
+    const int2 tile = tiles[t];
+    const LigandMeta m = meta[tile.x];
+    Xform tr[RB];
+    float e[RB];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      const int j = tile.y + lane + k * 64;
+      tr[k] = make_xform(poses, stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
+      e[k] = 0.f;
+    }
+    for (int il = 0; il < m.natlig; ++il) {
+      const float4 la = ligands[m.lig_off + il];
+      float lx[RB], ly[RB], lz[RB];
+#pragma unroll
+      for (int k = 0; k < RB; ++k) {
+        const float *mm = tr[k].m;
+        lx[k] = mm[3] + la.x * mm[0] + la.y * mm[1] + la.z * mm[2];
+        ly[k] = mm[7] + la.x * mm[4] + la.y * mm[5] + la.z * mm[6];
+        lz[k] = mm[11] + la.x * mm[8] + la.y * mm[9] + la.z * mm[10];
+      }
+      const PairCoef *cl = coef + size_t(m.coef_off) + size_t(il) * ngroups;
+      for (int g = 0; g < ngroups; ++g)
+        group_dispatch<RB, FORM>(protein, group_begin[g], group_begin[g + 1], cl[g], lx, ly, lz, e);
+    }
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      const int j = tile.y + lane + k * 64;
+      if (j < m.nposes) energies[size_t(m.pose_off) + size_t(j)] = e[k] * 0.5f;
+    }
+  }
+}
+
+template <int RB, int WG>
+hipError_t launch_rb_wg(const BatchArgs &a, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormB>), dim3(unsigned(blocks)), dim3(WG), 0, s,
+                     a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.counter,
+                     a.stride, a.ngroups, a.ntiles);
+  return hipGetLastError();
+}
+
+template <int RB>
+hipError_t launch_rb(const BatchArgs &a, int wg, int blocks, hipStream_t s) {
+  switch (wg) {
+    case 64: return launch_rb_wg<RB, 64>(a, blocks, s);
+    case 128: return launch_rb_wg<RB, 128>(a, blocks, s);
+    case 256: return launch_rb_wg<RB, 256>(a, blocks, s);
+    case 512: return launch_rb_wg<RB, 512>(a, blocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+bool batch_shape_supported(int rb, int wgsize) {
+  return (rb == 1 || rb == 2 || rb == 4 || rb == 8) && (wgsize == 64 || wgsize == 128 || wgsize == 256 ||
+                                                         wgsize == 512);
+}
+
+hipError_t launch_batch(const BatchArgs &a, int rb, int wgsize, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, a.counter);
+  if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+  switch (rb) {
+    case 1: return launch_rb<1>(a, wgsize, blocks, s);
+    case 2: return launch_rb<2>(a, wgsize, blocks, s);
+    case 4: return launch_rb<4>(a, wgsize, blocks, s);
+    case 8: return launch_rb<8>(a, wgsize, blocks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace bude

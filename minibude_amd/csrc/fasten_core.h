@@ -1,0 +1,101 @@
+// fasten_core.h -- device building blocks shared by the plain and batched
+// fasten_main kernels: pose transform, and the pair loop over one protein
+// group with its hinge-form and zero-weight specialisations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bude_internal.h"
+
+namespace bude {
+namespace core {
+
+__device__ __forceinline__ float sat(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+
+// 3x4 rigid transform of one pose (src/serial/fasten.hpp:32-50):
+// R = Rz(pan) * Ry(roll) * Rx(tilt), translation in the last column.
+struct Xform {
+  float m[12];
+};
+
+__device__ __forceinline__ Xform make_xform(const float *__restrict__ poses, size_t stride, size_t q) {
+  const float tilt = poses[q], roll = poses[stride + q], pan = poses[2 * stride + q];
+  float sx, cx, sy, cy, sz, cz;
+  sincosf(tilt, &sx, &cx);
+  sincosf(roll, &sy, &cy);
+  sincosf(pan, &sz, &cz);
+  Xform t;
+  t.m[0] = cy * cz;
+  t.m[1] = sx * sy * cz - cx * sz;
+  t.m[2] = cx * sy * cz + sx * sz;
+  t.m[3] = poses[3 * stride + q];
+  t.m[4] = cy * sz;
+  t.m[5] = sx * sy * sz + cx * cz;
+  t.m[6] = cx * sy * sz - sx * cz;
+  t.m[7] = poses[4 * stride + q];
+  t.m[8] = -sy;
+  t.m[9] = sx * cy;
+  t.m[10] = cx * cy;
+  t.m[11] = poses[5 * stride + q];
+  return t;
+}
+
+// Hinge forms of the charge / desolvation terms (see PairCoef):
+//   kFormB  sat(1 + b * neg_e1) with b = d - radij (one extra subtraction,
+//           matches the reference's rounding of 1 - b*elcdst1 exactly)
+//   kFormD  sat(c0 + d * neg_e1) (one rounding of c0 per type pair)
+enum HingeForm { kFormB = 0, kFormD = 1 };
+
+// All pairs of one protein group with the RB poses of this work-item.
+// CHG / DSLV drop the charge / desolvation hinge when the group's weight is
+// exactly zero for this ligand atom (adding fma(0, h, e) == e is a no-op, so
+// results are bit-identical).
+template <int RB, int FORM, bool CHG, bool DSLV>
+__device__ __forceinline__ void group_pairs(const float4 *__restrict__ prot, int i0, int i1, const PairCoef c,
+                                            const float (&lx)[RB], const float (&ly)[RB], const float (&lz)[RB],
+                                            float (&e)[RB]) {
+#pragma unroll 2
+  for (int i = i0; i < i1; ++i) {
+    const float4 p = prot[i];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      const float dx = lx[k] - p.x;
+      const float dy = ly[k] - p.y;
+      const float dz = lz[k] - p.z;
+      const float d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+      if constexpr (FORM == kFormB) {
+        const float b = d - c.radij;
+        e[k] = __builtin_fmaf(2.0f * kHardness, sat(b * c.neg_rr), e[k]);
+        if constexpr (CHG) e[k] = __builtin_fmaf(c.wq, sat(__builtin_fmaf(b, c.neg_e1, 1.0f)), e[k]);
+        if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(b, c.neg_rd, 1.0f)), e[k]);
+      } else {
+        e[k] = __builtin_fmaf(2.0f * kHardness, sat(__builtin_fmaf(d, c.neg_rr, 1.0f)), e[k]);
+        if constexpr (CHG) e[k] = __builtin_fmaf(c.wq, sat(__builtin_fmaf(d, c.neg_e1, c.c0)), e[k]);
+        if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(d, c.neg_rd, c.e0)), e[k]);
+      }
+    }
+  }
+}
+
+// One protein group: its pairs are summed into a fresh partial sum that is
+// then added to the pose total, so the running FP32 total takes one rounding
+// per (ligand atom, group) instead of three per pair (a shorter, smaller-
+// magnitude accumulation than the reference's single running sum).
+template <int RB, int FORM>
+__device__ __forceinline__ void group_dispatch(const float4 *__restrict__ prot, int i0, int i1, const PairCoef c,
+                                               const float (&lx)[RB], const float (&ly)[RB],
+                                               const float (&lz)[RB], float (&e)[RB]) {
+  float part[RB];
+#pragma unroll
+  for (int k = 0; k < RB; ++k) part[k] = 0.f;
+  const bool chg = c.wq != 0.f, dslv = c.wd != 0.f;  // wave-uniform
+  if (chg && dslv) group_pairs<RB, FORM, true, true>(prot, i0, i1, c, lx, ly, lz, part);
+  else if (chg) group_pairs<RB, FORM, true, false>(prot, i0, i1, c, lx, ly, lz, part);
+  else if (dslv) group_pairs<RB, FORM, false, true>(prot, i0, i1, c, lx, ly, lz, part);
+  else group_pairs<RB, FORM, false, false>(prot, i0, i1, c, lx, ly, lz, part);
+#pragma unroll
+  for (int k = 0; k < RB; ++k) e[k] += part[k];
+}
+
+}  // namespace core
+}  // namespace bude

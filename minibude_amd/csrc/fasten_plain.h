@@ -20,6 +20,7 @@ struct PlainArgs {
   int natlig;
   int ngroups;
   int ppwi;
+  int form;                  // 0: hinges in b = d - radij, 1: hinges in d
 };
 
 // Poses per register block actually held in VGPRs for (ppwi, wgsize).

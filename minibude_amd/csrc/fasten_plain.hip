@@ -21,69 +21,18 @@
 #include <hip/hip_runtime.h>
 
 #include "bude_internal.h"
+#include "fasten_core.h"
 #include "fasten_plain.h"
 
 namespace bude {
 
 namespace {
 
-__device__ __forceinline__ float sat(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
-
-// 3x4 rigid transform of one pose (src/serial/fasten.hpp:32-50):
-// R = Rz(pan) * Ry(roll) * Rx(tilt), translation in the last column.
-struct Xform {
-  float m[12];
-};
-
-__device__ __forceinline__ Xform make_xform(const float *__restrict__ poses, size_t stride, size_t q) {
-  const float tilt = poses[q], roll = poses[stride + q], pan = poses[2 * stride + q];
-  float sx, cx, sy, cy, sz, cz;
-  sincosf(tilt, &sx, &cx);
-  sincosf(roll, &sy, &cy);
-  sincosf(pan, &sz, &cz);
-  Xform t;
-  t.m[0] = cy * cz;
-  t.m[1] = sx * sy * cz - cx * sz;
-  t.m[2] = cx * sy * cz + sx * sz;
-  t.m[3] = poses[3 * stride + q];
-  t.m[4] = cy * sz;
-  t.m[5] = sx * sy * sz + cx * cz;
-  t.m[6] = cx * sy * sz - sx * cz;
-  t.m[7] = poses[4 * stride + q];
-  t.m[8] = -sy;
-  t.m[9] = sx * cy;
-  t.m[10] = cx * cy;
-  t.m[11] = poses[5 * stride + q];
-  return t;
-}
-
-template <int RB>
-__device__ __forceinline__ void group_pairs(const float4 *__restrict__ prot, int i0, int i1, const PairCoef c,
-                                            const float (&lx)[RB], const float (&ly)[RB], const float (&lz)[RB],
-                                            float (&e)[RB]) {
-#pragma unroll 2
-  for (int i = i0; i < i1; ++i) {
-    const float4 p = prot[i];
-#pragma unroll
-    for (int k = 0; k < RB; ++k) {
-      const float dx = lx[k] - p.x;
-      const float dy = ly[k] - p.y;
-      const float dz = lz[k] - p.z;
-      const float d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
-      const float b = d - c.radij;
-      const float st = sat(b * c.neg_rr);
-      const float ch = sat(__builtin_fmaf(b, c.neg_e1, 1.0f));
-      const float ds = sat(__builtin_fmaf(b, c.neg_rd, 1.0f));
-      e[k] = __builtin_fmaf(2.0f * kHardness, st, e[k]);
-      e[k] = __builtin_fmaf(c.wq, ch, e[k]);
-      e[k] = __builtin_fmaf(c.wd, ds, e[k]);
-    }
-  }
-}
+using namespace core;
 
 // Separate __restrict__ kernel arguments (not a struct) so the compiler can
 // prove the uniform loads are never clobbered and issue them as scalar loads.
-template <int RB, int WG>
+template <int RB, int WG, int FORM>
 __global__ __launch_bounds__(WG) void fasten_plain_kernel(
     const float4 *__restrict__ protein, const int32_t *__restrict__ group_begin, const PairCoef *__restrict__ coef,
     const float4 *__restrict__ ligand, const float *__restrict__ poses, float *__restrict__ energies, size_t stride,
@@ -113,7 +62,7 @@ __global__ __launch_bounds__(WG) void fasten_plain_kernel(
       }
       const PairCoef *cl = coef + size_t(il) * ngroups;
       for (int g = 0; g < ngroups; ++g)
-        group_pairs<RB>(protein, group_begin[g], group_begin[g + 1], cl[g], lx, ly, lz, e);
+        group_dispatch<RB, FORM>(protein, group_begin[g], group_begin[g + 1], cl[g], lx, ly, lz, e);
     }
 
 #pragma unroll
@@ -126,9 +75,14 @@ __global__ __launch_bounds__(WG) void fasten_plain_kernel(
 
 template <int RB, int WG>
 hipError_t launch_rb_wg(const PlainArgs &a, size_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_plain_kernel<RB, WG>), dim3(unsigned(blocks)), dim3(WG), 0, s,
-                     a.protein, a.group_begin, a.coef, a.ligand, a.poses, a.energies, a.stride, a.nposes, a.natlig,
-                     a.ngroups, a.ppwi);
+  if (a.form == kFormD)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_plain_kernel<RB, WG, kFormD>), dim3(unsigned(blocks)), dim3(WG), 0,
+                       s, a.protein, a.group_begin, a.coef, a.ligand, a.poses, a.energies, a.stride, a.nposes,
+                       a.natlig, a.ngroups, a.ppwi);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_plain_kernel<RB, WG, kFormB>), dim3(unsigned(blocks)), dim3(WG), 0,
+                       s, a.protein, a.group_begin, a.coef, a.ligand, a.poses, a.energies, a.stride, a.nposes,
+                       a.natlig, a.ngroups, a.ppwi);
   return hipGetLastError();
 }
 

@@ -55,6 +55,8 @@ PairCoef pair_coef(const BudeFFParams &p, const BudeFFParams &l) {
   c.wq = kCnstnt * (type_E ? -std::fabs(chrg_init) : chrg_init);
   c.neg_rd = dslv_on ? -(1.0f / distdslv) : 0.f;
   c.wd = dslv_on ? dslv_init : 0.f;
+  c.c0 = 1.0f + radij * elcdst1;  // radij * elcdst1 is exact (power of two)
+  c.e0 = dslv_on ? float(1.0 + double(radij) * double(1.0f / distdslv)) : 0.f;
   return c;
 }
 

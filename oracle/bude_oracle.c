@@ -207,8 +207,8 @@ void bude_oracle_transform_f64(double t0, double t1, double t2, double t3, doubl
 }
 
 static double pose_f64(size_t natlig, size_t natpro, const OAtom *protein, const OAtom *ligand, const OFF *ff,
-                       const double t[12]) {
-  double etot = 0.0;
+                       const double t[12], double *magnitude) {
+  double etot = 0.0, mag = 0.0;
   for (size_t il = 0; il < natlig; il++) {
     const OAtom la = ligand[il];
     const double lx = t[3] + la.x * t[0] + la.y * t[1] + la.z * t[2];
@@ -220,8 +220,10 @@ static double pose_f64(size_t natlig, size_t natpro, const OAtom *protein, const
       double terms[3];
       bude_oracle_pair_terms_f64(&ff[pa.type], &ff[la.type], sqrt(x * x + y * y + z * z), terms);
       etot += terms[0] + terms[1] + terms[2];
+      mag += fabs(terms[0]) + fabs(terms[1]) + fabs(terms[2]);
     }
   }
+  if (magnitude) *magnitude = mag * 0.5;
   return etot * 0.5;
 }
 
@@ -236,7 +238,25 @@ void bude_oracle_fasten_f64(size_t natlig, size_t natpro, const void *protein, c
     const size_t i = first + (size_t)k;
     double t[12];
     bude_oracle_transform_f64(p0[i], p1[i], p2[i], p3[i], p4[i], p5[i], t);
-    energies[k] = pose_f64(natlig, natpro, pro, lig, f, t);
+    energies[k] = pose_f64(natlig, natpro, pro, lig, f, t, NULL);
+  }
+}
+
+/* FP64 energies plus the pose's magnitude 0.5 * sum |steric| + |charge| +
+ * |dslv| over all pairs: the scale of the cancellation in the sum, which
+ * bounds what any FP32 evaluation can resolve (|error| ~ eps * magnitude). */
+void bude_oracle_fasten_f64_mag(size_t natlig, size_t natpro, const void *protein, const void *ligand, const void *ff,
+                                const float *p0, const float *p1, const float *p2, const float *p3, const float *p4,
+                                const float *p5, size_t first, size_t count, double *energies, double *magnitude) {
+  const OAtom *pro = (const OAtom *)protein;
+  const OAtom *lig = (const OAtom *)ligand;
+  const OFF *f = (const OFF *)ff;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long k = 0; k < (long long)count; k++) {
+    const size_t i = first + (size_t)k;
+    double t[12];
+    bude_oracle_transform_f64(p0[i], p1[i], p2[i], p3[i], p4[i], p5[i], t);
+    energies[k] = pose_f64(natlig, natpro, pro, lig, f, t, &magnitude[k]);
   }
 }
 

@@ -7,7 +7,11 @@ Fixtures:
   bm1/, bm2/      the reference's own decks and golden energies, copied
                   byte for byte from data/bm1 and data/bm2 (inputs and expected
                   outputs of the reference's only correctness check).
-  c0/, c1/        reference energies of the seeded synthetic workload decks:
+  c0/, c1/        fp64.f32: FP64-oracle energies and magnitudes (0.5 * sum of
+                  |steric| + |charge| + |desolvation| over all pairs), float32
+                  pairs [E, M] per pose, from oracle/liboracle.so; the
+                  magnitude is the scale of cancellation in each pose's sum.
+                  reference energies of the seeded synthetic workload decks:
                   ref_energies.f32 (float32, little-endian, one per pose)
                   computed by the reference's own per-pose kernel
                   (makedeck/ref-kernel.cpp:15-139, built strict by
@@ -50,6 +54,21 @@ def ref_energies(d: deckmod.Deck, first: int = 0, count: int | None = None) -> n
     return out
 
 
+def fp64_fixture(name: str) -> None:
+    d = deckmod.workload_deck(name)
+    lib = ctypes.CDLL(str(ROOT / "oracle" / "liboracle.so"))
+    P, S = ctypes.c_void_p, ctypes.c_size_t
+    e = np.zeros(d.nposes, np.float64)
+    m = np.zeros(d.nposes, np.float64)
+    rows = [P(d.poses[k].ctypes.data) for k in range(6)]
+    t = time.time()
+    lib.bude_oracle_fasten_f64_mag(S(d.natlig), S(d.natpro), P(d.protein.ctypes.data), P(d.ligand.ctypes.data),
+                                   P(d.forcefield.ctypes.data), *rows, S(0), S(d.nposes), P(e.ctypes.data),
+                                   P(m.ctypes.data))
+    np.stack([e, m], axis=1).astype("<f4").tofile(HERE / name / "fp64.f32")
+    print(f"{name}: FP64 energies + magnitudes in {time.time() - t:.1f}s")
+
+
 def copy_bm(name: str) -> None:
     src = REF / "data" / name
     dst = HERE / name
@@ -85,7 +104,7 @@ def synth_fixture(name: str) -> None:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--only", choices=["bm", "c0", "c1"])
+    ap.add_argument("--only", choices=["bm", "c0", "c1", "fp64"])
     a = ap.parse_args()
     if not REF_LIB.exists():
         print("build oracle/_ref first: make -C oracle ref", file=sys.stderr)
@@ -96,6 +115,9 @@ def main() -> int:
     for w in ("c0", "c1"):
         if a.only in (None, w):
             synth_fixture(w)
+    if a.only in (None, "fp64"):
+        for w in ("c0", "c1"):
+            fp64_fixture(w)
     return 0
 
 

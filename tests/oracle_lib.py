@@ -50,6 +50,15 @@ def fasten_f64(deck, first: int = 0, count: int | None = None) -> np.ndarray:
     return out
 
 
+def fasten_f64_mag(deck, first: int = 0, count: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """FP64 energies and magnitudes 0.5 * sum |term| (the cancellation scale)."""
+    count = deck.nposes - first if count is None else count
+    e = np.zeros(count, np.float64)
+    m = np.zeros(count, np.float64)
+    oracle().bude_oracle_fasten_f64_mag(*_args(deck, first, count), _P(e.ctypes.data), _P(m.ctypes.data))
+    return e, m
+
+
 def pair_terms_f64(pff, lff, d: float) -> np.ndarray:
     out = np.zeros(3, np.float64)
     p = np.ascontiguousarray(pff)
