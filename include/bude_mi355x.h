@@ -136,6 +136,82 @@ int bude_mi355x_batch_energies(bude_batch *b, size_t ligand, float *out);
 int bude_mi355x_batch_clear(bude_batch *b);
 int bude_mi355x_batch_destroy(bude_batch *b);
 
+/* ---- learned prescreen: an MLP pose-energy surrogate (21 pose features:
+ *      the 6 pose parameters, sin and cos of the 3 angles, and the 9 entries
+ *      of the rotation matrix they determine -> 256 -> 256 -> 256 -> 1, GELU,
+ *      BF16 weights, FP32 accumulation on MFMA),
+ *      trained on the GPU by this library's own forward/backward kernels and
+ *      Adam on energies the plain kernel computes; screening runs the
+ *      surrogate on every pose and the plain kernel on the lowest
+ *      keep_fraction (exhaustive plain scoring when the validation RMSE is
+ *      above max_val_rmse).  No counterpart in the reference; every energy it
+ *      returns is fasten_main's (src/serial/fasten.hpp:18-132). ---- */
+typedef struct bude_screen bude_screen;
+
+typedef struct BudeScreenConfig {
+  uint32_t batch;       /* training mini-batch, a multiple of 32 */
+  uint32_t epochs;      /* passes over the training split */
+  float lr;             /* peak Adam learning rate (warm-up, cosine decay) */
+  float clip_quantile;  /* training energies above this quantile are clipped to it */
+  float keep_fraction;  /* fraction of screened poses rescored exactly (lowest predictions) */
+  float max_val_rmse;   /* validation RMSE bound (standardised units) above which screening falls back */
+  float tail_quantile;  /* training energies below this quantile get loss weight tail_weight (0: none) */
+  float tail_weight;
+  uint64_t seed;        /* weight initialisation and shuffling */
+} BudeScreenConfig;
+
+typedef struct BudeScreenTrainStats {
+  double label_ms;   /* plain-kernel labelling of the training poses */
+  double train_ms;   /* all training steps */
+  float final_loss;  /* mean squared error of the last epoch (standardised units) */
+  float val_rmse;    /* RMSE on the held-out 1/16 of the poses (standardised units) */
+  float clip, mu, sigma;  /* target transform: y = (min(E, clip) - mu) / sigma */
+  uint32_t steps;
+  int32_t fallback;  /* 1 when val_rmse > max_val_rmse */
+} BudeScreenTrainStats;
+
+typedef struct BudeScreenResult {
+  double total_ms;    /* host wall time of the whole screen */
+  float score_ms;     /* surrogate (or exhaustive plain kernel in fallback) */
+  float select_ms;    /* top-k selection */
+  float rescore_ms;   /* gather + exact plain-kernel rescoring */
+  uint64_t nposes;
+  uint64_t nselected;
+  int32_t fallback;
+} BudeScreenResult;
+
+int bude_mi355x_screen_create(int device, const BudeAtom *protein, size_t natpro, const BudeAtom *ligand,
+                              size_t natlig, const BudeFFParams *forcefield, size_t ntypes,
+                              const BudeScreenConfig *cfg, bude_screen **out);
+/* Labels n poses (6 rows of n floats) with the plain kernel and trains on
+ * the first n - max(32, n/16), validating on the rest; loss_curve[e] = mean
+ * squared error of epoch e (up to loss_cap entries, may be NULL). */
+int bude_mi355x_screen_train(bude_screen *s, const float *const poses[6], size_t n, BudeScreenTrainStats *stats,
+                             float *loss_curve, size_t loss_cap);
+/* Makes n poses resident for screening. */
+int bude_mi355x_screen_load(bude_screen *s, const float *const poses[6], size_t n);
+/* Screens the loaded poses (see above). */
+int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res);
+/* The last screen's selected pose indices (ascending) and their exact energies. */
+int bude_mi355x_screen_selected(bude_screen *s, uint32_t *idx, float *energies, size_t cap, size_t *n);
+/* Surrogate prediction for every loaded pose, in energy units of the clipped scale. */
+int bude_mi355x_screen_predict(bude_screen *s, float *pred);
+/* Plain kernel on every loaded pose: the exhaustive baseline and judge. */
+int bude_mi355x_screen_exhaustive(bude_screen *s, float *energies, double *ms);
+int bude_mi355x_screen_destroy(bude_screen *s);
+/* One forward + backward pass of the surrogate on `batch` rows (a multiple
+ * of 32) of 21 features with caller weights W_l [in][out] (21x256, 256x256,
+ * 256x256, 256x1) and biases: predictions, mean squared error against
+ * target, and the FP32 gradients of that loss.  Exposes the training
+ * kernels to gradient checks. */
+int bude_mi355x_mlp_forward_backward(int device, size_t batch, const float *features, const float *target,
+                                     const float *const weights[4], const float *const biases[4], float *pred,
+                                     float *loss, float *const grad_w[4], float *const grad_b[4]);
+/* The fused inference kernel the screen uses, on n poses (6 rows of n
+ * floats) with caller weights as above: predictions per pose. */
+int bude_mi355x_mlp_predict(int device, const float *const poses[6], size_t n, const float *const weights[4],
+                            const float *const biases[4], float *pred);
+
 /* ---- deck tooling: the synthetic generator behind the benchmark decks
  *      (the role makedeck/main.cpp:49-234 plays for bm1/bm2).  Every value
  *      comes from std::mt19937 words through fixed integer/IEEE mappings, so

@@ -42,6 +42,8 @@ LIB_SOURCES = [
     "capi_plain.hip",
     "fasten_batch.hip",
     "capi_batch.hip",
+    "mlp.hip",
+    "capi_screen.hip",
 ]
 
 

@@ -45,6 +45,16 @@ EXPORTED = (
     "bude_mi355x_batch_energies",
     "bude_mi355x_batch_clear",
     "bude_mi355x_batch_destroy",
+    "bude_mi355x_screen_create",
+    "bude_mi355x_screen_train",
+    "bude_mi355x_screen_load",
+    "bude_mi355x_screen_run",
+    "bude_mi355x_screen_selected",
+    "bude_mi355x_screen_predict",
+    "bude_mi355x_screen_exhaustive",
+    "bude_mi355x_screen_destroy",
+    "bude_mi355x_mlp_forward_backward",
+    "bude_mi355x_mlp_predict",
     "bude_synth_forcefield",
     "bude_synth_protein",
     "bude_synth_ligand",

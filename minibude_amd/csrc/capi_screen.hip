@@ -1,0 +1,714 @@
+// capi_screen.hip -- C ABI of the learned prescreen: an MLP pose-energy
+// surrogate trained on the GPU (own BF16-MFMA forward/backward kernels,
+// Adam) on energies the plain kernel computes, then used to screen a pose
+// set: surrogate on every pose, exact plain-kernel rescoring of the lowest
+// keep_fraction, plain kernel on everything when the surrogate's validation
+// error is above the configured bound (fallback).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "bude_internal.h"
+#include "fasten_plain.h"
+#include "hip_util.h"
+#include "mlp.h"
+
+namespace {
+
+using bude::DeviceBuffer;
+using bude::mlp::bf16;
+constexpr int kK[4] = {bude::mlp::kIn, bude::mlp::kHidden, bude::mlp::kHidden, bude::mlp::kHidden};
+constexpr int kN[4] = {bude::mlp::kHidden, bude::mlp::kHidden, bude::mlp::kHidden, bude::mlp::kOut};
+
+struct Net {
+  DeviceBuffer w[4], b[4], mw[4], vw[4], mb[4], vb[4], gw[4], gb[4], w16[4], w16t[4];
+  int step = 0;
+};
+
+// Activations of one batch of `rows` poses (training keeps everything the
+// backward pass reads; inference uses feat, a[0], a[1] as ping-pong).
+struct Acts {
+  size_t rows = 0;
+  DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss;
+};
+
+int alloc_acts(Acts &A, size_t rows, bool training) {
+  A.rows = rows;
+  const size_t H = bude::mlp::kHidden, O = bude::mlp::kOut, I = bude::mlp::kIn;
+  int st = A.feat.alloc(rows * I * 2);
+  if (!st) st = A.pred.alloc(rows * 4);
+  for (int l = 0; l < 3 && !st; l++) st = A.a[l].alloc(rows * H * 2);
+  if (training) {
+    if (!st) st = A.feat_t.alloc(rows * I * 2);
+    for (int l = 0; l < 3 && !st; l++) {
+      st = A.z[l].alloc(rows * H * 4);
+      if (!st) st = A.a_t[l].alloc(rows * H * 2);
+      if (!st) st = A.dz[l].alloc(rows * H * 2);
+      if (!st) st = A.dz_t[l].alloc(rows * H * 2);
+    }
+    if (!st) st = A.dz3.alloc(rows * O * 2);
+    if (!st) st = A.dz3_t.alloc(rows * O * 2);
+    if (!st) st = A.loss.alloc(256);
+  }
+  return st;
+}
+
+bude::mlp::GemmArgs gemm_args(const void *a, const void *b, int M, int N, int K, int lda, int ldb, int epi) {
+  bude::mlp::GemmArgs g{};
+  g.a = static_cast<const bf16 *>(a);
+  g.b = static_cast<const bf16 *>(b);
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.lda = lda;
+  g.ldb = ldb;
+  g.epi = epi;
+  return g;
+}
+
+// Forward over `rows` rows already featurised in A.feat; training stores
+// z, a, a^T for the backward pass.
+int forward(const Net &net, Acts &A, size_t rows, bool training, hipStream_t s) {
+  const int R = int(rows);
+  const void *in = A.feat.ptr;
+  int in_ld = bude::mlp::kIn;
+  for (int l = 0; l < 3; l++) {
+    auto g = gemm_args(in, net.w16t[l].ptr, R, kN[l], kK[l], in_ld, kK[l], bude::mlp::kEpiHidden);
+    g.bias = static_cast<const float *>(net.b[l].ptr);
+    g.out_f32 = training ? static_cast<float *>(A.z[l].ptr) : nullptr;
+    g.out_bf16 = static_cast<bf16 *>(A.a[training ? l : (l & 1)].ptr);
+    g.out_t = training ? static_cast<bf16 *>(A.a_t[l].ptr) : nullptr;
+    g.ldo = kN[l];
+    g.ldt = R;
+    if (int st = bude::hip_check(bude::mlp::gemm(g, s), "mlp gemm forward")) return st;
+    in = g.out_bf16;
+    in_ld = kN[l];
+  }
+  auto g = gemm_args(in, net.w16t[3].ptr, R, kN[3], kK[3], in_ld, kK[3], bude::mlp::kEpiOut);
+  g.bias = static_cast<const float *>(net.b[3].ptr);
+  g.out_f32 = static_cast<float *>(A.pred.ptr);
+  return bude::hip_check(bude::mlp::gemm(g, s), "mlp gemm output");
+}
+
+// Backward from dz3 (already written by the loss kernel) into net.gw / gb.
+int backward(Net &net, Acts &A, size_t rows, hipStream_t s) {
+  const int R = int(rows);
+  const void *dz_t_next = A.dz3_t.ptr;  // [N_l][R]
+  const void *dz_next = A.dz3.ptr;      // [R][N_l]
+  for (int l = 3; l >= 0; l--) {
+    // dW_l = A_{l-1}^T dZ_l  -> [K_l][N_l]
+    const void *prev_t = l == 0 ? A.feat_t.ptr : A.a_t[l - 1].ptr;
+    auto gw = gemm_args(prev_t, dz_t_next, kK[l], kN[l], R, R, R, bude::mlp::kEpiF32);
+    gw.out_f32 = static_cast<float *>(net.gw[l].ptr);
+    gw.ldo = kN[l];
+    if (int st = bude::hip_check(bude::mlp::gemm(gw, s), "mlp gemm dW")) return st;
+    if (int st = bude::hip_check(bude::mlp::bias_grad(static_cast<const bf16 *>(dz_t_next), kN[l], R, R,
+                                                      static_cast<float *>(net.gb[l].ptr), s),
+                                 "mlp bias grad"))
+      return st;
+    if (l == 0) break;
+    // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1})  -> [R][K_l]
+    auto ga = gemm_args(dz_next, net.w16[l].ptr, R, kK[l], kN[l], kN[l], kN[l], bude::mlp::kEpiBackGelu);
+    ga.zprev = static_cast<const float *>(A.z[l - 1].ptr);
+    ga.out_bf16 = static_cast<bf16 *>(A.dz[l - 1].ptr);
+    ga.out_t = static_cast<bf16 *>(A.dz_t[l - 1].ptr);
+    ga.ldo = kK[l];
+    ga.ldt = R;
+    if (int st = bude::hip_check(bude::mlp::gemm(ga, s), "mlp gemm dA")) return st;
+    dz_next = A.dz[l - 1].ptr;
+    dz_t_next = A.dz_t[l - 1].ptr;
+  }
+  return BUDE_OK;
+}
+
+int adam_step(Net &net, float lr, hipStream_t s) {
+  net.step++;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float bc1 = 1.f - std::pow(b1, float(net.step)), bc2 = 1.f - std::pow(b2, float(net.step));
+  for (int l = 0; l < 4; l++) {
+    if (int st = bude::hip_check(
+            bude::mlp::adam(static_cast<float *>(net.w[l].ptr), static_cast<const float *>(net.gw[l].ptr),
+                            static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
+                            size_t(kK[l]) * kN[l], lr, b1, b2, eps, bc1, bc2, static_cast<bf16 *>(net.w16[l].ptr),
+                            static_cast<bf16 *>(net.w16t[l].ptr), kK[l], kN[l], s),
+            "adam W"))
+      return st;
+    if (int st = bude::hip_check(
+            bude::mlp::adam(static_cast<float *>(net.b[l].ptr), static_cast<const float *>(net.gb[l].ptr),
+                            static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), size_t(kN[l]),
+                            lr, b1, b2, eps, bc1, bc2, nullptr, nullptr, 1, kN[l], s),
+            "adam b"))
+      return st;
+  }
+  return BUDE_OK;
+}
+
+// Upload FP32 weights W_l [kK[l]][kN[l]] (zero padded) and biases; refresh BF16 copies.
+int set_weights(Net &net, const std::vector<std::vector<float>> &w, const std::vector<std::vector<float>> &b,
+                hipStream_t s) {
+  for (int l = 0; l < 4; l++) {
+    const size_t nw = size_t(kK[l]) * kN[l];
+    int st = net.w[l].upload(w[l].data(), nw * 4);
+    if (!st) st = net.b[l].upload(b[l].data(), size_t(kN[l]) * 4);
+    for (DeviceBuffer *d : {&net.mw[l], &net.vw[l], &net.gw[l]})
+      if (!st) {
+        st = d->alloc(nw * 4);
+        if (!st) st = bude::hip_check(hipMemset(d->ptr, 0, nw * 4), "hipMemset");
+      }
+    for (DeviceBuffer *d : {&net.mb[l], &net.vb[l], &net.gb[l]})
+      if (!st) {
+        st = d->alloc(size_t(kN[l]) * 4);
+        if (!st) st = bude::hip_check(hipMemset(d->ptr, 0, size_t(kN[l]) * 4), "hipMemset");
+      }
+    if (!st) st = net.w16[l].alloc(nw * 2);
+    if (!st) st = net.w16t[l].alloc(nw * 2);
+    if (!st)
+      st = bude::hip_check(bude::mlp::to_bf16(static_cast<const float *>(net.w[l].ptr), kK[l], kN[l],
+                                              static_cast<bf16 *>(net.w16[l].ptr),
+                                              static_cast<bf16 *>(net.w16t[l].ptr), s),
+                           "to_bf16");
+    if (st) return st;
+  }
+  net.step = 0;
+  return bude::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+// He-normal hidden weights, small output weights, zero biases; only the 12
+// real input rows and the one real output column are non-zero.
+void init_weights(uint64_t seed, std::vector<std::vector<float>> &w, std::vector<std::vector<float>> &b) {
+  std::mt19937_64 g(seed ^ 0x5eedf00dull);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  w.assign(4, {});
+  b.assign(4, {});
+  for (int l = 0; l < 4; l++) {
+    w[l].assign(size_t(kK[l]) * kN[l], 0.f);
+    b[l].assign(size_t(kN[l]), 0.f);
+    const int fan_in = l == 0 ? bude::mlp::kFeat : kK[l];
+    const int rows = l == 0 ? bude::mlp::kFeat : kK[l];
+    const int cols = l == 3 ? 1 : kN[l];
+    const float sd = l == 3 ? std::sqrt(1.f / float(fan_in)) : std::sqrt(2.f / float(fan_in));
+    for (int r = 0; r < rows; r++)
+      for (int c = 0; c < cols; c++) w[l][size_t(r) * kN[l] + c] = sd * nd(g);
+  }
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct bude_screen {
+  int device = 0;
+  int cus = 0;
+  BudeScreenConfig cfg{};
+  // plain-kernel deck (labels, rescoring, fallback)
+  size_t natlig = 0, ngroups = 0;
+  DeviceBuffer protein, group_begin, coef, ligand;
+  // surrogate
+  Net net;
+  bool trained = false, fallback = false;
+  float clip = 0.f, mu = 0.f, sigma = 1.f, tail = -3.0e38f, val_rmse = 0.f;
+  // candidate poses
+  size_t ncand = 0;
+  DeviceBuffer cand, pred, sel_idx, sel_n, hist, gathered, exact;
+  size_t nsel = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {};
+
+  ~bude_screen() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  bude::PlainArgs plain(const float *poses, size_t stride, size_t n, float *energies) const {
+    bude::PlainArgs a{};
+    a.protein = static_cast<const float4 *>(protein.ptr);
+    a.group_begin = static_cast<const int32_t *>(group_begin.ptr);
+    a.coef = static_cast<const bude::PairCoef *>(coef.ptr);
+    a.ligand = static_cast<const float4 *>(ligand.ptr);
+    a.poses = poses;
+    a.energies = energies;
+    a.stride = stride;
+    a.nposes = n;
+    a.natlig = int(natlig);
+    a.ngroups = int(ngroups);
+    a.ppwi = 1;
+    a.form = 0;
+    return a;
+  }
+};
+
+namespace {
+
+int upload_poses(DeviceBuffer &d, const float *const poses[6], size_t n) {
+  if (int st = d.alloc(6 * n * sizeof(float))) return st;
+  for (int k = 0; k < 6; k++)
+    if (int st = bude::hip_check(hipMemcpy(static_cast<float *>(d.ptr) + size_t(k) * n, poses[k], n * sizeof(float),
+                                           hipMemcpyHostToDevice),
+                                 "hipMemcpy poses"))
+      return st;
+  return BUDE_OK;
+}
+
+// Surrogate predictions (normalised target units) for n poses whose 6 rows
+// start at `poses` with row stride `stride`: the fused inference kernel on a
+// persistent grid of 2 work-groups per CU.
+int predict_all(bude_screen *s, const float *poses, size_t stride, size_t n, float *out) {
+  const bf16 *wt[4] = {static_cast<const bf16 *>(s->net.w16t[0].ptr), static_cast<const bf16 *>(s->net.w16t[1].ptr),
+                       static_cast<const bf16 *>(s->net.w16t[2].ptr), static_cast<const bf16 *>(s->net.w16t[3].ptr)};
+  const float *bias[4] = {static_cast<const float *>(s->net.b[0].ptr), static_cast<const float *>(s->net.b[1].ptr),
+                          static_cast<const float *>(s->net.b[2].ptr), static_cast<const float *>(s->net.b[3].ptr)};
+  if (int st = bude::hip_check(bude::mlp::infer(poses, stride, n, wt, static_cast<const bf16 *>(s->net.w16[3].ptr),
+                                                bias, out, 2 * s->cus, s->stream),
+                               "mlp infer"))
+    return st;
+  return bude::hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+}
+
+}  // namespace
+
+extern "C" int bude_mi355x_screen_create(int device, const BudeAtom *protein, size_t natpro, const BudeAtom *ligand,
+                                         size_t natlig, const BudeFFParams *forcefield, size_t ntypes,
+                                         const BudeScreenConfig *cfg, bude_screen **out) {
+  if (!out || !cfg) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_create: NULL argument");
+  *out = nullptr;
+  if (int st = bude::check_deck(protein, natpro, ligand, natlig, forcefield, ntypes, "bude_mi355x_screen_create"))
+    return st;
+  if (cfg->batch < 32 || cfg->batch % 32 || cfg->batch > (1u << 20) || !(cfg->keep_fraction > 0.f) ||
+      cfg->keep_fraction > 1.f || !(cfg->lr > 0.f) || !(cfg->clip_quantile > 0.f) || cfg->clip_quantile > 1.f ||
+      cfg->tail_quantile < 0.f || cfg->tail_quantile > 1.f || !(cfg->tail_weight > 0.f))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "bude_mi355x_screen_create: batch must be a multiple of 32 in [32, 2^20], keep_fraction and "
+                      "clip_quantile in (0, 1], tail_quantile in [0, 1], lr > 0, tail_weight > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || device < 0 || device >= ndev)
+    return bude::fail(BUDE_ERR_NO_DEVICE, "bude_mi355x_screen_create: no such HIP device");
+  if (int st = bude::hip_check(hipSetDevice(device), "hipSetDevice")) return st;
+
+  auto s = std::make_unique<bude_screen>();
+  s->device = device;
+  s->cfg = *cfg;
+  hipDeviceProp_t props{};
+  if (int st = bude::hip_check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties")) return st;
+  s->cus = props.multiProcessorCount;
+  s->natlig = natlig;
+  const bude::ProteinGroups groups = bude::group_protein(protein, natpro);
+  s->ngroups = groups.ngroups();
+  const std::vector<bude::PairCoef> coef = bude::coef_table(ligand, natlig, groups, forcefield);
+  std::vector<float> lig4(natlig * 4);
+  for (size_t i = 0; i < natlig; i++) {
+    lig4[4 * i + 0] = ligand[i].x;
+    lig4[4 * i + 1] = ligand[i].y;
+    lig4[4 * i + 2] = ligand[i].z;
+  }
+  int st = s->protein.upload(groups.xyzw.data(), groups.xyzw.size() * 4);
+  if (!st) st = s->group_begin.upload(groups.begin.data(), groups.begin.size() * 4);
+  if (!st) st = s->coef.upload(coef.data(), coef.size() * sizeof(bude::PairCoef));
+  if (!st) st = s->ligand.upload(lig4.data(), lig4.size() * 4);
+  if (!st) st = s->hist.alloc((2048 + 64) * 4);
+  if (!st) st = s->sel_n.alloc(256);
+  if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto &e : s->ev)
+    if (!st) st = bude::hip_check(hipEventCreate(&e), "hipEventCreate");
+  std::vector<std::vector<float>> w, b;
+  init_weights(cfg->seed, w, b);
+  if (!st) st = set_weights(s->net, w, b, s->stream);
+  if (st) return st;
+  *out = s.release();
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_screen_train(bude_screen *s, const float *const poses[6], size_t n,
+                                        BudeScreenTrainStats *stats, float *loss_curve, size_t loss_cap) {
+  if (!s || !poses || !stats) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_train: NULL argument");
+  const size_t B = s->cfg.batch;
+  const size_t nval = std::max<size_t>(32, n / 16);
+  if (n < 2 * B + nval || n > 0xffffffffu)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_train: need at least 2 batches + a validation "
+                                                 "split (n/16) of poses");
+  if (int st = bude::hip_check(hipSetDevice(s->device), "hipSetDevice")) return st;
+  *stats = BudeScreenTrainStats{};
+  hipStream_t st_ = s->stream;
+
+  // 1. exact labels from the plain kernel
+  DeviceBuffer tp, te;
+  if (int st = upload_poses(tp, poses, n)) return st;
+  if (int st = te.alloc(n * 4)) return st;
+  auto t0 = std::chrono::steady_clock::now();
+  const bude::PlainArgs pa = s->plain(static_cast<const float *>(tp.ptr), n, n, static_cast<float *>(te.ptr));
+  if (int st = bude::hip_check(bude::launch_plain(pa, 256, st_), "label launch")) return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize")) return st;
+  stats->label_ms = ms_since(t0);
+
+  // 2. target transform from the training split: clip at a quantile (the
+  //    screen ranks the low tail), standardise
+  const size_t ntrain = n - nval;
+  std::vector<float> e(n);
+  if (int st = bude::hip_check(hipMemcpy(e.data(), te.ptr, n * 4, hipMemcpyDeviceToHost), "hipMemcpy")) return st;
+  {
+    std::vector<float> sorted(e.begin(), e.begin() + long(ntrain));
+    const size_t q = std::min(ntrain - 1, size_t(double(s->cfg.clip_quantile) * double(ntrain)));
+    std::nth_element(sorted.begin(), sorted.begin() + long(q), sorted.end());
+    s->clip = sorted[q];
+    if (s->cfg.tail_quantile > 0.f) {
+      const size_t qt = std::min(ntrain - 1, size_t(double(s->cfg.tail_quantile) * double(ntrain)));
+      std::nth_element(sorted.begin(), sorted.begin() + long(qt), sorted.end());
+      s->tail = sorted[qt];
+    }
+    double sum = 0, sq = 0;
+    for (size_t i = 0; i < ntrain; i++) {
+      const double y = std::min(e[i], s->clip);
+      sum += y;
+      sq += y * y;
+    }
+    s->mu = float(sum / double(ntrain));
+    s->sigma = float(std::sqrt(std::max(1e-12, sq / double(ntrain) - double(s->mu) * s->mu)));
+  }
+
+  // 3. Adam over shuffled mini-batches; learning rate: linear warm-up then cosine decay
+  Acts A;
+  if (int st = alloc_acts(A, B, true)) return st;
+  std::vector<uint32_t> perm(ntrain);
+  std::iota(perm.begin(), perm.end(), 0u);
+  std::mt19937_64 rng(s->cfg.seed);
+  DeviceBuffer dperm;
+  if (int st = dperm.alloc(ntrain * 4)) return st;
+  const size_t steps_per_epoch = ntrain / B;
+  const size_t total = steps_per_epoch * std::max<uint32_t>(1, s->cfg.epochs);
+  const size_t warm = std::min<size_t>(200, total / 10 + 1);
+  t0 = std::chrono::steady_clock::now();
+  size_t step = 0;
+  for (uint32_t ep = 0; ep < std::max<uint32_t>(1, s->cfg.epochs); ep++) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    if (int st = bude::hip_check(hipMemcpyAsync(dperm.ptr, perm.data(), ntrain * 4, hipMemcpyHostToDevice, st_),
+                                 "hipMemcpyAsync"))
+      return st;
+    if (int st = bude::hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, st_), "hipMemsetAsync")) return st;
+    for (size_t it = 0; it < steps_per_epoch; it++, step++) {
+      const uint32_t *idx = static_cast<const uint32_t *>(dperm.ptr) + it * B;
+      if (int st = bude::hip_check(bude::mlp::features(static_cast<const float *>(tp.ptr), n, idx, B,
+                                                       static_cast<bf16 *>(A.feat.ptr),
+                                                       static_cast<bf16 *>(A.feat_t.ptr), int(B), st_),
+                                   "features"))
+        return st;
+      if (int st = forward(s->net, A, B, true, st_)) return st;
+      if (int st = bude::hip_check(
+              bude::mlp::loss_grad(static_cast<const float *>(A.pred.ptr), static_cast<const float *>(te.ptr), idx, B,
+                                   s->clip, s->mu, s->sigma, s->tail, s->cfg.tail_weight,
+                                   static_cast<bf16 *>(A.dz3.ptr),
+                                   static_cast<bf16 *>(A.dz3_t.ptr), int(B), static_cast<float *>(A.loss.ptr), st_),
+              "loss"))
+        return st;
+      if (int st = backward(s->net, A, B, st_)) return st;
+      const float warmf = step < warm ? float(step + 1) / float(warm) : 1.f;
+      const float prog = float(step) / float(std::max<size_t>(1, total - 1));
+      const float lr = s->cfg.lr * warmf * (0.05f + 0.95f * 0.5f * (1.f + std::cos(3.14159265f * prog)));
+      if (int st = adam_step(s->net, lr, st_)) return st;
+    }
+    float lsum = 0.f;
+    if (int st = bude::hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 4, hipMemcpyDeviceToHost, st_), "hipMemcpyAsync"))
+      return st;
+    if (int st = bude::hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize")) return st;
+    const float mse = lsum / float(steps_per_epoch * B);
+    if (loss_curve && ep < loss_cap) loss_curve[ep] = mse;
+    stats->final_loss = mse;
+  }
+  stats->train_ms = ms_since(t0);
+  stats->steps = uint32_t(step);
+
+  // 4. validation RMSE (normalised units) on the held-out split
+  DeviceBuffer vpred;
+  if (int st = vpred.alloc(nval * 4)) return st;
+  if (int st = predict_all(s, static_cast<const float *>(tp.ptr) + ntrain, n, nval, static_cast<float *>(vpred.ptr)))
+    return st;
+  {
+    std::vector<float> vp(nval);
+    if (int st = bude::hip_check(hipMemcpy(vp.data(), vpred.ptr, nval * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+      return st;
+    double se = 0;
+    for (size_t i = 0; i < nval; i++) {
+      const double y = (double(std::min(e[ntrain + i], s->clip)) - s->mu) / s->sigma;
+      se += (vp[i] - y) * (vp[i] - y);
+    }
+    s->val_rmse = float(std::sqrt(se / double(nval)));
+  }
+  s->trained = true;
+  s->fallback = !(s->val_rmse <= s->cfg.max_val_rmse);
+  stats->val_rmse = s->val_rmse;
+  stats->clip = s->clip;
+  stats->mu = s->mu;
+  stats->sigma = s->sigma;
+  stats->fallback = s->fallback ? 1 : 0;
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_screen_load(bude_screen *s, const float *const poses[6], size_t n) {
+  if (!s || !poses || n == 0 || n > 0xffffffffu)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_load: bad arguments");
+  if (int st = bude::hip_check(hipSetDevice(s->device), "hipSetDevice")) return st;
+  s->ncand = n;
+  s->nsel = 0;
+  int st = upload_poses(s->cand, poses, n);
+  if (!st) st = s->pred.alloc(n * 4);
+  if (!st) st = s->sel_idx.alloc(n * 4);
+  if (!st) st = s->gathered.alloc(6 * n * 4);
+  if (!st) st = s->exact.alloc(n * 4);
+  return st;
+}
+
+extern "C" int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res) {
+  if (!s || !res) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_run: NULL argument");
+  if (!s->trained || s->ncand == 0)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_run: train and load poses first");
+  if (int st = bude::hip_check(hipSetDevice(s->device), "hipSetDevice")) return st;
+  *res = BudeScreenResult{};
+  const size_t n = s->ncand;
+  const size_t k = std::max<size_t>(1, size_t(std::ceil(double(s->cfg.keep_fraction) * double(n))));
+  hipStream_t st_ = s->stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  float ms01 = 0, ms12 = 0, ms23 = 0;
+  if (s->fallback) {
+    // plain kernel on every pose, then the same selection on exact energies
+    if (int st = bude::hip_check(hipEventRecord(s->ev[0], st_), "hipEventRecord")) return st;
+    const bude::PlainArgs pa = s->plain(static_cast<const float *>(s->cand.ptr), n, n, static_cast<float *>(s->pred.ptr));
+    if (int st = bude::hip_check(bude::launch_plain(pa, 256, st_), "fallback launch")) return st;
+    if (int st = bude::hip_check(hipEventRecord(s->ev[1], st_), "hipEventRecord")) return st;
+  } else {
+    if (int st = bude::hip_check(hipEventRecord(s->ev[0], st_), "hipEventRecord")) return st;
+    if (int st = predict_all(s, static_cast<const float *>(s->cand.ptr), n, n, static_cast<float *>(s->pred.ptr)))
+      return st;
+    if (int st = bude::hip_check(hipEventRecord(s->ev[1], st_), "hipEventRecord")) return st;
+  }
+  if (int st = bude::hip_check(bude::mlp::select_lowest(static_cast<const float *>(s->pred.ptr), n, k,
+                                                        static_cast<uint32_t *>(s->hist.ptr),
+                                                        static_cast<uint32_t *>(s->sel_idx.ptr),
+                                                        static_cast<uint32_t *>(s->sel_n.ptr), st_),
+                               "select"))
+    return st;
+  uint32_t nsel = 0;
+  if (int st = bude::hip_check(hipMemcpyAsync(&nsel, s->sel_n.ptr, 4, hipMemcpyDeviceToHost, st_), "hipMemcpyAsync"))
+    return st;
+  if (int st = bude::hip_check(hipEventRecord(s->ev[2], st_), "hipEventRecord")) return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize")) return st;
+  s->nsel = nsel;
+  if (!s->fallback) {  // (in fallback the exact energies are already in pred)
+    if (int st = bude::hip_check(bude::mlp::gather_poses(static_cast<const float *>(s->cand.ptr), n,
+                                                         static_cast<const uint32_t *>(s->sel_idx.ptr),
+                                                         static_cast<const uint32_t *>(s->sel_n.ptr), nsel,
+                                                         static_cast<float *>(s->gathered.ptr), nsel, st_),
+                                 "gather"))
+      return st;
+    const bude::PlainArgs pa = s->plain(static_cast<const float *>(s->gathered.ptr), nsel, nsel,
+                                        static_cast<float *>(s->exact.ptr));
+    if (int st = bude::hip_check(bude::launch_plain(pa, 256, st_), "rescore launch")) return st;
+  }
+  if (int st = bude::hip_check(hipEventRecord(s->ev[3], st_), "hipEventRecord")) return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize")) return st;
+  (void)hipEventElapsedTime(&ms01, s->ev[0], s->ev[1]);
+  (void)hipEventElapsedTime(&ms12, s->ev[1], s->ev[2]);
+  (void)hipEventElapsedTime(&ms23, s->ev[2], s->ev[3]);
+  res->total_ms = ms_since(t0);
+  res->score_ms = ms01;
+  res->select_ms = ms12;
+  res->rescore_ms = ms23;
+  res->nposes = n;
+  res->nselected = nsel;
+  res->fallback = s->fallback ? 1 : 0;
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_screen_selected(bude_screen *s, uint32_t *idx, float *energies, size_t cap, size_t *n) {
+  if (!s || !n) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_selected: NULL argument");
+  *n = s->nsel;
+  if (!idx && !energies) return bude::ok();
+  if (cap < s->nsel) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_selected: buffer too small");
+  if (int st = bude::hip_check(hipSetDevice(s->device), "hipSetDevice")) return st;
+  std::vector<uint32_t> id(s->nsel);
+  if (int st = bude::hip_check(hipMemcpy(id.data(), s->sel_idx.ptr, s->nsel * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+    return st;
+  std::vector<float> ex(s->nsel);
+  if (s->fallback) {
+    std::vector<float> all(s->ncand);
+    if (int st = bude::hip_check(hipMemcpy(all.data(), s->pred.ptr, s->ncand * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+      return st;
+    for (size_t j = 0; j < s->nsel; j++) ex[j] = all[id[j]];
+  } else if (s->nsel) {
+    if (int st = bude::hip_check(hipMemcpy(ex.data(), s->exact.ptr, s->nsel * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+      return st;
+  }
+  // report in pose order
+  std::vector<size_t> ord(s->nsel);
+  std::iota(ord.begin(), ord.end(), size_t(0));
+  std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return id[a] < id[b]; });
+  for (size_t j = 0; j < s->nsel; j++) {
+    if (idx) idx[j] = id[ord[j]];
+    if (energies) energies[j] = ex[ord[j]];
+  }
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_screen_predict(bude_screen *s, float *pred) {
+  if (!s || !pred || s->ncand == 0)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_predict: load poses first");
+  if (int st = bude::hip_check(hipSetDevice(s->device), "hipSetDevice")) return st;
+  DeviceBuffer out;
+  if (int st = out.alloc(s->ncand * 4)) return st;
+  if (int st = predict_all(s, static_cast<const float *>(s->cand.ptr), s->ncand, s->ncand,
+                           static_cast<float *>(out.ptr)))
+    return st;
+  std::vector<float> p(s->ncand);
+  if (int st = bude::hip_check(hipMemcpy(p.data(), out.ptr, s->ncand * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+    return st;
+  for (size_t i = 0; i < s->ncand; i++) pred[i] = p[i] * s->sigma + s->mu;  // back to energy units (clipped scale)
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_screen_exhaustive(bude_screen *s, float *energies, double *ms) {
+  if (!s || s->ncand == 0) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_exhaustive: no poses");
+  if (int st = bude::hip_check(hipSetDevice(s->device), "hipSetDevice")) return st;
+  DeviceBuffer out;
+  if (int st = out.alloc(s->ncand * 4)) return st;
+  const bude::PlainArgs pa = s->plain(static_cast<const float *>(s->cand.ptr), s->ncand, s->ncand,
+                                      static_cast<float *>(out.ptr));
+  if (int st = bude::hip_check(hipEventRecord(s->ev[0], s->stream), "hipEventRecord")) return st;
+  if (int st = bude::hip_check(bude::launch_plain(pa, 256, s->stream), "exhaustive launch")) return st;
+  if (int st = bude::hip_check(hipEventRecord(s->ev[1], s->stream), "hipEventRecord")) return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize")) return st;
+  float t = 0.f;
+  (void)hipEventElapsedTime(&t, s->ev[0], s->ev[1]);
+  if (ms) *ms = t;
+  if (energies)
+    return bude::hip_check(hipMemcpy(energies, out.ptr, s->ncand * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_screen_destroy(bude_screen *s) {
+  if (!s) return bude::ok();
+  (void)hipSetDevice(s->device);
+  delete s;
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_mlp_forward_backward(int device, size_t batch, const float *features, const float *target,
+                                                const float *const weights[4], const float *const biases[4],
+                                                float *pred, float *loss, float *const grad_w[4],
+                                                float *const grad_b[4]) {
+  if (!features || !target || !weights || !biases || !pred || !loss || !grad_w || !grad_b || batch == 0 ||
+      batch % 32 || batch > (1u << 16))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "bude_mi355x_mlp_forward_backward: bad arguments (batch a multiple of 32, <= 65536)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || device < 0 || device >= ndev)
+    return bude::fail(BUDE_ERR_NO_DEVICE, "bude_mi355x_mlp_forward_backward: no such HIP device");
+  if (int st = bude::hip_check(hipSetDevice(device), "hipSetDevice")) return st;
+  hipStream_t s = nullptr;
+  if (int st = bude::hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) return st;
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{s};
+  // real shapes: 21x256, 256x256, 256x256, 256x1 -> padded device layout
+  const int realK[4] = {bude::mlp::kFeat, 256, 256, 256}, realN[4] = {256, 256, 256, 1};
+  std::vector<std::vector<float>> w(4), b(4);
+  for (int l = 0; l < 4; l++) {
+    w[l].assign(size_t(kK[l]) * kN[l], 0.f);
+    b[l].assign(size_t(kN[l]), 0.f);
+    for (int r = 0; r < realK[l]; r++)
+      for (int c = 0; c < realN[l]; c++) w[l][size_t(r) * kN[l] + c] = weights[l][size_t(r) * realN[l] + c];
+    for (int c = 0; c < realN[l]; c++) b[l][c] = biases[l][c];
+  }
+  Net net;
+  if (int st = set_weights(net, w, b, s)) return st;
+  Acts A;
+  if (int st = alloc_acts(A, batch, true)) return st;
+  std::vector<float> fpad(batch * bude::mlp::kIn, 0.f);
+  for (size_t i = 0; i < batch; i++)
+    for (int c = 0; c < bude::mlp::kFeat; c++) fpad[i * bude::mlp::kIn + c] = features[i * bude::mlp::kFeat + c];
+  DeviceBuffer f32, tgt;
+  if (int st = f32.upload(fpad.data(), fpad.size() * 4)) return st;
+  if (int st = tgt.upload(target, batch * 4)) return st;
+  if (int st = bude::hip_check(bude::mlp::to_bf16(static_cast<const float *>(f32.ptr), int(batch), bude::mlp::kIn,
+                                                  static_cast<bf16 *>(A.feat.ptr), static_cast<bf16 *>(A.feat_t.ptr),
+                                                  s),
+                               "to_bf16"))
+    return st;
+  if (int st = forward(net, A, batch, true, s)) return st;
+  if (int st = bude::hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, s), "hipMemsetAsync")) return st;
+  if (int st = bude::hip_check(
+          bude::mlp::loss_grad(static_cast<const float *>(A.pred.ptr), static_cast<const float *>(tgt.ptr), nullptr,
+                               batch, 3.0e38f, 0.f, 1.f, -3.0e38f, 1.f, static_cast<bf16 *>(A.dz3.ptr),
+                               static_cast<bf16 *>(A.dz3_t.ptr), int(batch), static_cast<float *>(A.loss.ptr), s),
+          "loss"))
+    return st;
+  if (int st = backward(net, A, batch, s)) return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
+  if (int st = bude::hip_check(hipMemcpy(pred, A.pred.ptr, batch * 4, hipMemcpyDeviceToHost), "hipMemcpy")) return st;
+  float lsum = 0.f;
+  if (int st = bude::hip_check(hipMemcpy(&lsum, A.loss.ptr, 4, hipMemcpyDeviceToHost), "hipMemcpy")) return st;
+  *loss = lsum / float(batch);
+  for (int l = 0; l < 4; l++) {
+    std::vector<float> gw(size_t(kK[l]) * kN[l]);
+    std::vector<float> gb(static_cast<size_t>(kN[l]));
+    if (int st = bude::hip_check(hipMemcpy(gw.data(), net.gw[l].ptr, gw.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+      return st;
+    if (int st = bude::hip_check(hipMemcpy(gb.data(), net.gb[l].ptr, gb.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+      return st;
+    for (int r = 0; r < realK[l]; r++)
+      for (int c = 0; c < realN[l]; c++) grad_w[l][size_t(r) * realN[l] + c] = gw[size_t(r) * kN[l] + c];
+    for (int c = 0; c < realN[l]; c++) grad_b[l][c] = gb[c];
+  }
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_mlp_predict(int device, const float *const poses[6], size_t n,
+                                       const float *const weights[4], const float *const biases[4], float *pred) {
+  if (!poses || !weights || !biases || !pred || n == 0 || n > 0xffffffffu)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_mlp_predict: bad arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || device < 0 || device >= ndev)
+    return bude::fail(BUDE_ERR_NO_DEVICE, "bude_mi355x_mlp_predict: no such HIP device");
+  if (int st = bude::hip_check(hipSetDevice(device), "hipSetDevice")) return st;
+  hipStream_t s = nullptr;
+  if (int st = bude::hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) return st;
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{s};
+  const int realK[4] = {bude::mlp::kFeat, 256, 256, 256}, realN[4] = {256, 256, 256, 1};
+  std::vector<std::vector<float>> w(4), b(4);
+  for (int l = 0; l < 4; l++) {
+    w[l].assign(size_t(kK[l]) * kN[l], 0.f);
+    b[l].assign(size_t(kN[l]), 0.f);
+    for (int r = 0; r < realK[l]; r++)
+      for (int c = 0; c < realN[l]; c++) w[l][size_t(r) * kN[l] + c] = weights[l][size_t(r) * realN[l] + c];
+    for (int c = 0; c < realN[l]; c++) b[l][c] = biases[l][c];
+  }
+  Net net;
+  if (int st = set_weights(net, w, b, s)) return st;
+  DeviceBuffer dp, out;
+  if (int st = upload_poses(dp, poses, n)) return st;
+  if (int st = out.alloc(n * 4)) return st;
+  hipDeviceProp_t props{};
+  if (int st = bude::hip_check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties")) return st;
+  const bf16 *wt[4] = {static_cast<const bf16 *>(net.w16t[0].ptr), static_cast<const bf16 *>(net.w16t[1].ptr),
+                       static_cast<const bf16 *>(net.w16t[2].ptr), static_cast<const bf16 *>(net.w16t[3].ptr)};
+  const float *bias[4] = {static_cast<const float *>(net.b[0].ptr), static_cast<const float *>(net.b[1].ptr),
+                          static_cast<const float *>(net.b[2].ptr), static_cast<const float *>(net.b[3].ptr)};
+  if (int st = bude::hip_check(bude::mlp::infer(static_cast<const float *>(dp.ptr), n, n, wt,
+                                                static_cast<const bf16 *>(net.w16[3].ptr), bias,
+                                                static_cast<float *>(out.ptr), 2 * props.multiProcessorCount, s),
+                               "mlp infer"))
+    return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
+  return bude::hip_check(hipMemcpy(pred, out.ptr, n * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+}

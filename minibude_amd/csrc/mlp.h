@@ -1,0 +1,88 @@
+// mlp.h -- the pose-energy surrogate: a 4-layer MLP (21 pose features ->
+// 256 -> 256 -> 256 -> 1, GELU) on BF16 MFMA with FP32 accumulation, its
+// training kernels (backward GEMMs, loss, Adam) and the top-k selection
+// used to pass the surrogate's best poses to exact rescoring.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bude {
+namespace mlp {
+
+using bf16 = uint16_t;  // storage type of BF16 tensors (round-to-nearest-even from FP32)
+
+constexpr int kFeat = 21;    // tilt, roll, pan, tx, ty, tz, sin x3, cos x3, rotation matrix x9
+constexpr int kIn = 32;      // feature width padded to the MFMA K step
+constexpr int kHidden = 256;
+constexpr int kOut = 32;     // output layer padded to one MFMA tile (column 0 is the energy)
+constexpr int kLayers = 4;
+
+// GEMM epilogues.
+enum Epilogue : int {
+  kEpiHidden = 0,  // z = acc + b[n]; a = gelu(z); store z (f32, optional), a (bf16), a^T (bf16, optional)
+  kEpiOut = 1,     // pred[m] = acc[m][0] + b[0]
+  kEpiBackGelu = 2,  // dz = acc * gelu'(zprev[m][n]); store dz (bf16) and dz^T (bf16)
+  kEpiF32 = 3,     // store acc (f32), e.g. a weight gradient
+};
+
+struct GemmArgs {
+  const bf16 *a;  // [M][lda], rows k-contiguous
+  const bf16 *b;  // [N][ldb], rows k-contiguous: C = A * B^T
+  int M, N, K, lda, ldb;
+  int epi;
+  const float *bias;   // [N]            (kEpiHidden, kEpiOut)
+  float *out_f32;      // [M][ldo]       (kEpiHidden z, kEpiF32 acc, kEpiOut pred[M])
+  bf16 *out_bf16;      // [M][ldo]       (kEpiHidden a, kEpiBackGelu dz)
+  bf16 *out_t;         // [N][ldt]       (transposed copy, optional)
+  const float *zprev;  // [M][ldo]       (kEpiBackGelu)
+  int ldo, ldt;
+};
+
+// C = A * B^T on 32x32 tiles (M, N, K multiples of 32).
+hipError_t gemm(const GemmArgs &g, hipStream_t s);
+
+// Features of poses idx[i] (or i when idx is null) from 6 pose rows of
+// `stride` floats: row i of out [n][kIn] (bf16), and out_t [kIn][ldt] if set.
+hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t,
+                    int ldt, hipStream_t s);
+
+// Weighted MSE on the transformed target: y = (min(E, clip) - mu) / sigma,
+// weight w = tail_weight when E < tail else 1.
+// dz[b][0] = 2 w (pred - y) / n, other columns 0 (also transposed);
+// loss_acc += sum (pred - y)^2 (f32 atomics, for monitoring).
+hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, float clip, float mu,
+                     float sigma, float tail, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt, float *loss_acc,
+                     hipStream_t s);
+
+// db[n] = sum_b dz_t[n][b] for n < N rows of length B.
+hipError_t bias_grad(const bf16 *dz_t, int N, int B, int ldt, float *db, hipStream_t s);
+
+// Adam step on n parameters; refreshes the BF16 copy w16 [rows][cols] and
+// its transpose w16t [cols][rows] (either may be null).
+hipError_t adam(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                float bc1, float bc2, bf16 *w16, bf16 *w16t, int rows, int cols, hipStream_t s);
+
+// FP32 -> BF16 of a rows x cols matrix and its transpose.
+hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s);
+
+// Top-k by value: threshold = k-th smallest of x[0..n) via three 11-bit
+// radix histogram passes on order-preserving keys, then every i with
+// x[i] <= threshold is appended to out_idx (count in *out_n).  hist needs
+// 2048 + 8 uint32; all device memory.
+hipError_t select_lowest(const float *x, size_t n, size_t k, uint32_t *hist, uint32_t *out_idx, uint32_t *out_n,
+                         hipStream_t s);
+
+// Fused inference: predictions (standardised units) of n poses through the
+// whole network, activations kept in LDS; wt[l] = W_l^T [N_l][K_l] (BF16),
+// w3 = W_3 [256][kOut] (column 0 used); `blocks` persistent work-groups.
+hipError_t infer(const float *poses, size_t stride, size_t n, const bf16 *const wt[4], const bf16 *w3,
+                 const float *const bias[4], float *out, int blocks, hipStream_t s);
+
+// Gather 6 pose rows for idx[0..n) into dst rows of stride dst_stride.
+hipError_t gather_poses(const float *src, size_t src_stride, const uint32_t *idx, const uint32_t *n_dev, size_t n_max,
+                        float *dst, size_t dst_stride, hipStream_t s);
+
+}  // namespace mlp
+}  // namespace bude

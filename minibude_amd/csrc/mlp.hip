@@ -1,0 +1,454 @@
+// mlp.hip -- kernels of the pose-energy surrogate (see mlp.h).
+//
+// GEMMs run on gfx950 BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
+// accumulate): one wavefront owns a 32x32 tile of C = A * B^T as 2x2 MFMA
+// blocks, both operands k-contiguous, so every fragment is one 16-byte load
+// per lane.  MFMA 16x16x32 lane layout (lane l, r = l % 16, g = l / 16):
+//   A fragment  A[r][8g .. 8g+7]       B fragment  B^T[8g .. 8g+7][r]
+//   accumulator C[4g .. 4g+3][r]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "mlp.h"
+
+namespace bude {
+namespace mlp {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16 to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return bf16(0x7fc0);  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                            // round to nearest even
+  return bf16(u >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+__device__ __forceinline__ bf16x8 load8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+// Surrogate inputs of pose q: angles / pi, translations / 5, sin and cos of
+// the angles, and the 9 entries of the rotation R = Rz(pan) Ry(roll) Rx(tilt)
+// they determine (src/serial/fasten.hpp:39-50); the rest of the kIn-wide row
+// is zero.
+__device__ __forceinline__ void pose_features(const float *__restrict__ poses, size_t stride, size_t q,
+                                              float (&f)[kIn]) {
+#pragma unroll
+  for (int c = 0; c < kIn; c++) f[c] = 0.f;
+  float sn[3], cs[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float ang = poses[size_t(a) * stride + q];
+    sincosf(ang, &sn[a], &cs[a]);
+    f[a] = ang * 0.31830988618379067f;
+    f[3 + a] = poses[size_t(3 + a) * stride + q] * 0.2f;
+    f[6 + a] = sn[a];
+    f[9 + a] = cs[a];
+  }
+  const float sx = sn[0], cx = cs[0], sy = sn[1], cy = cs[1], sz = sn[2], cz = cs[2];
+  f[12] = cy * cz;
+  f[13] = sx * sy * cz - cx * sz;
+  f[14] = cx * sy * cz + sx * sz;
+  f[15] = cy * sz;
+  f[16] = sx * sy * sz + cx * cz;
+  f[17] = cx * sy * sz - sx * cz;
+  f[18] = -sy;
+  f[19] = sx * cy;
+  f[20] = cx * cy;
+}
+
+__global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
+  const int lane = threadIdx.x;
+  const int n0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int r = lane & 15, grp = lane >> 4;
+  const bf16 *a0 = g.a + size_t(m0 + r) * g.lda + 8 * grp;
+  const bf16 *a1 = a0 + size_t(16) * g.lda;
+  const bf16 *b0 = g.b + size_t(n0 + r) * g.ldb + 8 * grp;
+  const bf16 *b1 = b0 + size_t(16) * g.ldb;
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k = 0; k < g.K; k += 32) {
+    const bf16x8 fa0 = load8(a0 + k), fa1 = load8(a1 + k);
+    const bf16x8 fb0 = load8(b0 + k), fb1 = load8(b1 + k);
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
+  }
+
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int n = n0 + 16 * j + r;
+      const int mb = m0 + 16 * i + 4 * grp;
+      if (g.epi == kEpiOut) {
+        if (n == 0)
+#pragma unroll
+          for (int q = 0; q < 4; q++) g.out_f32[mb + q] = acc[i][j][q] + g.bias[0];
+        continue;
+      }
+      bf16 t[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const size_t o = size_t(mb + q) * g.ldo + n;
+        const float c = acc[i][j][q];
+        if (g.epi == kEpiHidden) {
+          const float z = c + g.bias[n];
+          if (g.out_f32) g.out_f32[o] = z;
+          t[q] = to_bf16(gelu(z));
+          g.out_bf16[o] = t[q];
+        } else if (g.epi == kEpiBackGelu) {
+          t[q] = to_bf16(c * gelu_grad(g.zprev[o]));
+          g.out_bf16[o] = t[q];
+        } else {
+          g.out_f32[o] = c;
+        }
+      }
+      if (g.out_t && g.epi != kEpiF32) {
+        bf16 *dst = g.out_t + size_t(n) * g.ldt + mb;  // 4 consecutive rows -> one 8-byte store
+        *reinterpret_cast<uint2 *>(dst) =
+            make_uint2(uint32_t(t[0]) | (uint32_t(t[1]) << 16), uint32_t(t[2]) | (uint32_t(t[3]) << 16));
+      }
+    }
+  }
+}
+
+__global__ void features_kernel(const float *__restrict__ poses, size_t stride, const uint32_t *__restrict__ idx,
+                                size_t n, bf16 *__restrict__ out, bf16 *__restrict__ out_t, int ldt) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t q = idx ? idx[i] : i;
+  float f[kIn];
+  pose_features(poses, stride, q, f);
+  bf16 h[kIn];
+#pragma unroll
+  for (int c = 0; c < kIn; c++) h[c] = to_bf16(f[c]);
+  uint4 *dst = reinterpret_cast<uint4 *>(out + i * kIn);
+#pragma unroll
+  for (int v = 0; v < kIn / 8; v++)
+    dst[v] = make_uint4(uint32_t(h[8 * v + 0]) | (uint32_t(h[8 * v + 1]) << 16),
+                        uint32_t(h[8 * v + 2]) | (uint32_t(h[8 * v + 3]) << 16),
+                        uint32_t(h[8 * v + 4]) | (uint32_t(h[8 * v + 5]) << 16),
+                        uint32_t(h[8 * v + 6]) | (uint32_t(h[8 * v + 7]) << 16));
+  if (out_t)
+#pragma unroll
+    for (int c = 0; c < kIn; c++) out_t[size_t(c) * ldt + i] = h[c];
+}
+
+__global__ void loss_grad_kernel(const float *__restrict__ pred, const float *__restrict__ energy,
+                                 const uint32_t *__restrict__ idx, size_t n, float clip, float mu, float inv_sigma,
+                                 float tail, float tail_weight, bf16 *__restrict__ dz, bf16 *__restrict__ dz_t,
+                                 int ldt, float *__restrict__ loss) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  float sq = 0.f;
+  if (b < n) {
+    const float e = energy[idx ? idx[b] : b];
+    const float y = (fminf(e, clip) - mu) * inv_sigma;
+    const float d = pred[b] - y;
+    sq = d * d;
+    const float wgt = e < tail ? tail_weight : 1.0f;
+    const bf16 g = to_bf16(2.0f * wgt * d / float(n));
+    dz[b * kOut] = g;
+#pragma unroll
+    for (int c = 1; c < kOut; c++) dz[b * kOut + c] = 0;
+    dz_t[b] = g;
+#pragma unroll
+    for (int c = 1; c < kOut; c++) dz_t[size_t(c) * ldt + b] = 0;
+  }
+  // wave reduction of the squared error, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off);
+  if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, sq);
+}
+
+__global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__ dz_t, int B, int ldt,
+                                                        float *__restrict__ db) {
+  __shared__ float part[256];
+  const int n = blockIdx.x;
+  float s = 0.f;
+  for (int b = threadIdx.x; b < B; b += 256) s += from_bf16(dz_t[size_t(n) * ldt + b]);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[n] = part[0];
+}
+
+__global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, size_t n, float lr, float b1, float b2, float eps, float bc1,
+                            float bc2, bf16 *__restrict__ w16, bf16 *__restrict__ w16t, int rows, int cols) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  w[i] = wi;
+  const bf16 h = to_bf16(wi);
+  if (w16) w16[i] = h;
+  if (w16t) w16t[size_t(i % cols) * rows + i / cols] = h;
+}
+
+__global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, bf16 *__restrict__ w16,
+                               bf16 *__restrict__ w16t) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(rows) * cols) return;
+  const bf16 h = to_bf16(w[i]);
+  if (w16) w16[i] = h;
+  if (w16t) w16t[size_t(i % cols) * rows + i / cols] = h;
+}
+
+// ---- top-k selection --------------------------------------------------------
+
+__device__ __forceinline__ uint32_t order_key(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// hist[0..2047] bins; hist[2048] prefix, hist[2049] mask, hist[2050] k remaining
+__global__ void select_init_kernel(uint32_t *hist, uint32_t k, uint32_t *out_n) {
+  for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
+  if (threadIdx.x == 0) {
+    hist[2048] = 0;
+    hist[2049] = 0;
+    hist[2050] = k;
+    *out_n = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void select_hist_kernel(const float *__restrict__ x, size_t n, int shift,
+                                                          uint32_t digit_mask, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t local[2048];
+  for (int i = threadIdx.x; i < 2048; i += 256) local[i] = 0;
+  __syncthreads();
+  const uint32_t prefix = hist[2048], mask = hist[2049];
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    const uint32_t key = order_key(x[i]);
+    if ((key & mask) == prefix) atomicAdd(&local[(key >> shift) & digit_mask], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2048; i += 256)
+    if (local[i]) atomicAdd(&hist[i], local[i]);
+}
+
+// One work-group: find the bin holding the k-th smallest, extend the prefix.
+__global__ __launch_bounds__(64) void select_pick_kernel(uint32_t *hist, int shift, int bits) {
+  if (threadIdx.x == 0) {
+    const uint32_t k = hist[2050];
+    uint32_t cum = 0, bin = 0;
+    const uint32_t nb = 1u << bits;
+    for (bin = 0; bin < nb; bin++) {
+      if (cum + hist[bin] >= k) break;
+      cum += hist[bin];
+    }
+    if (bin == nb) bin = nb - 1;
+    hist[2048] |= bin << shift;
+    hist[2049] |= (nb - 1) << shift;
+    hist[2050] = k - cum;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2048; i += 64) hist[i] = 0;
+}
+
+__global__ void select_compact_kernel(const float *__restrict__ x, size_t n, const uint32_t *__restrict__ hist,
+                                      uint32_t *__restrict__ out_idx, uint32_t *__restrict__ out_n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (order_key(x[i]) <= hist[2048]) out_idx[atomicAdd(out_n, 1u)] = uint32_t(i);
+}
+
+__global__ void gather_poses_kernel(const float *__restrict__ src, size_t src_stride, const uint32_t *__restrict__ idx,
+                                    const uint32_t *__restrict__ n_dev, float *__restrict__ dst, size_t dst_stride) {
+  const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= *n_dev) return;
+  const size_t q = idx[j];
+#pragma unroll
+  for (int k = 0; k < 6; k++) dst[size_t(k) * dst_stride + j] = src[size_t(k) * src_stride + q];
+}
+
+// ---- fused inference ---------------------------------------------------------
+// One work-group (4 waves) scores tiles of kTile poses through the whole
+// network with the activations in LDS: features -> 3 x (MFMA GEMM + bias +
+// GELU) -> output dot product; wave w owns output columns [64w, 64w + 64) of
+// every hidden layer.  Weights stream from L2 (k-contiguous rows of W^T).
+constexpr int kTile = 64;
+constexpr int kLdsRow = kHidden + 8;  // bf16 per LDS row (padding staggers banks)
+
+__device__ __forceinline__ bf16x8 lds_load8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+template <int K>
+__device__ __forceinline__ void fused_layer(const bf16 *__restrict__ in, bf16 *__restrict__ out,
+                                            const bf16 *__restrict__ wt, const float *__restrict__ bias, int lane,
+                                            int wave) {
+  const int r = lane & 15, grp = lane >> 4;
+  floatx4 acc[kTile / 16][4];
+#pragma unroll
+  for (int i = 0; i < kTile / 16; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int k = 0; k < K; k += 32) {
+    bf16x8 fb[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) fb[j] = load8(wt + size_t(64 * wave + 16 * j + r) * K + k + 8 * grp);
+#pragma unroll
+    for (int i = 0; i < kTile / 16; i++) {
+      const bf16x8 fa = lds_load8(in + (16 * i + r) * kLdsRow + k + 8 * grp);
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[j], acc[i][j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int n = 64 * wave + 16 * j + r;
+    const float bn = bias[n];
+#pragma unroll
+    for (int i = 0; i < kTile / 16; i++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) out[(16 * i + 4 * grp + q) * kLdsRow + n] = to_bf16(gelu(acc[i][j][q] + bn));
+  }
+}
+
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict__ poses, size_t stride, size_t n,
+                                                        const bf16 *__restrict__ w0t, const float *__restrict__ b0,
+                                                        const bf16 *__restrict__ w1t, const float *__restrict__ b1,
+                                                        const bf16 *__restrict__ w2t, const float *__restrict__ b2,
+                                                        const bf16 *__restrict__ w3, const float *__restrict__ b3,
+                                                        float *__restrict__ out) {
+  __shared__ bf16 act[2][kTile * kLdsRow];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const size_t ntiles = (n + kTile - 1) / kTile;
+  for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const size_t base = tile * kTile;
+    // features of the tile's poses (rows past n are zero)
+    if (tid < kTile) {
+      const size_t i = base + tid;
+      bf16 *row = act[0] + tid * kLdsRow;
+      float f[kIn];
+      if (i < n) {
+        pose_features(poses, stride, i, f);
+      } else {
+#pragma unroll
+        for (int c = 0; c < kIn; c++) f[c] = 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < kIn; c++) row[c] = to_bf16(f[c]);
+    }
+    __syncthreads();
+    fused_layer<kIn>(act[0], act[1], w0t, b0, lane, wave);
+    __syncthreads();
+    fused_layer<kHidden>(act[1], act[0], w1t, b1, lane, wave);
+    __syncthreads();
+    fused_layer<kHidden>(act[0], act[1], w2t, b2, lane, wave);
+    __syncthreads();
+    // output layer: 4 threads per pose, 64 inputs each, combined by shuffles
+    {
+      const int row = tid >> 2, part = tid & 3;
+      const bf16 *a = act[1] + row * kLdsRow + 64 * part;
+      float sum = 0.f;
+#pragma unroll 8
+      for (int c = 0; c < 64; c++) sum += from_bf16(a[c]) * from_bf16(w3[size_t(64 * part + c) * kOut]);
+      sum += __shfl_xor(sum, 1);
+      sum += __shfl_xor(sum, 2);
+      if (part == 0 && base + row < n) out[base + row] = sum + b3[0];
+    }
+    __syncthreads();
+  }
+}
+
+unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
+
+}  // namespace
+
+hipError_t gemm(const GemmArgs &g, hipStream_t s) {
+  if (g.M % 32 || g.N % 32 || g.K % 32 || g.lda % 8 || g.ldb % 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(gemm_nt_kernel, dim3(unsigned(g.N / 32), unsigned(g.M / 32)), dim3(64), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t, int ldt,
+                    hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(features_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, poses, stride, idx, n, out, out_t,
+                     ldt);
+  return hipGetLastError();
+}
+
+hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, float clip, float mu,
+                     float sigma, float tail, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt, float *loss_acc,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, energy, idx, n, clip, mu,
+                     1.0f / sigma, tail, tail_weight, dz, dz_t, ldt, loss_acc);
+  return hipGetLastError();
+}
+
+hipError_t bias_grad(const bf16 *dz_t, int N, int B, int ldt, float *db, hipStream_t s) {
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(unsigned(N)), dim3(256), 0, s, dz_t, B, ldt, db);
+  return hipGetLastError();
+}
+
+hipError_t adam(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                float bc1, float bc2, bf16 *w16, bf16 *w16t, int rows, int cols, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, eps, bc1, bc2,
+                     w16, w16t, rows, cols);
+  return hipGetLastError();
+}
+
+hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s) {
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(size_t(rows) * cols, 256)), dim3(256), 0, s, w, rows, cols, w16,
+                     w16t);
+  return hipGetLastError();
+}
+
+hipError_t select_lowest(const float *x, size_t n, size_t k, uint32_t *hist, uint32_t *out_idx, uint32_t *out_n,
+                         hipStream_t s) {
+  if (k == 0 || k > n || n > 0xffffffffu) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(select_init_kernel, dim3(1), dim3(256), 0, s, hist, uint32_t(k), out_n);
+  const unsigned grid = unsigned(std::min<size_t>(blocks_for(n, 256), 2048));
+  const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
+  for (int p = 0; p < 3; p++) {
+    hipLaunchKernelGGL(select_hist_kernel, dim3(grid), dim3(256), 0, s, x, n, shifts[p], (1u << bits[p]) - 1u, hist);
+    hipLaunchKernelGGL(select_pick_kernel, dim3(1), dim3(64), 0, s, hist, shifts[p], bits[p]);
+  }
+  hipLaunchKernelGGL(select_compact_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, n, hist, out_idx, out_n);
+  return hipGetLastError();
+}
+
+hipError_t infer(const float *poses, size_t stride, size_t n, const bf16 *const wt[4], const bf16 *w3,
+                 const float *const bias[4], float *out, int blocks, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t tiles = (n + kTile - 1) / kTile;
+  const unsigned grid = unsigned(std::min<size_t>(tiles, size_t(blocks)));
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(256), 0, s, poses, stride, n, wt[0], bias[0], wt[1], bias[1],
+                     wt[2], bias[2], w3, bias[3], out);
+  return hipGetLastError();
+}
+
+hipError_t gather_poses(const float *src, size_t src_stride, const uint32_t *idx, const uint32_t *n_dev, size_t n_max,
+                        float *dst, size_t dst_stride, hipStream_t s) {
+  if (n_max == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_poses_kernel, dim3(blocks_for(n_max, 256)), dim3(256), 0, s, src, src_stride, idx, n_dev,
+                     dst, dst_stride);
+  return hipGetLastError();
+}
+
+}  // namespace mlp
+}  // namespace bude
