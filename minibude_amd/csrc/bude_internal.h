@@ -75,6 +75,12 @@ ProteinGroups group_protein(const BudeAtom *protein, size_t natpro);
 std::vector<PairCoef> coef_table(const BudeAtom *ligand, size_t natlig, const ProteinGroups &groups,
                                  const BudeFFParams *ff);
 
+// Hinge form used by every kernel: 1 (hinges in d with per-type-pair
+// offsets, the default) or 0 (hinges in b = d - radij); the environment
+// variable BUDE_MI355X_PLAIN_FORM=b selects 0.  Both evaluate the same energy
+// to within FP32 rounding.
+int plain_form();
+
 // Argument checks shared by the entry points.
 int check_deck(const BudeAtom *protein, size_t natpro, const BudeAtom *ligand, size_t natlig,
                const BudeFFParams *ff, size_t ntypes, const char *who);

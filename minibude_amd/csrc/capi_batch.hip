@@ -109,6 +109,7 @@ int launch_once(bude_batch *b, const BudeBatchConfig *cfg) {
   a.stride = b->total_poses;
   a.ngroups = int(b->ngroups);
   a.ntiles = nt;
+  a.form = bude::plain_form();
   const int waves = b->cus * 4 * cfg->waves_per_simd;
   const int blocks = std::max(1, waves / (cfg->wgsize / 64));
   return bude::hip_check(bude::launch_batch(a, cfg->ppwi, cfg->wgsize, blocks, b->stream), "fasten_batch launch");
@@ -274,7 +275,7 @@ extern "C" int bude_mi355x_batch_run_separate(bude_batch *b, size_t ppwi, size_t
       a.natlig = m.natlig;
       a.ngroups = int(b->ngroups);
       a.ppwi = int(ppwi);
-      a.form = 0;
+      a.form = bude::plain_form();
       if (int st = bude::hip_check(bude::launch_plain(a, int(wgsize), b->stream), "fasten_plain launch")) return st;
     }
     return BUDE_OK;

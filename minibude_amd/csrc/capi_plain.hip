@@ -20,17 +20,6 @@ constexpr size_t kPpwi[] = {1, 2, 4, 8, 16, 32, 64, 128};  // src/main.cpp:48 DE
 bool ppwi_supported(size_t p) { return std::find(std::begin(kPpwi), std::end(kPpwi), p) != std::end(kPpwi); }
 bool wgsize_supported(size_t w) { return w == 64 || w == 128 || w == 256 || w == 512 || w == 1024; }
 
-// Hinge form of the plain kernel; BUDE_MI355X_PLAIN_FORM=b|d overrides the
-// default (both forms compute the same energies to within rounding).
-int plain_form() {
-  static const int form = [] {
-    const char *v = std::getenv("BUDE_MI355X_PLAIN_FORM");
-    if (v && (v[0] == 'd' || v[0] == 'D')) return 1;
-    return 0;
-  }();
-  return form;
-}
-
 double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -188,7 +177,7 @@ extern "C" int bude_mi355x_fasten(bude_ctx *ctx, size_t ppwi, size_t wgsize, siz
   a.natlig = int(ctx->natlig);
   a.ngroups = int(ctx->ngroups);
   a.ppwi = int(ppwi);
-  a.form = plain_form();
+  a.form = bude::plain_form();
 
   for (size_t i = 0; i < warmup + iterations; i++) {
     const auto t0 = std::chrono::steady_clock::now();

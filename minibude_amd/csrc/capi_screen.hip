@@ -37,7 +37,7 @@ struct Net {
 // backward pass reads; inference uses feat, a[0], a[1] as ping-pong).
 struct Acts {
   size_t rows = 0;
-  DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss;
+  DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss, work;
 };
 
 int alloc_acts(Acts &A, size_t rows, bool training) {
@@ -57,6 +57,11 @@ int alloc_acts(Acts &A, size_t rows, bool training) {
     if (!st) st = A.dz3.alloc(rows * O * 2);
     if (!st) st = A.dz3_t.alloc(rows * O * 2);
     if (!st) st = A.loss.alloc(256);
+    // split-K partial sums of the largest weight gradient
+    size_t wmax = 0;
+    for (int l = 0; l < 4; l++)
+      wmax = std::max(wmax, size_t(bude::mlp::auto_ksplit(kK[l], kN[l], int(rows))) * kK[l] * kN[l]);
+    if (!st) st = A.work.alloc(wmax * 4);
   }
   return st;
 }
@@ -101,6 +106,10 @@ int forward(const Net &net, Acts &A, size_t rows, bool training, hipStream_t s) 
 // Backward from dz3 (already written by the loss kernel) into net.gw / gb.
 int backward(Net &net, Acts &A, size_t rows, hipStream_t s) {
   const int R = int(rows);
+  bude::mlp::BiasGradBatch bg{};
+  bg.count = 4;
+  bg.B = R;
+  bg.ldt = R;
   const void *dz_t_next = A.dz3_t.ptr;  // [N_l][R]
   const void *dz_next = A.dz3.ptr;      // [R][N_l]
   for (int l = 3; l >= 0; l--) {
@@ -109,11 +118,12 @@ int backward(Net &net, Acts &A, size_t rows, hipStream_t s) {
     auto gw = gemm_args(prev_t, dz_t_next, kK[l], kN[l], R, R, R, bude::mlp::kEpiF32);
     gw.out_f32 = static_cast<float *>(net.gw[l].ptr);
     gw.ldo = kN[l];
+    gw.ksplit = bude::mlp::auto_ksplit(kK[l], kN[l], R);
+    gw.work = static_cast<float *>(A.work.ptr);
     if (int st = bude::hip_check(bude::mlp::gemm(gw, s), "mlp gemm dW")) return st;
-    if (int st = bude::hip_check(bude::mlp::bias_grad(static_cast<const bf16 *>(dz_t_next), kN[l], R, R,
-                                                      static_cast<float *>(net.gb[l].ptr), s),
-                                 "mlp bias grad"))
-      return st;
+    bg.dz_t[l] = static_cast<const bf16 *>(dz_t_next);
+    bg.db[l] = static_cast<float *>(net.gb[l].ptr);
+    bg.N[l] = kN[l];
     if (l == 0) break;
     // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1})  -> [R][K_l]
     auto ga = gemm_args(dz_next, net.w16[l].ptr, R, kK[l], kN[l], kN[l], kN[l], bude::mlp::kEpiBackGelu);
@@ -126,29 +136,24 @@ int backward(Net &net, Acts &A, size_t rows, hipStream_t s) {
     dz_next = A.dz[l - 1].ptr;
     dz_t_next = A.dz_t[l - 1].ptr;
   }
-  return BUDE_OK;
+  return bude::hip_check(bude::mlp::bias_grad_multi(bg, s), "mlp bias grads");
 }
 
 int adam_step(Net &net, float lr, hipStream_t s) {
   net.step++;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const float bc1 = 1.f - std::pow(b1, float(net.step)), bc2 = 1.f - std::pow(b2, float(net.step));
+  bude::mlp::AdamBatch ab{};
   for (int l = 0; l < 4; l++) {
-    if (int st = bude::hip_check(
-            bude::mlp::adam(static_cast<float *>(net.w[l].ptr), static_cast<const float *>(net.gw[l].ptr),
-                            static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
-                            size_t(kK[l]) * kN[l], lr, b1, b2, eps, bc1, bc2, static_cast<bf16 *>(net.w16[l].ptr),
-                            static_cast<bf16 *>(net.w16t[l].ptr), kK[l], kN[l], s),
-            "adam W"))
-      return st;
-    if (int st = bude::hip_check(
-            bude::mlp::adam(static_cast<float *>(net.b[l].ptr), static_cast<const float *>(net.gb[l].ptr),
-                            static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), size_t(kN[l]),
-                            lr, b1, b2, eps, bc1, bc2, nullptr, nullptr, 1, kN[l], s),
-            "adam b"))
-      return st;
+    ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(net.gw[l].ptr),
+                   static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
+                   static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), kK[l], kN[l]};
+    ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(net.gb[l].ptr),
+                       static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), nullptr, nullptr, 1,
+                       kN[l]};
   }
-  return BUDE_OK;
+  ab.count = 8;
+  return bude::hip_check(bude::mlp::adam_multi(ab, lr, b1, b2, eps, bc1, bc2, s), "adam");
 }
 
 // Upload FP32 weights W_l [kK[l]][kN[l]] (zero padded) and biases; refresh BF16 copies.
@@ -243,7 +248,7 @@ struct bude_screen {
     a.natlig = int(natlig);
     a.ngroups = int(ngroups);
     a.ppwi = 1;
-    a.form = 0;
+    a.form = bude::plain_form();
     return a;
   }
 };

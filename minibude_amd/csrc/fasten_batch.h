@@ -31,6 +31,7 @@ struct BatchArgs {
   size_t stride;
   int ngroups;
   int ntiles;
+  int form;                     // hinge form (plain_form())
 };
 
 bool batch_shape_supported(int rb, int wgsize);

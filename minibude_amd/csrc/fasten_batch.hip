@@ -76,9 +76,14 @@ __global__ __launch_bounds__(WG) void fasten_batch_kernel(
 
 template <int RB, int WG>
 hipError_t launch_rb_wg(const BatchArgs &a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormB>), dim3(unsigned(blocks)), dim3(WG), 0, s,
-                     a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.counter,
-                     a.stride, a.ngroups, a.ntiles);
+  if (a.form == kFormD)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormD>), dim3(unsigned(blocks)), dim3(WG), 0, s,
+                       a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.counter,
+                       a.stride, a.ngroups, a.ntiles);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormB>), dim3(unsigned(blocks)), dim3(WG), 0, s,
+                       a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.counter,
+                       a.stride, a.ngroups, a.ntiles);
   return hipGetLastError();
 }
 

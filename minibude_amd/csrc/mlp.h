@@ -38,10 +38,18 @@ struct GemmArgs {
   bf16 *out_t;         // [N][ldt]       (transposed copy, optional)
   const float *zprev;  // [M][ldo]       (kEpiBackGelu)
   int ldo, ldt;
+  int ksplit;          // kEpiF32 only: > 1 splits K over ksplit work-groups per tile
+  float *work;         // [ksplit][M][ldo] partial sums when ksplit > 1
 };
 
-// C = A * B^T on 32x32 tiles (M, N, K multiples of 32).
+// C = A * B^T on 32x32 tiles (M, N, K multiples of 32).  With ksplit > 1
+// (kEpiF32) every tile's K range is split over ksplit work-groups writing
+// partial sums to `work`, then summed in a fixed order (deterministic).
 hipError_t gemm(const GemmArgs &g, hipStream_t s);
+
+// Split count that gives a kEpiF32 GEMM about 1,024 waves (K split in
+// multiples of 256).
+int auto_ksplit(int M, int N, int K);
 
 // Features of poses idx[i] (or i when idx is null) from 6 pose rows of
 // `stride` floats: row i of out [n][kIn] (bf16), and out_t [kIn][ldt] if set.
@@ -63,6 +71,30 @@ hipError_t bias_grad(const bf16 *dz_t, int N, int B, int ldt, float *db, hipStre
 // its transpose w16t [cols][rows] (either may be null).
 hipError_t adam(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
                 float bc1, float bc2, bf16 *w16, bf16 *w16t, int rows, int cols, hipStream_t s);
+
+// One Adam step over up to 8 parameter tensors in a single launch.
+struct AdamTensor {
+  float *w;
+  const float *g;
+  float *m, *v;
+  bf16 *w16, *w16t;  // optional BF16 copies (row-major and transposed)
+  int rows, cols;    // w is rows x cols row-major
+};
+struct AdamBatch {
+  AdamTensor t[8];
+  int count;
+};
+hipError_t adam_multi(const AdamBatch &b, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                      hipStream_t s);
+
+// Bias gradients of up to 4 layers in one launch: db_l[n] = sum_b dz_t_l[n][b].
+struct BiasGradBatch {
+  const bf16 *dz_t[4];
+  float *db[4];
+  int N[4];
+  int count, B, ldt;
+};
+hipError_t bias_grad_multi(const BiasGradBatch &b, hipStream_t s);
 
 // FP32 -> BF16 of a rows x cols matrix and its transpose.
 hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s);

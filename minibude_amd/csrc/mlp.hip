@@ -29,10 +29,25 @@ __device__ __forceinline__ bf16 to_bf16(float f) {
 
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float(uint32_t(h) << 16); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU in its tanh form, 0.5 z (1 + tanh(u)) with u = sqrt(2/pi) (z +
+// 0.044715 z^3), written branch-free as z * sigmoid(2u) = z / (1 + 2^q) with
+// q = -2u * log2(e): 4 VALU + v_exp_f32 + v_rcp_f32.  Saturates correctly
+// (2^q -> inf gives 0, 2^q -> 0 gives z).
+constexpr float kG1 = -2.3022082f;   // -2 sqrt(2/pi) log2(e)
+constexpr float kG3 = -0.10294324f;  // -2 sqrt(2/pi) 0.044715 log2(e)
 
+__device__ __forceinline__ float gelu_sigma(float z) {
+  const float q = z * __builtin_fmaf(z * z, kG3, kG1);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(q));
+}
+
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
+
+// d/dz [z s(z)], s = sigmoid(2u): s + z s (1 - s) 2u'(z),
+// 2u' = 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2).
 __device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+  const float s = gelu_sigma(z);
+  return s + z * s * (1.0f - s) * __builtin_fmaf(z * z, 0.21406444f, 1.5957691f);
 }
 
 __device__ __forceinline__ bf16x8 load8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
@@ -70,6 +85,7 @@ __device__ __forceinline__ void pose_features(const float *__restrict__ poses, s
 __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
   const int lane = threadIdx.x;
   const int n0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int kc = g.K / g.ksplit, kbeg = blockIdx.z * kc, kend = kbeg + kc;
   const int r = lane & 15, grp = lane >> 4;
   const bf16 *a0 = g.a + size_t(m0 + r) * g.lda + 8 * grp;
   const bf16 *a1 = a0 + size_t(16) * g.lda;
@@ -81,7 +97,7 @@ __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k = 0; k < g.K; k += 32) {
+  for (int k = kbeg; k < kend; k += 32) {
     const bf16x8 fa0 = load8(a0 + k), fa1 = load8(a1 + k);
     const bf16x8 fb0 = load8(b0 + k), fb1 = load8(b1 + k);
     acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
@@ -115,6 +131,8 @@ __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
         } else if (g.epi == kEpiBackGelu) {
           t[q] = to_bf16(c * gelu_grad(g.zprev[o]));
           g.out_bf16[o] = t[q];
+        } else if (g.ksplit > 1) {
+          g.work[size_t(blockIdx.z) * size_t(g.M) * g.ldo + o] = c;
         } else {
           g.out_f32[o] = c;
         }
@@ -126,6 +144,15 @@ __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
       }
     }
   }
+}
+
+// out[i] = sum over z of work[z][i], z ascending (fixed order).
+__global__ void ksplit_reduce_kernel(const float *__restrict__ work, size_t n, int splits, float *__restrict__ out) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.f;
+  for (int z = 0; z < splits; z++) acc += work[size_t(z) * n + i];
+  out[i] = acc;
 }
 
 __global__ void features_kernel(const float *__restrict__ poses, size_t stride, const uint32_t *__restrict__ idx,
@@ -205,6 +232,44 @@ __global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, 
   const bf16 h = to_bf16(wi);
   if (w16) w16[i] = h;
   if (w16t) w16t[size_t(i % cols) * rows + i / cols] = h;
+}
+
+__global__ void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int t = 0;
+  for (; t < b.count; t++) {
+    const size_t n = size_t(b.t[t].rows) * b.t[t].cols;
+    if (i < n) break;
+    i -= n;
+  }
+  if (t >= b.count) return;
+  const AdamTensor &T = b.t[t];
+  const float gi = T.g[i];
+  const float mi = b1 * T.m[i] + (1.f - b1) * gi;
+  const float vi = b2 * T.v[i] + (1.f - b2) * gi * gi;
+  T.m[i] = mi;
+  T.v[i] = vi;
+  const float wi = T.w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  T.w[i] = wi;
+  const bf16 h = to_bf16(wi);
+  if (T.w16) T.w16[i] = h;
+  if (T.w16t) T.w16t[size_t(i % T.cols) * T.rows + i / T.cols] = h;
+}
+
+__global__ __launch_bounds__(256) void bias_grad_multi_kernel(const BiasGradBatch b) {
+  __shared__ float part[256];
+  int layer = 0, n = blockIdx.x;
+  while (layer < b.count - 1 && n >= b.N[layer]) n -= b.N[layer++];
+  const bf16 *row = b.dz_t[layer] + size_t(n) * b.ldt;
+  float s = 0.f;
+  for (int i = threadIdx.x; i < b.B; i += 256) s += from_bf16(row[i]);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) b.db[layer][n] = part[0];
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, bf16 *__restrict__ w16,
@@ -378,10 +443,27 @@ unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
 
 }  // namespace
 
-hipError_t gemm(const GemmArgs &g, hipStream_t s) {
-  if (g.M % 32 || g.N % 32 || g.K % 32 || g.lda % 8 || g.ldb % 8) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(gemm_nt_kernel, dim3(unsigned(g.N / 32), unsigned(g.M / 32)), dim3(64), 0, s, g);
+hipError_t gemm(const GemmArgs &g0, hipStream_t s) {
+  GemmArgs g = g0;
+  if (g.ksplit < 1) g.ksplit = 1;
+  if (g.M % 32 || g.N % 32 || g.K % 32 || g.lda % 8 || g.ldb % 8 || (g.K / 32) % g.ksplit) return hipErrorInvalidValue;
+  if (g.ksplit > 1 && (g.epi != kEpiF32 || !g.work)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(gemm_nt_kernel, dim3(unsigned(g.N / 32), unsigned(g.M / 32), unsigned(g.ksplit)), dim3(64), 0, s,
+                     g);
+  if (g.ksplit > 1) {
+    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+    const size_t n = size_t(g.M) * g.ldo;
+    hipLaunchKernelGGL(ksplit_reduce_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.work, n, g.ksplit,
+                       g.out_f32);
+  }
   return hipGetLastError();
+}
+
+int auto_ksplit(int M, int N, int K) {
+  const int tiles = (M / 32) * (N / 32);
+  int split = 1;
+  while (split * 2 * tiles <= 1024 && (K / 32) % (split * 2) == 0 && K / (split * 2) >= 256) split *= 2;
+  return split;
 }
 
 hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t, int ldt,
@@ -409,6 +491,21 @@ hipError_t adam(float *w, const float *g, float *m, float *v, size_t n, float lr
                 float bc1, float bc2, bf16 *w16, bf16 *w16t, int rows, int cols, hipStream_t s) {
   hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, eps, bc1, bc2,
                      w16, w16t, rows, cols);
+  return hipGetLastError();
+}
+
+hipError_t adam_multi(const AdamBatch &b, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                      hipStream_t s) {
+  size_t total = 0;
+  for (int t = 0; t < b.count; t++) total += size_t(b.t[t].rows) * b.t[t].cols;
+  hipLaunchKernelGGL(adam_multi_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, b, lr, b1, b2, eps, bc1, bc2);
+  return hipGetLastError();
+}
+
+hipError_t bias_grad_multi(const BiasGradBatch &b, hipStream_t s) {
+  int rows = 0;
+  for (int l = 0; l < b.count; l++) rows += b.N[l];
+  hipLaunchKernelGGL(bias_grad_multi_kernel, dim3(unsigned(rows)), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 
 namespace bude {
@@ -89,6 +90,14 @@ std::vector<PairCoef> coef_table(const BudeAtom *ligand, size_t natlig, const Pr
   for (size_t il = 0; il < natlig; il++)
     for (size_t g = 0; g < ng; g++) t[il * ng + g] = pair_coef(ff[groups.type[g]], ff[ligand[il].type]);
   return t;
+}
+
+int plain_form() {
+  static const int form = [] {
+    const char *v = std::getenv("BUDE_MI355X_PLAIN_FORM");
+    return (v && (v[0] == 'b' || v[0] == 'B')) ? 0 : 1;
+  }();
+  return form;
 }
 
 int check_deck(const BudeAtom *protein, size_t natpro, const BudeAtom *ligand, size_t natlig,

@@ -2,8 +2,8 @@
 (numpy, FP64) -- test infrastructure only, the checker of the GPU training
 kernels; the product never imports it.
 
-Network: 21 features -> 256 -> 256 -> 256 -> 1, GELU (erf form) on the three
-hidden layers, loss = mean((pred - target)^2).  With ``bf16=True`` values are
+Network: 21 features -> 256 -> 256 -> 256 -> 1, GELU (tanh form,
+0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))) on the three hidden layers, loss = mean((pred - target)^2).  With ``bf16=True`` values are
 rounded to BF16 (round to nearest even) exactly where the GPU stores BF16:
 inputs, weights, hidden activations and the back-propagated dZ; everything
 else is FP64.  With ``bf16=False`` it is the plain FP64 network, whose
@@ -15,9 +15,6 @@ import math
 
 import numpy as np
 
-_erf = np.vectorize(math.erf, otypes=[np.float64])
-
-
 def to_bf16(x: np.ndarray) -> np.ndarray:
     """Round to BF16 (nearest even) and return as float64."""
     f = np.asarray(x, np.float32)
@@ -26,12 +23,16 @@ def to_bf16(x: np.ndarray) -> np.ndarray:
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
+_C = math.sqrt(2.0 / math.pi)
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + _erf(z / math.sqrt(2.0)))
+    return 0.5 * z * (1.0 + np.tanh(_C * (z + 0.044715 * z ** 3)))
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + _erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
+    t = np.tanh(_C * (z + 0.044715 * z ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * _C * (1.0 + 3 * 0.044715 * z * z)
 
 
 def forward_backward(feat, target, W, b, bf16: bool = True):
