@@ -62,6 +62,9 @@ DEFAULT_WGSIZE = 256
 # persistent ragged-batch configuration chosen by bude_mi355x_batch_autotune on
 # the 256-ligand workload: ppwi, wgsize, waves per SIMD, heaviest-first, graph
 BATCH_CONFIG = (1, 64, 8, 0, 1)
+# screen operating points: the workload's 2 % first, then larger fractions
+# until the exact top-1 % recall reaches 0.99
+SCREEN_KEEP = (0.02, 0.03, 0.04, 0.05, 0.06, 0.08)
 TRAFFIC_SUMMARY = ROOT / "profiles" / "fasten_plain_c1_traffic.json"
 CPU_SAMPLE_POSES = 65536
 CPU_ITERS = 2
@@ -178,33 +181,44 @@ def screen() -> dict:
     from minibude_amd import screen as S
     w = S.screen_workload()
     cfg = S.default_config()
-    log(f"screen: training on {w.train_poses.shape[1]} poses, screening {w.screen_poses.shape[1]}")
+    n = w.screen_poses.shape[1]
+    log(f"screen: training on {w.train_poses.shape[1]} poses, screening {n}")
+    points = []
     with S.Screen(w.protein, w.ligand, w.forcefield, cfg, 0) as sc:
         st, _ = sc.train(w.train_poses)
         sc.load(w.screen_poses)
-        runs = [sc.run() for _ in range(3)]
-        r = min(runs, key=lambda x: x.total_ms)
-        idx, _ = sc.selected()
         exact, ex_ms = sc.exhaustive()
-    n = w.screen_poses.shape[1]
+        # the workload's operating point (2 % rescored), then the smallest
+        # fraction whose recall of the exact top 1 % reaches 0.99
+        for keep in SCREEN_KEEP:
+            sc.set_keep(keep)
+            r = min((sc.run() for _ in range(3)), key=lambda x: x.total_ms)
+            idx, _ = sc.selected()
+            rec = S.recall(idx, exact, 0.01)
+            points.append({
+                "keep_fraction": keep,
+                "poses_per_s": round(n / (r.total_ms * 1e-3), 1),
+                "screen_ms": round(r.total_ms, 3),
+                "surrogate_ms": round(r.score_ms, 3),
+                "select_ms": round(r.select_ms, 3),
+                "rescore_ms": round(r.rescore_ms, 3),
+                "speedup_vs_exhaustive": round(ex_ms / r.total_ms, 2),
+                "recall_top1pct": round(rec, 4),
+                "recall_top0.1pct": round(S.recall(idx, exact, 0.001), 4),
+                "fallback": bool(r.fallback),
+            })
+            log(f"screen: {points[-1]}")
+            if keep != SCREEN_KEEP[0] and rec >= 0.99:
+                break
     out = {
         "workload": (f"c3: 4,096-atom protein, 26-atom ligand; surrogate trained on {w.train_poses.shape[1]} plain-"
-                     f"kernel energies, {n} fresh poses screened, lowest {cfg.keep_fraction:.0%} rescored exactly"),
-        "poses_per_s": round(n / (r.total_ms * 1e-3), 1),
-        "screen_ms": round(r.total_ms, 3),
-        "surrogate_ms": round(r.score_ms, 3),
-        "select_ms": round(r.select_ms, 3),
-        "rescore_ms": round(r.rescore_ms, 3),
+                     f"kernel energies, {n} fresh poses screened, lowest keep_fraction rescored exactly"),
         "exhaustive_ms": round(ex_ms, 3),
         "exhaustive_poses_per_s": round(n / (ex_ms * 1e-3), 1),
-        "speedup_vs_exhaustive": round(ex_ms / r.total_ms, 2),
-        "recall_top1pct": round(S.recall(idx, exact, 0.01), 4),
-        "recall_top0.1pct": round(S.recall(idx, exact, 0.001), 4),
         "val_rmse": round(st.val_rmse, 4),
         "train_s": round((st.label_ms + st.train_ms) * 1e-3, 2),
-        "fallback": bool(r.fallback),
+        "points": points,
     }
-    log(f"screen: {out}")
     return out
 
 

@@ -192,6 +192,8 @@ int bude_mi355x_screen_train(bude_screen *s, const float *const poses[6], size_t
 int bude_mi355x_screen_load(bude_screen *s, const float *const poses[6], size_t n);
 /* Screens the loaded poses (see above). */
 int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res);
+/* Changes the fraction the next screen_run rescores exactly (no retraining). */
+int bude_mi355x_screen_set_keep(bude_screen *s, float keep_fraction);
 /* The last screen's selected pose indices (ascending) and their exact energies. */
 int bude_mi355x_screen_selected(bude_screen *s, uint32_t *idx, float *energies, size_t cap, size_t *n);
 /* Surrogate prediction for every loaded pose, in energy units of the clipped scale. */

@@ -49,6 +49,7 @@ EXPORTED = (
     "bude_mi355x_screen_train",
     "bude_mi355x_screen_load",
     "bude_mi355x_screen_run",
+    "bude_mi355x_screen_set_keep",
     "bude_mi355x_screen_selected",
     "bude_mi355x_screen_predict",
     "bude_mi355x_screen_exhaustive",

@@ -532,6 +532,13 @@ extern "C" int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res) {
   return bude::ok();
 }
 
+extern "C" int bude_mi355x_screen_set_keep(bude_screen *s, float keep_fraction) {
+  if (!s || !(keep_fraction > 0.f) || keep_fraction > 1.f)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_set_keep: keep_fraction must be in (0, 1]");
+  s->cfg.keep_fraction = keep_fraction;
+  return bude::ok();
+}
+
 extern "C" int bude_mi355x_screen_selected(bude_screen *s, uint32_t *idx, float *energies, size_t cap, size_t *n) {
   if (!s || !n) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_screen_selected: NULL argument");
   *n = s->nsel;

@@ -40,8 +40,8 @@ class ScreenResult(_c.Structure):
 
 
 def default_config(**kw) -> ScreenConfig:
-    c = dict(batch=8192, epochs=60, lr=2e-3, clip_quantile=0.1, keep_fraction=0.02, max_val_rmse=0.8,
-             tail_quantile=0.02, tail_weight=4.0, seed=0)
+    c = dict(batch=8192, epochs=60, lr=4e-3, clip_quantile=0.1, keep_fraction=0.02, max_val_rmse=0.8,
+             tail_quantile=0.05, tail_weight=8.0, seed=0)
     c.update(kw)
     return ScreenConfig(**c)
 
@@ -53,6 +53,7 @@ _SIG = {
                                  _c.c_size_t],
     "bude_mi355x_screen_load": [_c.c_void_p, _c.POINTER(_c.c_void_p), _c.c_size_t],
     "bude_mi355x_screen_run": [_c.c_void_p, _c.c_void_p],
+    "bude_mi355x_screen_set_keep": [_c.c_void_p, _c.c_float],
     "bude_mi355x_screen_selected": [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_size_t)],
     "bude_mi355x_screen_predict": [_c.c_void_p, _c.c_void_p],
     "bude_mi355x_screen_exhaustive": [_c.c_void_p, _c.c_void_p, _c.POINTER(_c.c_double)],
@@ -107,6 +108,10 @@ class Screen:
         r = ScreenResult()
         capi.check(_lib().bude_mi355x_screen_run(self._h, _c.byref(r)))
         return r
+
+    def set_keep(self, keep_fraction: float) -> None:
+        capi.check(_lib().bude_mi355x_screen_set_keep(self._h, keep_fraction))
+        self.cfg.keep_fraction = keep_fraction
 
     def selected(self) -> tuple[np.ndarray, np.ndarray]:
         n = _c.c_size_t()

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT
+from minibude_amd import capi
 from minibude_amd import deck as D
 from minibude_amd import screen as S
 from minibude_amd.model import Context
@@ -54,15 +55,21 @@ def test_fallback_selects_exact_lowest(gpu_device, small_workload):
         st, _ = s.train(w.train_poses)
         assert st.fallback == 1
         s.load(w.screen_poses)
-        r = s.run()
-        assert r.fallback == 1
-        idx, e = s.selected()
         exact, _ = s.exhaustive()
-    k = int(np.ceil(0.02 * len(exact)))
-    assert len(idx) >= k
-    thr = np.sort(exact)[k - 1]
-    assert set(idx.tolist()) == set(np.nonzero(exact <= thr)[0].tolist())
-    assert np.array_equal(e, exact[idx])
+        # the configured fraction, then others set without retraining
+        for keep in (0.02, 0.001, 0.05):
+            if keep != 0.02:
+                s.set_keep(keep)
+            r = s.run()
+            assert r.fallback == 1
+            idx, e = s.selected()
+            k = int(np.ceil(keep * len(exact)))
+            assert len(idx) >= k
+            thr = np.sort(exact)[k - 1]
+            assert set(idx.tolist()) == set(np.nonzero(exact <= thr)[0].tolist())
+            assert np.array_equal(e, exact[idx])
+        with pytest.raises(capi.BudeInvalidArgument):
+            s.set_keep(0.0)
     assert S.recall(idx, exact) == 1.0
 
 
