@@ -31,7 +31,12 @@ in HBM.  Prints ONE JSON line:
   screen        the learned prescreen (surrogate trained on 8M plain-kernel
                 energies, 16.8M fresh poses screened, lowest 2 % rescored
                 exactly) vs exhaustive plain scoring: poses/s and top-1 %
-                recall
+                recall, at 2 % rescored and at the fraction reaching 0.99
+  combined      16 proteins x 64 ligands x 262,144 poses per pair (512 x
+                1,048,576 with --combined-full): both surrogates (per-protein
+                MLP, single-atom energy maps), per-pair lowest 2 % rescored in
+                one ragged batched launch per protein, vs exhaustive batched
+                scoring of every pair: poses/s and per-pair top-1 % recall
 """
 from __future__ import annotations
 
@@ -65,6 +70,10 @@ BATCH_CONFIG = (1, 64, 8, 0, 1)
 # screen operating points: the workload's 2 % first, then larger fractions
 # until the exact top-1 % recall reaches 0.99
 SCREEN_KEEP = (0.02, 0.03, 0.04, 0.05, 0.06, 0.08)
+# combined screen sizes: the default keeps the run to seconds with recall on
+# every pair; the full size is the workload's own
+COMBINED_DEFAULT = dict(nproteins=16, nligands=64, nposes=1 << 18)
+COMBINED_FULL = dict(nproteins=16, nligands=512, nposes=1 << 20)
 TRAFFIC_SUMMARY = ROOT / "profiles" / "fasten_plain_c1_traffic.json"
 CPU_SAMPLE_POSES = 65536
 CPU_ITERS = 2
@@ -252,6 +261,68 @@ def traffic() -> float | None:
     return None
 
 
+def combined(full: bool) -> dict:
+    """Both surrogates of the combined screen on the same workload: every
+    protein screened (timed), exact energies of every pair for recall (all
+    proteins at the default size, the first one at full size, where the
+    exhaustive time of the rest is extrapolated from its interaction rate)."""
+    from minibude_amd import combined as C
+    shape = COMBINED_FULL if full else COMBINED_DEFAULT
+    t0 = time.time()
+    w = C.combined_workload(**shape)
+    per_protein = [len(p) * sum(len(l) * q.shape[1] for l, q in zip(w.ligands, w.poses)) for p in w.proteins]
+    log(f"combined: {len(w.proteins)} proteins x {len(w.ligands)} ligands x {shape['nposes']} poses "
+        f"({w.total_poses} pair-poses, {w.interactions:.3e} interactions), generated in {time.time() - t0:.1f} s")
+    out = {
+        "workload": (f"c4: {len(w.proteins)} proteins of 900-6000 atoms x {len(w.ligands)} ligands of 8-96 atoms x "
+                     f"{shape['nposes']} poses per pair" + ("" if full else " (reduced from 512 ligands x 2^20 poses)")),
+        "pairs": w.pairs,
+        "pair_poses": w.total_poses,
+        "interactions": w.interactions,
+    }
+    for kind in ("field", "mlp"):
+        cfg = C.default_config(kind)
+        screen_ms, ex_ms, ex_inter, rec, fb, labels, sel = 0.0, 0.0, 0, [], 0, 0, 0
+        parts = {"label_ms": 0.0, "train_ms": 0.0, "score_ms": 0.0, "select_ms": 0.0, "rescore_ms": 0.0}
+        with C.Combined(w.forcefield, w.ligands, w.poses, cfg, 0) as c:
+            for p, protein in enumerate(w.proteins):
+                r = c.screen(protein)
+                screen_ms += r.total_ms
+                fb += r.fallback_pairs
+                labels += r.labels
+                sel += r.nselected
+                for k in parts:
+                    parts[k] += getattr(r, k)
+                if not full or p == 0:
+                    ex_ms += c.exhaustive()
+                    ex_inter += per_protein[p]
+                    rec.extend(c.recall(0.01).tolist())
+        rec = np.array(rec)
+        ex_total_ms = ex_ms * w.interactions / ex_inter
+        res = {
+            "poses_per_s": round(w.total_poses / (screen_ms * 1e-3), 1),
+            "screen_ms": round(screen_ms, 2),
+            "exhaustive_ms": round(ex_total_ms, 2),
+            "exhaustive_poses_per_s": round(w.total_poses / (ex_total_ms * 1e-3), 1),
+            "speedup_vs_exhaustive": round(ex_total_ms / screen_ms, 2),
+            "recall_top1pct_mean": round(float(rec.mean()), 4),
+            "recall_top1pct_min": round(float(rec.min()), 4),
+            "recall_top1pct_p05": round(float(np.percentile(rec, 5)), 4),
+            "pairs_recall_ge_0.99": round(float((rec >= 0.99).mean()), 4),
+            "recall_pairs_measured": int(len(rec)),
+            "fallback_pairs": fb,
+            "labels": labels,
+            "rescored": sel,
+            "phases_ms": {k: round(v, 2) for k, v in parts.items()},
+            "config": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in cfg.as_dict().items()},
+        }
+        if full:
+            res["exhaustive_note"] = "measured on the first protein's pairs, extrapolated by interactions"
+        log(f"combined {kind}: {res}")
+        out[kind] = res
+    return out
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +333,9 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-screen", action="store_true")
+    ap.add_argument("--no-combined", action="store_true")
+    ap.add_argument("--combined-full", action="store_true",
+                    help="combined screen at 16 proteins x 512 ligands x 2^20 poses (recall on the first protein)")
     ap.add_argument("--copy-calibration", type=int, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
     a = ap.parse_args(argv)
@@ -343,6 +417,7 @@ def main(argv: list[str] | None = None) -> int:
         "parity": parity,
         "batched": None if a.no_batched else batched(),
         "screen": None if a.no_screen else screen(),
+        "combined": None if a.no_combined else combined(a.combined_full),
         "device": device_name(0),
     }
     print(json.dumps(out), flush=True)

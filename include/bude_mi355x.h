@@ -201,6 +201,91 @@ int bude_mi355x_screen_predict(bude_screen *s, float *pred);
 /* Plain kernel on every loaded pose: the exhaustive baseline and judge. */
 int bude_mi355x_screen_exhaustive(bude_screen *s, float *energies, double *ms);
 int bude_mi355x_screen_destroy(bude_screen *s);
+
+/* ---- combined screen: a fixed set of ligands, each with its own pose set,
+ *      screened against one protein after another.  Per protein: exact
+ *      energies of labels_per_pair sampled poses of every (protein, ligand)
+ *      pair, a surrogate validated per pair on them, the surrogate over
+ *      every pose of every pair, and exact rescoring of each pair's lowest
+ *      keep_fraction in one ragged batched launch; a pair whose calibrated
+ *      validation RMSE (below) exceeds max_val_rmse is scored exhaustively
+ *      instead.
+ *      Surrogates: BUDE_SURROGATE_MLP, the learned prescreen's network with
+ *      11 ligand descriptor inputs, one per protein, trained on 15/16 of the
+ *      samples; BUDE_SURROGATE_FIELD, single-ligand-atom energy maps of the
+ *      protein on a grid (one per ligand atom type, computed by the pair loop
+ *      of fasten_main) summed over the ligand's atoms by trilinear
+ *      interpolation.  No counterpart in the reference; every energy
+ *      returned is fasten_main's (src/serial/fasten.hpp:18-132). ---- */
+typedef struct bude_combined bude_combined;
+
+enum { BUDE_SURROGATE_MLP = 0, BUDE_SURROGATE_FIELD = 1 };
+
+typedef struct BudeCombinedConfig {
+  uint32_t surrogate;        /* BUDE_SURROGATE_MLP or BUDE_SURROGATE_FIELD */
+  uint32_t labels_per_pair;  /* sampled poses per pair scored exactly (2 .. 2^24) */
+  float keep_fraction;       /* per pair: the lowest fraction of surrogate scores is rescored exactly */
+  float max_val_rmse;        /* bound on a pair's calibrated validation RMSE (standardised units) */
+  float clip_quantile;       /* per-pair target transform, as the learned prescreen */
+  float tail_quantile;
+  float tail_weight;
+  uint32_t batch;            /* MLP: training mini-batch, a multiple of 32 */
+  uint32_t epochs;           /* MLP: passes over the training samples */
+  float lr;                  /* MLP: peak Adam learning rate */
+  float grid_spacing;        /* FIELD: map grid spacing in Angstrom (0.05 .. 4) */
+  uint64_t seed;             /* sampling, initialisation, shuffling */
+} BudeCombinedConfig;
+
+typedef struct BudeCombinedResult {
+  double total_ms;           /* host wall time of the whole protein */
+  float prepare_ms;          /* protein grouping, pair tables, uploads (host) */
+  float label_ms;            /* exact scoring of the sampled poses */
+  float train_ms;            /* MLP training, or building the maps */
+  float score_ms;            /* surrogate over every pose + exhaustive scoring of fallback pairs */
+  float select_ms;           /* per-pair top-k */
+  float rescore_ms;          /* gather + exact rescoring of the selections */
+  uint64_t nposes;           /* poses screened (all pairs) */
+  uint64_t nselected;        /* poses with exact energies returned */
+  uint64_t labels;           /* sampled poses scored exactly */
+  uint32_t fallback_pairs;
+} BudeCombinedResult;
+
+typedef struct BudePairResult {
+  uint32_t nposes, nselected;
+  int32_t fallback;          /* 1: scored exhaustively */
+  float val_rmse;            /* surrogate vs exact on the pair's validation samples (clipped, standardised) */
+  float val_rmse_calibrated; /* the same after the least-squares increasing affine map of the scores onto the
+                                targets (the part of the error that changes the ranking); fallback criterion */
+} BudePairResult;
+
+/* ligands[i] has natlig[i] atoms and poses[i] 6 rows of nposes[i] floats (as
+ * bude_mi355x_batch_create); at most 2^32 - 1 poses in all.  The poses stay
+ * resident in device memory for every protein screened. */
+int bude_mi355x_combined_create(int device, const BudeFFParams *forcefield, size_t ntypes, size_t nligands,
+                                const BudeAtom *const *ligands, const size_t *natlig, const float *const *poses,
+                                const size_t *nposes, const BudeCombinedConfig *cfg, bude_combined **out);
+/* Screens every ligand's poses against this protein. */
+int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *protein, size_t natpro, BudeCombinedResult *res);
+int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePairResult *r);
+/* The last screen's selection of one pair: pose indices (ascending, within
+ * the ligand's pose set) and their exact energies. */
+int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, uint32_t *idx, float *energies, size_t cap,
+                                  size_t *n);
+/* Surrogate score of every pose of one pair (exact energies for a fallback
+ * pair): standardised units for the MLP, energy units for the field. */
+int bude_mi355x_combined_scores(bude_combined *c, size_t ligand, float *out);
+/* Exact energies of every pose of every pair against the last protein (one
+ * batched launch, for measuring recall); then _exact / _recall read them. */
+int bude_mi355x_combined_exhaustive(bude_combined *c, double *device_ms);
+int bude_mi355x_combined_exact(bude_combined *c, size_t ligand, float *out);
+/* recall[l] = fraction of pair l's exact lowest round(top_fraction * n)
+ * poses that the last screen selected. */
+int bude_mi355x_combined_recall(bude_combined *c, float top_fraction, float *recall);
+/* FIELD: the map grid (point (i,j,k) at lo + h*(i,j,k)), and the map of one
+ * forcefield type, points in (i, j, k) row-major order. */
+int bude_mi355x_combined_grid(bude_combined *c, float lo[3], float *h, int32_t n[3]);
+int bude_mi355x_combined_map(bude_combined *c, int32_t type, float *out);
+int bude_mi355x_combined_destroy(bude_combined *c);
 /* One forward + backward pass of the surrogate on `batch` rows (a multiple
  * of 32) of 21 features with caller weights W_l [in][out] (21x256, 256x256,
  * 256x256, 256x1) and biases: predictions, mean squared error against

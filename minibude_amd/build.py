@@ -43,7 +43,11 @@ LIB_SOURCES = [
     "fasten_batch.hip",
     "capi_batch.hip",
     "mlp.hip",
+    "surrogate.hip",
     "capi_screen.hip",
+    "select.hip",
+    "field.hip",
+    "capi_combined.hip",
 ]
 
 

@@ -50,6 +50,17 @@ EXPORTED = (
     "bude_mi355x_screen_load",
     "bude_mi355x_screen_run",
     "bude_mi355x_screen_set_keep",
+    "bude_mi355x_combined_create",
+    "bude_mi355x_combined_screen",
+    "bude_mi355x_combined_pair",
+    "bude_mi355x_combined_selected",
+    "bude_mi355x_combined_scores",
+    "bude_mi355x_combined_exhaustive",
+    "bude_mi355x_combined_exact",
+    "bude_mi355x_combined_recall",
+    "bude_mi355x_combined_grid",
+    "bude_mi355x_combined_map",
+    "bude_mi355x_combined_destroy",
     "bude_mi355x_screen_selected",
     "bude_mi355x_screen_predict",
     "bude_mi355x_screen_exhaustive",

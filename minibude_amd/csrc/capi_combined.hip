@@ -1,0 +1,885 @@
+// capi_combined.hip -- C ABI of the combined screen: a fixed set of ligands,
+// each with its own pose set, screened against one protein after another.
+// Per protein: exact energies of a few sampled poses of every (protein,
+// ligand) pair (one ragged batched launch), a surrogate (the per-protein MLP
+// or the protein's single-atom energy maps) validated per pair on them, the
+// surrogate over every pose of every pair, per-pair top-k of its scores, and
+// exact rescoring of every pair's selection in one ragged batched launch;
+// pairs whose validation error is too large are scored exhaustively instead.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "bude_internal.h"
+#include "fasten_batch.h"
+#include "field.h"
+#include "hip_util.h"
+#include "mlp.h"
+#include "select.h"
+#include "surrogate.h"
+
+namespace {
+
+using bude::DeviceBuffer;
+
+// Persistent batched launch shape used for every exact pass: tiles of 64
+// poses, 64-lane work-groups, 8 waves per SIMD (the batch autotuner's choice
+// on ragged ligand sets).
+constexpr int kBatchPpwi = 1, kBatchWg = 64, kBatchWavesPerSimd = 8;
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+__global__ void gather_index_kernel(const float *__restrict__ src, size_t src_stride, const uint32_t *__restrict__ idx,
+                                    size_t n, float *__restrict__ dst, size_t dst_stride) {
+  const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const size_t q = idx[j];
+#pragma unroll
+  for (int k = 0; k < 6; k++) dst[size_t(k) * dst_stride + j] = src[size_t(k) * src_stride + q];
+}
+
+// Row j of the gathered selection: ligand l with goff[l] <= j < goff[l+1],
+// pose sel[pose_off[l] + (j - goff[l])] of that ligand.
+__global__ void gather_selected_kernel(const float *__restrict__ src, size_t src_stride,
+                                       const uint32_t *__restrict__ sel, const int64_t *__restrict__ pose_off,
+                                       const int64_t *__restrict__ goff, int nlig, size_t n, float *__restrict__ dst,
+                                       size_t dst_stride) {
+  const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int lo = 0, hi = nlig - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (goff[mid] <= int64_t(j)) lo = mid;
+    else hi = mid - 1;
+  }
+  const size_t q = size_t(pose_off[lo]) + sel[size_t(pose_off[lo]) + (j - size_t(goff[lo]))];
+#pragma unroll
+  for (int k = 0; k < 6; k++) dst[size_t(k) * dst_stride + j] = src[size_t(k) * src_stride + q];
+}
+
+unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
+
+// Eleven numbers describing a ligand to the per-protein MLP: atom count,
+// radius of gyration, the normalised second-moment tensor (6) in the
+// ligand's own frame (with the rotation-matrix inputs this orients the
+// ligand's shape), and the centroid.
+std::vector<float> descriptor(const BudeAtom *a, size_t n) {
+  double c[3] = {0, 0, 0};
+  for (size_t i = 0; i < n; i++) {
+    c[0] += a[i].x;
+    c[1] += a[i].y;
+    c[2] += a[i].z;
+  }
+  for (double &v : c) v /= double(n);
+  double m[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
+  for (size_t i = 0; i < n; i++) {
+    const double x = a[i].x - c[0], y = a[i].y - c[1], z = a[i].z - c[2];
+    m[0] += x * x;
+    m[1] += y * y;
+    m[2] += z * z;
+    m[3] += x * y;
+    m[4] += x * z;
+    m[5] += y * z;
+  }
+  for (double &v : m) v /= double(n);
+  const double rg2 = std::max(1e-12, m[0] + m[1] + m[2]);
+  std::vector<float> d(bude::mlp::kDesc, 0.f);
+  d[0] = float(double(n) / 96.0);
+  d[1] = float(std::sqrt(rg2) / 5.0);
+  for (int k = 0; k < 6; k++) d[2 + k] = float(m[k] / rg2);
+  for (int k = 0; k < 3; k++) d[8 + k] = float(c[k] / 5.0);
+  return d;
+}
+
+struct Norm {
+  float clip = 0.f, mu = 0.f, sigma = 1.f, tail = -3.0e38f;
+};
+
+// Target transform of one pair from its training energies (as the learned
+// prescreen: clip at a quantile, standardise, tail threshold).
+Norm norm_of(std::vector<float> e, float clip_q, float tail_q) {
+  Norm r;
+  if (e.empty()) return r;
+  const size_t n = e.size();
+  const size_t q = std::min(n - 1, size_t(double(clip_q) * double(n)));
+  std::nth_element(e.begin(), e.begin() + long(q), e.end());
+  r.clip = e[q];
+  if (tail_q > 0.f) {
+    const size_t qt = std::min(n - 1, size_t(double(tail_q) * double(n)));
+    std::nth_element(e.begin(), e.begin() + long(qt), e.end());
+    r.tail = e[qt];
+  }
+  double sum = 0, sq = 0;
+  for (float v : e) {
+    const double y = std::min(v, r.clip);
+    sum += y;
+    sq += y * y;
+  }
+  r.mu = float(sum / double(n));
+  r.sigma = float(std::sqrt(std::max(1e-12, sq / double(n) - double(r.mu) * r.mu)));
+  return r;
+}
+
+}  // namespace
+
+struct bude_combined {
+  int device = 0, cus = 0;
+  BudeCombinedConfig cfg{};
+  std::vector<BudeFFParams> ff;
+  // ligands (fixed for the object's life)
+  size_t nlig = 0, total = 0;
+  std::vector<std::vector<BudeAtom>> lig;
+  std::vector<int32_t> lig_off;
+  std::vector<size_t> nposes, pose_off;
+  std::vector<int32_t> slot_of_type;  // forcefield type -> map slot (-1: unused)
+  std::vector<int32_t> slot_type;     // map slot -> forcefield type
+  DeviceBuffer ligands;               // float4: x, y, z, slot bits
+  DeviceBuffer poses;                 // 6 rows of `total`
+  DeviceBuffer pred, sel_idx, sel_n, exact_all;
+  DeviceBuffer desc;                  // [nlig][kDesc]
+  bude::sel::Plan plan;
+  bude::field::Grid grid{};
+  // per protein
+  bool screened = false;
+  size_t ngroups = 0, proteins_done = 0;
+  DeviceBuffer protein, group_begin, coef, slot_coef, maps, counter;
+  std::vector<int32_t> coef_off;
+  // per-call work tables (each its own buffer; uploaded only while the stream is idle)
+  DeviceBuffer meta_label, tiles_label, meta_fb, tiles_fb, meta_rs, tiles_rs, meta_all, tiles_all;
+  DeviceBuffer tb_label, tb_all, lab_src, lab_poses, lab_e, lab_lig, lab_pred, norms, bias0;
+  DeviceBuffer goff, poff, gathered, exact_sel;
+  // MLP
+  bude::surrogate::Net net;
+  // per pair results of the last protein
+  std::vector<uint32_t> nsel;
+  std::vector<int64_t> gofs;
+  std::vector<int32_t> fallback;
+  std::vector<float> val_rmse, val_rmse_cal;
+  bool have_exact = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+
+  ~bude_combined() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+struct Span {
+  size_t lig, off, n;  // ligand, first pose in the pose array, poses
+};
+
+// One persistent batched launch scoring spans[i].n poses of ligand
+// spans[i].lig starting at row spans[i].off of `poses` (row stride
+// `stride`), energies to energies[off + j].  Tiles of the heaviest ligands
+// first.  meta / tiles are this call's own work buffers; the stream must be
+// idle (they are overwritten synchronously).
+int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride,
+                 float *energies, DeviceBuffer &meta, DeviceBuffer &tiles) {
+  std::vector<bude::LigandMeta> m(c->nlig);
+  for (size_t l = 0; l < c->nlig; l++) {
+    m[l].lig_off = c->lig_off[l];
+    m[l].natlig = int32_t(c->lig[l].size());
+    m[l].coef_off = c->coef_off[l];
+  }
+  std::vector<size_t> order;
+  for (size_t i = 0; i < spans.size(); i++)
+    if (spans[i].n) {
+      m[spans[i].lig].nposes = int32_t(spans[i].n);
+      m[spans[i].lig].pose_off = int64_t(spans[i].off);
+      order.push_back(i);
+    }
+  if (order.empty()) return BUDE_OK;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return c->lig[spans[a].lig].size() > c->lig[spans[b].lig].size(); });
+  std::vector<int2> t;
+  for (size_t i : order)
+    for (size_t j = 0; j < spans[i].n; j += 64 * kBatchPpwi) t.push_back(make_int2(int(spans[i].lig), int(j)));
+  if (t.size() > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: too many tiles");
+  if (int st = meta.reserve(m.size() * sizeof(bude::LigandMeta))) return st;
+  if (int st = bude::hip_check(hipMemcpy(meta.ptr, m.data(), m.size() * sizeof(bude::LigandMeta),
+                                         hipMemcpyHostToDevice),
+                               "hipMemcpy meta"))
+    return st;
+  if (int st = tiles.reserve(t.size() * sizeof(int2))) return st;
+  if (int st = bude::hip_check(hipMemcpy(tiles.ptr, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice),
+                               "hipMemcpy tiles"))
+    return st;
+  bude::BatchArgs a{};
+  a.protein = static_cast<const float4 *>(c->protein.ptr);
+  a.group_begin = static_cast<const int32_t *>(c->group_begin.ptr);
+  a.coef = static_cast<const bude::PairCoef *>(c->coef.ptr);
+  a.ligands = static_cast<const float4 *>(c->ligands.ptr);
+  a.meta = static_cast<const bude::LigandMeta *>(meta.ptr);
+  a.tiles = static_cast<const int2 *>(tiles.ptr);
+  a.poses = poses;
+  a.energies = energies;
+  a.counter = static_cast<unsigned int *>(c->counter.ptr);
+  a.stride = stride;
+  a.ngroups = int(c->ngroups);
+  a.ntiles = int(t.size());
+  a.form = bude::plain_form();
+  const int blocks = std::max(1, c->cus * 4 * kBatchWavesPerSimd / (kBatchWg / 64));
+  return bude::hip_check(bude::launch_batch(a, kBatchPpwi, kBatchWg, blocks, c->stream), "combined batch launch");
+}
+
+// Field scores of spans (as exact_ragged) from the current maps.
+int field_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride, float *out,
+                 DeviceBuffer &meta, DeviceBuffer &tile_begin) {
+  std::vector<bude::LigandMeta> m(c->nlig);
+  std::vector<int64_t> tb(c->nlig + 1, 0);
+  for (size_t l = 0; l < c->nlig; l++) {
+    m[l].lig_off = c->lig_off[l];
+    m[l].natlig = int32_t(c->lig[l].size());
+  }
+  for (const Span &s : spans) {
+    m[s.lig].nposes = int32_t(s.n);
+    m[s.lig].pose_off = int64_t(s.off);
+  }
+  for (size_t l = 0; l < c->nlig; l++) tb[l + 1] = tb[l] + int64_t((size_t(m[l].nposes) + 63) / 64);
+  if (tb[c->nlig] == 0) return BUDE_OK;
+  if (int st = meta.reserve(m.size() * sizeof(bude::LigandMeta))) return st;
+  if (int st = bude::hip_check(hipMemcpy(meta.ptr, m.data(), m.size() * sizeof(bude::LigandMeta),
+                                         hipMemcpyHostToDevice),
+                               "hipMemcpy meta"))
+    return st;
+  if (int st = tile_begin.reserve(tb.size() * sizeof(int64_t))) return st;
+  if (int st = bude::hip_check(hipMemcpy(tile_begin.ptr, tb.data(), tb.size() * sizeof(int64_t),
+                                         hipMemcpyHostToDevice),
+                               "hipMemcpy tile_begin"))
+    return st;
+  bude::field::ScoreArgs a{};
+  a.maps = static_cast<const float *>(c->maps.ptr);
+  a.grid = c->grid;
+  a.ligands = static_cast<const float4 *>(c->ligands.ptr);
+  a.meta = static_cast<const bude::LigandMeta *>(meta.ptr);
+  a.tile_begin = static_cast<const int64_t *>(tile_begin.ptr);
+  a.nligands = int(c->nlig);
+  a.poses = poses;
+  a.stride = stride;
+  a.out = out;
+  return bude::hip_check(bude::field::score(a, tb[c->nlig], c->stream), "field score");
+}
+
+int sync(bude_combined *c) { return bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
+
+int record(bude_combined *c, int i) { return bude::hip_check(hipEventRecord(c->ev[i], c->stream), "hipEventRecord"); }
+
+float elapsed(bude_combined *c, int a, int b) {
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->ev[a], c->ev[b]);
+  return ms;
+}
+
+int check_config(const BudeCombinedConfig *cfg) {
+  if (!cfg) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: config is NULL");
+  if (cfg->surrogate != BUDE_SURROGATE_MLP && cfg->surrogate != BUDE_SURROGATE_FIELD)
+    return bude::fail(BUDE_ERR_UNSUPPORTED, "combined: surrogate must be BUDE_SURROGATE_MLP or BUDE_SURROGATE_FIELD");
+  if (!(cfg->keep_fraction > 0.f) || cfg->keep_fraction > 1.f || !(cfg->clip_quantile > 0.f) ||
+      cfg->clip_quantile > 1.f || cfg->tail_quantile < 0.f || cfg->tail_quantile > 1.f || !(cfg->tail_weight > 0.f) ||
+      cfg->max_val_rmse < 0.f)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "combined: keep_fraction and clip_quantile in (0, 1], tail_quantile in [0, 1], tail_weight > 0, "
+                      "max_val_rmse >= 0");
+  if (cfg->labels_per_pair < 2 || cfg->labels_per_pair > (1u << 24))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: labels_per_pair must be in [2, 2^24]");
+  if (cfg->surrogate == BUDE_SURROGATE_MLP &&
+      (cfg->batch < 32 || cfg->batch % 32 || cfg->batch > (1u << 20) || !(cfg->lr > 0.f)))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: MLP batch must be a multiple of 32 in [32, 2^20], lr > 0");
+  if (cfg->surrogate == BUDE_SURROGATE_FIELD && !(cfg->grid_spacing >= 0.05f && cfg->grid_spacing <= 4.f))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: grid_spacing must be in [0.05, 4]");
+  return BUDE_OK;
+}
+
+// Map grid covering every position a ligand atom can take: the poses rotate
+// about the ligand frame's origin, so |R x| = |x| <= r, and translate by t.
+int make_grid(bude_combined *c, const float *const *poses) {
+  double r = 0;
+  for (const auto &l : c->lig)
+    for (const BudeAtom &a : l) r = std::max(r, std::sqrt(double(a.x) * a.x + double(a.y) * a.y + double(a.z) * a.z));
+  double tmin[3] = {1e300, 1e300, 1e300}, tmax[3] = {-1e300, -1e300, -1e300};
+  for (size_t l = 0; l < c->nlig; l++)
+    for (int k = 0; k < 3; k++) {
+      const float *row = poses[l] + size_t(3 + k) * c->nposes[l];
+      for (size_t j = 0; j < c->nposes[l]; j++)
+        if (std::isfinite(row[j])) {
+          tmin[k] = std::min(tmin[k], double(row[j]));
+          tmax[k] = std::max(tmax[k], double(row[j]));
+        }
+    }
+  const double h = c->cfg.grid_spacing;
+  size_t points = 1;
+  for (int k = 0; k < 3; k++) {
+    if (tmin[k] > tmax[k]) tmin[k] = tmax[k] = 0;
+    // lo on a multiple of h: grid coordinates lo + i*h are then exact in
+    // FP32 for spacings that are powers of two
+    const double lo = std::floor((tmin[k] - r - h) / h) * h, hi = tmax[k] + r + h;
+    c->grid.lo[k] = float(lo);
+    const double n = std::ceil((hi - lo) / h) + 1;
+    if (n > 4096) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: grid too large (raise grid_spacing)");
+    c->grid.n[k] = std::max(2, int(n));
+    points *= size_t(c->grid.n[k]);
+  }
+  c->grid.h = float(h);
+  if (points > 0x7fffffffu || points * c->slot_type.size() * 4 > (size_t(32) << 30))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: maps would exceed 32 GiB (raise grid_spacing)");
+  return BUDE_OK;
+}
+
+}  // namespace
+
+extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *forcefield, size_t ntypes, size_t nligands,
+                                           const BudeAtom *const *ligands, const size_t *natlig,
+                                           const float *const *poses, const size_t *nposes,
+                                           const BudeCombinedConfig *cfg, bude_combined **out) {
+  if (!out) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_create: out is NULL");
+  *out = nullptr;
+  if (int st = check_config(cfg)) return st;
+  if (!forcefield || ntypes == 0 || nligands == 0 || nligands > (1u << 20) || !ligands || !natlig || !poses ||
+      !nposes)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_create: empty forcefield or ligand set");
+  size_t total = 0;
+  for (size_t i = 0; i < nligands; i++) {
+    const std::string who = "bude_mi355x_combined_create: ligand " + std::to_string(i);
+    // the protein comes later; check the ligand against the forcefield with itself as a stand-in protein
+    if (int st = bude::check_deck(ligands[i], natlig[i], ligands[i], natlig[i], forcefield, ntypes, who.c_str()))
+      return st;
+    if (!poses[i] || nposes[i] == 0 || nposes[i] > 0x7fffffffu)
+      return bude::fail(BUDE_ERR_INVALID_ARGUMENT, who + ": no poses (or more than 2^31)");
+    total += nposes[i];
+  }
+  if (total > 0xffffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_create: over 2^32 poses");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || device < 0 || device >= ndev)
+    return bude::fail(BUDE_ERR_NO_DEVICE, "bude_mi355x_combined_create: no such HIP device");
+  if (int st = bude::hip_check(hipSetDevice(device), "hipSetDevice")) return st;
+
+  auto c = std::make_unique<bude_combined>();
+  c->device = device;
+  c->cfg = *cfg;
+  hipDeviceProp_t props{};
+  if (int st = bude::hip_check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties")) return st;
+  c->cus = props.multiProcessorCount;
+  c->ff.assign(forcefield, forcefield + ntypes);
+  c->nlig = nligands;
+  c->total = total;
+  c->slot_of_type.assign(ntypes, -1);
+  std::vector<float> lig4, desc;
+  size_t off = 0;
+  for (size_t i = 0; i < nligands; i++) {
+    c->lig.emplace_back(ligands[i], ligands[i] + natlig[i]);
+    c->lig_off.push_back(int32_t(lig4.size() / 4));
+    for (size_t a = 0; a < natlig[i]; a++) {
+      const BudeAtom &at = ligands[i][a];
+      int32_t &slot = c->slot_of_type[size_t(at.type)];
+      if (slot < 0) {
+        slot = int32_t(c->slot_type.size());
+        c->slot_type.push_back(at.type);
+      }
+      float w;
+      std::memcpy(&w, &slot, 4);
+      lig4.insert(lig4.end(), {at.x, at.y, at.z, w});
+    }
+    if (lig4.size() / 4 > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: too many ligand atoms");
+    const std::vector<float> d = descriptor(ligands[i], natlig[i]);
+    desc.insert(desc.end(), d.begin(), d.end());
+    c->nposes.push_back(nposes[i]);
+    c->pose_off.push_back(off);
+    off += nposes[i];
+  }
+  if (cfg->surrogate == BUDE_SURROGATE_FIELD)
+    if (int st = make_grid(c.get(), poses)) return st;
+
+  std::vector<bude::sel::Segment> segs(nligands);
+  for (size_t i = 0; i < nligands; i++) {
+    const double kf = std::ceil(double(cfg->keep_fraction) * double(nposes[i]));
+    segs[i] = bude::sel::Segment{int64_t(c->pose_off[i]), uint32_t(nposes[i]),
+                                 uint32_t(std::min<double>(double(nposes[i]), std::max(1.0, kf)))};
+  }
+  std::vector<int64_t> poff(c->pose_off.begin(), c->pose_off.end());
+  int st = c->plan.prepare(segs);
+  if (!st) st = c->ligands.upload(lig4.data(), lig4.size() * 4);
+  if (!st) st = c->desc.upload(desc.data(), desc.size() * 4);
+  if (!st) st = c->poff.upload(poff.data(), poff.size() * sizeof(int64_t));
+  if (!st) st = c->poses.alloc(6 * total * sizeof(float));
+  for (size_t i = 0; i < nligands && !st; i++)
+    for (int k = 0; k < 6 && !st; k++)
+      st = bude::hip_check(hipMemcpy(static_cast<float *>(c->poses.ptr) + size_t(k) * total + c->pose_off[i],
+                                     poses[i] + size_t(k) * nposes[i], nposes[i] * sizeof(float),
+                                     hipMemcpyHostToDevice),
+                           "hipMemcpy poses");
+  if (!st) st = c->pred.alloc(total * sizeof(float));
+  if (!st) st = c->sel_idx.alloc(total * sizeof(uint32_t));
+  if (!st) st = c->sel_n.alloc(nligands * sizeof(uint32_t));
+  if (!st) st = c->counter.alloc(256);
+  if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto &e : c->ev)
+    if (!st) st = bude::hip_check(hipEventCreate(&e), "hipEventCreate");
+  if (!st) st = bude::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  if (st) return st;
+  *out = c.release();
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *protein, size_t natpro,
+                                           BudeCombinedResult *res) {
+  if (!c || !res) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_screen: NULL argument");
+  if (int st = bude::check_deck(protein, natpro, c->lig[0].data(), c->lig[0].size(), c->ff.data(), c->ff.size(),
+                                "bude_mi355x_combined_screen"))
+    return st;
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  *res = BudeCombinedResult{};
+  c->screened = false;
+  c->have_exact = false;
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t L = c->nlig;
+  const BudeCombinedConfig &cfg = c->cfg;
+  const bool mlp = cfg.surrogate == BUDE_SURROGATE_MLP;
+
+  // 1. protein tables: grouped atoms, pair coefficients of every ligand atom
+  //    (and of every map slot)
+  const bude::ProteinGroups groups = bude::group_protein(protein, natpro);
+  c->ngroups = groups.ngroups();
+  std::vector<bude::PairCoef> coef;
+  c->coef_off.assign(L, 0);
+  for (size_t l = 0; l < L; l++) {
+    c->coef_off[l] = int32_t(coef.size());
+    const std::vector<bude::PairCoef> t = bude::coef_table(c->lig[l].data(), c->lig[l].size(), groups, c->ff.data());
+    coef.insert(coef.end(), t.begin(), t.end());
+    if (coef.size() > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: pair tables too large");
+  }
+  if (int st = c->protein.upload(groups.xyzw.data(), groups.xyzw.size() * 4)) return st;
+  if (int st = c->group_begin.upload(groups.begin.data(), groups.begin.size() * 4)) return st;
+  if (int st = c->coef.upload(coef.data(), coef.size() * sizeof(bude::PairCoef))) return st;
+  if (!mlp) {
+    std::vector<BudeAtom> probe(c->slot_type.size());
+    for (size_t s = 0; s < probe.size(); s++) probe[s] = BudeAtom{0.f, 0.f, 0.f, c->slot_type[s]};
+    const std::vector<bude::PairCoef> sc = bude::coef_table(probe.data(), probe.size(), groups, c->ff.data());
+    if (int st = c->slot_coef.upload(sc.data(), sc.size() * sizeof(bude::PairCoef))) return st;
+  }
+  res->prepare_ms = float(ms_since(t0));
+
+  // 2. sampled poses of every pair: first 15/16 train the MLP, the rest
+  //    validate (the field validates on all of them)
+  std::mt19937_64 rng(cfg.seed + 0x9e3779b97f4a7c15ull * (c->proteins_done + 1));
+  std::vector<uint32_t> src, lab_lig;
+  std::vector<size_t> lab_off(L + 1, 0), lab_train(L, 0);
+  for (size_t l = 0; l < L; l++) {
+    const size_t n = c->nposes[l], m = std::min<size_t>(cfg.labels_per_pair, n);
+    std::vector<uint32_t> pick;
+    if (m == n) {
+      pick.resize(n);
+      std::iota(pick.begin(), pick.end(), 0u);
+      std::shuffle(pick.begin(), pick.end(), rng);
+    } else {  // Floyd's sampling without replacement, then shuffled
+      std::vector<uint32_t> chosen;
+      std::vector<bool> seen;
+      if (n <= (size_t(1) << 26)) seen.assign(n, false);
+      std::vector<uint32_t> sorted;
+      for (size_t j = n - m; j < n; j++) {
+        const uint32_t t = uint32_t(std::uniform_int_distribution<size_t>(0, j)(rng));
+        bool dup;
+        if (!seen.empty()) dup = seen[t];
+        else dup = std::binary_search(sorted.begin(), sorted.end(), t);
+        const uint32_t v = dup ? uint32_t(j) : t;
+        chosen.push_back(v);
+        if (!seen.empty()) seen[v] = true;
+        else sorted.insert(std::upper_bound(sorted.begin(), sorted.end(), v), v);
+      }
+      pick = std::move(chosen);
+      std::shuffle(pick.begin(), pick.end(), rng);
+    }
+    for (uint32_t j : pick) {
+      src.push_back(uint32_t(c->pose_off[l] + j));
+      lab_lig.push_back(uint32_t(l));
+    }
+    lab_off[l + 1] = src.size();
+    lab_train[l] = mlp ? m - std::max<size_t>(1, m / 16) : 0;
+  }
+  const size_t nlab = src.size();
+  if (int st = c->lab_src.upload(src.data(), nlab * 4)) return st;
+  if (int st = c->lab_lig.upload(lab_lig.data(), nlab * 4)) return st;
+  if (int st = c->lab_poses.reserve(6 * nlab * 4)) return st;
+  if (int st = c->lab_e.reserve(nlab * 4)) return st;
+  if (int st = c->lab_pred.reserve(nlab * 4)) return st;
+  if (int st = record(c, 0)) return st;
+  hipLaunchKernelGGL(gather_index_kernel, dim3(blocks_for(nlab, 256)), dim3(256), 0, c->stream,
+                     static_cast<const float *>(c->poses.ptr), c->total, static_cast<const uint32_t *>(c->lab_src.ptr),
+                     nlab, static_cast<float *>(c->lab_poses.ptr), nlab);
+  if (int st = bude::hip_check(hipGetLastError(), "gather labels")) return st;
+  std::vector<Span> lab_spans(L);
+  for (size_t l = 0; l < L; l++) lab_spans[l] = Span{l, lab_off[l], lab_off[l + 1] - lab_off[l]};
+  if (int st = exact_ragged(c, lab_spans, static_cast<const float *>(c->lab_poses.ptr), nlab,
+                            static_cast<float *>(c->lab_e.ptr), c->meta_label, c->tiles_label))
+    return st;
+  if (int st = record(c, 1)) return st;
+  std::vector<float> le(nlab);
+  if (int st = bude::hip_check(hipMemcpyAsync(le.data(), c->lab_e.ptr, nlab * 4, hipMemcpyDeviceToHost, c->stream),
+                               "hipMemcpyAsync"))
+    return st;
+  if (int st = sync(c)) return st;
+  res->label_ms = elapsed(c, 0, 1);
+
+  // per-pair target transform from the training part (all samples for the field)
+  std::vector<Norm> norms(L);
+  std::vector<bude::mlp::TargetNorm> tn(L);
+  for (size_t l = 0; l < L; l++) {
+    const size_t b = lab_off[l], e = (mlp && lab_train[l] > 0) ? b + lab_train[l] : lab_off[l + 1];
+    norms[l] = norm_of(std::vector<float>(le.begin() + long(b), le.begin() + long(e)), cfg.clip_quantile,
+                       cfg.tail_quantile);
+    tn[l] = bude::mlp::TargetNorm{norms[l].clip, norms[l].mu, 1.0f / norms[l].sigma, norms[l].tail};
+  }
+
+  // 3. the surrogate: train the MLP, or build the maps
+  if (int st = record(c, 2)) return st;
+  if (mlp) {
+    if (int st = c->norms.upload(tn.data(), tn.size() * sizeof(tn[0]))) return st;
+    std::vector<std::vector<float>> w, b;
+    bude::surrogate::init_weights(cfg.seed + c->proteins_done, bude::mlp::kIn, w, b);
+    if (int st = bude::surrogate::set_weights(c->net, w, b, c->stream)) return st;
+    std::vector<uint32_t> train;
+    for (size_t l = 0; l < L; l++)
+      for (size_t j = 0; j < lab_train[l]; j++) train.push_back(uint32_t(lab_off[l] + j));
+    if (train.size() < cfg.batch)
+      return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: fewer training labels than one MLP batch");
+    bude::surrogate::TrainData td;
+    td.poses = static_cast<const float *>(c->lab_poses.ptr);
+    td.stride = nlab;
+    td.energy = static_cast<const float *>(c->lab_e.ptr);
+    td.table = static_cast<const bude::mlp::TargetNorm *>(c->norms.ptr);
+    td.lig = static_cast<const uint32_t *>(c->lab_lig.ptr);
+    td.desc = static_cast<const float *>(c->desc.ptr);
+    bude::surrogate::TrainParams tp;
+    tp.batch = cfg.batch;
+    tp.epochs = cfg.epochs;
+    tp.lr = cfg.lr;
+    tp.tail_weight = cfg.tail_weight;
+    tp.seed = cfg.seed + c->proteins_done;
+    if (int st = bude::surrogate::train(c->net, td, std::move(train), tp, c->stream, nullptr, 0, nullptr)) return st;
+    if (int st = c->bias0.reserve(L * bude::mlp::kHidden * 4)) return st;
+    if (int st = bude::hip_check(bude::mlp::fold_bias(static_cast<const float *>(c->net.b[0].ptr),
+                                                      static_cast<const bude::mlp::bf16 *>(c->net.w16[0].ptr),
+                                                      static_cast<const float *>(c->desc.ptr), int(L),
+                                                      static_cast<float *>(c->bias0.ptr), c->stream),
+                                 "fold_bias"))
+      return st;
+  } else {
+    const size_t np = c->grid.points();
+    if (int st = c->maps.reserve(np * c->slot_type.size() * 4)) return st;
+    if (int st = bude::hip_check(
+            bude::field::build_maps(static_cast<const float4 *>(c->protein.ptr),
+                                    static_cast<const int32_t *>(c->group_begin.ptr),
+                                    static_cast<const bude::PairCoef *>(c->slot_coef.ptr), int(c->ngroups),
+                                    int(c->slot_type.size()), c->grid, bude::plain_form(),
+                                    static_cast<float *>(c->maps.ptr), c->stream),
+            "build maps"))
+      return st;
+  }
+  if (int st = record(c, 3)) return st;
+
+  // 4. validation per pair: RMSE of the surrogate on the clipped,
+  //    standardised scale of that pair
+  float *lp = static_cast<float *>(c->lab_pred.ptr);
+  const float *lpos = static_cast<const float *>(c->lab_poses.ptr);
+  if (mlp) {
+    for (size_t l = 0; l < L; l++) {
+      const size_t b = lab_off[l] + lab_train[l], n = lab_off[l + 1] - b;
+      if (n == 0) continue;
+      if (int st = bude::surrogate::predict(c->net, lpos + b, nlab, n, lp + b, 2 * c->cus, c->stream,
+                                            static_cast<const float *>(c->bias0.ptr) + l * bude::mlp::kHidden))
+        return st;
+    }
+  } else {
+    if (int st = field_ragged(c, lab_spans, lpos, nlab, lp, c->meta_fb, c->tb_label)) return st;
+  }
+  std::vector<float> lpred(nlab);
+  if (int st = bude::hip_check(hipMemcpyAsync(lpred.data(), lp, nlab * 4, hipMemcpyDeviceToHost, c->stream),
+                               "hipMemcpyAsync"))
+    return st;
+  if (int st = sync(c)) return st;
+  res->train_ms = elapsed(c, 2, 3);
+  c->fallback.assign(L, 0);
+  c->val_rmse.assign(L, 0.f);
+  c->val_rmse_cal.assign(L, 0.f);
+  for (size_t l = 0; l < L; l++) {
+    const size_t b = lab_off[l] + lab_train[l], e = lab_off[l + 1];
+    if (e <= b) continue;
+    double se = 0, sy = 0, sp = 0, syy = 0, spp = 0, syp = 0;
+    for (size_t i = b; i < e; i++) {
+      const double y = (double(std::min(le[i], norms[l].clip)) - norms[l].mu) / norms[l].sigma;
+      const double p = mlp ? double(lpred[i])
+                           : (double(std::min(lpred[i], norms[l].clip)) - norms[l].mu) / norms[l].sigma;
+      se += (p - y) * (p - y);
+      sy += y;
+      sp += p;
+      syy += y * y;
+      spp += p * p;
+      syp += y * p;
+    }
+    const double n = double(e - b);
+    const double vy = std::max(0.0, syy / n - (sy / n) * (sy / n));
+    const double vp = std::max(0.0, spp / n - (sp / n) * (sp / n));
+    const double cv = syp / n - (sy / n) * (sp / n);
+    c->val_rmse[l] = float(std::sqrt(se / n));
+    // residual of the best increasing affine map of the scores onto the
+    // targets: the error that matters for a ranking
+    c->val_rmse_cal[l] = float(std::sqrt(vp > 0 && cv > 0 ? std::max(0.0, vy - cv * cv / vp) : vy));
+    c->fallback[l] = !(c->val_rmse_cal[l] <= cfg.max_val_rmse);
+    res->fallback_pairs += uint32_t(c->fallback[l]);
+  }
+
+  // 5. the surrogate over every pose of every pair; exhaustive exact scoring
+  //    of the fallback pairs (their "scores" are exact energies)
+  float *pred = static_cast<float *>(c->pred.ptr);
+  const float *poses = static_cast<const float *>(c->poses.ptr);
+  std::vector<Span> sur, fb;
+  for (size_t l = 0; l < L; l++)
+    (c->fallback[l] ? fb : sur).push_back(Span{l, c->pose_off[l], c->nposes[l]});
+  if (int st = record(c, 4)) return st;
+  if (mlp) {
+    for (const Span &s : sur)
+      if (int st = bude::surrogate::predict(c->net, poses + s.off, c->total, s.n, pred + s.off, 2 * c->cus, c->stream,
+                                            static_cast<const float *>(c->bias0.ptr) + s.lig * bude::mlp::kHidden))
+        return st;
+  } else {
+    if (int st = field_ragged(c, sur, poses, c->total, pred, c->meta_all, c->tb_all)) return st;
+  }
+  if (int st = exact_ragged(c, fb, poses, c->total, pred, c->meta_fb, c->tiles_fb)) return st;
+  if (int st = record(c, 5)) return st;
+
+  // 6. per-pair top-k of the scores
+  if (int st = bude::hip_check(bude::sel::select_lowest_segments(c->plan, pred, static_cast<uint32_t *>(c->sel_idx.ptr),
+                                                                 static_cast<uint32_t *>(c->sel_n.ptr), c->stream),
+                               "select"))
+    return st;
+  c->nsel.assign(L, 0);
+  if (int st = bude::hip_check(hipMemcpyAsync(c->nsel.data(), c->sel_n.ptr, L * 4, hipMemcpyDeviceToHost, c->stream),
+                               "hipMemcpyAsync"))
+    return st;
+  if (int st = record(c, 6)) return st;
+  if (int st = sync(c)) return st;
+
+  // 7. exact rescoring of every non-fallback pair's selection in one launch
+  c->gofs.assign(L + 1, 0);
+  for (size_t l = 0; l < L; l++) c->gofs[l + 1] = c->gofs[l] + (c->fallback[l] ? 0 : int64_t(c->nsel[l]));
+  const size_t G = size_t(c->gofs[L]);
+  if (G) {
+    if (int st = c->goff.reserve((L + 1) * sizeof(int64_t))) return st;
+    if (int st = bude::hip_check(hipMemcpy(c->goff.ptr, c->gofs.data(), (L + 1) * sizeof(int64_t),
+                                           hipMemcpyHostToDevice),
+                                 "hipMemcpy goff"))
+      return st;
+    if (int st = c->gathered.reserve(6 * G * 4)) return st;
+    if (int st = c->exact_sel.reserve(G * 4)) return st;
+    const size_t gstride = c->gathered.bytes / 24;
+    hipLaunchKernelGGL(gather_selected_kernel, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, poses, c->total,
+                       static_cast<const uint32_t *>(c->sel_idx.ptr), static_cast<const int64_t *>(c->poff.ptr),
+                       static_cast<const int64_t *>(c->goff.ptr), int(L), G, static_cast<float *>(c->gathered.ptr),
+                       gstride);
+    if (int st = bude::hip_check(hipGetLastError(), "gather selected")) return st;
+    std::vector<Span> rs;
+    for (size_t l = 0; l < L; l++)
+      if (!c->fallback[l]) rs.push_back(Span{l, size_t(c->gofs[l]), c->nsel[l]});
+    if (int st = exact_ragged(c, rs, static_cast<const float *>(c->gathered.ptr), gstride,
+                              static_cast<float *>(c->exact_sel.ptr), c->meta_rs, c->tiles_rs))
+      return st;
+  }
+  if (int st = record(c, 7)) return st;
+  if (int st = sync(c)) return st;
+  res->score_ms = elapsed(c, 4, 5);
+  res->select_ms = elapsed(c, 5, 6);
+  res->rescore_ms = elapsed(c, 6, 7);
+  res->total_ms = ms_since(t0);
+  res->nposes = c->total;
+  for (size_t l = 0; l < L; l++) res->nselected += c->nsel[l];
+  res->labels = nlab;
+  c->screened = true;
+  c->proteins_done++;
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePairResult *r) {
+  if (!c || !r || ligand >= c->nlig) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_pair: bad arguments");
+  if (!c->screened) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_pair: no protein screened yet");
+  r->nposes = uint32_t(c->nposes[ligand]);
+  r->nselected = c->nsel[ligand];
+  r->fallback = c->fallback[ligand];
+  r->val_rmse = c->val_rmse[ligand];
+  r->val_rmse_calibrated = c->val_rmse_cal[ligand];
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, uint32_t *idx, float *energies, size_t cap,
+                                             size_t *n) {
+  if (!c || !n || ligand >= c->nlig)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_selected: bad arguments");
+  if (!c->screened) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_selected: no protein screened yet");
+  const size_t k = c->nsel[ligand];
+  *n = k;
+  if (!idx && !energies) return bude::ok();
+  if (cap < k) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_selected: buffer too small");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  std::vector<uint32_t> id(k);
+  std::vector<float> ex(k);
+  const size_t po = c->pose_off[ligand];
+  if (k) {
+    if (int st = bude::hip_check(hipMemcpy(id.data(), static_cast<uint32_t *>(c->sel_idx.ptr) + po, k * 4,
+                                           hipMemcpyDeviceToHost),
+                                 "hipMemcpy"))
+      return st;
+    if (c->fallback[ligand]) {  // the scores of a fallback pair are its exact energies
+      std::vector<float> all(c->nposes[ligand]);
+      if (int st = bude::hip_check(hipMemcpy(all.data(), static_cast<float *>(c->pred.ptr) + po, all.size() * 4,
+                                             hipMemcpyDeviceToHost),
+                                   "hipMemcpy"))
+        return st;
+      for (size_t j = 0; j < k; j++) ex[j] = all[id[j]];
+    } else if (int st = bude::hip_check(hipMemcpy(ex.data(), static_cast<float *>(c->exact_sel.ptr) + c->gofs[ligand],
+                                                  k * 4, hipMemcpyDeviceToHost),
+                                        "hipMemcpy")) {
+      return st;
+    }
+  }
+  std::vector<size_t> ord(k);
+  std::iota(ord.begin(), ord.end(), size_t(0));
+  std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return id[a] < id[b]; });
+  for (size_t j = 0; j < k; j++) {
+    if (idx) idx[j] = id[ord[j]];
+    if (energies) energies[j] = ex[ord[j]];
+  }
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_scores(bude_combined *c, size_t ligand, float *out) {
+  if (!c || !out || ligand >= c->nlig)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_scores: bad arguments");
+  if (!c->screened) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_scores: no protein screened yet");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  return bude::hip_check(hipMemcpy(out, static_cast<float *>(c->pred.ptr) + c->pose_off[ligand],
+                                   c->nposes[ligand] * 4, hipMemcpyDeviceToHost),
+                         "hipMemcpy");
+}
+
+extern "C" int bude_mi355x_combined_exhaustive(bude_combined *c, double *device_ms) {
+  if (!c) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exhaustive: NULL argument");
+  if (!c->screened)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exhaustive: no protein screened yet");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  if (int st = c->exact_all.reserve(c->total * 4)) return st;
+  std::vector<Span> all(c->nlig);
+  for (size_t l = 0; l < c->nlig; l++) all[l] = Span{l, c->pose_off[l], c->nposes[l]};
+  if (int st = record(c, 0)) return st;
+  if (int st = exact_ragged(c, all, static_cast<const float *>(c->poses.ptr), c->total,
+                            static_cast<float *>(c->exact_all.ptr), c->meta_all, c->tiles_all))
+    return st;
+  if (int st = record(c, 1)) return st;
+  if (int st = sync(c)) return st;
+  if (device_ms) *device_ms = elapsed(c, 0, 1);
+  c->have_exact = true;
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_exact(bude_combined *c, size_t ligand, float *out) {
+  if (!c || !out || ligand >= c->nlig)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exact: bad arguments");
+  if (!c->have_exact)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exact: run bude_mi355x_combined_exhaustive first");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  return bude::hip_check(hipMemcpy(out, static_cast<float *>(c->exact_all.ptr) + c->pose_off[ligand],
+                                   c->nposes[ligand] * 4, hipMemcpyDeviceToHost),
+                         "hipMemcpy");
+}
+
+extern "C" int bude_mi355x_combined_recall(bude_combined *c, float top_fraction, float *recall) {
+  if (!c || !recall || !(top_fraction > 0.f) || top_fraction > 1.f)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_recall: bad arguments");
+  if (!c->have_exact)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_recall: run bude_mi355x_combined_exhaustive first");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  // the exact top set of every pair by the same per-segment selection
+  std::vector<bude::sel::Segment> segs(c->nlig);
+  std::vector<uint32_t> kt(c->nlig);
+  for (size_t l = 0; l < c->nlig; l++) {
+    kt[l] = uint32_t(std::max(1.0, std::nearbyint(double(top_fraction) * double(c->nposes[l]))));
+    segs[l] = bude::sel::Segment{int64_t(c->pose_off[l]), uint32_t(c->nposes[l]), kt[l]};
+  }
+  bude::sel::Plan plan;
+  DeviceBuffer tidx, tn;
+  int st = plan.prepare(segs);
+  if (!st) st = tidx.alloc(c->total * 4);
+  if (!st) st = tn.alloc(c->nlig * 4);
+  if (!st)
+    st = bude::hip_check(bude::sel::select_lowest_segments(plan, static_cast<const float *>(c->exact_all.ptr),
+                                                           static_cast<uint32_t *>(tidx.ptr),
+                                                           static_cast<uint32_t *>(tn.ptr), c->stream),
+                         "select");
+  if (!st) st = sync(c);
+  if (st) return st;
+  std::vector<uint32_t> tcount(c->nlig);
+  if (int e = bude::hip_check(hipMemcpy(tcount.data(), tn.ptr, c->nlig * 4, hipMemcpyDeviceToHost), "hipMemcpy"))
+    return e;
+  for (size_t l = 0; l < c->nlig; l++) {
+    const size_t po = c->pose_off[l];
+    std::vector<uint32_t> t(tcount[l]), s(c->nsel[l]);
+    if (int e = bude::hip_check(hipMemcpy(t.data(), static_cast<uint32_t *>(tidx.ptr) + po, t.size() * 4,
+                                          hipMemcpyDeviceToHost),
+                                "hipMemcpy"))
+      return e;
+    if (int e = bude::hip_check(hipMemcpy(s.data(), static_cast<uint32_t *>(c->sel_idx.ptr) + po, s.size() * 4,
+                                          hipMemcpyDeviceToHost),
+                                "hipMemcpy"))
+      return e;
+    std::vector<char> mark(c->nposes[l], 0);
+    for (uint32_t j : s) mark[j] = 1;
+    size_t hits = 0;
+    for (uint32_t j : t) hits += mark[j];
+    recall[l] = float(double(std::min<size_t>(hits, kt[l])) / double(kt[l]));
+  }
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_grid(bude_combined *c, float lo[3], float *h, int32_t n[3]) {
+  if (!c || !lo || !h || !n) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_grid: NULL argument");
+  if (c->cfg.surrogate != BUDE_SURROGATE_FIELD)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_grid: not a field screen");
+  for (int k = 0; k < 3; k++) {
+    lo[k] = c->grid.lo[k];
+    n[k] = c->grid.n[k];
+  }
+  *h = c->grid.h;
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_map(bude_combined *c, int32_t type, float *out) {
+  if (!c || !out || type < 0 || size_t(type) >= c->slot_of_type.size())
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_map: bad arguments");
+  if (c->cfg.surrogate != BUDE_SURROGATE_FIELD || !c->screened)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_map: no field screen run yet");
+  const int32_t slot = c->slot_of_type[size_t(type)];
+  if (slot < 0) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_map: no ligand atom has this type");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  const size_t np = c->grid.points();
+  return bude::hip_check(hipMemcpy(out, static_cast<float *>(c->maps.ptr) + size_t(slot) * np, np * 4,
+                                   hipMemcpyDeviceToHost),
+                         "hipMemcpy");
+}
+
+extern "C" int bude_mi355x_combined_destroy(bude_combined *c) {
+  if (!c) return bude::ok();
+  (void)hipSetDevice(c->device);
+  delete c;
+  return bude::ok();
+}

@@ -20,190 +20,16 @@
 #include "fasten_plain.h"
 #include "hip_util.h"
 #include "mlp.h"
+#include "surrogate.h"
 
 namespace {
 
 using bude::DeviceBuffer;
 using bude::mlp::bf16;
-constexpr int kK[4] = {bude::mlp::kIn, bude::mlp::kHidden, bude::mlp::kHidden, bude::mlp::kHidden};
-constexpr int kN[4] = {bude::mlp::kHidden, bude::mlp::kHidden, bude::mlp::kHidden, bude::mlp::kOut};
-
-struct Net {
-  DeviceBuffer w[4], b[4], mw[4], vw[4], mb[4], vb[4], gw[4], gb[4], w16[4], w16t[4];
-  int step = 0;
-};
-
-// Activations of one batch of `rows` poses (training keeps everything the
-// backward pass reads; inference uses feat, a[0], a[1] as ping-pong).
-struct Acts {
-  size_t rows = 0;
-  DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss, work;
-};
-
-int alloc_acts(Acts &A, size_t rows, bool training) {
-  A.rows = rows;
-  const size_t H = bude::mlp::kHidden, O = bude::mlp::kOut, I = bude::mlp::kIn;
-  int st = A.feat.alloc(rows * I * 2);
-  if (!st) st = A.pred.alloc(rows * 4);
-  for (int l = 0; l < 3 && !st; l++) st = A.a[l].alloc(rows * H * 2);
-  if (training) {
-    if (!st) st = A.feat_t.alloc(rows * I * 2);
-    for (int l = 0; l < 3 && !st; l++) {
-      st = A.z[l].alloc(rows * H * 4);
-      if (!st) st = A.a_t[l].alloc(rows * H * 2);
-      if (!st) st = A.dz[l].alloc(rows * H * 2);
-      if (!st) st = A.dz_t[l].alloc(rows * H * 2);
-    }
-    if (!st) st = A.dz3.alloc(rows * O * 2);
-    if (!st) st = A.dz3_t.alloc(rows * O * 2);
-    if (!st) st = A.loss.alloc(256);
-    // split-K partial sums of the largest weight gradient
-    size_t wmax = 0;
-    for (int l = 0; l < 4; l++)
-      wmax = std::max(wmax, size_t(bude::mlp::auto_ksplit(kK[l], kN[l], int(rows))) * kK[l] * kN[l]);
-    if (!st) st = A.work.alloc(wmax * 4);
-  }
-  return st;
-}
-
-bude::mlp::GemmArgs gemm_args(const void *a, const void *b, int M, int N, int K, int lda, int ldb, int epi) {
-  bude::mlp::GemmArgs g{};
-  g.a = static_cast<const bf16 *>(a);
-  g.b = static_cast<const bf16 *>(b);
-  g.M = M;
-  g.N = N;
-  g.K = K;
-  g.lda = lda;
-  g.ldb = ldb;
-  g.epi = epi;
-  return g;
-}
-
-// Forward over `rows` rows already featurised in A.feat; training stores
-// z, a, a^T for the backward pass.
-int forward(const Net &net, Acts &A, size_t rows, bool training, hipStream_t s) {
-  const int R = int(rows);
-  const void *in = A.feat.ptr;
-  int in_ld = bude::mlp::kIn;
-  for (int l = 0; l < 3; l++) {
-    auto g = gemm_args(in, net.w16t[l].ptr, R, kN[l], kK[l], in_ld, kK[l], bude::mlp::kEpiHidden);
-    g.bias = static_cast<const float *>(net.b[l].ptr);
-    g.out_f32 = training ? static_cast<float *>(A.z[l].ptr) : nullptr;
-    g.out_bf16 = static_cast<bf16 *>(A.a[training ? l : (l & 1)].ptr);
-    g.out_t = training ? static_cast<bf16 *>(A.a_t[l].ptr) : nullptr;
-    g.ldo = kN[l];
-    g.ldt = R;
-    if (int st = bude::hip_check(bude::mlp::gemm(g, s), "mlp gemm forward")) return st;
-    in = g.out_bf16;
-    in_ld = kN[l];
-  }
-  auto g = gemm_args(in, net.w16t[3].ptr, R, kN[3], kK[3], in_ld, kK[3], bude::mlp::kEpiOut);
-  g.bias = static_cast<const float *>(net.b[3].ptr);
-  g.out_f32 = static_cast<float *>(A.pred.ptr);
-  return bude::hip_check(bude::mlp::gemm(g, s), "mlp gemm output");
-}
-
-// Backward from dz3 (already written by the loss kernel) into net.gw / gb.
-int backward(Net &net, Acts &A, size_t rows, hipStream_t s) {
-  const int R = int(rows);
-  bude::mlp::BiasGradBatch bg{};
-  bg.count = 4;
-  bg.B = R;
-  bg.ldt = R;
-  const void *dz_t_next = A.dz3_t.ptr;  // [N_l][R]
-  const void *dz_next = A.dz3.ptr;      // [R][N_l]
-  for (int l = 3; l >= 0; l--) {
-    // dW_l = A_{l-1}^T dZ_l  -> [K_l][N_l]
-    const void *prev_t = l == 0 ? A.feat_t.ptr : A.a_t[l - 1].ptr;
-    auto gw = gemm_args(prev_t, dz_t_next, kK[l], kN[l], R, R, R, bude::mlp::kEpiF32);
-    gw.out_f32 = static_cast<float *>(net.gw[l].ptr);
-    gw.ldo = kN[l];
-    gw.ksplit = bude::mlp::auto_ksplit(kK[l], kN[l], R);
-    gw.work = static_cast<float *>(A.work.ptr);
-    if (int st = bude::hip_check(bude::mlp::gemm(gw, s), "mlp gemm dW")) return st;
-    bg.dz_t[l] = static_cast<const bf16 *>(dz_t_next);
-    bg.db[l] = static_cast<float *>(net.gb[l].ptr);
-    bg.N[l] = kN[l];
-    if (l == 0) break;
-    // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1})  -> [R][K_l]
-    auto ga = gemm_args(dz_next, net.w16[l].ptr, R, kK[l], kN[l], kN[l], kN[l], bude::mlp::kEpiBackGelu);
-    ga.zprev = static_cast<const float *>(A.z[l - 1].ptr);
-    ga.out_bf16 = static_cast<bf16 *>(A.dz[l - 1].ptr);
-    ga.out_t = static_cast<bf16 *>(A.dz_t[l - 1].ptr);
-    ga.ldo = kK[l];
-    ga.ldt = R;
-    if (int st = bude::hip_check(bude::mlp::gemm(ga, s), "mlp gemm dA")) return st;
-    dz_next = A.dz[l - 1].ptr;
-    dz_t_next = A.dz_t[l - 1].ptr;
-  }
-  return bude::hip_check(bude::mlp::bias_grad_multi(bg, s), "mlp bias grads");
-}
-
-int adam_step(Net &net, float lr, hipStream_t s) {
-  net.step++;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float bc1 = 1.f - std::pow(b1, float(net.step)), bc2 = 1.f - std::pow(b2, float(net.step));
-  bude::mlp::AdamBatch ab{};
-  for (int l = 0; l < 4; l++) {
-    ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(net.gw[l].ptr),
-                   static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
-                   static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), kK[l], kN[l]};
-    ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(net.gb[l].ptr),
-                       static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), nullptr, nullptr, 1,
-                       kN[l]};
-  }
-  ab.count = 8;
-  return bude::hip_check(bude::mlp::adam_multi(ab, lr, b1, b2, eps, bc1, bc2, s), "adam");
-}
-
-// Upload FP32 weights W_l [kK[l]][kN[l]] (zero padded) and biases; refresh BF16 copies.
-int set_weights(Net &net, const std::vector<std::vector<float>> &w, const std::vector<std::vector<float>> &b,
-                hipStream_t s) {
-  for (int l = 0; l < 4; l++) {
-    const size_t nw = size_t(kK[l]) * kN[l];
-    int st = net.w[l].upload(w[l].data(), nw * 4);
-    if (!st) st = net.b[l].upload(b[l].data(), size_t(kN[l]) * 4);
-    for (DeviceBuffer *d : {&net.mw[l], &net.vw[l], &net.gw[l]})
-      if (!st) {
-        st = d->alloc(nw * 4);
-        if (!st) st = bude::hip_check(hipMemset(d->ptr, 0, nw * 4), "hipMemset");
-      }
-    for (DeviceBuffer *d : {&net.mb[l], &net.vb[l], &net.gb[l]})
-      if (!st) {
-        st = d->alloc(size_t(kN[l]) * 4);
-        if (!st) st = bude::hip_check(hipMemset(d->ptr, 0, size_t(kN[l]) * 4), "hipMemset");
-      }
-    if (!st) st = net.w16[l].alloc(nw * 2);
-    if (!st) st = net.w16t[l].alloc(nw * 2);
-    if (!st)
-      st = bude::hip_check(bude::mlp::to_bf16(static_cast<const float *>(net.w[l].ptr), kK[l], kN[l],
-                                              static_cast<bf16 *>(net.w16[l].ptr),
-                                              static_cast<bf16 *>(net.w16t[l].ptr), s),
-                           "to_bf16");
-    if (st) return st;
-  }
-  net.step = 0;
-  return bude::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-}
-
-// He-normal hidden weights, small output weights, zero biases; only the 12
-// real input rows and the one real output column are non-zero.
-void init_weights(uint64_t seed, std::vector<std::vector<float>> &w, std::vector<std::vector<float>> &b) {
-  std::mt19937_64 g(seed ^ 0x5eedf00dull);
-  std::normal_distribution<float> nd(0.f, 1.f);
-  w.assign(4, {});
-  b.assign(4, {});
-  for (int l = 0; l < 4; l++) {
-    w[l].assign(size_t(kK[l]) * kN[l], 0.f);
-    b[l].assign(size_t(kN[l]), 0.f);
-    const int fan_in = l == 0 ? bude::mlp::kFeat : kK[l];
-    const int rows = l == 0 ? bude::mlp::kFeat : kK[l];
-    const int cols = l == 3 ? 1 : kN[l];
-    const float sd = l == 3 ? std::sqrt(1.f / float(fan_in)) : std::sqrt(2.f / float(fan_in));
-    for (int r = 0; r < rows; r++)
-      for (int c = 0; c < cols; c++) w[l][size_t(r) * kN[l] + c] = sd * nd(g);
-  }
-}
+using bude::surrogate::Acts;
+using bude::surrogate::Net;
+using bude::surrogate::kK;
+using bude::surrogate::kN;
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -269,14 +95,7 @@ int upload_poses(DeviceBuffer &d, const float *const poses[6], size_t n) {
 // start at `poses` with row stride `stride`: the fused inference kernel on a
 // persistent grid of 2 work-groups per CU.
 int predict_all(bude_screen *s, const float *poses, size_t stride, size_t n, float *out) {
-  const bf16 *wt[4] = {static_cast<const bf16 *>(s->net.w16t[0].ptr), static_cast<const bf16 *>(s->net.w16t[1].ptr),
-                       static_cast<const bf16 *>(s->net.w16t[2].ptr), static_cast<const bf16 *>(s->net.w16t[3].ptr)};
-  const float *bias[4] = {static_cast<const float *>(s->net.b[0].ptr), static_cast<const float *>(s->net.b[1].ptr),
-                          static_cast<const float *>(s->net.b[2].ptr), static_cast<const float *>(s->net.b[3].ptr)};
-  if (int st = bude::hip_check(bude::mlp::infer(poses, stride, n, wt, static_cast<const bf16 *>(s->net.w16[3].ptr),
-                                                bias, out, 2 * s->cus, s->stream),
-                               "mlp infer"))
-    return st;
+  if (int st = bude::surrogate::predict(s->net, poses, stride, n, out, 2 * s->cus, s->stream)) return st;
   return bude::hip_check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
 }
 
@@ -326,8 +145,8 @@ extern "C" int bude_mi355x_screen_create(int device, const BudeAtom *protein, si
   for (auto &e : s->ev)
     if (!st) st = bude::hip_check(hipEventCreate(&e), "hipEventCreate");
   std::vector<std::vector<float>> w, b;
-  init_weights(cfg->seed, w, b);
-  if (!st) st = set_weights(s->net, w, b, s->stream);
+  bude::surrogate::init_weights(cfg->seed, bude::mlp::kFeat, w, b);
+  if (!st) st = bude::surrogate::set_weights(s->net, w, b, s->stream);
   if (st) return st;
   *out = s.release();
   return bude::ok();
@@ -380,56 +199,26 @@ extern "C" int bude_mi355x_screen_train(bude_screen *s, const float *const poses
     s->sigma = float(std::sqrt(std::max(1e-12, sq / double(ntrain) - double(s->mu) * s->mu)));
   }
 
-  // 3. Adam over shuffled mini-batches; learning rate: linear warm-up then cosine decay
-  Acts A;
-  if (int st = alloc_acts(A, B, true)) return st;
+  // 3. Adam over shuffled mini-batches of the training split
   std::vector<uint32_t> perm(ntrain);
   std::iota(perm.begin(), perm.end(), 0u);
-  std::mt19937_64 rng(s->cfg.seed);
-  DeviceBuffer dperm;
-  if (int st = dperm.alloc(ntrain * 4)) return st;
-  const size_t steps_per_epoch = ntrain / B;
-  const size_t total = steps_per_epoch * std::max<uint32_t>(1, s->cfg.epochs);
-  const size_t warm = std::min<size_t>(200, total / 10 + 1);
+  bude::surrogate::TrainData td;
+  td.poses = static_cast<const float *>(tp.ptr);
+  td.stride = n;
+  td.energy = static_cast<const float *>(te.ptr);
+  td.norm = bude::mlp::TargetNorm{s->clip, s->mu, 1.0f / s->sigma, s->tail};
+  bude::surrogate::TrainParams tpar;
+  tpar.batch = uint32_t(B);
+  tpar.epochs = s->cfg.epochs;
+  tpar.lr = s->cfg.lr;
+  tpar.tail_weight = s->cfg.tail_weight;
+  tpar.seed = s->cfg.seed;
+  bude::surrogate::TrainReport rep;
   t0 = std::chrono::steady_clock::now();
-  size_t step = 0;
-  for (uint32_t ep = 0; ep < std::max<uint32_t>(1, s->cfg.epochs); ep++) {
-    std::shuffle(perm.begin(), perm.end(), rng);
-    if (int st = bude::hip_check(hipMemcpyAsync(dperm.ptr, perm.data(), ntrain * 4, hipMemcpyHostToDevice, st_),
-                                 "hipMemcpyAsync"))
-      return st;
-    if (int st = bude::hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, st_), "hipMemsetAsync")) return st;
-    for (size_t it = 0; it < steps_per_epoch; it++, step++) {
-      const uint32_t *idx = static_cast<const uint32_t *>(dperm.ptr) + it * B;
-      if (int st = bude::hip_check(bude::mlp::features(static_cast<const float *>(tp.ptr), n, idx, B,
-                                                       static_cast<bf16 *>(A.feat.ptr),
-                                                       static_cast<bf16 *>(A.feat_t.ptr), int(B), st_),
-                                   "features"))
-        return st;
-      if (int st = forward(s->net, A, B, true, st_)) return st;
-      if (int st = bude::hip_check(
-              bude::mlp::loss_grad(static_cast<const float *>(A.pred.ptr), static_cast<const float *>(te.ptr), idx, B,
-                                   s->clip, s->mu, s->sigma, s->tail, s->cfg.tail_weight,
-                                   static_cast<bf16 *>(A.dz3.ptr),
-                                   static_cast<bf16 *>(A.dz3_t.ptr), int(B), static_cast<float *>(A.loss.ptr), st_),
-              "loss"))
-        return st;
-      if (int st = backward(s->net, A, B, st_)) return st;
-      const float warmf = step < warm ? float(step + 1) / float(warm) : 1.f;
-      const float prog = float(step) / float(std::max<size_t>(1, total - 1));
-      const float lr = s->cfg.lr * warmf * (0.05f + 0.95f * 0.5f * (1.f + std::cos(3.14159265f * prog)));
-      if (int st = adam_step(s->net, lr, st_)) return st;
-    }
-    float lsum = 0.f;
-    if (int st = bude::hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 4, hipMemcpyDeviceToHost, st_), "hipMemcpyAsync"))
-      return st;
-    if (int st = bude::hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize")) return st;
-    const float mse = lsum / float(steps_per_epoch * B);
-    if (loss_curve && ep < loss_cap) loss_curve[ep] = mse;
-    stats->final_loss = mse;
-  }
+  if (int st = bude::surrogate::train(s->net, td, std::move(perm), tpar, st_, loss_curve, loss_cap, &rep)) return st;
+  stats->final_loss = rep.final_loss;
   stats->train_ms = ms_since(t0);
-  stats->steps = uint32_t(step);
+  stats->steps = rep.steps;
 
   // 4. validation RMSE (normalised units) on the held-out split
   DeviceBuffer vpred;
@@ -640,9 +429,9 @@ extern "C" int bude_mi355x_mlp_forward_backward(int device, size_t batch, const 
     for (int c = 0; c < realN[l]; c++) b[l][c] = biases[l][c];
   }
   Net net;
-  if (int st = set_weights(net, w, b, s)) return st;
+  if (int st = bude::surrogate::set_weights(net, w, b, s)) return st;
   Acts A;
-  if (int st = alloc_acts(A, batch, true)) return st;
+  if (int st = bude::surrogate::alloc_acts(A, batch, true)) return st;
   std::vector<float> fpad(batch * bude::mlp::kIn, 0.f);
   for (size_t i = 0; i < batch; i++)
     for (int c = 0; c < bude::mlp::kFeat; c++) fpad[i * bude::mlp::kIn + c] = features[i * bude::mlp::kFeat + c];
@@ -654,15 +443,16 @@ extern "C" int bude_mi355x_mlp_forward_backward(int device, size_t batch, const 
                                                   s),
                                "to_bf16"))
     return st;
-  if (int st = forward(net, A, batch, true, s)) return st;
+  if (int st = bude::surrogate::forward(net, A, batch, true, s)) return st;
   if (int st = bude::hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, s), "hipMemsetAsync")) return st;
   if (int st = bude::hip_check(
           bude::mlp::loss_grad(static_cast<const float *>(A.pred.ptr), static_cast<const float *>(tgt.ptr), nullptr,
-                               batch, 3.0e38f, 0.f, 1.f, -3.0e38f, 1.f, static_cast<bf16 *>(A.dz3.ptr),
+                               batch, bude::mlp::TargetNorm{3.0e38f, 0.f, 1.f, -3.0e38f}, nullptr, nullptr, 1.f,
+                               static_cast<bf16 *>(A.dz3.ptr),
                                static_cast<bf16 *>(A.dz3_t.ptr), int(batch), static_cast<float *>(A.loss.ptr), s),
           "loss"))
     return st;
-  if (int st = backward(net, A, batch, s)) return st;
+  if (int st = bude::surrogate::backward(net, A, batch, s)) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
   if (int st = bude::hip_check(hipMemcpy(pred, A.pred.ptr, batch * 4, hipMemcpyDeviceToHost), "hipMemcpy")) return st;
   float lsum = 0.f;
@@ -706,7 +496,7 @@ extern "C" int bude_mi355x_mlp_predict(int device, const float *const poses[6], 
     for (int c = 0; c < realN[l]; c++) b[l][c] = biases[l][c];
   }
   Net net;
-  if (int st = set_weights(net, w, b, s)) return st;
+  if (int st = bude::surrogate::set_weights(net, w, b, s)) return st;
   DeviceBuffer dp, out;
   if (int st = upload_poses(dp, poses, n)) return st;
   if (int st = out.alloc(n * 4)) return st;

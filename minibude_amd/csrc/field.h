@@ -1,0 +1,55 @@
+// field.h -- per-protein energy maps of single ligand atoms, and the pose
+// scorer that sums trilinear interpolations of them over a ligand's atoms.
+//
+// fasten_main's energy is a sum over ligand atoms of terms that depend only
+// on that atom's forcefield type and its position after the pose transform
+// (src/serial/fasten.hpp:56-131: the pair parameters of a3 depend on the
+// (ligand type, protein type) pair, the pair energy on the distance).  So
+// for one protein, E(pose) = sum_il F[type(il)](R x_il + t) with
+// F[t](p) = the energy of one ligand atom of type t at p.  The maps hold F
+// exactly (computed by the same pair loop as the plain kernel) at the points
+// of a regular grid; the scorer interpolates between them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bude_internal.h"
+#include "fasten_batch.h"
+
+namespace bude {
+namespace field {
+
+struct Grid {
+  float lo[3];  // coordinates of grid point (0, 0, 0)
+  float h;      // spacing
+  int n[3];     // points per axis, each >= 2
+  __host__ __device__ size_t points() const { return size_t(n[0]) * size_t(n[1]) * size_t(n[2]); }
+};
+
+// maps[(s * n0 + ix) * n1 + iy) * n2 + iz] = energy of one ligand atom whose
+// pair coefficients are coef[s * ngroups + g] (g over the protein groups) at
+// point lo + h * (ix, iy, iz): 0.5 * the sum of the pair energies, groups in
+// order, exactly as the plain kernel sums one ligand atom.
+hipError_t build_maps(const float4 *protein, const int32_t *group_begin, const PairCoef *coef, int ngroups,
+                      int nslots, const Grid &g, int form, float *maps, hipStream_t s);
+
+struct ScoreArgs {
+  const float *maps;          // nslots maps of g.points() floats
+  Grid grid;
+  const float4 *ligands;      // x, y, z, map slot (int bits)
+  const LigandMeta *meta;     // lig_off, natlig, nposes, pose_off of every ligand
+  const int64_t *tile_begin;  // nligands + 1: first 64-pose tile of each ligand
+  int nligands;
+  const float *poses;         // 6 rows of `stride` floats
+  size_t stride;
+  float *out;                 // out[meta.pose_off + j]
+};
+
+// Interpolated energy of every pose of every ligand: one wave per tile of 64
+// poses of one ligand; ntiles = tile_begin[nligands].
+hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s);
+
+}  // namespace field
+}  // namespace bude

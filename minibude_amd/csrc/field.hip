@@ -1,0 +1,110 @@
+// field.hip -- single-atom energy maps and the interpolating pose scorer.
+#include <hip/hip_runtime.h>
+
+#include "fasten_core.h"
+#include "field.h"
+
+namespace bude {
+namespace field {
+
+namespace {
+
+using namespace core;
+
+template <int FORM>
+__global__ __launch_bounds__(256) void map_kernel(const float4 *__restrict__ protein,
+                                                  const int32_t *__restrict__ group_begin,
+                                                  const PairCoef *__restrict__ coef, int ngroups, const Grid g,
+                                                  float *__restrict__ maps) {
+  const size_t np = g.points();
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const size_t q = i < np ? i : np - 1;
+  const size_t n12 = size_t(g.n[1]) * size_t(g.n[2]);
+  const int ix = int(q / n12), iy = int((q / size_t(g.n[2])) % size_t(g.n[1])), iz = int(q % size_t(g.n[2]));
+  float lx[1] = {g.lo[0] + float(ix) * g.h};
+  float ly[1] = {g.lo[1] + float(iy) * g.h};
+  float lz[1] = {g.lo[2] + float(iz) * g.h};
+  float e[1] = {0.f};
+  const PairCoef *c = coef + size_t(blockIdx.y) * size_t(ngroups);
+  for (int k = 0; k < ngroups; ++k) group_dispatch<1, FORM>(protein, group_begin[k], group_begin[k + 1], c[k], lx, ly, lz, e);
+  if (i < np) maps[size_t(blockIdx.y) * np + i] = e[0] * 0.5f;
+}
+
+// Cell index and fraction along one axis, clamped to the grid.
+__device__ __forceinline__ void cell(float w, float lo, float inv_h, int n, int &i0, float &f) {
+  const float u = (w - lo) * inv_h;
+  i0 = min(max(int(floorf(u)), 0), n - 2);
+  f = __builtin_amdgcn_fmed3f(u - float(i0), 0.0f, 1.0f);
+}
+
+__global__ __launch_bounds__(256) void score_kernel(const float *__restrict__ maps, const Grid g,
+                                                    const float4 *__restrict__ ligands,
+                                                    const LigandMeta *__restrict__ meta,
+                                                    const int64_t *__restrict__ tile_begin, int nligands,
+                                                    const float *__restrict__ poses, size_t stride,
+                                                    float *__restrict__ out, int64_t ntiles) {
+  const int lane = threadIdx.x & 63;
+  const int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (t >= ntiles) return;
+  // ligand of this tile: last l with tile_begin[l] <= t (wave-uniform search)
+  int lo = 0, hi = nligands - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_begin[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const LigandMeta m = meta[lo];
+  const int j = int(t - tile_begin[lo]) * 64 + lane;
+  const Xform x = make_xform(poses, stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
+  const float inv_h = 1.0f / g.h;
+  const size_t np = g.points();
+  const int s1 = g.n[2], s0 = g.n[1] * g.n[2];
+  float e = 0.f;
+  for (int il = 0; il < m.natlig; ++il) {
+    const float4 la = ligands[m.lig_off + il];
+    const float wx = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
+    const float wy = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
+    const float wz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
+    int ix, iy, iz;
+    float fx, fy, fz;
+    cell(wx, g.lo[0], inv_h, g.n[0], ix, fx);
+    cell(wy, g.lo[1], inv_h, g.n[1], iy, fy);
+    cell(wz, g.lo[2], inv_h, g.n[2], iz, fz);
+    const float *mp = maps + size_t(__float_as_int(la.w)) * np + size_t(ix * s0 + iy * s1 + iz);
+    const float v000 = mp[0], v001 = mp[1], v010 = mp[s1], v011 = mp[s1 + 1];
+    const float v100 = mp[s0], v101 = mp[s0 + 1], v110 = mp[s0 + s1], v111 = mp[s0 + s1 + 1];
+    const float c00 = __builtin_fmaf(fz, v001 - v000, v000), c01 = __builtin_fmaf(fz, v011 - v010, v010);
+    const float c10 = __builtin_fmaf(fz, v101 - v100, v100), c11 = __builtin_fmaf(fz, v111 - v110, v110);
+    const float c0 = __builtin_fmaf(fy, c01 - c00, c00), c1 = __builtin_fmaf(fy, c11 - c10, c10);
+    e += __builtin_fmaf(fx, c1 - c0, c0);
+  }
+  if (j < m.nposes) out[size_t(m.pose_off) + size_t(j)] = e;
+}
+
+}  // namespace
+
+hipError_t build_maps(const float4 *protein, const int32_t *group_begin, const PairCoef *coef, int ngroups,
+                      int nslots, const Grid &g, int form, float *maps, hipStream_t s) {
+  const size_t np = g.points();
+  if (np == 0 || nslots <= 0 || nslots > 65535 || g.n[0] < 2 || g.n[1] < 2 || g.n[2] < 2 || np > 0x7fffffffu ||
+      (np + 255) / 256 > 0x7fffffffu)
+    return hipErrorInvalidValue;
+  const dim3 grid(unsigned((np + 255) / 256), unsigned(nslots));
+  if (form == kFormD)
+    hipLaunchKernelGGL(map_kernel<kFormD>, grid, dim3(256), 0, s, protein, group_begin, coef, ngroups, g, maps);
+  else
+    hipLaunchKernelGGL(map_kernel<kFormB>, grid, dim3(256), 0, s, protein, group_begin, coef, ngroups, g, maps);
+  return hipGetLastError();
+}
+
+hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  if (a.nligands <= 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 || (ntiles + 3) / 4 > 0x7fffffff)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a.maps, a.grid, a.ligands,
+                     a.meta, a.tile_begin, a.nligands, a.poses, a.stride, a.out, ntiles);
+  return hipGetLastError();
+}
+
+}  // namespace field
+}  // namespace bude

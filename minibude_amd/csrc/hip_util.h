@@ -41,6 +41,8 @@ struct DeviceBuffer {
     bytes = n;
     return BUDE_OK;
   }
+  // Keeps the current allocation when it already holds n bytes.
+  int reserve(size_t n) { return (ptr && bytes >= n) ? BUDE_OK : alloc(n); }
   int upload(const void *src, size_t n) {
     if (int st = alloc(n)) return st;
     return n ? hip_check(hipMemcpy(ptr, src, n, hipMemcpyHostToDevice), "hipMemcpy H2D") : BUDE_OK;

@@ -18,6 +18,13 @@ constexpr int kIn = 32;      // feature width padded to the MFMA K step
 constexpr int kHidden = 256;
 constexpr int kOut = 32;     // output layer padded to one MFMA tile (column 0 is the energy)
 constexpr int kLayers = 4;
+constexpr int kDesc = kIn - kFeat;  // ligand descriptor inputs (zero when one ligand is screened)
+
+// Target transform of one ligand's energies: y = (min(E, clip) - mu) * inv_sigma;
+// energies below `tail` get the tail loss weight.
+struct TargetNorm {
+  float clip, mu, inv_sigma, tail;
+};
 
 // GEMM epilogues.
 enum Epilogue : int {
@@ -53,16 +60,25 @@ int auto_ksplit(int M, int N, int K);
 
 // Features of poses idx[i] (or i when idx is null) from 6 pose rows of
 // `stride` floats: row i of out [n][kIn] (bf16), and out_t [kIn][ldt] if set.
+// With desc [nlig][kDesc] and lig (ligand of each pose row), columns kFeat..
+// hold desc[lig[q]]; otherwise they are zero.
 hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t,
-                    int ldt, hipStream_t s);
+                    int ldt, hipStream_t s, const float *desc = nullptr, const uint32_t *lig = nullptr);
 
-// Weighted MSE on the transformed target: y = (min(E, clip) - mu) / sigma,
-// weight w = tail_weight when E < tail else 1.
+// Weighted MSE on the transformed target of pose q = idx[b] (or b):
+// y = (min(E, clip) - mu) * inv_sigma with the TargetNorm table[lig[q]]
+// (device memory) when table is set, else `norm`; weight tail_weight when
+// E < tail else 1.
 // dz[b][0] = 2 w (pred - y) / n, other columns 0 (also transposed);
 // loss_acc += sum (pred - y)^2 (f32 atomics, for monitoring).
-hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, float clip, float mu,
-                     float sigma, float tail, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt, float *loss_acc,
-                     hipStream_t s);
+hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, TargetNorm norm,
+                     const TargetNorm *table, const uint32_t *lig, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt,
+                     float *loss_acc, hipStream_t s);
+
+// Per-ligand first-layer bias with the descriptor inputs folded in, for the
+// fused inference kernel (whose descriptor columns are zero):
+// out[l][j] = b[j] + sum_c bf16(desc[l][c]) * w16[kFeat + c][j], w16 = W_0 [kIn][kHidden].
+hipError_t fold_bias(const float *b, const bf16 *w16, const float *desc, int nlig, float *out, hipStream_t s);
 
 // db[n] = sum_b dz_t[n][b] for n < N rows of length B.
 hipError_t bias_grad(const bf16 *dz_t, int N, int B, int ldt, float *db, hipStream_t s);

@@ -156,12 +156,18 @@ __global__ void ksplit_reduce_kernel(const float *__restrict__ work, size_t n, i
 }
 
 __global__ void features_kernel(const float *__restrict__ poses, size_t stride, const uint32_t *__restrict__ idx,
-                                size_t n, bf16 *__restrict__ out, bf16 *__restrict__ out_t, int ldt) {
+                                size_t n, bf16 *__restrict__ out, bf16 *__restrict__ out_t, int ldt,
+                                const float *__restrict__ desc, const uint32_t *__restrict__ lig) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t q = idx ? idx[i] : i;
   float f[kIn];
   pose_features(poses, stride, q, f);
+  if (desc) {
+    const float *d = desc + size_t(lig[q]) * kDesc;
+#pragma unroll
+    for (int c = 0; c < kDesc; c++) f[kFeat + c] = d[c];
+  }
   bf16 h[kIn];
 #pragma unroll
   for (int c = 0; c < kIn; c++) h[c] = to_bf16(f[c]);
@@ -178,17 +184,20 @@ __global__ void features_kernel(const float *__restrict__ poses, size_t stride, 
 }
 
 __global__ void loss_grad_kernel(const float *__restrict__ pred, const float *__restrict__ energy,
-                                 const uint32_t *__restrict__ idx, size_t n, float clip, float mu, float inv_sigma,
-                                 float tail, float tail_weight, bf16 *__restrict__ dz, bf16 *__restrict__ dz_t,
-                                 int ldt, float *__restrict__ loss) {
+                                 const uint32_t *__restrict__ idx, size_t n, TargetNorm norm,
+                                 const TargetNorm *__restrict__ table, const uint32_t *__restrict__ lig,
+                                 float tail_weight, bf16 *__restrict__ dz, bf16 *__restrict__ dz_t, int ldt,
+                                 float *__restrict__ loss) {
   const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float sq = 0.f;
   if (b < n) {
-    const float e = energy[idx ? idx[b] : b];
-    const float y = (fminf(e, clip) - mu) * inv_sigma;
-    const float d = pred[b] - y;
+    const size_t q = idx ? idx[b] : b;
+    const float e = energy[q];
+    const TargetNorm t = table ? table[lig[q]] : norm;
+    const float y = (fminf(e, t.clip) - t.mu) * t.inv_sigma;
+  const float d = pred[b] - y;
     sq = d * d;
-    const float wgt = e < tail ? tail_weight : 1.0f;
+    const float wgt = e < t.tail ? tail_weight : 1.0f;
     const bf16 g = to_bf16(2.0f * wgt * d / float(n));
     dz[b * kOut] = g;
 #pragma unroll
@@ -439,6 +448,17 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict_
   }
 }
 
+__global__ __launch_bounds__(256) void fold_bias_kernel(const float *__restrict__ b, const bf16 *__restrict__ w16,
+                                                        const float *__restrict__ desc, float *__restrict__ out) {
+  const int l = blockIdx.x, j = threadIdx.x;
+  float acc = b[j];
+#pragma unroll
+  for (int c = 0; c < kDesc; c++)
+    acc = __builtin_fmaf(from_bf16(to_bf16(desc[size_t(l) * kDesc + c])), from_bf16(w16[size_t(kFeat + c) * kHidden + j]),
+                         acc);
+  out[size_t(l) * kHidden + j] = acc;
+}
+
 unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
 
 }  // namespace
@@ -467,18 +487,26 @@ int auto_ksplit(int M, int N, int K) {
 }
 
 hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t, int ldt,
-                    hipStream_t s) {
+                    hipStream_t s, const float *desc, const uint32_t *lig) {
   if (n == 0) return hipSuccess;
+  if (desc && !lig) return hipErrorInvalidValue;
   hipLaunchKernelGGL(features_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, poses, stride, idx, n, out, out_t,
-                     ldt);
+                     ldt, desc, lig);
   return hipGetLastError();
 }
 
-hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, float clip, float mu,
-                     float sigma, float tail, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt, float *loss_acc,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, energy, idx, n, clip, mu,
-                     1.0f / sigma, tail, tail_weight, dz, dz_t, ldt, loss_acc);
+hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, TargetNorm norm,
+                     const TargetNorm *table, const uint32_t *lig, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt,
+                     float *loss_acc, hipStream_t s) {
+  if (table && !lig) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, energy, idx, n, norm, table,
+                     lig, tail_weight, dz, dz_t, ldt, loss_acc);
+  return hipGetLastError();
+}
+
+hipError_t fold_bias(const float *b, const bf16 *w16, const float *desc, int nlig, float *out, hipStream_t s) {
+  if (nlig <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fold_bias_kernel, dim3(unsigned(nlig)), dim3(kHidden), 0, s, b, w16, desc, out);
   return hipGetLastError();
 }
 

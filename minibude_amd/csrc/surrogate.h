@@ -1,0 +1,86 @@
+// surrogate.h -- training state of the MLP pose-energy surrogate (weights,
+// Adam moments, activations) and the training loop shared by the learned
+// prescreen of one ligand and the per-protein surrogate of a combined screen.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "hip_util.h"
+#include "mlp.h"
+
+namespace bude {
+namespace surrogate {
+
+using mlp::bf16;
+constexpr int kK[4] = {mlp::kIn, mlp::kHidden, mlp::kHidden, mlp::kHidden};
+constexpr int kN[4] = {mlp::kHidden, mlp::kHidden, mlp::kHidden, mlp::kOut};
+
+struct Net {
+  DeviceBuffer w[4], b[4], mw[4], vw[4], mb[4], vb[4], gw[4], gb[4], w16[4], w16t[4];
+  int step = 0;
+};
+
+// Activations of one batch of `rows` poses (training keeps everything the
+// backward pass reads).
+struct Acts {
+  size_t rows = 0;
+  DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss, work;
+};
+
+int alloc_acts(Acts &A, size_t rows, bool training);
+// Forward over `rows` rows already featurised in A.feat.
+int forward(const Net &net, Acts &A, size_t rows, bool training, hipStream_t s);
+// Backward from dz3 (written by the loss kernel) into net.gw / gb.
+int backward(Net &net, Acts &A, size_t rows, hipStream_t s);
+int adam_step(Net &net, float lr, hipStream_t s);
+// Uploads FP32 weights W_l [kK[l]][kN[l]] (zero padded) and biases, clears
+// the Adam state and refreshes the BF16 copies.
+int set_weights(Net &net, const std::vector<std::vector<float>> &w, const std::vector<std::vector<float>> &b,
+                hipStream_t s);
+// He-normal hidden weights, small output weights, zero biases; `inputs`
+// real input rows (the rest of the padded input stays zero) and one output.
+void init_weights(uint64_t seed, int inputs, std::vector<std::vector<float>> &w, std::vector<std::vector<float>> &b);
+
+// What the training loop reads, all device memory: label poses (6 rows of
+// `stride` floats), their exact energies, and either one TargetNorm for all
+// labels (table null) or a per-ligand table with the ligand of each label
+// and the per-ligand descriptor inputs.
+struct TrainData {
+  const float *poses = nullptr;
+  size_t stride = 0;
+  const float *energy = nullptr;
+  mlp::TargetNorm norm{};
+  const mlp::TargetNorm *table = nullptr;
+  const uint32_t *lig = nullptr;
+  const float *desc = nullptr;
+};
+
+struct TrainParams {
+  uint32_t batch = 8192, epochs = 1;
+  float lr = 1e-3f, tail_weight = 1.f;
+  uint64_t seed = 0;
+};
+
+struct TrainReport {
+  float final_loss = 0.f;
+  uint32_t steps = 0;
+};
+
+// Adam over mini-batches of the label indices `train` (reshuffled every
+// epoch with a seeded mt19937_64); learning rate: linear warm-up, then
+// cosine decay to 5 %.  loss_curve[e] (if set, e < cap) = epoch e's mean
+// squared error in standardised units.
+int train(Net &net, const TrainData &d, std::vector<uint32_t> train, const TrainParams &p, hipStream_t s,
+          float *loss_curve, size_t cap, TrainReport *rep);
+
+// The fused inference kernel over n poses (6 rows of `stride` floats) on a
+// persistent grid of `blocks` work-groups; bias0 replaces the first-layer
+// bias when set (a per-ligand bias with descriptor inputs folded in).
+int predict(const Net &net, const float *poses, size_t stride, size_t n, float *out, int blocks, hipStream_t s,
+            const float *bias0 = nullptr);
+
+}  // namespace surrogate
+}  // namespace bude

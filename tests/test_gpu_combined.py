@@ -1,0 +1,205 @@
+"""The combined screen on the GPU: single-atom energy maps against the plain
+kernel, the interpolating scorer against a numpy restatement, the energy
+decomposition on grid points, per-pair selection / exact rescoring / recall
+for both surrogates, and per-pair fallback."""
+import numpy as np
+import pytest
+
+from minibude_amd import combined as C
+from minibude_amd import deck as D
+from minibude_amd import screen as S
+from minibude_amd.model import Context
+
+pytestmark = pytest.mark.gpu
+
+
+def small_workload(nposes=8192, nligands=5, nproteins=2, seed=11):
+    return C.combined_workload(nproteins=nproteins, nligands=nligands, nposes=nposes, seed=seed,
+                               natpro_range=(300, 900), natlig_range=(4, 24))
+
+
+def plain_energies(protein, ligand, ff, poses, device):
+    with Context(D.Deck(protein, ligand, ff, np.ascontiguousarray(poses, np.float32)), device) as c:
+        return c.fasten(1, 64).energies
+
+
+def rotations(p):
+    a = p[:3].astype(np.float64)
+    sx, sy, sz = np.sin(a)
+    cx, cy, cz = np.cos(a)
+    return np.array([[cy * cz, sx * sy * cz - cx * sz, cx * sy * cz + sx * sz],
+                     [cy * sz, sx * sy * sz + cx * cz, cx * sy * sz - sx * cz],
+                     [-sy, sx * cy, cx * cy]]).transpose(2, 0, 1)  # (n, 3, 3)
+
+
+def trilinear(maps, lo, h, n, pos):
+    u = (pos - lo) / h
+    i0 = np.clip(np.floor(u).astype(np.int64), 0, np.array(n) - 2)
+    f = np.clip(u - i0, 0.0, 1.0)
+    out = 0.0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = (f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * \
+                    (f[:, 2] if dz else 1 - f[:, 2])
+                out = out + w * maps[i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz]
+    return out
+
+
+def test_field_maps_are_single_atom_plain_energies(gpu_device):
+    w = small_workload(nposes=2048, nligands=3, nproteins=1)
+    cfg = C.default_config("field", grid_spacing=1.0, max_val_rmse=1e9)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        c.screen(w.proteins[0])
+        lo, h, n = c.grid()
+        assert h == 1.0 and np.all(lo == np.round(lo))  # grid coordinates exact in FP32
+        t = int(w.ligands[0]["type"][0])
+        m = c.map(t)
+    g = [lo[k] + h * np.arange(n[k], dtype=np.float32) for k in range(3)]
+    X, Y, Z = np.meshgrid(*g, indexing="ij")
+    poses = np.zeros((6, X.size), np.float32)
+    poses[3], poses[4], poses[5] = X.ravel(), Y.ravel(), Z.ravel()
+    atom = np.array([(0.0, 0.0, 0.0, t)], dtype=w.ligands[0].dtype)
+    want = plain_energies(w.proteins[0], atom, w.forcefield, poses, gpu_device)
+    # same pair loop, same group order, same point: the same FP32 value
+    assert np.array_equal(m.ravel().view(np.uint32), want.view(np.uint32))
+
+
+def test_field_score_is_trilinear_sum_over_atoms(gpu_device):
+    w = small_workload(nposes=4096, nligands=3, nproteins=1)
+    cfg = C.default_config("field", grid_spacing=0.5, max_val_rmse=1e9)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        c.screen(w.proteins[0])
+        lo, h, n = c.grid()
+        for l in range(len(w.ligands)):
+            assert c.pair(l).fallback == 0
+            got = c.scores(l)
+            lig = w.ligands[l]
+            p = w.poses[l]
+            R = rotations(p)
+            t = p[3:].T.astype(np.float64)
+            want = np.zeros(p.shape[1])
+            mag = np.zeros(p.shape[1])
+            maps = {}
+            for a in range(len(lig)):
+                ty = int(lig["type"][a])
+                if ty not in maps:
+                    maps[ty] = c.map(ty).astype(np.float64)
+                x = np.array([lig["x"][a], lig["y"][a], lig["z"][a]], np.float64)
+                v = trilinear(maps[ty], lo.astype(np.float64), h, n, R @ x + t)
+                want += v
+                mag += np.abs(v)
+            # FP32 interpolation and sum vs FP64: relative to the sum of |terms|
+            assert np.max(np.abs(got - want) / np.maximum(mag, 1.0)) < 1e-4, f"ligand {l}"
+
+
+def test_field_is_exact_when_atoms_sit_on_grid_points(gpu_device):
+    # identity rotations, translations and atom coordinates on the 0.5 grid:
+    # the field score is the sum of exact single-atom energies, i.e. the
+    # exact pose energy up to FP32 summation order
+    ff = D.synth_forcefield(21, 34)
+    protein = D.synth_protein(21, 700, 34)
+    rng = np.random.default_rng(3)
+    lig = np.zeros(12, dtype=protein.dtype)
+    for k in "xyz":
+        lig[k] = rng.integers(-12, 13, size=12) * 0.5
+    lig["type"] = rng.integers(0, 34, size=12)
+    n = 1024
+    poses = np.zeros((6, n), np.float32)
+    poses[3:] = rng.integers(-10, 11, size=(3, n)) * 0.5
+    cfg = C.default_config("field", grid_spacing=0.5, max_val_rmse=1e9, labels_per_pair=64)
+    with C.Combined(ff, [lig], [poses], cfg, gpu_device) as c:
+        c.screen(protein)
+        got = c.scores(0)
+        lo, h, _ = c.grid()
+        maps = {int(t): c.map(int(t)).astype(np.float64) for t in np.unique(lig["type"])}
+    exact = plain_energies(protein, lig, ff, poses, gpu_device)
+    total = np.zeros(n)
+    mag = np.zeros(n)
+    for a in range(len(lig)):
+        pos = np.stack([lig[k][a] + poses[3 + i] for i, k in enumerate("xyz")], 1)
+        ijk = np.rint((pos - lo) / h).astype(np.int64)
+        v = maps[int(lig["type"][a])][ijk[:, 0], ijk[:, 1], ijk[:, 2]]
+        total += v
+        mag += np.abs(v)
+    mag = np.maximum(mag, 1.0)
+    # the scorer reads the map values exactly at grid points ...
+    assert np.max(np.abs(got - total) / mag) < 1e-5
+    # ... and their sum over the ligand's atoms is the pose energy
+    assert np.max(np.abs(exact - total) / mag) < 1e-5
+
+
+@pytest.mark.parametrize("surrogate", ["field", "mlp"])
+def test_selection_rescoring_and_recall(gpu_device, surrogate):
+    w = small_workload()
+    kw = dict(keep_fraction=0.05, max_val_rmse=1e9)
+    if surrogate == "mlp":
+        kw.update(labels_per_pair=2048, batch=1024, epochs=4)
+    cfg = C.default_config(surrogate, **kw)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        for p, protein in enumerate(w.proteins):
+            r = c.screen(protein)
+            assert r.fallback_pairs == 0 and r.nposes == sum(q.shape[1] for q in w.poses)
+            c.exhaustive()
+            rec = c.recall(0.01)
+            for l in range(len(w.ligands)):
+                idx, e = c.selected(l)
+                score = c.scores(l)
+                exact = c.exact(l)
+                k = int(np.ceil(0.05 * len(score)))
+                thr = np.sort(score)[k - 1]
+                # the selection is the lowest-k of the surrogate's scores (ties kept)
+                assert set(idx.tolist()) == set(np.nonzero(score <= thr)[0].tolist()), (p, l)
+                assert np.all(np.diff(idx.astype(np.int64)) > 0)
+                # rescored energies are the batched kernel's exact energies, bit for bit
+                assert np.array_equal(e.view(np.uint32), exact[idx].view(np.uint32)), (p, l)
+                assert rec[l] == pytest.approx(S.recall(idx, exact, 0.01), abs=1e-6), (p, l)
+            if p == 0:
+                # and those equal the plain kernel's for one pair
+                want = plain_energies(protein, w.ligands[0], w.forcefield, w.poses[0], gpu_device)
+                assert np.array_equal(c.exact(0).view(np.uint32), want.view(np.uint32))
+
+
+def test_fallback_pairs_are_scored_exhaustively(gpu_device):
+    w = small_workload(nposes=4096, nligands=3, nproteins=1)
+    cfg = C.default_config("field", max_val_rmse=0.0, keep_fraction=0.02)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        r = c.screen(w.proteins[0])
+        assert r.fallback_pairs == 3
+        c.exhaustive()
+        assert np.all(c.recall(0.01) == 1.0)
+        for l in range(3):
+            assert c.pair(l).fallback == 1
+            exact = c.exact(l)
+            assert np.array_equal(c.scores(l).view(np.uint32), exact.view(np.uint32))
+            idx, e = c.selected(l)
+            assert np.array_equal(e, exact[idx])
+
+
+def test_field_screen_recall(gpu_device):
+    # moderately sized pairs: the field keeps the exact top 1 % in its lowest 2 %
+    w = C.combined_workload(nproteins=1, nligands=8, nposes=65536, seed=12)
+    cfg = C.default_config("field")
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        r = c.screen(w.proteins[0])
+        c.exhaustive()
+        rec = c.recall(0.01)
+    print(f"field recall@1% (2% kept): mean {rec.mean():.4f} min {rec.min():.4f}; {r.as_dict()}", flush=True)
+    assert r.fallback_pairs == 0
+    assert rec.min() >= 0.97 and rec.mean() >= 0.99
+
+
+def test_mlp_combined_learns(gpu_device):
+    # the per-protein MLP against the same screen untrained (one step at a
+    # vanishing learning rate): training has to buy recall
+    w = C.combined_workload(nproteins=1, nligands=4, nposes=32768, seed=13, natpro_range=(1500, 2500))
+    out = {}
+    for name, kw in (("trained", dict(epochs=12, lr=4e-3)), ("untrained", dict(epochs=1, lr=1e-30))):
+        cfg = C.default_config("mlp", labels_per_pair=8192, batch=1024, max_val_rmse=1e9, **kw)
+        with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+            c.screen(w.proteins[0])
+            c.exhaustive()
+            out[name] = (c.recall(0.01), np.array([c.pair(l).val_rmse_calibrated for l in range(4)]))
+        print(f"mlp {name}: recall {out[name][0]}, calibrated val_rmse {out[name][1]}", flush=True)
+    assert np.all(out["trained"][1] < out["untrained"][1])
+    assert out["trained"][0].mean() > max(0.15, 3 * out["untrained"][0].mean())
