@@ -74,6 +74,7 @@ SCREEN_KEEP = (0.02, 0.03, 0.04, 0.05, 0.06, 0.08)
 # every pair; the full size is the workload's own
 COMBINED_DEFAULT = dict(nproteins=16, nligands=64, nposes=1 << 18)
 COMBINED_FULL = dict(nproteins=16, nligands=512, nposes=1 << 20)
+COMBINED_FIELD_KEEPS = (0.01, 0.0125, 0.015)
 TRAFFIC_SUMMARY = ROOT / "profiles" / "fasten_plain_c1_traffic.json"
 CPU_SAMPLE_POSES = 65536
 CPU_ITERS = 2
@@ -219,6 +220,24 @@ def screen() -> dict:
             log(f"screen: {points[-1]}")
             if keep != SCREEN_KEEP[0] and rec >= 0.99:
                 break
+    # the same workload through the combined screen's field surrogate (one pair)
+    from minibude_amd import combined as C
+    with C.Combined(w.forcefield, [w.ligand], [w.screen_poses], C.default_config("field"), 0) as c:
+        runs = [c.screen(w.protein) for _ in range(3)]
+        r = min(runs, key=lambda x: x.total_ms)
+        idx, _ = c.selected(0)
+    field = {
+        "keep_fraction": 0.02,
+        "poses_per_s": round(n / (r.total_ms * 1e-3), 1),
+        "screen_ms": round(r.total_ms, 3),
+        "speedup_vs_exhaustive": round(ex_ms / r.total_ms, 2),
+        "recall_top1pct": round(S.recall(idx, exact, 0.01), 4),
+        "recall_top0.1pct": round(S.recall(idx, exact, 0.001), 4),
+        "fallback": bool(r.fallback_pairs),
+        "phases_ms": {k: round(getattr(r, k), 3) for k in ("label_ms", "train_ms", "score_ms", "select_ms",
+                                                           "rescore_ms")},
+    }
+    log(f"screen (field surrogate): {field}")
     out = {
         "workload": (f"c3: 4,096-atom protein, 26-atom ligand; surrogate trained on {w.train_poses.shape[1]} plain-"
                      f"kernel energies, {n} fresh poses screened, lowest keep_fraction rescored exactly"),
@@ -227,6 +246,7 @@ def screen() -> dict:
         "val_rmse": round(st.val_rmse, 4),
         "train_s": round((st.label_ms + st.train_ms) * 1e-3, 2),
         "points": points,
+        "field_surrogate": field,
     }
     return out
 
@@ -261,7 +281,7 @@ def traffic() -> float | None:
     return None
 
 
-def combined(full: bool) -> dict:
+def combined(full: bool, kinds=("field", "mlp")) -> dict:
     """Both surrogates of the combined screen on the same workload: every
     protein screened (timed), exact energies of every pair for recall (all
     proteins at the default size, the first one at full size, where the
@@ -280,10 +300,12 @@ def combined(full: bool) -> dict:
         "pair_poses": w.total_poses,
         "interactions": w.interactions,
     }
-    for kind in ("field", "mlp"):
+    for kind in kinds:
         cfg = C.default_config(kind)
         screen_ms, ex_ms, ex_inter, rec, fb, labels, sel = 0.0, 0.0, 0, [], 0, 0, 0
         parts = {"label_ms": 0.0, "train_ms": 0.0, "score_ms": 0.0, "select_ms": 0.0, "rescore_ms": 0.0}
+        sweep = {k: [0.0, []] for k in (COMBINED_FIELD_KEEPS if kind == "field" else ())}
+        keep0 = float(cfg.keep_fraction)
         with C.Combined(w.forcefield, w.ligands, w.poses, cfg, 0) as c:
             for p, protein in enumerate(w.proteins):
                 r = c.screen(protein)
@@ -297,6 +319,13 @@ def combined(full: bool) -> dict:
                     ex_ms += c.exhaustive()
                     ex_inter += per_protein[p]
                     rec.extend(c.recall(0.01).tolist())
+                    # smaller rescored fractions on the same protein
+                    for keep, acc in sweep.items():
+                        c.set_keep(keep)
+                        acc[0] += c.screen(protein).total_ms
+                        acc[1].extend(c.recall(0.01).tolist())
+                    if sweep:
+                        c.set_keep(keep0)
         rec = np.array(rec)
         ex_total_ms = ex_ms * w.interactions / ex_inter
         res = {
@@ -314,8 +343,16 @@ def combined(full: bool) -> dict:
             "labels": labels,
             "rescored": sel,
             "phases_ms": {k: round(v, 2) for k, v in parts.items()},
-            "config": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in cfg.as_dict().items()},
+            "config": {**{k: (round(v, 6) if isinstance(v, float) else v) for k, v in cfg.as_dict().items()},
+                       "keep_fraction": keep0},
         }
+        if sweep:
+            ex_meas = ex_ms  # the sweep ran on the proteins whose exhaustive time was measured
+            res["keep_sweep"] = [{"keep_fraction": k, "speedup_vs_exhaustive": round(ex_meas / v[0], 2),
+                                  "recall_top1pct_mean": round(float(np.mean(v[1])), 4),
+                                  "recall_top1pct_min": round(float(np.min(v[1])), 4),
+                                  "pairs_recall_ge_0.99": round(float((np.array(v[1]) >= 0.99).mean()), 4)}
+                                 for k, v in sweep.items()]
         if full:
             res["exhaustive_note"] = "measured on the first protein's pairs, extrapolated by interactions"
         log(f"combined {kind}: {res}")
@@ -336,6 +373,7 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--no-combined", action="store_true")
     ap.add_argument("--combined-full", action="store_true",
                     help="combined screen at 16 proteins x 512 ligands x 2^20 poses (recall on the first protein)")
+    ap.add_argument("--combined-surrogates", default="field,mlp", help="comma list of field, mlp")
     ap.add_argument("--copy-calibration", type=int, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
     a = ap.parse_args(argv)
@@ -417,7 +455,8 @@ def main(argv: list[str] | None = None) -> int:
         "parity": parity,
         "batched": None if a.no_batched else batched(),
         "screen": None if a.no_screen else screen(),
-        "combined": None if a.no_combined else combined(a.combined_full),
+        "combined": None if a.no_combined else combined(a.combined_full,
+                                                         tuple(a.combined_surrogates.split(","))),
         "device": device_name(0),
     }
     print(json.dumps(out), flush=True)

@@ -266,6 +266,8 @@ int bude_mi355x_combined_create(int device, const BudeFFParams *forcefield, size
                                 const size_t *nposes, const BudeCombinedConfig *cfg, bude_combined **out);
 /* Screens every ligand's poses against this protein. */
 int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *protein, size_t natpro, BudeCombinedResult *res);
+/* Changes the per-pair fraction later screens rescore exactly. */
+int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fraction);
 int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePairResult *r);
 /* The last screen's selection of one pair: pose indices (ascending, within
  * the ligand's pose set) and their exact energies. */
@@ -275,7 +277,8 @@ int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, uint32_t *idx
  * pair): standardised units for the MLP, energy units for the field. */
 int bude_mi355x_combined_scores(bude_combined *c, size_t ligand, float *out);
 /* Exact energies of every pose of every pair against the last protein (one
- * batched launch, for measuring recall); then _exact / _recall read them. */
+ * batched launch, for measuring recall); _exact / _recall read them while
+ * the protein screened last is the same one (e.g. after set_keep). */
 int bude_mi355x_combined_exhaustive(bude_combined *c, double *device_ms);
 int bude_mi355x_combined_exact(bude_combined *c, size_t ligand, float *out);
 /* recall[l] = fraction of pair l's exact lowest round(top_fraction * n)

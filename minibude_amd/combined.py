@@ -63,11 +63,17 @@ def default_config(surrogate: str = "field", **kw) -> CombinedConfig:
     return CombinedConfig(**c)
 
 
+def _copy_config(cfg: CombinedConfig) -> CombinedConfig:
+    """A copy (the object keeps its own configuration)."""
+    return CombinedConfig(**cfg.as_dict())
+
+
 _P, _S = _c.c_void_p, _c.c_size_t
 _SIG = {
     "bude_mi355x_combined_create": [_c.c_int, _P, _S, _S, _c.POINTER(_P), _c.POINTER(_S), _c.POINTER(_P),
                                     _c.POINTER(_S), _P, _c.POINTER(_P)],
     "bude_mi355x_combined_screen": [_P, _P, _S, _P],
+    "bude_mi355x_combined_set_keep": [_P, _c.c_float],
     "bude_mi355x_combined_pair": [_P, _S, _P],
     "bude_mi355x_combined_selected": [_P, _S, _P, _P, _S, _c.POINTER(_S)],
     "bude_mi355x_combined_scores": [_P, _S, _P],
@@ -95,7 +101,7 @@ class Combined:
     """A ligand set with its poses resident in HBM, screened per protein."""
 
     def __init__(self, forcefield, ligands, poses, cfg: CombinedConfig | None = None, device: int = 0):
-        self.cfg = cfg or default_config()
+        self.cfg = _copy_config(cfg or default_config())
         self._h = _c.c_void_p()
         n = len(ligands)
         self._ff = np.ascontiguousarray(forcefield)
@@ -116,6 +122,10 @@ class Combined:
         r = CombinedResult()
         capi.check(_lib().bude_mi355x_combined_screen(self._h, capi.ptr(pro), len(pro), _c.byref(r)))
         return r
+
+    def set_keep(self, keep_fraction: float) -> None:
+        capi.check(_lib().bude_mi355x_combined_set_keep(self._h, keep_fraction))
+        self.cfg.keep_fraction = keep_fraction
 
     def pair(self, ligand: int) -> PairResult:
         r = PairResult()

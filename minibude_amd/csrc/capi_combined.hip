@@ -152,7 +152,7 @@ struct bude_combined {
   // per protein
   bool screened = false;
   size_t ngroups = 0, proteins_done = 0;
-  DeviceBuffer protein, group_begin, coef, slot_coef, maps, counter;
+  DeviceBuffer protein, group_begin, coef, slot_coef, maps, cells, counter;
   std::vector<int32_t> coef_off;
   // per-call work tables (each its own buffer; uploaded only while the stream is idle)
   DeviceBuffer meta_label, tiles_label, meta_fb, tiles_fb, meta_rs, tiles_rs, meta_all, tiles_all;
@@ -165,14 +165,19 @@ struct bude_combined {
   std::vector<int64_t> gofs;
   std::vector<int32_t> fallback;
   std::vector<float> val_rmse, val_rmse_cal;
+  // exact energies of every pair (exhaustive) and the protein they belong to
+  uint64_t protein_sum = 0, exact_sum = 0;
   bool have_exact = false;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};
+  // stream: exact passes, selection, MLP; stream2: maps and field scoring,
+  // overlapping the exact scoring of the sampled poses
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipEvent_t ev[12] = {};
 
   ~bude_combined() {
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
   }
 };
 
@@ -238,7 +243,7 @@ int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
 
 // Field scores of spans (as exact_ragged) from the current maps.
 int field_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride, float *out,
-                 DeviceBuffer &meta, DeviceBuffer &tile_begin) {
+                 DeviceBuffer &meta, DeviceBuffer &tile_begin, hipStream_t stream) {
   std::vector<bude::LigandMeta> m(c->nlig);
   std::vector<int64_t> tb(c->nlig + 1, 0);
   for (size_t l = 0; l < c->nlig; l++) {
@@ -262,7 +267,7 @@ int field_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
                                "hipMemcpy tile_begin"))
     return st;
   bude::field::ScoreArgs a{};
-  a.maps = static_cast<const float *>(c->maps.ptr);
+  a.cells = static_cast<const uint4 *>(c->cells.ptr);
   a.grid = c->grid;
   a.ligands = static_cast<const float4 *>(c->ligands.ptr);
   a.meta = static_cast<const bude::LigandMeta *>(meta.ptr);
@@ -271,12 +276,14 @@ int field_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
   a.poses = poses;
   a.stride = stride;
   a.out = out;
-  return bude::hip_check(bude::field::score(a, tb[c->nlig], c->stream), "field score");
+  return bude::hip_check(bude::field::score(a, tb[c->nlig], stream), "field score");
 }
 
 int sync(bude_combined *c) { return bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
 
-int record(bude_combined *c, int i) { return bude::hip_check(hipEventRecord(c->ev[i], c->stream), "hipEventRecord"); }
+int record(bude_combined *c, int i, hipStream_t s = nullptr) {
+  return bude::hip_check(hipEventRecord(c->ev[i], s ? s : c->stream), "hipEventRecord");
+}
 
 float elapsed(bude_combined *c, int a, int b) {
   float ms = 0.f;
@@ -426,6 +433,7 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
   if (!st) st = c->sel_n.alloc(nligands * sizeof(uint32_t));
   if (!st) st = c->counter.alloc(256);
   if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking), "hipStreamCreate");
   for (auto &e : c->ev)
     if (!st) st = bude::hip_check(hipEventCreate(&e), "hipEventCreate");
   if (!st) st = bude::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -443,7 +451,12 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   *res = BudeCombinedResult{};
   c->screened = false;
-  c->have_exact = false;
+  {  // FNV-1a of the protein: exact energies stay valid while it is unchanged
+    uint64_t hsum = 1469598103934665603ull ^ uint64_t(natpro);
+    const auto *bytes = reinterpret_cast<const unsigned char *>(protein);
+    for (size_t i = 0; i < natpro * sizeof(BudeAtom); i++) hsum = (hsum ^ bytes[i]) * 1099511628211ull;
+    c->protein_sum = hsum;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   const size_t L = c->nlig;
   const BudeCombinedConfig &cfg = c->cfg;
@@ -483,25 +496,20 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     if (m == n) {
       pick.resize(n);
       std::iota(pick.begin(), pick.end(), 0u);
-      std::shuffle(pick.begin(), pick.end(), rng);
-    } else {  // Floyd's sampling without replacement, then shuffled
-      std::vector<uint32_t> chosen;
+    } else {  // Floyd's sampling without replacement
       std::vector<bool> seen;
-      if (n <= (size_t(1) << 26)) seen.assign(n, false);
       std::vector<uint32_t> sorted;
+      if (n <= (size_t(1) << 26)) seen.assign(n, false);
       for (size_t j = n - m; j < n; j++) {
         const uint32_t t = uint32_t(std::uniform_int_distribution<size_t>(0, j)(rng));
-        bool dup;
-        if (!seen.empty()) dup = seen[t];
-        else dup = std::binary_search(sorted.begin(), sorted.end(), t);
+        const bool dup = seen.empty() ? std::binary_search(sorted.begin(), sorted.end(), t) : bool(seen[t]);
         const uint32_t v = dup ? uint32_t(j) : t;
-        chosen.push_back(v);
+        pick.push_back(v);
         if (!seen.empty()) seen[v] = true;
         else sorted.insert(std::upper_bound(sorted.begin(), sorted.end(), v), v);
       }
-      pick = std::move(chosen);
-      std::shuffle(pick.begin(), pick.end(), rng);
     }
+    std::shuffle(pick.begin(), pick.end(), rng);
     for (uint32_t j : pick) {
       src.push_back(uint32_t(c->pose_off[l] + j));
       lab_lig.push_back(uint32_t(l));
@@ -515,21 +523,58 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   if (int st = c->lab_poses.reserve(6 * nlab * 4)) return st;
   if (int st = c->lab_e.reserve(nlab * 4)) return st;
   if (int st = c->lab_pred.reserve(nlab * 4)) return st;
+  std::vector<Span> lab_spans(L), all(L);
+  for (size_t l = 0; l < L; l++) {
+    lab_spans[l] = Span{l, lab_off[l], lab_off[l + 1] - lab_off[l]};
+    all[l] = Span{l, c->pose_off[l], c->nposes[l]};
+  }
+  float *pred = static_cast<float *>(c->pred.ptr);
+  const float *poses = static_cast<const float *>(c->poses.ptr);
+  float *lp = static_cast<float *>(c->lab_pred.ptr);
+  const float *lpos = static_cast<const float *>(c->lab_poses.ptr);
+
+  // 3. field (second stream, overlapping step 4): maps, cells, then the
+  //    interpolated score of every pose of every pair
+  if (!mlp) {
+    const size_t np = c->grid.points(), nslots = c->slot_type.size();
+    if (int st = c->maps.reserve(np * nslots * 4)) return st;
+    if (int st = c->cells.reserve(c->grid.cells() * nslots * 16)) return st;
+    if (int st = record(c, 2, c->stream2)) return st;
+    if (int st = bude::hip_check(
+            bude::field::build_maps(static_cast<const float4 *>(c->protein.ptr),
+                                    static_cast<const int32_t *>(c->group_begin.ptr),
+                                    static_cast<const bude::PairCoef *>(c->slot_coef.ptr), int(c->ngroups),
+                                    int(nslots), c->grid, bude::plain_form(), static_cast<float *>(c->maps.ptr),
+                                    c->stream2),
+            "build maps"))
+      return st;
+    if (int st = bude::hip_check(bude::field::pack_cells(static_cast<const float *>(c->maps.ptr), int(nslots),
+                                                         c->grid, static_cast<uint4 *>(c->cells.ptr), c->stream2),
+                                 "pack cells"))
+      return st;
+    if (int st = record(c, 3, c->stream2)) return st;
+    if (int st = field_ragged(c, all, poses, c->total, pred, c->meta_all, c->tb_all, c->stream2)) return st;
+    if (int st = record(c, 4, c->stream2)) return st;
+  }
+
+  // 4. exact energies of the sampled poses (one ragged batched launch)
   if (int st = record(c, 0)) return st;
-  hipLaunchKernelGGL(gather_index_kernel, dim3(blocks_for(nlab, 256)), dim3(256), 0, c->stream,
-                     static_cast<const float *>(c->poses.ptr), c->total, static_cast<const uint32_t *>(c->lab_src.ptr),
-                     nlab, static_cast<float *>(c->lab_poses.ptr), nlab);
+  hipLaunchKernelGGL(gather_index_kernel, dim3(blocks_for(nlab, 256)), dim3(256), 0, c->stream, poses, c->total,
+                     static_cast<const uint32_t *>(c->lab_src.ptr), nlab, static_cast<float *>(c->lab_poses.ptr),
+                     nlab);
   if (int st = bude::hip_check(hipGetLastError(), "gather labels")) return st;
-  std::vector<Span> lab_spans(L);
-  for (size_t l = 0; l < L; l++) lab_spans[l] = Span{l, lab_off[l], lab_off[l + 1] - lab_off[l]};
-  if (int st = exact_ragged(c, lab_spans, static_cast<const float *>(c->lab_poses.ptr), nlab,
-                            static_cast<float *>(c->lab_e.ptr), c->meta_label, c->tiles_label))
+  if (int st = exact_ragged(c, lab_spans, lpos, nlab, static_cast<float *>(c->lab_e.ptr), c->meta_label,
+                            c->tiles_label))
     return st;
   if (int st = record(c, 1)) return st;
   std::vector<float> le(nlab);
   if (int st = bude::hip_check(hipMemcpyAsync(le.data(), c->lab_e.ptr, nlab * 4, hipMemcpyDeviceToHost, c->stream),
                                "hipMemcpyAsync"))
     return st;
+  if (!mlp) {  // field scores of the sampled poses, once the cells exist
+    if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->ev[3], 0), "hipStreamWaitEvent")) return st;
+    if (int st = field_ragged(c, lab_spans, lpos, nlab, lp, c->meta_fb, c->tb_label, c->stream)) return st;
+  }
   if (int st = sync(c)) return st;
   res->label_ms = elapsed(c, 0, 1);
 
@@ -543,9 +588,9 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     tn[l] = bude::mlp::TargetNorm{norms[l].clip, norms[l].mu, 1.0f / norms[l].sigma, norms[l].tail};
   }
 
-  // 3. the surrogate: train the MLP, or build the maps
-  if (int st = record(c, 2)) return st;
+  // 5. MLP: train on the training part, predict the validation part
   if (mlp) {
+    if (int st = record(c, 2)) return st;
     if (int st = c->norms.upload(tn.data(), tn.size() * sizeof(tn[0]))) return st;
     std::vector<std::vector<float>> w, b;
     bude::surrogate::init_weights(cfg.seed + c->proteins_done, bude::mlp::kIn, w, b);
@@ -556,7 +601,7 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     if (train.size() < cfg.batch)
       return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: fewer training labels than one MLP batch");
     bude::surrogate::TrainData td;
-    td.poses = static_cast<const float *>(c->lab_poses.ptr);
+    td.poses = lpos;
     td.stride = nlab;
     td.energy = static_cast<const float *>(c->lab_e.ptr);
     td.table = static_cast<const bude::mlp::TargetNorm *>(c->norms.ptr);
@@ -576,34 +621,14 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
                                                       static_cast<float *>(c->bias0.ptr), c->stream),
                                  "fold_bias"))
       return st;
-  } else {
-    const size_t np = c->grid.points();
-    if (int st = c->maps.reserve(np * c->slot_type.size() * 4)) return st;
-    if (int st = bude::hip_check(
-            bude::field::build_maps(static_cast<const float4 *>(c->protein.ptr),
-                                    static_cast<const int32_t *>(c->group_begin.ptr),
-                                    static_cast<const bude::PairCoef *>(c->slot_coef.ptr), int(c->ngroups),
-                                    int(c->slot_type.size()), c->grid, bude::plain_form(),
-                                    static_cast<float *>(c->maps.ptr), c->stream),
-            "build maps"))
-      return st;
-  }
-  if (int st = record(c, 3)) return st;
-
-  // 4. validation per pair: RMSE of the surrogate on the clipped,
-  //    standardised scale of that pair
-  float *lp = static_cast<float *>(c->lab_pred.ptr);
-  const float *lpos = static_cast<const float *>(c->lab_poses.ptr);
-  if (mlp) {
+    if (int st = record(c, 3)) return st;
     for (size_t l = 0; l < L; l++) {
-      const size_t b = lab_off[l] + lab_train[l], n = lab_off[l + 1] - b;
+      const size_t b0 = lab_off[l] + lab_train[l], n = lab_off[l + 1] - b0;
       if (n == 0) continue;
-      if (int st = bude::surrogate::predict(c->net, lpos + b, nlab, n, lp + b, 2 * c->cus, c->stream,
+      if (int st = bude::surrogate::predict(c->net, lpos + b0, nlab, n, lp + b0, 2 * c->cus, c->stream,
                                             static_cast<const float *>(c->bias0.ptr) + l * bude::mlp::kHidden))
         return st;
     }
-  } else {
-    if (int st = field_ragged(c, lab_spans, lpos, nlab, lp, c->meta_fb, c->tb_label)) return st;
   }
   std::vector<float> lpred(nlab);
   if (int st = bude::hip_check(hipMemcpyAsync(lpred.data(), lp, nlab * 4, hipMemcpyDeviceToHost, c->stream),
@@ -611,6 +636,8 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     return st;
   if (int st = sync(c)) return st;
   res->train_ms = elapsed(c, 2, 3);
+
+  // 6. validation per pair on the clipped, standardised scale of that pair
   c->fallback.assign(L, 0);
   c->val_rmse.assign(L, 0.f);
   c->val_rmse_cal.assign(L, 0.f);
@@ -641,26 +668,28 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     res->fallback_pairs += uint32_t(c->fallback[l]);
   }
 
-  // 5. the surrogate over every pose of every pair; exhaustive exact scoring
-  //    of the fallback pairs (their "scores" are exact energies)
-  float *pred = static_cast<float *>(c->pred.ptr);
-  const float *poses = static_cast<const float *>(c->poses.ptr);
-  std::vector<Span> sur, fb;
+  // 7. MLP scores of every pose of every non-fallback pair (the field's are
+  //    on stream2 already); exhaustive exact scores of the fallback pairs
+  //    overwrite theirs
+  std::vector<Span> fb;
   for (size_t l = 0; l < L; l++)
-    (c->fallback[l] ? fb : sur).push_back(Span{l, c->pose_off[l], c->nposes[l]});
-  if (int st = record(c, 4)) return st;
+    if (c->fallback[l]) fb.push_back(all[l]);
   if (mlp) {
-    for (const Span &s : sur)
-      if (int st = bude::surrogate::predict(c->net, poses + s.off, c->total, s.n, pred + s.off, 2 * c->cus, c->stream,
-                                            static_cast<const float *>(c->bias0.ptr) + s.lig * bude::mlp::kHidden))
-        return st;
-  } else {
-    if (int st = field_ragged(c, sur, poses, c->total, pred, c->meta_all, c->tb_all)) return st;
+    if (int st = record(c, 4)) return st;
+    for (size_t l = 0; l < L; l++)
+      if (!c->fallback[l])
+        if (int st = bude::surrogate::predict(c->net, poses + c->pose_off[l], c->total, c->nposes[l],
+                                              pred + c->pose_off[l], 2 * c->cus, c->stream,
+                                              static_cast<const float *>(c->bias0.ptr) + l * bude::mlp::kHidden))
+          return st;
+  } else if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->ev[4], 0), "hipStreamWaitEvent")) {
+    return st;
   }
-  if (int st = exact_ragged(c, fb, poses, c->total, pred, c->meta_fb, c->tiles_fb)) return st;
   if (int st = record(c, 5)) return st;
+  if (int st = exact_ragged(c, fb, poses, c->total, pred, c->meta_fb, c->tiles_fb)) return st;
+  if (int st = record(c, 6)) return st;
 
-  // 6. per-pair top-k of the scores
+  // 8. per-pair top-k of the scores
   if (int st = bude::hip_check(bude::sel::select_lowest_segments(c->plan, pred, static_cast<uint32_t *>(c->sel_idx.ptr),
                                                                  static_cast<uint32_t *>(c->sel_n.ptr), c->stream),
                                "select"))
@@ -669,10 +698,10 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   if (int st = bude::hip_check(hipMemcpyAsync(c->nsel.data(), c->sel_n.ptr, L * 4, hipMemcpyDeviceToHost, c->stream),
                                "hipMemcpyAsync"))
     return st;
-  if (int st = record(c, 6)) return st;
+  if (int st = record(c, 7)) return st;
   if (int st = sync(c)) return st;
 
-  // 7. exact rescoring of every non-fallback pair's selection in one launch
+  // 9. exact rescoring of every non-fallback pair's selection in one launch
   c->gofs.assign(L + 1, 0);
   for (size_t l = 0; l < L; l++) c->gofs[l + 1] = c->gofs[l] + (c->fallback[l] ? 0 : int64_t(c->nsel[l]));
   const size_t G = size_t(c->gofs[L]);
@@ -697,17 +726,33 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
                               static_cast<float *>(c->exact_sel.ptr), c->meta_rs, c->tiles_rs))
       return st;
   }
-  if (int st = record(c, 7)) return st;
+  if (int st = record(c, 8)) return st;
   if (int st = sync(c)) return st;
-  res->score_ms = elapsed(c, 4, 5);
-  res->select_ms = elapsed(c, 5, 6);
-  res->rescore_ms = elapsed(c, 6, 7);
+  res->score_ms = elapsed(c, mlp ? 4 : 3, mlp ? 5 : 4) + elapsed(c, 5, 6);
+  res->select_ms = elapsed(c, 6, 7);
+  res->rescore_ms = elapsed(c, 7, 8);
   res->total_ms = ms_since(t0);
   res->nposes = c->total;
   for (size_t l = 0; l < L; l++) res->nselected += c->nsel[l];
   res->labels = nlab;
   c->screened = true;
   c->proteins_done++;
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fraction) {
+  if (!c || !(keep_fraction > 0.f) || keep_fraction > 1.f)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_set_keep: keep_fraction must be in (0, 1]");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return st;
+  std::vector<bude::sel::Segment> segs(c->nlig);
+  for (size_t i = 0; i < c->nlig; i++) {
+    const double kf = std::ceil(double(keep_fraction) * double(c->nposes[i]));
+    segs[i] = bude::sel::Segment{int64_t(c->pose_off[i]), uint32_t(c->nposes[i]),
+                                 uint32_t(std::min<double>(double(c->nposes[i]), std::max(1.0, kf)))};
+  }
+  if (int st = c->plan.prepare(segs)) return st;
+  c->cfg.keep_fraction = keep_fraction;
   return bude::ok();
 }
 
@@ -789,14 +834,16 @@ extern "C" int bude_mi355x_combined_exhaustive(bude_combined *c, double *device_
   if (int st = sync(c)) return st;
   if (device_ms) *device_ms = elapsed(c, 0, 1);
   c->have_exact = true;
+  c->exact_sum = c->protein_sum;
   return bude::ok();
 }
 
 extern "C" int bude_mi355x_combined_exact(bude_combined *c, size_t ligand, float *out) {
   if (!c || !out || ligand >= c->nlig)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exact: bad arguments");
-  if (!c->have_exact)
-    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exact: run bude_mi355x_combined_exhaustive first");
+  if (!c->have_exact || c->exact_sum != c->protein_sum)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "bude_mi355x_combined_exact: run bude_mi355x_combined_exhaustive for this protein first");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   return bude::hip_check(hipMemcpy(out, static_cast<float *>(c->exact_all.ptr) + c->pose_off[ligand],
                                    c->nposes[ligand] * 4, hipMemcpyDeviceToHost),
@@ -806,8 +853,9 @@ extern "C" int bude_mi355x_combined_exact(bude_combined *c, size_t ligand, float
 extern "C" int bude_mi355x_combined_recall(bude_combined *c, float top_fraction, float *recall) {
   if (!c || !recall || !(top_fraction > 0.f) || top_fraction > 1.f)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_recall: bad arguments");
-  if (!c->have_exact)
-    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_recall: run bude_mi355x_combined_exhaustive first");
+  if (!c->have_exact || c->exact_sum != c->protein_sum)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "bude_mi355x_combined_recall: run bude_mi355x_combined_exhaustive for this protein first");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   // the exact top set of every pair by the same per-segment selection
   std::vector<bude::sel::Segment> segs(c->nlig);

@@ -26,6 +26,7 @@ struct Grid {
   float h;      // spacing
   int n[3];     // points per axis, each >= 2
   __host__ __device__ size_t points() const { return size_t(n[0]) * size_t(n[1]) * size_t(n[2]); }
+  __host__ __device__ size_t cells() const { return size_t(n[0] - 1) * size_t(n[1] - 1) * size_t(n[2] - 1); }
 };
 
 // maps[(s * n0 + ix) * n1 + iy) * n2 + iz] = energy of one ligand atom whose
@@ -35,8 +36,16 @@ struct Grid {
 hipError_t build_maps(const float4 *protein, const int32_t *group_begin, const PairCoef *coef, int ngroups,
                       int nslots, const Grid &g, int form, float *maps, hipStream_t s);
 
+// Cells of the maps: cell (ix, iy, iz) of slot s holds its 8 corner values
+// [v000 v001 v010 v011 v100 v101 v110 v111] (index bits x y z) as FP16
+// (round to nearest even, clamped to +-65504) in 16 contiguous bytes, so an
+// interpolation is one 16-byte read instead of 8 scattered ones and the
+// cells of every slot stay small enough for the device's last-level cache.
+// cells has nslots * g.cells() entries.
+hipError_t pack_cells(const float *maps, int nslots, const Grid &g, uint4 *cells, hipStream_t s);
+
 struct ScoreArgs {
-  const float *maps;          // nslots maps of g.points() floats
+  const uint4 *cells;         // nslots * grid.cells() packed cells (pack_cells)
   Grid grid;
   const float4 *ligands;      // x, y, z, map slot (int bits)
   const LigandMeta *meta;     // lig_off, natlig, nposes, pose_off of every ligand

@@ -37,7 +37,30 @@ __device__ __forceinline__ void cell(float w, float lo, float inv_h, int n, int 
   f = __builtin_amdgcn_fmed3f(u - float(i0), 0.0f, 1.0f);
 }
 
-__global__ __launch_bounds__(256) void score_kernel(const float *__restrict__ maps, const Grid g,
+union Cell {
+  uint4 u;
+  _Float16 h[8];
+};
+
+__device__ __forceinline__ _Float16 to_half(float v) { return _Float16(fminf(fmaxf(v, -65504.0f), 65504.0f)); }
+
+__global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ maps, const Grid g,
+                                                   uint4 *__restrict__ cells) {
+  const size_t nc = g.cells(), np = g.points();
+  const size_t c = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (c >= nc) return;
+  const size_t c2 = size_t(g.n[2] - 1), c12 = size_t(g.n[1] - 1) * c2;
+  const size_t ix = c / c12, iy = (c / c2) % size_t(g.n[1] - 1), iz = c % c2;
+  const size_t s1 = size_t(g.n[2]), s0 = size_t(g.n[1]) * s1;
+  const float *m = maps + size_t(blockIdx.y) * np + ix * s0 + iy * s1 + iz;
+  const size_t off[8] = {0, 1, s1, s1 + 1, s0, s0 + 1, s0 + s1, s0 + s1 + 1};
+  Cell v;
+#pragma unroll
+  for (int k = 0; k < 8; k++) v.h[k] = to_half(m[off[k]]);
+  cells[size_t(blockIdx.y) * nc + c] = v.u;
+}
+
+__global__ __launch_bounds__(256) void score_kernel(const uint4 *__restrict__ cells, const Grid g,
                                                     const float4 *__restrict__ ligands,
                                                     const LigandMeta *__restrict__ meta,
                                                     const int64_t *__restrict__ tile_begin, int nligands,
@@ -57,8 +80,8 @@ __global__ __launch_bounds__(256) void score_kernel(const float *__restrict__ ma
   const int j = int(t - tile_begin[lo]) * 64 + lane;
   const Xform x = make_xform(poses, stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
   const float inv_h = 1.0f / g.h;
-  const size_t np = g.points();
-  const int s1 = g.n[2], s0 = g.n[1] * g.n[2];
+  const size_t nc = g.cells();
+  const int c2 = g.n[2] - 1, c12 = (g.n[1] - 1) * c2;
   float e = 0.f;
   for (int il = 0; il < m.natlig; ++il) {
     const float4 la = ligands[m.lig_off + il];
@@ -70,11 +93,13 @@ __global__ __launch_bounds__(256) void score_kernel(const float *__restrict__ ma
     cell(wx, g.lo[0], inv_h, g.n[0], ix, fx);
     cell(wy, g.lo[1], inv_h, g.n[1], iy, fy);
     cell(wz, g.lo[2], inv_h, g.n[2], iz, fz);
-    const float *mp = maps + size_t(__float_as_int(la.w)) * np + size_t(ix * s0 + iy * s1 + iz);
-    const float v000 = mp[0], v001 = mp[1], v010 = mp[s1], v011 = mp[s1 + 1];
-    const float v100 = mp[s0], v101 = mp[s0 + 1], v110 = mp[s0 + s1], v111 = mp[s0 + s1 + 1];
-    const float c00 = __builtin_fmaf(fz, v001 - v000, v000), c01 = __builtin_fmaf(fz, v011 - v010, v010);
-    const float c10 = __builtin_fmaf(fz, v101 - v100, v100), c11 = __builtin_fmaf(fz, v111 - v110, v110);
+    Cell v;
+    v.u = cells[size_t(__float_as_int(la.w)) * nc + size_t(ix * c12 + iy * c2 + iz)];
+    float q[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) q[k] = float(v.h[k]);  // v000 v001 v010 v011 v100 v101 v110 v111
+    const float c00 = __builtin_fmaf(fz, q[1] - q[0], q[0]), c01 = __builtin_fmaf(fz, q[3] - q[2], q[2]);
+    const float c10 = __builtin_fmaf(fz, q[5] - q[4], q[4]), c11 = __builtin_fmaf(fz, q[7] - q[6], q[6]);
     const float c0 = __builtin_fmaf(fy, c01 - c00, c00), c1 = __builtin_fmaf(fy, c11 - c10, c10);
     e += __builtin_fmaf(fx, c1 - c0, c0);
   }
@@ -97,11 +122,19 @@ hipError_t build_maps(const float4 *protein, const int32_t *group_begin, const P
   return hipGetLastError();
 }
 
+hipError_t pack_cells(const float *maps, int nslots, const Grid &g, uint4 *cells, hipStream_t s) {
+  const size_t nc = g.cells();
+  if (nc == 0 || nslots <= 0 || nslots > 65535 || (nc + 255) / 256 > 0x7fffffffu || nc > 0x7fffffffu)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pack_kernel, dim3(unsigned((nc + 255) / 256), unsigned(nslots)), dim3(256), 0, s, maps, g, cells);
+  return hipGetLastError();
+}
+
 hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (a.nligands <= 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 || (ntiles + 3) / 4 > 0x7fffffff)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a.maps, a.grid, a.ligands,
+  hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a.cells, a.grid, a.ligands,
                      a.meta, a.tile_begin, a.nligands, a.poses, a.stride, a.out, ntiles);
   return hipGetLastError();
 }

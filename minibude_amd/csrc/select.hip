@@ -91,27 +91,48 @@ __global__ __launch_bounds__(64) void seg_pick_kernel(uint32_t *__restrict__ his
   for (int i = lane; i < kBins; i += 64) h[i] = 0;
 }
 
+// Two passes over the chunk: count (per wave, over its quarter of the
+// chunk), one atomic per chunk for the chunk's output base, then write with
+// per-wave running offsets.  (An atomic per wave and step contends on the
+// few per-segment counters.)
 __global__ __launch_bounds__(256) void seg_compact_kernel(const Segment *__restrict__ segs,
                                                           const Chunk *__restrict__ chunks,
                                                           const float *__restrict__ x, const uint32_t *__restrict__ hist,
                                                           uint32_t *__restrict__ out_idx, uint32_t *__restrict__ out_n) {
+  __shared__ uint32_t wave_base[4];
   const Chunk c = chunks[blockIdx.x];
   const Segment sg = segs[c.seg];
   const uint32_t thr = hist[size_t(c.seg) * kHistStride + kBins];
   const uint32_t end = min(c.begin + kChunk, sg.n);
   const float *xs = x + sg.off;
-  const int lane = threadIdx.x & 63;
-  // wave-uniform trip count; one counter atomic per wave and step
-  for (uint32_t j0 = c.begin + (threadIdx.x & ~63u); j0 < end; j0 += 256) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr uint32_t kQuarter = kChunk / 4;
+  const uint32_t wb = c.begin + uint32_t(wave) * kQuarter, we = min(wb + kQuarter, end);
+  uint32_t count = 0;
+  for (uint32_t j0 = wb; j0 < we; j0 += 64) {
     const uint32_t j = j0 + lane;
-    const bool take = j < end && order_key(xs[j]) <= thr;
+    count += uint32_t(__popcll(__ballot(j < we && order_key(xs[j]) <= thr)));
+  }
+  if (lane == 0) wave_base[wave] = count;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t c0 = wave_base[0], c1 = wave_base[1], c2 = wave_base[2], c3 = wave_base[3];
+    const uint32_t base = (c0 + c1 + c2 + c3) ? atomicAdd(&out_n[c.seg], c0 + c1 + c2 + c3) : 0u;
+    wave_base[0] = base;
+    wave_base[1] = base + c0;
+    wave_base[2] = base + c0 + c1;
+    wave_base[3] = base + c0 + c1 + c2;
+  }
+  __syncthreads();
+  if (count == 0) return;
+  uint32_t pos = wave_base[wave];
+  for (uint32_t j0 = wb; j0 < we; j0 += 64) {
+    const uint32_t j = j0 + lane;
+    const bool take = j < we && order_key(xs[j]) <= thr;
     const unsigned long long m = __ballot(take);
-    if (m == 0) continue;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&out_n[c.seg], uint32_t(__popcll(m)));
-    base = __shfl(base, 0);
     const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-    if (take) out_idx[sg.off + base + below] = j;
+    if (take) out_idx[sg.off + pos + below] = j;
+    pos += uint32_t(__popcll(m));
   }
 }
 

@@ -83,8 +83,8 @@ def test_field_score_is_trilinear_sum_over_atoms(gpu_device):
             maps = {}
             for a in range(len(lig)):
                 ty = int(lig["type"][a])
-                if ty not in maps:
-                    maps[ty] = c.map(ty).astype(np.float64)
+                if ty not in maps:  # the scorer reads the corner values as FP16
+                    maps[ty] = np.clip(c.map(ty), -65504, 65504).astype(np.float16).astype(np.float64)
                 x = np.array([lig["x"][a], lig["y"][a], lig["z"][a]], np.float64)
                 v = trilinear(maps[ty], lo.astype(np.float64), h, n, R @ x + t)
                 want += v
@@ -114,19 +114,21 @@ def test_field_is_exact_when_atoms_sit_on_grid_points(gpu_device):
         lo, h, _ = c.grid()
         maps = {int(t): c.map(int(t)).astype(np.float64) for t in np.unique(lig["type"])}
     exact = plain_energies(protein, lig, ff, poses, gpu_device)
-    total = np.zeros(n)
-    mag = np.zeros(n)
+    total, total16, mag = np.zeros(n), np.zeros(n), np.zeros(n)
     for a in range(len(lig)):
         pos = np.stack([lig[k][a] + poses[3 + i] for i, k in enumerate("xyz")], 1)
         ijk = np.rint((pos - lo) / h).astype(np.int64)
         v = maps[int(lig["type"][a])][ijk[:, 0], ijk[:, 1], ijk[:, 2]]
         total += v
+        total16 += np.clip(v, -65504, 65504).astype(np.float16).astype(np.float64)
         mag += np.abs(v)
     mag = np.maximum(mag, 1.0)
-    # the scorer reads the map values exactly at grid points ...
-    assert np.max(np.abs(got - total) / mag) < 1e-5
-    # ... and their sum over the ligand's atoms is the pose energy
+    # the scorer reads the (FP16-stored) map values exactly at grid points ...
+    assert np.max(np.abs(got - total16) / mag) < 1e-5
+    # ... the sum of the single-atom energies over the ligand is the pose energy ...
     assert np.max(np.abs(exact - total) / mag) < 1e-5
+    # ... so on grid points the field is exact up to the FP16 storage rounding
+    assert np.max(np.abs(got - exact) / mag) < 2.0 ** -11 + 1e-5
 
 
 @pytest.mark.parametrize("surrogate", ["field", "mlp"])
@@ -158,6 +160,32 @@ def test_selection_rescoring_and_recall(gpu_device, surrogate):
                 # and those equal the plain kernel's for one pair
                 want = plain_energies(protein, w.ligands[0], w.forcefield, w.poses[0], gpu_device)
                 assert np.array_equal(c.exact(0).view(np.uint32), want.view(np.uint32))
+
+
+def test_set_keep_rescreens_with_the_new_fraction(gpu_device):
+    w = small_workload(nposes=4096, nligands=3, nproteins=2)
+    cfg = C.default_config("field", max_val_rmse=1e9)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        c.screen(w.proteins[0])
+        c.exhaustive()
+        r2 = c.recall(0.01)
+        for keep in (0.05, 0.01):
+            c.set_keep(keep)
+            c.screen(w.proteins[0])  # same protein: its exact energies stay usable
+            rk = c.recall(0.01)
+            for l in range(3):
+                idx, _ = c.selected(l)
+                assert len(idx) >= int(np.ceil(keep * 4096))
+                score = c.scores(l)
+                thr = np.sort(score)[int(np.ceil(keep * 4096)) - 1]
+                assert set(idx.tolist()) == set(np.nonzero(score <= thr)[0].tolist())
+            if keep > 0.02:
+                assert np.all(rk >= r2)
+        with pytest.raises(ValueError):
+            c.set_keep(0.0)
+        c.screen(w.proteins[1])  # another protein: the exact energies are stale
+        with pytest.raises(ValueError, match="this protein"):
+            c.recall(0.01)
 
 
 def test_fallback_pairs_are_scored_exhaustively(gpu_device):
