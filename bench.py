@@ -281,13 +281,15 @@ def traffic() -> float | None:
     return None
 
 
-def combined(full: bool, kinds=("field", "mlp")) -> dict:
+def combined(full: bool, kinds=("field", "mlp"), nproteins: int = 0) -> dict:
     """Both surrogates of the combined screen on the same workload: every
     protein screened (timed), exact energies of every pair for recall (all
     proteins at the default size, the first one at full size, where the
     exhaustive time of the rest is extrapolated from its interaction rate)."""
     from minibude_amd import combined as C
-    shape = COMBINED_FULL if full else COMBINED_DEFAULT
+    shape = dict(COMBINED_FULL if full else COMBINED_DEFAULT)
+    if nproteins:
+        shape["nproteins"] = nproteins
     t0 = time.time()
     w = C.combined_workload(**shape)
     per_protein = [len(p) * sum(len(l) * q.shape[1] for l, q in zip(w.ligands, w.poses)) for p in w.proteins]
@@ -295,7 +297,8 @@ def combined(full: bool, kinds=("field", "mlp")) -> dict:
         f"({w.total_poses} pair-poses, {w.interactions:.3e} interactions), generated in {time.time() - t0:.1f} s")
     out = {
         "workload": (f"c4: {len(w.proteins)} proteins of 900-6000 atoms x {len(w.ligands)} ligands of 8-96 atoms x "
-                     f"{shape['nposes']} poses per pair" + ("" if full else " (reduced from 512 ligands x 2^20 poses)")),
+                     f"{shape['nposes']} poses per pair" +
+                     ("" if full and not nproteins else " (reduced from 16 proteins x 512 ligands x 2^20 poses)")),
         "pairs": w.pairs,
         "pair_poses": w.total_poses,
         "interactions": w.interactions,
@@ -374,6 +377,7 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--combined-full", action="store_true",
                     help="combined screen at 16 proteins x 512 ligands x 2^20 poses (recall on the first protein)")
     ap.add_argument("--combined-surrogates", default="field,mlp", help="comma list of field, mlp")
+    ap.add_argument("--combined-proteins", type=int, default=0, help="screen only the first N proteins")
     ap.add_argument("--copy-calibration", type=int, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
     a = ap.parse_args(argv)
@@ -455,8 +459,8 @@ def main(argv: list[str] | None = None) -> int:
         "parity": parity,
         "batched": None if a.no_batched else batched(),
         "screen": None if a.no_screen else screen(),
-        "combined": None if a.no_combined else combined(a.combined_full,
-                                                         tuple(a.combined_surrogates.split(","))),
+        "combined": None if a.no_combined else combined(a.combined_full, tuple(a.combined_surrogates.split(",")),
+                                                         a.combined_proteins),
         "device": device_name(0),
     }
     print(json.dumps(out), flush=True)
