@@ -474,14 +474,14 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     coef.insert(coef.end(), t.begin(), t.end());
     if (coef.size() > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: pair tables too large");
   }
-  if (int st = c->protein.upload(groups.xyzw.data(), groups.xyzw.size() * 4)) return st;
-  if (int st = c->group_begin.upload(groups.begin.data(), groups.begin.size() * 4)) return st;
-  if (int st = c->coef.upload(coef.data(), coef.size() * sizeof(bude::PairCoef))) return st;
+  if (int st = c->protein.assign(groups.xyzw.data(), groups.xyzw.size() * 4)) return st;
+  if (int st = c->group_begin.assign(groups.begin.data(), groups.begin.size() * 4)) return st;
+  if (int st = c->coef.assign(coef.data(), coef.size() * sizeof(bude::PairCoef))) return st;
   if (!mlp) {
     std::vector<BudeAtom> probe(c->slot_type.size());
     for (size_t s = 0; s < probe.size(); s++) probe[s] = BudeAtom{0.f, 0.f, 0.f, c->slot_type[s]};
     const std::vector<bude::PairCoef> sc = bude::coef_table(probe.data(), probe.size(), groups, c->ff.data());
-    if (int st = c->slot_coef.upload(sc.data(), sc.size() * sizeof(bude::PairCoef))) return st;
+    if (int st = c->slot_coef.assign(sc.data(), sc.size() * sizeof(bude::PairCoef))) return st;
   }
   res->prepare_ms = float(ms_since(t0));
 
@@ -518,8 +518,8 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     lab_train[l] = mlp ? m - std::max<size_t>(1, m / 16) : 0;
   }
   const size_t nlab = src.size();
-  if (int st = c->lab_src.upload(src.data(), nlab * 4)) return st;
-  if (int st = c->lab_lig.upload(lab_lig.data(), nlab * 4)) return st;
+  if (int st = c->lab_src.assign(src.data(), nlab * 4)) return st;
+  if (int st = c->lab_lig.assign(lab_lig.data(), nlab * 4)) return st;
   if (int st = c->lab_poses.reserve(6 * nlab * 4)) return st;
   if (int st = c->lab_e.reserve(nlab * 4)) return st;
   if (int st = c->lab_pred.reserve(nlab * 4)) return st;
@@ -591,7 +591,7 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   // 5. MLP: train on the training part, predict the validation part
   if (mlp) {
     if (int st = record(c, 2)) return st;
-    if (int st = c->norms.upload(tn.data(), tn.size() * sizeof(tn[0]))) return st;
+    if (int st = c->norms.assign(tn.data(), tn.size() * sizeof(tn[0]))) return st;
     std::vector<std::vector<float>> w, b;
     bude::surrogate::init_weights(cfg.seed + c->proteins_done, bude::mlp::kIn, w, b);
     if (int st = bude::surrogate::set_weights(c->net, w, b, c->stream)) return st;

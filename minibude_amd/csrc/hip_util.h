@@ -43,6 +43,11 @@ struct DeviceBuffer {
   }
   // Keeps the current allocation when it already holds n bytes.
   int reserve(size_t n) { return (ptr && bytes >= n) ? BUDE_OK : alloc(n); }
+  // Copies n bytes in, reusing the allocation when it is large enough.
+  int assign(const void *src, size_t n) {
+    if (int st = reserve(n)) return st;
+    return n ? hip_check(hipMemcpy(ptr, src, n, hipMemcpyHostToDevice), "hipMemcpy H2D") : BUDE_OK;
+  }
   int upload(const void *src, size_t n) {
     if (int st = alloc(n)) return st;
     return n ? hip_check(hipMemcpy(ptr, src, n, hipMemcpyHostToDevice), "hipMemcpy H2D") : BUDE_OK;
