@@ -155,7 +155,7 @@ struct bude_combined {
   DeviceBuffer protein, group_begin, coef, slot_coef, maps, cells, counter;
   std::vector<int32_t> coef_off;
   // per-call work tables (each its own buffer; uploaded only while the stream is idle)
-  DeviceBuffer meta_label, tiles_label, meta_fb, tiles_fb, meta_rs, tiles_rs, meta_all, tiles_all;
+  DeviceBuffer meta_label, tiles_label, meta_lf, meta_fb, tiles_fb, meta_rs, tiles_rs, meta_all, tiles_all;
   DeviceBuffer tb_label, tb_all, lab_src, lab_poses, lab_e, lab_lig, lab_pred, norms, bias0;
   DeviceBuffer goff, poff, gathered, exact_sel;
   // MLP
@@ -187,13 +187,14 @@ struct Span {
   size_t lig, off, n;  // ligand, first pose in the pose array, poses
 };
 
-// One persistent batched launch scoring spans[i].n poses of ligand
-// spans[i].lig starting at row spans[i].off of `poses` (row stride
-// `stride`), energies to energies[off + j].  Tiles of the heaviest ligands
-// first.  meta / tiles are this call's own work buffers; the stream must be
-// idle (they are overwritten synchronously).
-int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride,
-                 float *energies, DeviceBuffer &meta, DeviceBuffer &tiles) {
+// Work tables of one persistent batched launch scoring spans[i].n poses of
+// ligand spans[i].lig starting at row spans[i].off of the pose array:
+// per-ligand meta and the tile list (heaviest ligands first), uploaded into
+// this launch's own buffers (synchronously: nothing may be reading them).
+// Returns the tile count in *ntiles (0: nothing to score).
+int prepare_exact(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer &meta, DeviceBuffer &tiles,
+                  int *ntiles) {
+  *ntiles = 0;
   std::vector<bude::LigandMeta> m(c->nlig);
   for (size_t l = 0; l < c->nlig; l++) {
     m[l].lig_off = c->lig_off[l];
@@ -214,15 +215,17 @@ int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
   for (size_t i : order)
     for (size_t j = 0; j < spans[i].n; j += 64 * kBatchPpwi) t.push_back(make_int2(int(spans[i].lig), int(j)));
   if (t.size() > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: too many tiles");
-  if (int st = meta.reserve(m.size() * sizeof(bude::LigandMeta))) return st;
-  if (int st = bude::hip_check(hipMemcpy(meta.ptr, m.data(), m.size() * sizeof(bude::LigandMeta),
-                                         hipMemcpyHostToDevice),
-                               "hipMemcpy meta"))
-    return st;
-  if (int st = tiles.reserve(t.size() * sizeof(int2))) return st;
-  if (int st = bude::hip_check(hipMemcpy(tiles.ptr, t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice),
-                               "hipMemcpy tiles"))
-    return st;
+  if (int st = meta.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
+  if (int st = tiles.assign(t.data(), t.size() * sizeof(int2))) return st;
+  *ntiles = int(t.size());
+  return BUDE_OK;
+}
+
+// The launch prepared by prepare_exact: energies[off + j] of every span, on
+// the first stream (which owns the work-queue counter).
+int launch_exact(bude_combined *c, const float *poses, size_t stride, float *energies, const DeviceBuffer &meta,
+                 const DeviceBuffer &tiles, int ntiles) {
+  if (ntiles == 0) return BUDE_OK;
   bude::BatchArgs a{};
   a.protein = static_cast<const float4 *>(c->protein.ptr);
   a.group_begin = static_cast<const int32_t *>(c->group_begin.ptr);
@@ -235,15 +238,24 @@ int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
   a.counter = static_cast<unsigned int *>(c->counter.ptr);
   a.stride = stride;
   a.ngroups = int(c->ngroups);
-  a.ntiles = int(t.size());
+  a.ntiles = ntiles;
   a.form = bude::plain_form();
   const int blocks = std::max(1, c->cus * 4 * kBatchWavesPerSimd / (kBatchWg / 64));
   return bude::hip_check(bude::launch_batch(a, kBatchPpwi, kBatchWg, blocks, c->stream), "combined batch launch");
 }
 
-// Field scores of spans (as exact_ragged) from the current maps.
-int field_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride, float *out,
-                 DeviceBuffer &meta, DeviceBuffer &tile_begin, hipStream_t stream) {
+int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride,
+                 float *energies, DeviceBuffer &meta, DeviceBuffer &tiles) {
+  int nt = 0;
+  if (int st = prepare_exact(c, spans, meta, tiles, &nt)) return st;
+  return launch_exact(c, poses, stride, energies, meta, tiles, nt);
+}
+
+// Work tables of a field scoring of spans (as prepare_exact): meta and the
+// per-ligand tile prefix.  Returns the tile count in *ntiles.
+int prepare_field(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer &meta, DeviceBuffer &tile_begin,
+                  int64_t *ntiles) {
+  *ntiles = 0;
   std::vector<bude::LigandMeta> m(c->nlig);
   std::vector<int64_t> tb(c->nlig + 1, 0);
   for (size_t l = 0; l < c->nlig; l++) {
@@ -256,16 +268,17 @@ int field_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
   }
   for (size_t l = 0; l < c->nlig; l++) tb[l + 1] = tb[l] + int64_t((size_t(m[l].nposes) + 63) / 64);
   if (tb[c->nlig] == 0) return BUDE_OK;
-  if (int st = meta.reserve(m.size() * sizeof(bude::LigandMeta))) return st;
-  if (int st = bude::hip_check(hipMemcpy(meta.ptr, m.data(), m.size() * sizeof(bude::LigandMeta),
-                                         hipMemcpyHostToDevice),
-                               "hipMemcpy meta"))
-    return st;
-  if (int st = tile_begin.reserve(tb.size() * sizeof(int64_t))) return st;
-  if (int st = bude::hip_check(hipMemcpy(tile_begin.ptr, tb.data(), tb.size() * sizeof(int64_t),
-                                         hipMemcpyHostToDevice),
-                               "hipMemcpy tile_begin"))
-    return st;
+  if (int st = meta.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
+  if (int st = tile_begin.assign(tb.data(), tb.size() * sizeof(int64_t))) return st;
+  *ntiles = tb[c->nlig];
+  return BUDE_OK;
+}
+
+// Interpolated field scores (out[off + j]) of the spans prepared by
+// prepare_field, from the current cells.
+int launch_field(bude_combined *c, const float *poses, size_t stride, float *out, const DeviceBuffer &meta,
+                 const DeviceBuffer &tile_begin, int64_t ntiles, hipStream_t stream) {
+  if (ntiles == 0) return BUDE_OK;
   bude::field::ScoreArgs a{};
   a.cells = static_cast<const uint4 *>(c->cells.ptr);
   a.grid = c->grid;
@@ -276,7 +289,7 @@ int field_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
   a.poses = poses;
   a.stride = stride;
   a.out = out;
-  return bude::hip_check(bude::field::score(a, tb[c->nlig], stream), "field score");
+  return bude::hip_check(bude::field::score(a, ntiles, stream), "field score");
 }
 
 int sync(bude_combined *c) { return bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
@@ -533,12 +546,33 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   float *lp = static_cast<float *>(c->lab_pred.ptr);
   const float *lpos = static_cast<const float *>(c->lab_poses.ptr);
 
-  // 3. field (second stream, overlapping step 4): maps, cells, then the
-  //    interpolated score of every pose of every pair
+  // 3. every work table of the exact and field passes first: an upload
+  //    issued while a kernel fills the device waits for it
+  int nt_label = 0;
+  int64_t ft_all = 0, ft_label = 0;
+  if (int st = prepare_exact(c, lab_spans, c->meta_label, c->tiles_label, &nt_label)) return st;
   if (!mlp) {
     const size_t np = c->grid.points(), nslots = c->slot_type.size();
     if (int st = c->maps.reserve(np * nslots * 4)) return st;
     if (int st = c->cells.reserve(c->grid.cells() * nslots * 16)) return st;
+    if (int st = prepare_field(c, all, c->meta_all, c->tb_all, &ft_all)) return st;
+    if (int st = prepare_field(c, lab_spans, c->meta_lf, c->tb_label, &ft_label)) return st;
+  }
+
+  // 4. exact energies of the sampled poses (one ragged batched launch) on the
+  //    first stream, and, for the field, maps -> cells -> scores of every
+  //    pose of every pair on the second stream, concurrently
+  if (int st = record(c, 0)) return st;
+  hipLaunchKernelGGL(gather_index_kernel, dim3(blocks_for(nlab, 256)), dim3(256), 0, c->stream, poses, c->total,
+                     static_cast<const uint32_t *>(c->lab_src.ptr), nlab, static_cast<float *>(c->lab_poses.ptr),
+                     nlab);
+  if (int st = bude::hip_check(hipGetLastError(), "gather labels")) return st;
+  if (int st = launch_exact(c, lpos, nlab, static_cast<float *>(c->lab_e.ptr), c->meta_label, c->tiles_label,
+                            nt_label))
+    return st;
+  if (int st = record(c, 1)) return st;
+  if (!mlp) {
+    const size_t nslots = c->slot_type.size();
     if (int st = record(c, 2, c->stream2)) return st;
     if (int st = bude::hip_check(
             bude::field::build_maps(static_cast<const float4 *>(c->protein.ptr),
@@ -553,27 +587,16 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
                                  "pack cells"))
       return st;
     if (int st = record(c, 3, c->stream2)) return st;
-    if (int st = field_ragged(c, all, poses, c->total, pred, c->meta_all, c->tb_all, c->stream2)) return st;
+    if (int st = launch_field(c, poses, c->total, pred, c->meta_all, c->tb_all, ft_all, c->stream2)) return st;
     if (int st = record(c, 4, c->stream2)) return st;
   }
-
-  // 4. exact energies of the sampled poses (one ragged batched launch)
-  if (int st = record(c, 0)) return st;
-  hipLaunchKernelGGL(gather_index_kernel, dim3(blocks_for(nlab, 256)), dim3(256), 0, c->stream, poses, c->total,
-                     static_cast<const uint32_t *>(c->lab_src.ptr), nlab, static_cast<float *>(c->lab_poses.ptr),
-                     nlab);
-  if (int st = bude::hip_check(hipGetLastError(), "gather labels")) return st;
-  if (int st = exact_ragged(c, lab_spans, lpos, nlab, static_cast<float *>(c->lab_e.ptr), c->meta_label,
-                            c->tiles_label))
-    return st;
-  if (int st = record(c, 1)) return st;
   std::vector<float> le(nlab);
   if (int st = bude::hip_check(hipMemcpyAsync(le.data(), c->lab_e.ptr, nlab * 4, hipMemcpyDeviceToHost, c->stream),
                                "hipMemcpyAsync"))
     return st;
   if (!mlp) {  // field scores of the sampled poses, once the cells exist
     if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->ev[3], 0), "hipStreamWaitEvent")) return st;
-    if (int st = field_ragged(c, lab_spans, lpos, nlab, lp, c->meta_fb, c->tb_label, c->stream)) return st;
+    if (int st = launch_field(c, lpos, nlab, lp, c->meta_lf, c->tb_label, ft_label, c->stream)) return st;
   }
   if (int st = sync(c)) return st;
   res->label_ms = elapsed(c, 0, 1);

@@ -41,8 +41,13 @@ struct DeviceBuffer {
     bytes = n;
     return BUDE_OK;
   }
-  // Keeps the current allocation when it already holds n bytes.
-  int reserve(size_t n) { return (ptr && bytes >= n) ? BUDE_OK : alloc(n); }
+  // Keeps the current allocation when it already holds n bytes; a growing
+  // buffer gets a quarter of headroom, so sizes that vary a little from call
+  // to call settle on one allocation.
+  int reserve(size_t n) {
+    if (ptr && bytes >= n) return BUDE_OK;
+    return alloc(ptr ? n + n / 4 : n);
+  }
   // Copies n bytes in, reusing the allocation when it is large enough.
   int assign(const void *src, size_t n) {
     if (int st = reserve(n)) return st;
