@@ -198,7 +198,7 @@ int bude_mi355x_screen_set_keep(bude_screen *s, float keep_fraction);
 int bude_mi355x_screen_selected(bude_screen *s, uint32_t *idx, float *energies, size_t cap, size_t *n);
 /* Surrogate prediction for every loaded pose, in energy units of the clipped scale. */
 int bude_mi355x_screen_predict(bude_screen *s, float *pred);
-/* Plain kernel on every loaded pose: the exhaustive baseline and judge. */
+/* Plain kernel on every loaded pose: the exhaustive baseline, and the exact energies recall is measured on. */
 int bude_mi355x_screen_exhaustive(bude_screen *s, float *energies, double *ms);
 int bude_mi355x_screen_destroy(bude_screen *s);
 

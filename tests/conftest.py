@@ -24,7 +24,7 @@ def pytest_configure(config):
     drv = ROOT / "minibude_amd" / "bin" / "bude-mi355x"
     host_srcs = list((ROOT / "minibude_amd" / "host").glob("*.cpp"))
     if not lib.exists() or (host_srcs and not drv.exists()):
-        # Build only when nothing is built (the GPU box uses the shipped build).
+        # Build only when nothing is built (a machine without the toolchain runs the shipped build).
         from minibude_amd import build
         build.build()
     if not (ROOT / "oracle" / "liboracle.so").exists():
