@@ -359,55 +359,68 @@ __global__ void gather_poses_kernel(const float *__restrict__ src, size_t src_st
 }
 
 // ---- fused inference ---------------------------------------------------------
-// One work-group (4 waves) scores tiles of kTile poses through the whole
-// network with the activations in LDS: features -> 3 x (MFMA GEMM + bias +
-// GELU) -> output dot product; wave w owns output columns [64w, 64w + 64) of
-// every hidden layer.  Weights stream from L2 (k-contiguous rows of W^T).
-constexpr int kTile = 64;
+// One work-group (8 waves) scores tiles of kTile poses through the whole
+// network with the activations in one LDS buffer: features -> 3 x (MFMA GEMM
+// + bias + GELU) -> output dot product; wave w owns output columns
+// [32w, 32w + 32) of every hidden layer and holds them in registers until
+// every wave has read the layer's input, then overwrites the buffer.  Weights
+// stream from L2 (k-contiguous rows of W^T), the next k-step's fragments
+// loaded while the current one multiplies; 128-pose tiles halve the weight
+// traffic per pose against 64-pose tiles, and one 66 KiB buffer leaves room
+// for two work-groups per CU.
+constexpr int kTile = 128;
+constexpr int kInferThreads = 512;
 constexpr int kLdsRow = kHidden + 8;  // bf16 per LDS row (padding staggers banks)
 
 __device__ __forceinline__ bf16x8 lds_load8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 
 template <int K>
-__device__ __forceinline__ void fused_layer(const bf16 *__restrict__ in, bf16 *__restrict__ out,
-                                            const bf16 *__restrict__ wt, const float *__restrict__ bias, int lane,
-                                            int wave) {
+__device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *__restrict__ wt,
+                                            const float *__restrict__ bias, int lane, int wave) {
   const int r = lane & 15, grp = lane >> 4;
-  floatx4 acc[kTile / 16][4];
+  floatx4 acc[kTile / 16][2];
 #pragma unroll
   for (int i = 0; i < kTile / 16; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k = 0; k < K; k += 32) {
-    bf16x8 fb[4];
+    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const bf16 *wrow = wt + size_t(32 * wave + r) * K + 8 * grp;
+  bf16x8 fb[2];
 #pragma unroll
-    for (int j = 0; j < 4; j++) fb[j] = load8(wt + size_t(64 * wave + 16 * j + r) * K + k + 8 * grp);
+  for (int j = 0; j < 2; j++) fb[j] = load8(wrow + size_t(16 * j) * K);
+#pragma unroll 1
+  for (int k = 0; k < K; k += 32) {
+    bf16x8 fn[2];
+    const int kn = k + 32 < K ? k + 32 : k;  // (the last step reloads its own fragments)
+#pragma unroll
+    for (int j = 0; j < 2; j++) fn[j] = load8(wrow + size_t(16 * j) * K + kn);
 #pragma unroll
     for (int i = 0; i < kTile / 16; i++) {
-      const bf16x8 fa = lds_load8(in + (16 * i + r) * kLdsRow + k + 8 * grp);
+      const bf16x8 fa = lds_load8(buf + (16 * i + r) * kLdsRow + k + 8 * grp);
 #pragma unroll
-      for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[j], acc[i][j], 0, 0, 0);
     }
-  }
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int n = 64 * wave + 16 * j + r;
+    for (int j = 0; j < 2; j++) fb[j] = fn[j];
+  }
+  __syncthreads();  // every wave has read this layer's input
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int n = 32 * wave + 16 * j + r;
     const float bn = bias[n];
 #pragma unroll
     for (int i = 0; i < kTile / 16; i++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) out[(16 * i + 4 * grp + q) * kLdsRow + n] = to_bf16(gelu(acc[i][j][q] + bn));
+      for (int q = 0; q < 4; q++) buf[(16 * i + 4 * grp + q) * kLdsRow + n] = to_bf16(gelu(acc[i][j][q] + bn));
   }
+  __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict__ poses, size_t stride, size_t n,
-                                                        const bf16 *__restrict__ w0t, const float *__restrict__ b0,
-                                                        const bf16 *__restrict__ w1t, const float *__restrict__ b1,
-                                                        const bf16 *__restrict__ w2t, const float *__restrict__ b2,
-                                                        const bf16 *__restrict__ w3, const float *__restrict__ b3,
-                                                        float *__restrict__ out) {
-  __shared__ bf16 act[2][kTile * kLdsRow];
+__global__ __launch_bounds__(kInferThreads) void mlp_infer_kernel(
+    const float *__restrict__ poses, size_t stride, size_t n, const bf16 *__restrict__ w0t,
+    const float *__restrict__ b0, const bf16 *__restrict__ w1t, const float *__restrict__ b1,
+    const bf16 *__restrict__ w2t, const float *__restrict__ b2, const bf16 *__restrict__ w3,
+    const float *__restrict__ b3, float *__restrict__ out) {
+  __shared__ bf16 act[kTile * kLdsRow];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const size_t ntiles = (n + kTile - 1) / kTile;
   for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -415,7 +428,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict_
     // features of the tile's poses (rows past n are zero)
     if (tid < kTile) {
       const size_t i = base + tid;
-      bf16 *row = act[0] + tid * kLdsRow;
+      bf16 *row = act + tid * kLdsRow;
       float f[kIn];
       if (i < n) {
         pose_features(poses, stride, i, f);
@@ -427,16 +440,13 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const float *__restrict_
       for (int c = 0; c < kIn; c++) row[c] = to_bf16(f[c]);
     }
     __syncthreads();
-    fused_layer<kIn>(act[0], act[1], w0t, b0, lane, wave);
-    __syncthreads();
-    fused_layer<kHidden>(act[1], act[0], w1t, b1, lane, wave);
-    __syncthreads();
-    fused_layer<kHidden>(act[0], act[1], w2t, b2, lane, wave);
-    __syncthreads();
+    fused_layer<kIn>(act, w0t, b0, lane, wave);
+    fused_layer<kHidden>(act, w1t, b1, lane, wave);
+    fused_layer<kHidden>(act, w2t, b2, lane, wave);
     // output layer: 4 threads per pose, 64 inputs each, combined by shuffles
     {
       const int row = tid >> 2, part = tid & 3;
-      const bf16 *a = act[1] + row * kLdsRow + 64 * part;
+      const bf16 *a = act + row * kLdsRow + 64 * part;
       float sum = 0.f;
 #pragma unroll 8
       for (int c = 0; c < 64; c++) sum += from_bf16(a[c]) * from_bf16(w3[size_t(64 * part + c) * kOut]);
@@ -562,7 +572,7 @@ hipError_t infer(const float *poses, size_t stride, size_t n, const bf16 *const 
   if (n == 0) return hipSuccess;
   const size_t tiles = (n + kTile - 1) / kTile;
   const unsigned grid = unsigned(std::min<size_t>(tiles, size_t(blocks)));
-  hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(256), 0, s, poses, stride, n, wt[0], bias[0], wt[1], bias[1],
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(kInferThreads), 0, s, poses, stride, n, wt[0], bias[0], wt[1], bias[1],
                      wt[2], bias[2], w3, bias[3], out);
   return hipGetLastError();
 }

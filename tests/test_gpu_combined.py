@@ -18,6 +18,11 @@ def small_workload(nposes=8192, nligands=5, nproteins=2, seed=11):
                                natpro_range=(300, 900), natlig_range=(4, 24))
 
 
+def kept(keep, n):
+    """Poses a pair keeps: ceil(keep * n) with keep as the C float the library holds."""
+    return max(1, int(np.ceil(float(np.float32(keep)) * n)))
+
+
 def plain_energies(protein, ligand, ff, poses, device):
     with Context(D.Deck(protein, ligand, ff, np.ascontiguousarray(poses, np.float32)), device) as c:
         return c.fasten(1, 64).energies
@@ -148,7 +153,7 @@ def test_selection_rescoring_and_recall(gpu_device, surrogate):
                 idx, e = c.selected(l)
                 score = c.scores(l)
                 exact = c.exact(l)
-                k = int(np.ceil(0.05 * len(score)))
+                k = kept(0.05, len(score))
                 thr = np.sort(score)[k - 1]
                 # the selection is the lowest-k of the surrogate's scores (ties kept)
                 assert set(idx.tolist()) == set(np.nonzero(score <= thr)[0].tolist()), (p, l)
@@ -175,9 +180,9 @@ def test_set_keep_rescreens_with_the_new_fraction(gpu_device):
             rk = c.recall(0.01)
             for l in range(3):
                 idx, _ = c.selected(l)
-                assert len(idx) >= int(np.ceil(keep * 4096))
+                assert len(idx) >= kept(keep, 4096)
                 score = c.scores(l)
-                thr = np.sort(score)[int(np.ceil(keep * 4096)) - 1]
+                thr = np.sort(score)[kept(keep, 4096) - 1]
                 assert set(idx.tolist()) == set(np.nonzero(score <= thr)[0].tolist())
             if keep > 0.02:
                 assert np.all(rk >= r2)
@@ -202,6 +207,42 @@ def test_fallback_pairs_are_scored_exhaustively(gpu_device):
             assert np.array_equal(c.scores(l).view(np.uint32), exact.view(np.uint32))
             idx, e = c.selected(l)
             assert np.array_equal(e, exact[idx])
+
+
+def test_mixed_fallback_and_ragged_edges(gpu_device):
+    # pose counts of 1, 63, 64, 65 and 3000, a threshold between the pairs'
+    # validation errors (some pairs fall back, others are screened), and a
+    # keep fraction of 1 on the same ligand set
+    w = small_workload(nposes=3000, nligands=5, nproteins=1, seed=14)
+    poses = [w.poses[0][:, :1], w.poses[1][:, :63], w.poses[2][:, :64], w.poses[3][:, :65], w.poses[4]]
+    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48, keep_fraction=0.05)
+    with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
+        c.screen(w.proteins[0])
+        cal = np.array([c.pair(l).val_rmse_calibrated for l in range(5)])
+    thr = float(np.sort(cal)[2])  # the two largest fall back
+    for keep in (0.05, 1.0):
+        # a fresh object draws the same samples, so the same pairs fall back
+        cfg = C.default_config("field", max_val_rmse=thr, labels_per_pair=48, keep_fraction=keep)
+        with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
+            r = c.screen(w.proteins[0])
+            c.exhaustive()
+            fb = [c.pair(l).fallback for l in range(5)]
+            assert r.fallback_pairs == sum(fb) and 0 < sum(fb) < 5
+            for l in range(5):
+                n = poses[l].shape[1]
+                exact = c.exact(l)
+                want = plain_energies(w.proteins[0], w.ligands[l], w.forcefield, poses[l], gpu_device)
+                assert np.array_equal(exact.view(np.uint32), want.view(np.uint32)), l
+                idx, e = c.selected(l)
+                k = kept(keep, n)
+                score = c.scores(l)
+                assert set(idx.tolist()) == set(np.nonzero(score <= np.sort(score)[k - 1])[0].tolist()), (keep, l)
+                assert np.array_equal(e.view(np.uint32), exact[idx].view(np.uint32)), (keep, l)
+                if fb[l]:
+                    assert np.array_equal(score.view(np.uint32), exact.view(np.uint32))
+                if keep == 1.0:
+                    assert len(idx) == n
+            assert np.all(c.recall(0.01)[np.array(fb) == 1] == 1.0)
 
 
 def test_field_screen_recall(gpu_device):

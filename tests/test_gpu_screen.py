@@ -63,7 +63,7 @@ def test_fallback_selects_exact_lowest(gpu_device, small_workload):
             r = s.run()
             assert r.fallback == 1
             idx, e = s.selected()
-            k = int(np.ceil(keep * len(exact)))
+            k = int(np.ceil(float(np.float32(keep)) * len(exact)))  # keep as the C float the library holds
             assert len(idx) >= k
             thr = np.sort(exact)[k - 1]
             assert set(idx.tolist()) == set(np.nonzero(exact <= thr)[0].tolist())
