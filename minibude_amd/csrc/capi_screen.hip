@@ -20,6 +20,7 @@
 #include "fasten_plain.h"
 #include "hip_util.h"
 #include "mlp.h"
+#include "select.h"
 #include "surrogate.h"
 
 namespace {
@@ -50,7 +51,9 @@ struct bude_screen {
   float clip = 0.f, mu = 0.f, sigma = 1.f, tail = -3.0e38f, val_rmse = 0.f;
   // candidate poses
   size_t ncand = 0;
-  DeviceBuffer cand, pred, sel_idx, sel_n, hist, gathered, exact;
+  DeviceBuffer cand, pred, sel_idx, sel_n, gathered, exact;
+  bude::sel::Plan plan;  // one segment: the loaded poses, k = ceil(keep_fraction * n)
+  size_t plan_n = 0, plan_k = 0;
   size_t nsel = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {};
@@ -139,7 +142,6 @@ extern "C" int bude_mi355x_screen_create(int device, const BudeAtom *protein, si
   if (!st) st = s->group_begin.upload(groups.begin.data(), groups.begin.size() * 4);
   if (!st) st = s->coef.upload(coef.data(), coef.size() * sizeof(bude::PairCoef));
   if (!st) st = s->ligand.upload(lig4.data(), lig4.size() * 4);
-  if (!st) st = s->hist.alloc((2048 + 64) * 4);
   if (!st) st = s->sel_n.alloc(256);
   if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
   for (auto &e : s->ev)
@@ -270,6 +272,11 @@ extern "C" int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res) {
   const size_t k = std::max<size_t>(1, size_t(std::ceil(double(s->cfg.keep_fraction) * double(n))));
   hipStream_t st_ = s->stream;
   const auto t0 = std::chrono::steady_clock::now();
+  if (s->plan_n != n || s->plan_k != k) {  // (uploads before any launch of this run)
+    if (int st = s->plan.prepare({bude::sel::Segment{0, uint32_t(n), uint32_t(k)}})) return st;
+    s->plan_n = n;
+    s->plan_k = k;
+  }
   float ms01 = 0, ms12 = 0, ms23 = 0;
   if (s->fallback) {
     // plain kernel on every pose, then the same selection on exact energies
@@ -283,10 +290,9 @@ extern "C" int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res) {
       return st;
     if (int st = bude::hip_check(hipEventRecord(s->ev[1], st_), "hipEventRecord")) return st;
   }
-  if (int st = bude::hip_check(bude::mlp::select_lowest(static_cast<const float *>(s->pred.ptr), n, k,
-                                                        static_cast<uint32_t *>(s->hist.ptr),
-                                                        static_cast<uint32_t *>(s->sel_idx.ptr),
-                                                        static_cast<uint32_t *>(s->sel_n.ptr), st_),
+  if (int st = bude::hip_check(bude::sel::select_lowest_segments(s->plan, static_cast<const float *>(s->pred.ptr),
+                                                                 static_cast<uint32_t *>(s->sel_idx.ptr),
+                                                                 static_cast<uint32_t *>(s->sel_n.ptr), st_),
                                "select"))
     return st;
   uint32_t nsel = 0;

@@ -1,7 +1,7 @@
 // mlp.h -- the pose-energy surrogate: a 4-layer MLP (21 pose features ->
 // 256 -> 256 -> 256 -> 1, GELU) on BF16 MFMA with FP32 accumulation, its
-// training kernels (backward GEMMs, loss, Adam) and the top-k selection
-// used to pass the surrogate's best poses to exact rescoring.
+// training kernels (backward GEMMs, loss, Adam), and the gather that passes
+// the surrogate's best poses to exact rescoring.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -114,13 +114,6 @@ hipError_t bias_grad_multi(const BiasGradBatch &b, hipStream_t s);
 
 // FP32 -> BF16 of a rows x cols matrix and its transpose.
 hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s);
-
-// Top-k by value: threshold = k-th smallest of x[0..n) via three 11-bit
-// radix histogram passes on order-preserving keys, then every i with
-// x[i] <= threshold is appended to out_idx (count in *out_n).  hist needs
-// 2048 + 8 uint32; all device memory.
-hipError_t select_lowest(const float *x, size_t n, size_t k, uint32_t *hist, uint32_t *out_idx, uint32_t *out_n,
-                         hipStream_t s);
 
 // Fused inference: predictions (standardised units) of n poses through the
 // whole network, activations kept in LDS; wt[l] = W_l^T [N_l][K_l] (BF16),

@@ -290,65 +290,6 @@ __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, 
   if (w16t) w16t[size_t(i % cols) * rows + i / cols] = h;
 }
 
-// ---- top-k selection --------------------------------------------------------
-
-__device__ __forceinline__ uint32_t order_key(float x) {
-  const uint32_t u = __float_as_uint(x);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-// hist[0..2047] bins; hist[2048] prefix, hist[2049] mask, hist[2050] k remaining
-__global__ void select_init_kernel(uint32_t *hist, uint32_t k, uint32_t *out_n) {
-  for (int i = threadIdx.x; i < 2048; i += blockDim.x) hist[i] = 0;
-  if (threadIdx.x == 0) {
-    hist[2048] = 0;
-    hist[2049] = 0;
-    hist[2050] = k;
-    *out_n = 0;
-  }
-}
-
-__global__ __launch_bounds__(256) void select_hist_kernel(const float *__restrict__ x, size_t n, int shift,
-                                                          uint32_t digit_mask, uint32_t *__restrict__ hist) {
-  __shared__ uint32_t local[2048];
-  for (int i = threadIdx.x; i < 2048; i += 256) local[i] = 0;
-  __syncthreads();
-  const uint32_t prefix = hist[2048], mask = hist[2049];
-  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
-    const uint32_t key = order_key(x[i]);
-    if ((key & mask) == prefix) atomicAdd(&local[(key >> shift) & digit_mask], 1u);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < 2048; i += 256)
-    if (local[i]) atomicAdd(&hist[i], local[i]);
-}
-
-// One work-group: find the bin holding the k-th smallest, extend the prefix.
-__global__ __launch_bounds__(64) void select_pick_kernel(uint32_t *hist, int shift, int bits) {
-  if (threadIdx.x == 0) {
-    const uint32_t k = hist[2050];
-    uint32_t cum = 0, bin = 0;
-    const uint32_t nb = 1u << bits;
-    for (bin = 0; bin < nb; bin++) {
-      if (cum + hist[bin] >= k) break;
-      cum += hist[bin];
-    }
-    if (bin == nb) bin = nb - 1;
-    hist[2048] |= bin << shift;
-    hist[2049] |= (nb - 1) << shift;
-    hist[2050] = k - cum;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < 2048; i += 64) hist[i] = 0;
-}
-
-__global__ void select_compact_kernel(const float *__restrict__ x, size_t n, const uint32_t *__restrict__ hist,
-                                      uint32_t *__restrict__ out_idx, uint32_t *__restrict__ out_n) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (order_key(x[i]) <= hist[2048]) out_idx[atomicAdd(out_n, 1u)] = uint32_t(i);
-}
-
 __global__ void gather_poses_kernel(const float *__restrict__ src, size_t src_stride, const uint32_t *__restrict__ idx,
                                     const uint32_t *__restrict__ n_dev, float *__restrict__ dst, size_t dst_stride) {
   const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -550,20 +491,6 @@ hipError_t bias_grad_multi(const BiasGradBatch &b, hipStream_t s) {
 hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s) {
   hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(size_t(rows) * cols, 256)), dim3(256), 0, s, w, rows, cols, w16,
                      w16t);
-  return hipGetLastError();
-}
-
-hipError_t select_lowest(const float *x, size_t n, size_t k, uint32_t *hist, uint32_t *out_idx, uint32_t *out_n,
-                         hipStream_t s) {
-  if (k == 0 || k > n || n > 0xffffffffu) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(select_init_kernel, dim3(1), dim3(256), 0, s, hist, uint32_t(k), out_n);
-  const unsigned grid = unsigned(std::min<size_t>(blocks_for(n, 256), 2048));
-  const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
-  for (int p = 0; p < 3; p++) {
-    hipLaunchKernelGGL(select_hist_kernel, dim3(grid), dim3(256), 0, s, x, n, shifts[p], (1u << bits[p]) - 1u, hist);
-    hipLaunchKernelGGL(select_pick_kernel, dim3(1), dim3(64), 0, s, hist, shifts[p], bits[p]);
-  }
-  hipLaunchKernelGGL(select_compact_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, n, hist, out_idx, out_n);
   return hipGetLastError();
 }
 
