@@ -186,10 +186,10 @@ def batched() -> dict:
     return out
 
 
-def screen() -> dict:
+def screen(epochs: int = 0) -> dict:
     from minibude_amd import screen as S
     w = S.screen_workload()
-    cfg = S.default_config()
+    cfg = S.default_config(**({"epochs": epochs} if epochs else {}))
     n = w.screen_poses.shape[1]
     log(f"screen: training on {w.train_poses.shape[1]} poses, screening {n}")
     points = []
@@ -243,6 +243,7 @@ def screen() -> dict:
         "exhaustive_ms": round(ex_ms, 3),
         "exhaustive_poses_per_s": round(n / (ex_ms * 1e-3), 1),
         "val_rmse": round(st.val_rmse, 4),
+        "epochs": int(cfg.epochs),
         "train_s": round((st.label_ms + st.train_ms) * 1e-3, 2),
         "points": points,
         "field_surrogate": field,
@@ -373,6 +374,8 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-screen", action="store_true")
     ap.add_argument("--no-combined", action="store_true")
+    ap.add_argument("--screen-epochs", type=int, default=0, help="training epochs of the learned prescreen "
+                    "(default: its configuration's 60; fewer only to keep a kernel trace short)")
     ap.add_argument("--combined-full", action="store_true",
                     help="combined screen at 16 proteins x 512 ligands x 2^20 poses (recall on the first protein)")
     ap.add_argument("--combined-surrogates", default="field,mlp", help="comma list of field, mlp")
@@ -457,7 +460,7 @@ def main(argv: list[str] | None = None) -> int:
         "valid": valid,
         "parity": parity,
         "batched": None if a.no_batched else batched(),
-        "screen": None if a.no_screen else screen(),
+        "screen": None if a.no_screen else screen(a.screen_epochs),
         "combined": None if a.no_combined else combined(a.combined_full, tuple(a.combined_surrogates.split(",")),
                                                          a.combined_proteins),
         "device": device_name(0),
