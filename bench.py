@@ -162,6 +162,31 @@ def validation(e: np.ndarray, ref: np.ndarray) -> dict:
     return out
 
 
+SWEEP_PPWI = (1, 2, 4, 8, 16, 32, 64, 128)
+SWEEP_WGSIZE = (64, 128, 256, 512, 1024)
+
+
+def sweep(ctx: Context, ref: np.ndarray | None, warmup: int, steps: int) -> dict:
+    """The single-GPU workload's autotune sweep (the reference driver's
+    ppwi x wgsize report, src/main.cpp:517-565): hipEvent ms per launch and
+    validity of every point, the best by summed time as the driver picks it."""
+    inter = None
+    rows = []
+    for p in SWEEP_PPWI:
+        for wg in SWEEP_WGSIZE:
+            s = ctx.fasten(p, wg, warmup=warmup, iterations=steps, want_energies=ref is not None)
+            dev = float(np.mean(s.device_ms[warmup:]))
+            inter = inter or metrics.interactions_per_pass(ctx.deck.nposes, ctx.deck.natlig, ctx.deck.natpro)
+            v = validation(s.energies, ref)["valid"] if ref is not None else None
+            rows.append({"ppwi": p, "wgsize": wg, "ms": round(dev, 3),
+                         "g_interactions_per_s": round(inter / dev / 1e6, 1),
+                         "tflops": round(metrics.flops_per_pass(ctx.deck.nposes, ctx.deck.natlig, ctx.deck.natpro, p)
+                                         / (dev * 1e-3) / 1e12, 2), "valid": v})
+            log(f"sweep: {rows[-1]}")
+    best = min(rows, key=lambda r: r["ms"])
+    return {"points": rows, "best": best}
+
+
 def batched() -> dict:
     from minibude_amd.batch import Batch, ragged_workload
     from minibude_amd.capi import BatchConfig
@@ -374,6 +399,8 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-screen", action="store_true")
     ap.add_argument("--no-combined", action="store_true")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also run the ppwi x wgsize autotune sweep of the plain kernel (40 points)")
     ap.add_argument("--screen-epochs", type=int, default=0, help="training epochs of the learned prescreen "
                     "(default: its configuration's 60; fewer only to keep a kernel trace short)")
     ap.add_argument("--combined-full", action="store_true",
@@ -405,6 +432,7 @@ def main(argv: list[str] | None = None) -> int:
         log(f"uploaded in {ctx.context_ms:.1f} ms; {a.warmup} warmup + {a.steps} timed steps "
             f"(ppwi={a.ppwi}, wgsize={a.wgsize})")
         s = ctx.fasten(a.ppwi, a.wgsize, warmup=a.warmup, iterations=a.steps, want_energies=True)
+        sweep_out = sweep(ctx, ref, 1, 2) if a.sweep else None
     host = np.array(s.kernel_ms[a.warmup:])
     dev = np.array(s.device_ms[a.warmup:])
     ms_per_step = float(host.mean())
@@ -459,6 +487,7 @@ def main(argv: list[str] | None = None) -> int:
         "cpu_baseline": cpu,
         "valid": valid,
         "parity": parity,
+        "sweep": sweep_out,
         "batched": None if a.no_batched else batched(),
         "screen": None if a.no_screen else screen(a.screen_epochs),
         "combined": None if a.no_combined else combined(a.combined_full, tuple(a.combined_surrogates.split(",")),
