@@ -46,6 +46,17 @@ __device__ __forceinline__ Xform make_xform(const float *__restrict__ poses, siz
 //   kFormD  sat(c0 + d * neg_e1) (one rounding of c0 per type pair)
 enum HingeForm { kFormB = 0, kFormD = 1 };
 
+// A VGPR copy of a wave-uniform value, opaque to the compiler.  A VALU
+// instruction reads at most one SGPR, so fma(d, neg_e1, c0) needs one of the
+// two group coefficients in a VGPR; left to itself the compiler re-copies it
+// inside the pair loop (it reuses the copy's register for the result), one
+// extra VALU per pair.  Copied once per group here, the value stays live.
+__device__ __forceinline__ float vgpr_copy(float s) {
+  float v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+  return v;
+}
+
 // All pairs of one protein group with the RB poses of this work-item.
 // CHG / DSLV drop the charge / desolvation hinge when the group's weight is
 // exactly zero for this ligand atom (adding fma(0, h, e) == e is a no-op, so
@@ -54,6 +65,11 @@ template <int RB, int FORM, bool CHG, bool DSLV>
 __device__ __forceinline__ void group_pairs(const float4 *__restrict__ prot, int i0, int i1, const PairCoef c,
                                             const float (&lx)[RB], const float (&ly)[RB], const float (&lz)[RB],
                                             float (&e)[RB]) {
+  float c0 = 0.f, e0 = 0.f;
+  if constexpr (FORM == kFormD) {
+    if constexpr (CHG) c0 = vgpr_copy(c.c0);
+    if constexpr (DSLV) e0 = vgpr_copy(c.e0);
+  }
 #pragma unroll 2
   for (int i = i0; i < i1; ++i) {
     const float4 p = prot[i];
@@ -70,8 +86,8 @@ __device__ __forceinline__ void group_pairs(const float4 *__restrict__ prot, int
         if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(b, c.neg_rd, 1.0f)), e[k]);
       } else {
         e[k] = __builtin_fmaf(2.0f * kHardness, sat(__builtin_fmaf(d, c.neg_rr, 1.0f)), e[k]);
-        if constexpr (CHG) e[k] = __builtin_fmaf(c.wq, sat(__builtin_fmaf(d, c.neg_e1, c.c0)), e[k]);
-        if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(d, c.neg_rd, c.e0)), e[k]);
+        if constexpr (CHG) e[k] = __builtin_fmaf(c.wq, sat(__builtin_fmaf(d, c.neg_e1, c0)), e[k]);
+        if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(d, c.neg_rd, e0)), e[k]);
       }
     }
   }
