@@ -245,6 +245,33 @@ def test_mixed_fallback_and_ragged_edges(gpu_device):
             assert np.all(c.recall(0.01)[np.array(fb) == 1] == 1.0)
 
 
+def test_errors_are_reported_not_faked(gpu_device):
+    w = small_workload(nposes=256, nligands=2, nproteins=1)
+    with C.Combined(w.forcefield, w.ligands, w.poses, C.default_config("field"), gpu_device) as c:
+        for call in (lambda: c.pair(0), lambda: c.selected(0), lambda: c.scores(0)):
+            with pytest.raises(ValueError, match="no protein screened"):
+                call()
+        bad = w.proteins[0].copy()
+        bad["type"][3] = 999
+        with pytest.raises(ValueError, match="protein atom 3"):
+            c.screen(bad)
+        c.screen(w.proteins[0])
+        with pytest.raises(ValueError, match="exhaustive"):
+            c.recall(0.01)
+        with pytest.raises(ValueError, match="exhaustive"):
+            c.exact(0)
+        used = set(np.concatenate([l["type"] for l in w.ligands]).tolist())
+        unused = next(t for t in range(len(w.forcefield)) if t not in used)
+        with pytest.raises(ValueError, match="no ligand atom has this type"):
+            c.map(unused)
+        with pytest.raises(ValueError):
+            c.pair(2)
+    cfg = C.default_config("mlp", labels_per_pair=64, batch=64, epochs=1)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        with pytest.raises(ValueError, match="not a field screen"):
+            c.grid()
+
+
 def test_field_screen_recall(gpu_device):
     # moderately sized pairs: the field keeps the exact top 1 % in its lowest 2 %
     w = C.combined_workload(nproteins=1, nligands=8, nposes=65536, seed=12)
