@@ -57,7 +57,39 @@ __device__ __forceinline__ float vgpr_copy(float s) {
   return v;
 }
 
-// All pairs of one protein group with the RB poses of this work-item.
+// The pairs of one protein atom p with the RB poses of this work-item.
+template <int RB, int FORM, bool CHG, bool DSLV>
+__device__ __forceinline__ void atom_pairs(const float4 p, const PairCoef &c, float c0, float e0,
+                                           const float (&lx)[RB], const float (&ly)[RB], const float (&lz)[RB],
+                                           float (&e)[RB]) {
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    const float dx = lx[k] - p.x;
+    const float dy = ly[k] - p.y;
+    const float dz = lz[k] - p.z;
+    const float d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+    if constexpr (FORM == kFormB) {
+      const float b = d - c.radij;
+      e[k] = __builtin_fmaf(2.0f * kHardness, sat(b * c.neg_rr), e[k]);
+      if constexpr (CHG) e[k] = __builtin_fmaf(c.wq, sat(__builtin_fmaf(b, c.neg_e1, 1.0f)), e[k]);
+      if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(b, c.neg_rd, 1.0f)), e[k]);
+    } else {
+      e[k] = __builtin_fmaf(2.0f * kHardness, sat(__builtin_fmaf(d, c.neg_rr, 1.0f)), e[k]);
+      if constexpr (CHG) e[k] = __builtin_fmaf(c.wq, sat(__builtin_fmaf(d, c.neg_e1, c0)), e[k]);
+      if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(d, c.neg_rd, e0)), e[k]);
+    }
+  }
+}
+
+// All pairs of one protein group with the RB poses of this work-item, atoms
+// in order.  The protein coordinates are wave-uniform scalar loads, and a
+// scalar load can only be waited for together with every other one in
+// flight, so the loop runs on two 4-atom buffers: the first use of the
+// current buffer waits for it (the only load outstanding), the next buffer is
+// requested right after, and the current buffer's other three atoms cover its
+// latency.  The scheduling barriers keep the compiler from hoisting the next
+// request above that first use (which would make the wait cover both).  A
+// wave then keeps its VALU busy even when it is alone on its SIMD.
 // CHG / DSLV drop the charge / desolvation hinge when the group's weight is
 // exactly zero for this ligand atom (adding fma(0, h, e) == e is a no-op, so
 // results are bit-identical).
@@ -70,27 +102,41 @@ __device__ __forceinline__ void group_pairs(const float4 *__restrict__ prot, int
     if constexpr (CHG) c0 = vgpr_copy(c.c0);
     if constexpr (DSLV) e0 = vgpr_copy(c.e0);
   }
-#pragma unroll 2
-  for (int i = i0; i < i1; ++i) {
-    const float4 p = prot[i];
-#pragma unroll
-    for (int k = 0; k < RB; ++k) {
-      const float dx = lx[k] - p.x;
-      const float dy = ly[k] - p.y;
-      const float dz = lz[k] - p.z;
-      const float d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
-      if constexpr (FORM == kFormB) {
-        const float b = d - c.radij;
-        e[k] = __builtin_fmaf(2.0f * kHardness, sat(b * c.neg_rr), e[k]);
-        if constexpr (CHG) e[k] = __builtin_fmaf(c.wq, sat(__builtin_fmaf(b, c.neg_e1, 1.0f)), e[k]);
-        if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(b, c.neg_rd, 1.0f)), e[k]);
-      } else {
-        e[k] = __builtin_fmaf(2.0f * kHardness, sat(__builtin_fmaf(d, c.neg_rr, 1.0f)), e[k]);
-        if constexpr (CHG) e[k] = __builtin_fmaf(c.wq, sat(__builtin_fmaf(d, c.neg_e1, c0)), e[k]);
-        if constexpr (DSLV) e[k] = __builtin_fmaf(c.wd, sat(__builtin_fmaf(d, c.neg_rd, e0)), e[k]);
-      }
+  int i = i0;
+  if (i1 - i >= 8) {
+    float4 x0 = prot[i], x1 = prot[i + 1], x2 = prot[i + 2], x3 = prot[i + 3];
+    // invariant: x0..x3 hold atoms i..i+3 and i + 4 <= i1
+    for (; i + 12 <= i1; i += 8) {
+      atom_pairs<RB, FORM, CHG, DSLV>(x0, c, c0, e0, lx, ly, lz, e);
+      __builtin_amdgcn_sched_barrier(0);
+      const float4 y0 = prot[i + 4], y1 = prot[i + 5], y2 = prot[i + 6], y3 = prot[i + 7];
+      __builtin_amdgcn_sched_barrier(0);
+      atom_pairs<RB, FORM, CHG, DSLV>(x1, c, c0, e0, lx, ly, lz, e);
+      atom_pairs<RB, FORM, CHG, DSLV>(x2, c, c0, e0, lx, ly, lz, e);
+      atom_pairs<RB, FORM, CHG, DSLV>(x3, c, c0, e0, lx, ly, lz, e);
+      atom_pairs<RB, FORM, CHG, DSLV>(y0, c, c0, e0, lx, ly, lz, e);
+      __builtin_amdgcn_sched_barrier(0);
+      x0 = prot[i + 8];
+      x1 = prot[i + 9];
+      x2 = prot[i + 10];
+      x3 = prot[i + 11];
+      __builtin_amdgcn_sched_barrier(0);
+      atom_pairs<RB, FORM, CHG, DSLV>(y1, c, c0, e0, lx, ly, lz, e);
+      atom_pairs<RB, FORM, CHG, DSLV>(y2, c, c0, e0, lx, ly, lz, e);
+      atom_pairs<RB, FORM, CHG, DSLV>(y3, c, c0, e0, lx, ly, lz, e);
     }
+    atom_pairs<RB, FORM, CHG, DSLV>(x0, c, c0, e0, lx, ly, lz, e);
+    atom_pairs<RB, FORM, CHG, DSLV>(x1, c, c0, e0, lx, ly, lz, e);
+    atom_pairs<RB, FORM, CHG, DSLV>(x2, c, c0, e0, lx, ly, lz, e);
+    atom_pairs<RB, FORM, CHG, DSLV>(x3, c, c0, e0, lx, ly, lz, e);
+    i += 4;
   }
+  for (; i + 2 <= i1; i += 2) {
+    const float4 p0 = prot[i], p1 = prot[i + 1];
+    atom_pairs<RB, FORM, CHG, DSLV>(p0, c, c0, e0, lx, ly, lz, e);
+    atom_pairs<RB, FORM, CHG, DSLV>(p1, c, c0, e0, lx, ly, lz, e);
+  }
+  if (i < i1) atom_pairs<RB, FORM, CHG, DSLV>(prot[i], c, c0, e0, lx, ly, lz, e);
 }
 
 // One protein group: its pairs are summed into a fresh partial sum that is
