@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "bude_internal.h"
+#include "fasten_batch.h"
 #include "fasten_plain.h"
 #include "hip_util.h"
 #include "mlp.h"
@@ -52,6 +53,10 @@ struct bude_screen {
   // candidate poses
   size_t ncand = 0;
   DeviceBuffer cand, pred, sel_idx, sel_n, gathered, exact;
+  // rescoring: the persistent batched kernel over tiles (0, 64 j) of the
+  // gathered poses, one ligand whose pose count is the selection's size
+  DeviceBuffer rs_tiles, rs_meta, rs_counter;
+  bude::LigandMeta rs_meta_host{};
   bude::sel::Plan plan;  // one segment: the loaded poses, k = ceil(keep_fraction * n)
   size_t plan_n = 0, plan_k = 0;
   size_t nsel = 0;
@@ -92,6 +97,38 @@ int upload_poses(DeviceBuffer &d, const float *const poses[6], size_t n) {
                                  "hipMemcpy poses"))
       return st;
   return BUDE_OK;
+}
+
+// Exact energies of the nsel gathered poses into exact[0, nsel): the
+// persistent batched kernel (bit-identical to the plain kernel) on tiles of
+// 64 poses pulled from a queue by a grid of 3 waves per SIMD, so the SIMDs
+// finish together on a pass of a few thousand tiles.  The stream is idle
+// here (the selection's size was just read back), so the one-ligand table is
+// uploaded now.
+int rescore(bude_screen *s, size_t nsel) {
+  if (nsel == 0) return BUDE_OK;
+  s->rs_meta_host = bude::LigandMeta{};
+  s->rs_meta_host.natlig = int32_t(s->natlig);
+  s->rs_meta_host.nposes = int32_t(nsel);
+  if (int st = bude::hip_check(hipMemcpyAsync(s->rs_meta.ptr, &s->rs_meta_host, sizeof(bude::LigandMeta),
+                                              hipMemcpyHostToDevice, s->stream),
+                               "hipMemcpyAsync rescoring table"))
+    return st;
+  bude::BatchArgs a{};
+  a.protein = static_cast<const float4 *>(s->protein.ptr);
+  a.group_begin = static_cast<const int32_t *>(s->group_begin.ptr);
+  a.coef = static_cast<const bude::PairCoef *>(s->coef.ptr);
+  a.ligands = static_cast<const float4 *>(s->ligand.ptr);
+  a.meta = static_cast<const bude::LigandMeta *>(s->rs_meta.ptr);
+  a.tiles = static_cast<const int2 *>(s->rs_tiles.ptr);
+  a.poses = static_cast<const float *>(s->gathered.ptr);
+  a.energies = static_cast<float *>(s->exact.ptr);
+  a.counter = static_cast<unsigned int *>(s->rs_counter.ptr);
+  a.stride = nsel;
+  a.ngroups = int(s->ngroups);
+  a.ntiles = int((nsel + 63) / 64);
+  a.form = bude::plain_form();
+  return bude::hip_check(bude::launch_batch(a, 1, 64, std::max(1, s->cus * 4 * 3), s->stream), "rescore launch");
 }
 
 // Surrogate predictions (normalised target units) for n poses whose 6 rows
@@ -259,6 +296,13 @@ extern "C" int bude_mi355x_screen_load(bude_screen *s, const float *const poses[
   if (!st) st = s->sel_idx.alloc(n * 4);
   if (!st) st = s->gathered.alloc(6 * n * 4);
   if (!st) st = s->exact.alloc(n * 4);
+  if (!st) {
+    std::vector<int2> t((n + 63) / 64);
+    for (size_t j = 0; j < t.size(); j++) t[j] = make_int2(0, int(64 * j));
+    st = s->rs_tiles.assign(t.data(), t.size() * sizeof(int2));
+  }
+  if (!st) st = s->rs_counter.alloc(sizeof(unsigned int));
+  if (!st) st = s->rs_meta.alloc(sizeof(bude::LigandMeta));
   return st;
 }
 
@@ -308,9 +352,7 @@ extern "C" int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res) {
                                                          static_cast<float *>(s->gathered.ptr), nsel, st_),
                                  "gather"))
       return st;
-    const bude::PlainArgs pa = s->plain(static_cast<const float *>(s->gathered.ptr), nsel, nsel,
-                                        static_cast<float *>(s->exact.ptr));
-    if (int st = bude::hip_check(bude::launch_plain(pa, 256, st_), "rescore launch")) return st;
+    if (int st = rescore(s, nsel)) return st;
   }
   if (int st = bude::hip_check(hipEventRecord(s->ev[3], st_), "hipEventRecord")) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize")) return st;
