@@ -31,13 +31,14 @@ namespace {
 using bude::DeviceBuffer;
 
 // Persistent batched launch shape used for every exact pass: tiles of 64
-// poses, 64-lane work-groups.  The grid depth follows the tile count: a pass
-// of fewer than 16 tiles per SIMD (the rescoring of a few per cent of the
-// poses) runs 3 waves per SIMD, so every wave pulls about two tiles from the
-// heaviest-first queue and the SIMDs finish together; larger passes run 4,
-// enough to keep the VALU busy (measured on ragged 64-ligand sets: 12.6 vs
-// 15.3 ms at 8 waves per SIMD for 5,243 tiles, 136.2 vs 137.3 ms for 65,536).
-constexpr int kBatchPpwi = 1, kBatchWg = 64;
+// poses on a grid pinned to the same number of waves on every SIMD.  The
+// depth follows the tile count: a pass of fewer than 16 tiles per SIMD (the
+// rescoring of a few per cent of the poses) runs 3 waves per SIMD, so every
+// wave pulls about two tiles from the heaviest-first queue and the SIMDs
+// finish together; larger passes run 4, enough to keep the VALU busy
+// (measured on ragged 64-ligand sets: 12.6 vs 15.3 ms at 8 waves per SIMD
+// for 5,243 tiles, 136.2 vs 137.3 ms for 65,536).
+constexpr int kBatchPpwi = 1;
 
 int batch_waves_per_simd(int ntiles, int cus) { return ntiles < 16 * 4 * cus ? 3 : 4; }
 
@@ -140,6 +141,7 @@ Norm norm_of(std::vector<float> e, float clip_q, float tail_q) {
 
 struct bude_combined {
   int device = 0, cus = 0;
+  size_t lds_per_cu = 0;
   BudeCombinedConfig cfg{};
   std::vector<BudeFFParams> ff;
   // ligands (fixed for the object's life)
@@ -246,8 +248,9 @@ int launch_exact(bude_combined *c, const float *poses, size_t stride, float *ene
   a.ngroups = int(c->ngroups);
   a.ntiles = ntiles;
   a.form = bude::plain_form();
-  const int blocks = std::max(1, c->cus * 4 * batch_waves_per_simd(ntiles, c->cus) / (kBatchWg / 64));
-  return bude::hip_check(bude::launch_batch(a, kBatchPpwi, kBatchWg, blocks, c->stream), "combined batch launch");
+  const bude::PinnedGrid g = bude::pinned_grid(c->cus, c->lds_per_cu, batch_waves_per_simd(ntiles, c->cus));
+  a.lds_bytes = g.lds_bytes;
+  return bude::hip_check(bude::launch_batch(a, kBatchPpwi, g.wgsize, g.blocks, c->stream), "combined batch launch");
 }
 
 int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride,
@@ -399,6 +402,7 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
   hipDeviceProp_t props{};
   if (int st = bude::hip_check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties")) return st;
   c->cus = props.multiProcessorCount;
+  c->lds_per_cu = props.maxSharedMemoryPerMultiProcessor;
   c->ff.assign(forcefield, forcefield + ntypes);
   c->nlig = nligands;
   c->total = total;

@@ -42,6 +42,7 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 struct bude_screen {
   int device = 0;
   int cus = 0;
+  size_t lds_per_cu = 0;
   BudeScreenConfig cfg{};
   // plain-kernel deck (labels, rescoring, fallback)
   size_t natlig = 0, ngroups = 0;
@@ -101,8 +102,8 @@ int upload_poses(DeviceBuffer &d, const float *const poses[6], size_t n) {
 
 // Exact energies of the nsel gathered poses into exact[0, nsel): the
 // persistent batched kernel (bit-identical to the plain kernel) on tiles of
-// 64 poses pulled from a queue by a grid of 3 waves per SIMD, so the SIMDs
-// finish together on a pass of a few thousand tiles.  The stream is idle
+// 64 poses pulled from a queue by a grid pinned to 3 waves on every SIMD, so
+// the SIMDs finish together on a pass of a few thousand tiles.  The stream is idle
 // here (the selection's size was just read back), so the one-ligand table is
 // uploaded now.
 int rescore(bude_screen *s, size_t nsel) {
@@ -128,7 +129,9 @@ int rescore(bude_screen *s, size_t nsel) {
   a.ngroups = int(s->ngroups);
   a.ntiles = int((nsel + 63) / 64);
   a.form = bude::plain_form();
-  return bude::hip_check(bude::launch_batch(a, 1, 64, std::max(1, s->cus * 4 * 3), s->stream), "rescore launch");
+  const bude::PinnedGrid g = bude::pinned_grid(s->cus, s->lds_per_cu, 3);
+  a.lds_bytes = g.lds_bytes;
+  return bude::hip_check(bude::launch_batch(a, 1, g.wgsize, g.blocks, s->stream), "rescore launch");
 }
 
 // Surrogate predictions (normalised target units) for n poses whose 6 rows
@@ -165,6 +168,7 @@ extern "C" int bude_mi355x_screen_create(int device, const BudeAtom *protein, si
   hipDeviceProp_t props{};
   if (int st = bude::hip_check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties")) return st;
   s->cus = props.multiProcessorCount;
+  s->lds_per_cu = props.maxSharedMemoryPerMultiProcessor;
   s->natlig = natlig;
   const bude::ProteinGroups groups = bude::group_protein(protein, natpro);
   s->ngroups = groups.ngroups();

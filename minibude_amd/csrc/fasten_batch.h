@@ -32,9 +32,24 @@ struct BatchArgs {
   int ngroups;
   int ntiles;
   int form;                     // hinge form (plain_form())
+  unsigned lds_bytes;           // dynamic LDS per work-group, unused by the kernel: sized so that exactly
+                                // n work-groups fit on a CU, it pins the waves per SIMD (0: no pinning)
 };
 
 bool batch_shape_supported(int rb, int wgsize);
+
+// A persistent grid pinned to `wps` waves on every SIMD: work-groups of 256
+// work-items (one wave per SIMD), `wps` of them per CU, each carrying unused
+// dynamic LDS sized so that no more than `wps` fit on a CU.  Left to itself
+// the dispatcher can stack waves unevenly over the SIMDs of a launch that
+// does not fill the machine, and a SIMD holding one or two waves runs its
+// tiles at a fraction of the rate of one holding four.  lds_bytes is 0 (no
+// pinning) when the device's LDS does not allow it.
+struct PinnedGrid {
+  int wgsize, blocks;
+  unsigned lds_bytes;
+};
+PinnedGrid pinned_grid(int cus, size_t lds_per_cu, int wps);
 
 // Counter reset + one persistent launch of `blocks` work-groups of `wgsize`
 // work-items, each wave scoring tiles of 64 * rb poses.

@@ -14,6 +14,8 @@
 // of a ligand is masked).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bude_internal.h"
 #include "fasten_batch.h"
 #include "fasten_core.h"
@@ -77,13 +79,13 @@ __global__ __launch_bounds__(WG) void fasten_batch_kernel(
 template <int RB, int WG>
 hipError_t launch_rb_wg(const BatchArgs &a, int blocks, hipStream_t s) {
   if (a.form == kFormD)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormD>), dim3(unsigned(blocks)), dim3(WG), 0, s,
-                       a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.counter,
-                       a.stride, a.ngroups, a.ntiles);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormD>), dim3(unsigned(blocks)), dim3(WG),
+                       a.lds_bytes, s, a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses,
+                       a.energies, a.counter, a.stride, a.ngroups, a.ntiles);
   else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormB>), dim3(unsigned(blocks)), dim3(WG), 0, s,
-                       a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.counter,
-                       a.stride, a.ngroups, a.ntiles);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormB>), dim3(unsigned(blocks)), dim3(WG),
+                       a.lds_bytes, s, a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses,
+                       a.energies, a.counter, a.stride, a.ngroups, a.ntiles);
   return hipGetLastError();
 }
 
@@ -103,6 +105,19 @@ hipError_t launch_rb(const BatchArgs &a, int wg, int blocks, hipStream_t s) {
 bool batch_shape_supported(int rb, int wgsize) {
   return (rb == 1 || rb == 2 || rb == 4 || rb == 8) && (wgsize == 64 || wgsize == 128 || wgsize == 256 ||
                                                          wgsize == 512);
+}
+
+PinnedGrid pinned_grid(int cus, size_t lds_per_cu, int wps) {
+  PinnedGrid g{256, std::max(1, cus * wps), 0u};
+  if (wps < 1) return g;
+  // whole KiB, at most the 64 KiB a work-group may allocate, with 2 KiB of
+  // slack per work-group for the allocation granularity: wps work-groups fit
+  // and wps + 1 do not
+  const size_t per = lds_per_cu / size_t(wps);
+  if (per <= 3072) return g;
+  const size_t b = std::min<size_t>(per - 2048, 65536) & ~size_t(1023);
+  if (size_t(wps) * (b + 2048) <= lds_per_cu && size_t(wps + 1) * b > lds_per_cu) g.lds_bytes = unsigned(b);
+  return g;
 }
 
 hipError_t launch_batch(const BatchArgs &a, int rb, int wgsize, int blocks, hipStream_t s) {
