@@ -318,6 +318,47 @@ int bude_synth_ligand(uint64_t seed, size_t natlig, size_t ntypes, float radius,
  * U[-max_trans, max_trans).  Pose i is the same for every first/count. */
 int bude_synth_poses(uint64_t seed, size_t first, size_t count, float max_trans, float *rows6, size_t stride);
 
+/* ---- deck tooling: decks from real molecules (the makedeck program,
+ *      makedeck/main.cpp:49-234).  Host only: no device is touched.  A deck
+ *      built from the same bhff/mol2 files, pose seed and pose count is
+ *      byte-identical to makedeck's (the bm2 deck is reproduced exactly). ---- */
+typedef struct bude_forcefield bude_forcefield;
+/* Reads a BUDE bhff forcefield (bude::readForceField,
+ * makedeck/input-utils.cpp:72-119): every 2-column "RESIDUE COUNT" line is
+ * followed by COUNT 9-column atom rows "type hbchar radius hphb _ _ _ scaling
+ * elsc"; an entry's parameters are {hbchar code, radius*scaling, hphb,
+ * elsc}.  Lines whose first non-blank character is '#' or '%' are comments.
+ * Malformed rows fail with BUDE_ERR_INVALID_ARGUMENT and "path:line". */
+int bude_forcefield_read(const char *bhff_path, bude_forcefield **out);
+/* Residue groups and total entries. */
+int bude_forcefield_info(const bude_forcefield *ff, size_t *nresidues, size_t *nentries);
+/* The deck's forcefield rows: all entries, residue groups in residue-name
+ * order, entries in file order (makedeck/main.cpp:146-149).  out may be NULL
+ * to query *n; otherwise cap must be >= the entry count. */
+int bude_forcefield_params(const bude_forcefield *ff, BudeFFParams *out, size_t cap, size_t *n);
+int bude_forcefield_destroy(bude_forcefield *ff);
+/* Heavy atoms of a Tripos mol2 file against a forcefield (bude::readMol2,
+ * makedeck/input-utils.cpp:122-202): the rows between "@<TRIPOS>ATOM" and
+ * the next "@<TRIPOS>" line, hydrogens (type "H"/"h") skipped.  With a
+ * one-residue forcefield an atom is looked up by its Sybyl type (column 6;
+ * rows must have 9 columns), otherwise by residue (first 3 characters of
+ * column 8) and atom name (column 2; rows must have 10 columns).  An atom's
+ * type is the entry's index within its residue group, as makedeck writes
+ * it.  "@<BUDE>CONF n" blocks after the atoms are counted in
+ * *nconformations (may be NULL).  atoms may be NULL to query *natoms;
+ * otherwise cap must be >= the heavy-atom count. */
+int bude_mol2_read(const bude_forcefield *ff, const char *mol2_path, BudeAtom *atoms, size_t cap, size_t *natoms,
+                   size_t *nconformations);
+/* nposes distinct poses drawn from the grid values[k][0..counts[k]) of the
+ * six pose fields (bude::generatePoses, makedeck/input-utils.cpp:205-262):
+ * indices drawn per field in the order tilt, roll, pan, x, y, z from
+ * std::mt19937(seed), duplicates dropped, poses emitted in the order of
+ * makedeck's hash set.  Angle values are degrees and come out in radians.
+ * rows[k] receives nposes floats.  Fails with BUDE_ERR_INVALID_ARGUMENT when
+ * nposes exceeds the number of grid combinations. */
+int bude_poses_generate(size_t nposes, uint64_t seed, const float *const values[6], const size_t counts[6],
+                        float *const rows[6]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@ INCLUDE = ROOT / "include"
 OBJ = PKG / "lib" / "obj"
 LIB = PKG / "lib" / "libbude_mi355x.so"
 DRIVER = PKG / "bin" / "bude-mi355x"
+MAKEDECK = PKG / "bin" / "bude-makedeck"
 
 ARCH = "gfx950"
 # Packed FP32 (v_pk_*_f32) is switched off for all device code.
@@ -38,6 +39,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra"]
 LIB_SOURCES = [
     "prep.cpp",
     "deck_synth.cpp",
+    "deck_io.cpp",
     "fasten_plain.hip",
     "capi_plain.hip",
     "fasten_batch.hip",
@@ -100,10 +102,22 @@ def build_driver() -> Path | None:
     return DRIVER
 
 
+def build_makedeck() -> Path:
+    """bin/bude-makedeck: decks from bhff/mol2 files (the makedeck program)."""
+    srcs = sorted((HOST / "makedeck").glob("*.cpp"))
+    MAKEDECK.parent.mkdir(parents=True, exist_ok=True)
+    tmp = MAKEDECK.with_name(MAKEDECK.name + ".tmp")
+    _run(["g++", *HOST_FLAGS, f"-I{INCLUDE}", *map(str, srcs), "-o", str(tmp),
+          f"-L{LIB.parent}", "-lbude_mi355x", "-Wl,-rpath,$ORIGIN/../lib"])
+    os.replace(tmp, MAKEDECK)
+    return MAKEDECK
+
+
 def build(lib_only: bool = False) -> None:
     build_library()
     if not lib_only:
         build_driver()
+        build_makedeck()
 
 
 def main(argv: list[str] | None = None) -> int:
@@ -111,7 +125,7 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--lib-only", action="store_true")
     args = ap.parse_args(argv)
     build(lib_only=args.lib_only)
-    print(f"built {LIB}" + ("" if args.lib_only else f" and {DRIVER}"))
+    print(f"built {LIB}" + ("" if args.lib_only else f", {DRIVER} and {MAKEDECK}"))
     return 0
 
 

@@ -72,6 +72,12 @@ EXPORTED = (
     "bude_synth_protein",
     "bude_synth_ligand",
     "bude_synth_poses",
+    "bude_forcefield_read",
+    "bude_forcefield_info",
+    "bude_forcefield_params",
+    "bude_forcefield_destroy",
+    "bude_mol2_read",
+    "bude_poses_generate",
 )
 
 
@@ -135,6 +141,12 @@ def _declare(lib: ctypes.CDLL) -> None:
         "bude_synth_protein": (_I, [ctypes.c_uint64, _S, _S, ctypes.c_float, _P]),
         "bude_synth_ligand": (_I, [ctypes.c_uint64, _S, _S, ctypes.c_float, _P]),
         "bude_synth_poses": (_I, [ctypes.c_uint64, _S, _S, ctypes.c_float, _P, _S]),
+        "bude_forcefield_read": (_I, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+        "bude_forcefield_info": (_I, [_P, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+        "bude_forcefield_params": (_I, [_P, _P, _S, ctypes.POINTER(_S)]),
+        "bude_forcefield_destroy": (_I, [_P]),
+        "bude_mol2_read": (_I, [_P, ctypes.c_char_p, _P, _S, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+        "bude_poses_generate": (_I, [_S, ctypes.c_uint64, ctypes.POINTER(_P), ctypes.POINTER(_S), ctypes.POINTER(_P)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

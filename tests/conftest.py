@@ -22,8 +22,10 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     lib = ROOT / "minibude_amd" / "lib" / "libbude_mi355x.so"
     drv = ROOT / "minibude_amd" / "bin" / "bude-mi355x"
+    mkdeck = ROOT / "minibude_amd" / "bin" / "bude-makedeck"
     host_srcs = list((ROOT / "minibude_amd" / "host").glob("*.cpp"))
-    if not lib.exists() or (host_srcs and not drv.exists()):
+    mkdeck_srcs = list((ROOT / "minibude_amd" / "host" / "makedeck").glob("*.cpp"))
+    if not lib.exists() or (host_srcs and not drv.exists()) or (mkdeck_srcs and not mkdeck.exists()):
         # Build only when nothing is built (a machine without the toolchain runs the shipped build).
         from minibude_amd import build
         build.build()
