@@ -20,12 +20,14 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ bf16 to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return bf16(0x7fc0);  // NaN
-  u += 0x7fffu + ((u >> 16) & 1u);                            // round to nearest even
-  return bf16(u >> 16);
+// FP32 -> BF16, round to nearest even: v_cvt_pk_bf16_f32, two values per
+// instruction (lo in bits 0-15).
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float floatx2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(floatx2_t{lo, hi}, bf16x2_t));
 }
+__device__ __forceinline__ bf16 to_bf16(float f) { return bf16(pack_bf16(f, 0.f) & 0xffffu); }
 
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float(uint32_t(h) << 16); }
 
@@ -301,14 +303,18 @@ __global__ void gather_poses_kernel(const float *__restrict__ src, size_t src_st
 
 // ---- fused inference ---------------------------------------------------------
 // One work-group (8 waves) scores tiles of kTile poses through the whole
-// network with the activations in one LDS buffer: features -> 3 x (MFMA GEMM
-// + bias + GELU) -> output dot product; wave w owns output columns
-// [32w, 32w + 32) of every hidden layer and holds them in registers until
-// every wave has read the layer's input, then overwrites the buffer.  Weights
-// stream from L2 (k-contiguous rows of W^T), the next k-step's fragments
-// loaded while the current one multiplies; 128-pose tiles halve the weight
-// traffic per pose against 64-pose tiles, and one 66 KiB buffer leaves room
-// for two work-groups per CU.
+// network with the activations in one LDS buffer (pose-major rows):
+// features -> 3 x (MFMA GEMM + bias + GELU) -> output layer, also on MFMA.
+// A hidden layer computes its transpose, D[neuron][pose] = W . act^T, so the
+// weights are the A operand (streamed from L2 as k-contiguous rows of W^T,
+// the next k-step's fragments loaded while the current one multiplies) and
+// the activations the B operand (16-byte LDS reads along k); a lane's four
+// results are then four consecutive neurons of one pose, which bias, GELU,
+// two packed BF16 conversions and one 8-byte LDS store finish.  Wave w owns
+// neurons [32w, 32w + 32) and holds them in registers until every wave has
+// read the layer's input, then overwrites the buffer.  128-pose tiles halve
+// the weight traffic per pose against 64-pose tiles, and one 66 KiB buffer
+// leaves room for two work-groups per CU.
 constexpr int kTile = 128;
 constexpr int kInferThreads = 512;
 constexpr int kLdsRow = kHidden + 8;  // bf16 per LDS row (padding staggers banks)
@@ -319,15 +325,15 @@ template <int K>
 __device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *__restrict__ wt,
                                             const float *__restrict__ bias, int lane, int wave) {
   const int r = lane & 15, grp = lane >> 4;
-  floatx4 acc[kTile / 16][2];
+  floatx4 acc[2][kTile / 16];  // [neuron block of 16][pose block of 16]
 #pragma unroll
-  for (int i = 0; i < kTile / 16; i++)
+  for (int j = 0; j < 2; j++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < kTile / 16; i++) acc[j][i] = floatx4{0.f, 0.f, 0.f, 0.f};
   const bf16 *wrow = wt + size_t(32 * wave + r) * K + 8 * grp;
-  bf16x8 fb[2];
+  bf16x8 fw[2];
 #pragma unroll
-  for (int j = 0; j < 2; j++) fb[j] = load8(wrow + size_t(16 * j) * K);
+  for (int j = 0; j < 2; j++) fw[j] = load8(wrow + size_t(16 * j) * K);
 #pragma unroll 1
   for (int k = 0; k < K; k += 32) {
     bf16x8 fn[2];
@@ -336,40 +342,53 @@ __device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *
     for (int j = 0; j < 2; j++) fn[j] = load8(wrow + size_t(16 * j) * K + kn);
 #pragma unroll
     for (int i = 0; i < kTile / 16; i++) {
+      // (two halves: the compiler would otherwise hold all eight B fragments)
+      if (i == kTile / 32) __builtin_amdgcn_sched_barrier(0);
       const bf16x8 fa = lds_load8(buf + (16 * i + r) * kLdsRow + k + 8 * grp);
 #pragma unroll
-      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; j++) acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw[j], fa, acc[j][i], 0, 0, 0);
     }
 #pragma unroll
-    for (int j = 0; j < 2; j++) fb[j] = fn[j];
+    for (int j = 0; j < 2; j++) fw[j] = fn[j];
   }
   __syncthreads();  // every wave has read this layer's input
 #pragma unroll
   for (int j = 0; j < 2; j++) {
-    const int n = 32 * wave + 16 * j + r;
-    const float bn = bias[n];
+    const int n0 = 32 * wave + 16 * j + 4 * grp;  // this lane's four neurons
+    const float4 b4 = *reinterpret_cast<const float4 *>(bias + n0);
 #pragma unroll
-    for (int i = 0; i < kTile / 16; i++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) buf[(16 * i + 4 * grp + q) * kLdsRow + n] = to_bf16(gelu(acc[i][j][q] + bn));
+    for (int i = 0; i < kTile / 16; i++) {
+      const floatx4 z = acc[j][i];
+      uint2 v;
+      v.x = pack_bf16(gelu(z[0] + b4.x), gelu(z[1] + b4.y));
+      v.y = pack_bf16(gelu(z[2] + b4.z), gelu(z[3] + b4.w));
+      *reinterpret_cast<uint2 *>(buf + (16 * i + r) * kLdsRow + n0) = v;
+    }
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kInferThreads) void mlp_infer_kernel(
+// (4 waves per SIMD, two work-groups per CU: within the 128 registers this
+// allows, the compiler spills a few kernel-lifetime values once and reloads
+// them a few times per layer, which measured faster than the 146 registers
+// and one work-group per CU it picks otherwise)
+__global__ __launch_bounds__(kInferThreads, 4) void mlp_infer_kernel(
     const float *__restrict__ poses, size_t stride, size_t n, const bf16 *__restrict__ w0t,
     const float *__restrict__ b0, const bf16 *__restrict__ w1t, const float *__restrict__ b1,
     const bf16 *__restrict__ w2t, const float *__restrict__ b2, const bf16 *__restrict__ w3,
     const float *__restrict__ b3, float *__restrict__ out) {
   __shared__ bf16 act[kTile * kLdsRow];
+  __shared__ bf16 w3s[kHidden];  // column 0 of W_3: the output neuron's weights
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < kHidden) w3s[tid] = w3[size_t(tid) * kOut];
+  const float bias3 = b3[0];
   const size_t ntiles = (n + kTile - 1) / kTile;
   for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const size_t base = tile * kTile;
-    // features of the tile's poses (rows past n are zero)
+    // features of the tile's poses (rows past n are zero), as packed pairs
     if (tid < kTile) {
       const size_t i = base + tid;
-      bf16 *row = act + tid * kLdsRow;
+      uint32_t *row = reinterpret_cast<uint32_t *>(act + tid * kLdsRow);
       float f[kIn];
       if (i < n) {
         pose_features(poses, stride, i, f);
@@ -378,22 +397,25 @@ __global__ __launch_bounds__(kInferThreads) void mlp_infer_kernel(
         for (int c = 0; c < kIn; c++) f[c] = 0.f;
       }
 #pragma unroll
-      for (int c = 0; c < kIn; c++) row[c] = to_bf16(f[c]);
+      for (int c = 0; c < kIn; c += 2) row[c / 2] = pack_bf16(f[c], f[c + 1]);
     }
     __syncthreads();
     fused_layer<kIn>(act, w0t, b0, lane, wave);
     fused_layer<kHidden>(act, w1t, b1, lane, wave);
     fused_layer<kHidden>(act, w2t, b2, lane, wave);
-    // output layer: 4 threads per pose, 64 inputs each, combined by shuffles
+    // output layer on MFMA: wave w scores poses [16w, 16w + 16) as row 0 of
+    // D = W_3^T . act^T (A rows 1-15 are zero); lanes 0-15 hold the results
     {
-      const int row = tid >> 2, part = tid & 3;
-      const bf16 *a = act + row * kLdsRow + 64 * part;
-      float sum = 0.f;
-#pragma unroll 8
-      for (int c = 0; c < 64; c++) sum += from_bf16(a[c]) * from_bf16(w3[size_t(64 * part + c) * kOut]);
-      sum += __shfl_xor(sum, 1);
-      sum += __shfl_xor(sum, 2);
-      if (part == 0 && base + row < n) out[base + row] = sum + b3[0];
+      const int r = lane & 15, grp = lane >> 4;
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+      const bf16x8 zero = bf16x8{};
+#pragma unroll
+      for (int k = 0; k < kHidden; k += 32) {
+        const bf16x8 fb = lds_load8(act + (16 * wave + r) * kLdsRow + k + 8 * grp);
+        const bf16x8 fa = r == 0 ? lds_load8(w3s + k + 8 * grp) : zero;
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc, 0, 0, 0);
+      }
+      if (lane < 16 && base + 16 * wave + lane < n) out[base + 16 * wave + lane] = acc[0] + bias3;
     }
     __syncthreads();
   }
