@@ -174,7 +174,7 @@ typedef struct BudeScreenResult {
   double total_ms;    /* host wall time of the whole screen */
   float score_ms;     /* surrogate (or exhaustive plain kernel in fallback) */
   float select_ms;    /* top-k selection */
-  float rescore_ms;   /* gather + exact plain-kernel rescoring */
+  float rescore_ms;   /* gather + exact rescoring (the plain kernel's energies, bit for bit) */
   uint64_t nposes;
   uint64_t nselected;
   int32_t fallback;
