@@ -111,11 +111,12 @@ int main(int argc, char **argv) {
   if (out_dir.empty()) die("--out is a required parameter");
 
   const fs::path dir(out_dir);
-  if (fs::is_regular_file(dir) || fs::is_directory(dir)) {
-    if (!force) die("Output path already exists (either a file or directory), pass --force to delete/overwrite them");
-    fs::remove_all(dir);
-  }
   std::error_code ec;
+  if (fs::is_regular_file(dir, ec) || fs::is_directory(dir, ec)) {
+    if (!force) die("Output path already exists (either a file or directory), pass --force to delete/overwrite them");
+    fs::remove_all(dir, ec);
+    if (ec) die("Unable to delete " + dir.string() + ": " + ec.message());
+  }
   if (!fs::create_directories(dir, ec)) die("Unable to create output directory: " + dir.string());
 
   bude_forcefield *ff = nullptr;
