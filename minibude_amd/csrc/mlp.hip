@@ -351,6 +351,10 @@ __device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *
 #pragma unroll
     for (int j = 0; j < 2; j++) fw[j] = fn[j];
   }
+  // The epilogue (bias, GELU, conversion) is the VALU-heavy phase; raising
+  // the wave's issue priority through it lets the other work-group's MFMA
+  // waves fill the gaps instead of delaying it (-4 to -5 % per launch).
+  __builtin_amdgcn_s_setprio(2);
   __syncthreads();  // every wave has read this layer's input
 #pragma unroll
   for (int j = 0; j < 2; j++) {
@@ -365,6 +369,7 @@ __device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *
       *reinterpret_cast<uint2 *>(buf + (16 * i + r) * kLdsRow + n0) = v;
     }
   }
+  __builtin_amdgcn_s_setprio(0);
   __syncthreads();
 }
 
