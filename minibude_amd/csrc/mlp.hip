@@ -309,8 +309,9 @@ __global__ void gather_poses_kernel(const float *__restrict__ src, size_t src_st
 // weights are the A operand (streamed from L2 as k-contiguous rows of W^T,
 // the next k-step's fragments loaded while the current one multiplies) and
 // the activations the B operand (16-byte LDS reads along k); a lane's four
-// results are then four consecutive neurons of one pose, which bias, GELU,
-// two packed BF16 conversions and one 8-byte LDS store finish.  Wave w owns
+// results are then four consecutive neurons of one pose (accumulated onto
+// their biases), which GELU, two packed BF16 conversions and one 8-byte LDS
+// store finish.  Wave w owns
 // neurons [32w, 32w + 32) and holds them in registers until every wave has
 // read the layer's input, then overwrites the buffer.  128-pose tiles halve
 // the weight traffic per pose against 64-pose tiles, and one 66 KiB buffer
@@ -325,11 +326,13 @@ template <int K>
 __device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *__restrict__ wt,
                                             const float *__restrict__ bias, int lane, int wave) {
   const int r = lane & 15, grp = lane >> 4;
-  floatx4 acc[2][kTile / 16];  // [neuron block of 16][pose block of 16]
+  floatx4 acc[2][kTile / 16];  // [neuron block of 16][pose block of 16], from the biases
 #pragma unroll
-  for (int j = 0; j < 2; j++)
+  for (int j = 0; j < 2; j++) {
+    const float4 b4 = *reinterpret_cast<const float4 *>(bias + 32 * wave + 16 * j + 4 * grp);
 #pragma unroll
-    for (int i = 0; i < kTile / 16; i++) acc[j][i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < kTile / 16; i++) acc[j][i] = floatx4{b4.x, b4.y, b4.z, b4.w};
+  }
   const bf16 *wrow = wt + size_t(32 * wave + r) * K + 8 * grp;
   bf16x8 fw[2];
 #pragma unroll
@@ -351,7 +354,7 @@ __device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *
 #pragma unroll
     for (int j = 0; j < 2; j++) fw[j] = fn[j];
   }
-  // The epilogue (bias, GELU, conversion) is the VALU-heavy phase; raising
+  // The epilogue (GELU, conversion) is the VALU-heavy phase; raising
   // the wave's issue priority through it lets the other work-group's MFMA
   // waves fill the gaps instead of delaying it (-4 to -5 % per launch).
   __builtin_amdgcn_s_setprio(2);
@@ -359,13 +362,12 @@ __device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *
 #pragma unroll
   for (int j = 0; j < 2; j++) {
     const int n0 = 32 * wave + 16 * j + 4 * grp;  // this lane's four neurons
-    const float4 b4 = *reinterpret_cast<const float4 *>(bias + n0);
 #pragma unroll
     for (int i = 0; i < kTile / 16; i++) {
       const floatx4 z = acc[j][i];
       uint2 v;
-      v.x = pack_bf16(gelu(z[0] + b4.x), gelu(z[1] + b4.y));
-      v.y = pack_bf16(gelu(z[2] + b4.z), gelu(z[3] + b4.w));
+      v.x = pack_bf16(gelu(z[0]), gelu(z[1]));
+      v.y = pack_bf16(gelu(z[2]), gelu(z[3]));
       *reinterpret_cast<uint2 *>(buf + (16 * i + r) * kLdsRow + n0) = v;
     }
   }
