@@ -158,3 +158,31 @@ def test_model_interface_like_reference_driver(gpu_device, bm1):
     assert all(h >= dv * 0.5 for h, dv in zip(s.kernel_ms, s.device_ms))
     r = metrics.evaluate(bm1.ref_energies, s.energies, s.kernel_ms, 2, 3, bm1.nposes, bm1.natlig, bm1.natpro, 16)
     assert r.valid and r.interactions_per_sec > 0
+
+
+def test_every_group_size(gpu_device):
+    # The pair loop walks each protein-type group in pipelined blocks of four
+    # atoms with a remainder of up to seven; give the groups every size from
+    # 0 to 23 (and some larger ones) so each loop exit is taken, and check
+    # every register-block width against the FP64 oracle.
+    ntypes = 34
+    sizes = [t if t < 24 else 24 + 3 * (t - 24) for t in range(ntypes)]
+    pro = D.synth_protein(31, sum(sizes), ntypes)
+    pro["type"] = np.repeat(np.arange(ntypes, dtype=np.int32), sizes)
+    pro = pro[np.random.default_rng(5).permutation(len(pro))]  # groups are formed by the library
+    d = D.Deck(pro, D.synth_ligand(32, 12, ntypes), D.synth_forcefield(31, ntypes), D.synth_poses(33, 2000))
+    e64, mag = O.fasten_f64_mag(d, 0, d.nposes)
+    eps = 2.0 ** -24
+    with Context(d, gpu_device) as c:
+        first = None
+        for p, w in ((1, 64), (2, 128), (4, 256), (8, 64)):
+            e = c.fasten(p, w).energies
+            ok = mag > 0
+            assert np.all(e[~ok] == 0.0)
+            noise = float(np.max(np.abs(e.astype(np.float64) - e64)[ok] / mag[ok]) / eps)
+            print(f"group sizes 0-23 and larger, ppwi {p} wgsize {w}: max |E-E64|/M = {noise:.2f} eps", flush=True)
+            assert noise < 16.0
+            if first is None:
+                first = e
+            else:  # the register block only changes which lane holds a pose, never the sums
+                assert np.array_equal(e.view(np.uint32), first.view(np.uint32))
