@@ -273,6 +273,13 @@ int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePairResult *r
  * the ligand's pose set) and their exact energies. */
 int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, uint32_t *idx, float *energies, size_t cap,
                                   size_t *n);
+/* The last screen's sampled poses of one pair (what the MLP trained on, or
+ * the field was validated on): pose indices within the ligand's pose set,
+ * in sampling order, and their exact energies, which come from a split
+ * kernel (four lanes per pose) and match the tiled kernel's to rounding.
+ * idx and energies may both be NULL to query *n. */
+int bude_mi355x_combined_labels(const bude_combined *c, size_t ligand, uint32_t *idx, float *energies, size_t cap,
+                                size_t *n);
 /* Surrogate score of every pose of one pair (exact energies for a fallback
  * pair): standardised units for the MLP, energy units for the field. */
 int bude_mi355x_combined_scores(bude_combined *c, size_t ligand, float *out);

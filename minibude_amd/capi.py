@@ -72,6 +72,7 @@ EXPORTED = (
     "bude_synth_protein",
     "bude_synth_ligand",
     "bude_synth_poses",
+    "bude_mi355x_combined_labels",
     "bude_forcefield_read",
     "bude_forcefield_info",
     "bude_forcefield_params",

@@ -76,6 +76,7 @@ _SIG = {
     "bude_mi355x_combined_set_keep": [_P, _c.c_float],
     "bude_mi355x_combined_pair": [_P, _S, _P],
     "bude_mi355x_combined_selected": [_P, _S, _P, _P, _S, _c.POINTER(_S)],
+    "bude_mi355x_combined_labels": [_P, _S, _P, _P, _S, _c.POINTER(_S)],
     "bude_mi355x_combined_scores": [_P, _S, _P],
     "bude_mi355x_combined_exhaustive": [_P, _c.POINTER(_c.c_double)],
     "bude_mi355x_combined_exact": [_P, _S, _P],
@@ -131,6 +132,16 @@ class Combined:
         r = PairResult()
         capi.check(_lib().bude_mi355x_combined_pair(self._h, ligand, _c.byref(r)))
         return r
+
+    def labels(self, ligand: int) -> tuple[np.ndarray, np.ndarray]:
+        """The last screen's sampled poses of one pair and their exact energies."""
+        n = _c.c_size_t()
+        capi.check(_lib().bude_mi355x_combined_labels(self._h, ligand, None, None, 0, _c.byref(n)))
+        idx = np.zeros(n.value, np.uint32)
+        e = np.zeros(n.value, np.float32)
+        capi.check(_lib().bude_mi355x_combined_labels(self._h, ligand, capi.ptr(idx), capi.ptr(e), n.value,
+                                                      _c.byref(n)))
+        return idx, e
 
     def selected(self, ligand: int) -> tuple[np.ndarray, np.ndarray]:
         n = _c.c_size_t()

@@ -166,6 +166,10 @@ struct bude_combined {
   DeviceBuffer meta_label, tiles_label, meta_lf, meta_fb, tiles_fb, meta_rs, tiles_rs, meta_all, tiles_all;
   DeviceBuffer tb_label, tb_all, lab_src, lab_poses, lab_e, lab_lig, lab_pred, norms, bias0;
   DeviceBuffer goff, poff, gathered, exact_sel;
+  // the last screen's sampled poses: pose index within the pair, energy, per-pair offsets
+  std::vector<uint32_t> lab_j;
+  std::vector<float> lab_energy;
+  std::vector<size_t> lab_offs;
   // MLP
   bude::surrogate::Net net;
   // per pair results of the last protein
@@ -201,7 +205,7 @@ struct Span {
 // this launch's own buffers (synchronously: nothing may be reading them).
 // Returns the tile count in *ntiles (0: nothing to score).
 int prepare_exact(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer &meta, DeviceBuffer &tiles,
-                  int *ntiles) {
+                  int *ntiles, size_t tile_poses = 64 * kBatchPpwi) {
   *ntiles = 0;
   std::vector<bude::LigandMeta> m(c->nlig);
   for (size_t l = 0; l < c->nlig; l++) {
@@ -221,7 +225,7 @@ int prepare_exact(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer
                    [&](size_t a, size_t b) { return c->lig[spans[a].lig].size() > c->lig[spans[b].lig].size(); });
   std::vector<int2> t;
   for (size_t i : order)
-    for (size_t j = 0; j < spans[i].n; j += 64 * kBatchPpwi) t.push_back(make_int2(int(spans[i].lig), int(j)));
+    for (size_t j = 0; j < spans[i].n; j += tile_poses) t.push_back(make_int2(int(spans[i].lig), int(j)));
   if (t.size() > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: too many tiles");
   if (int st = meta.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
   if (int st = tiles.assign(t.data(), t.size() * sizeof(int2))) return st;
@@ -251,6 +255,30 @@ int launch_exact(bude_combined *c, const float *poses, size_t stride, float *ene
   const bude::PinnedGrid g = bude::pinned_grid(c->cus, c->lds_per_cu, batch_waves_per_simd(ntiles, c->cus));
   a.lds_bytes = g.lds_bytes;
   return bude::hip_check(bude::launch_batch(a, kBatchPpwi, g.wgsize, g.blocks, c->stream), "combined batch launch");
+}
+
+// The sampled-pose pass (about a thousand poses per pair): the split form,
+// 16-pose tiles, four lanes per pose, so the short pass fills the SIMDs
+// instead of leaving one latency-bound wave on each.  Its energies only
+// train or validate the surrogate; every energy the screen returns comes
+// from the tiled kernel.
+int launch_sampled(bude_combined *c, const float *poses, size_t stride, float *energies, const DeviceBuffer &meta,
+                   const DeviceBuffer &tiles, int ntiles) {
+  if (ntiles == 0) return BUDE_OK;
+  bude::BatchArgs a{};
+  a.protein = static_cast<const float4 *>(c->protein.ptr);
+  a.group_begin = static_cast<const int32_t *>(c->group_begin.ptr);
+  a.coef = static_cast<const bude::PairCoef *>(c->coef.ptr);
+  a.ligands = static_cast<const float4 *>(c->ligands.ptr);
+  a.meta = static_cast<const bude::LigandMeta *>(meta.ptr);
+  a.tiles = static_cast<const int2 *>(tiles.ptr);
+  a.poses = poses;
+  a.energies = energies;
+  a.stride = stride;
+  a.ngroups = int(c->ngroups);
+  a.ntiles = ntiles;
+  a.form = bude::plain_form();
+  return bude::hip_check(bude::launch_split(a, c->stream), "combined sampled-pose launch");
 }
 
 int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride,
@@ -560,7 +588,7 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   //    issued while a kernel fills the device waits for it
   int nt_label = 0;
   int64_t ft_all = 0, ft_label = 0;
-  if (int st = prepare_exact(c, lab_spans, c->meta_label, c->tiles_label, &nt_label)) return st;
+  if (int st = prepare_exact(c, lab_spans, c->meta_label, c->tiles_label, &nt_label, bude::kSplitTile)) return st;
   if (!mlp) {
     const size_t np = c->grid.points(), nslots = c->slot_type.size();
     if (int st = c->maps.reserve(np * nslots * 4)) return st;
@@ -577,8 +605,8 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
                      static_cast<const uint32_t *>(c->lab_src.ptr), nlab, static_cast<float *>(c->lab_poses.ptr),
                      nlab);
   if (int st = bude::hip_check(hipGetLastError(), "gather labels")) return st;
-  if (int st = launch_exact(c, lpos, nlab, static_cast<float *>(c->lab_e.ptr), c->meta_label, c->tiles_label,
-                            nt_label))
+  if (int st = launch_sampled(c, lpos, nlab, static_cast<float *>(c->lab_e.ptr), c->meta_label, c->tiles_label,
+                              nt_label))
     return st;
   if (int st = record(c, 1)) return st;
   if (!mlp) {
@@ -610,6 +638,11 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   }
   if (int st = sync(c)) return st;
   res->label_ms = elapsed(c, 0, 1);
+  c->lab_energy = le;
+  c->lab_offs = lab_off;
+  c->lab_j.resize(nlab);
+  for (size_t l = 0; l < L; l++)
+    for (size_t i = lab_off[l]; i < lab_off[l + 1]; i++) c->lab_j[i] = uint32_t(src[i] - c->pose_off[l]);
 
   // per-pair target transform from the training part (all samples for the field)
   std::vector<Norm> norms(L);
@@ -837,6 +870,22 @@ extern "C" int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, ui
   for (size_t j = 0; j < k; j++) {
     if (idx) idx[j] = id[ord[j]];
     if (energies) energies[j] = ex[ord[j]];
+  }
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_labels(const bude_combined *c, size_t ligand, uint32_t *idx, float *energies,
+                                           size_t cap, size_t *n) {
+  if (!c || !n) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: NULL argument");
+  if (!c->screened) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: screen a protein first");
+  if (ligand >= c->nlig) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: no such ligand");
+  const size_t b = c->lab_offs[ligand], e = c->lab_offs[ligand + 1];
+  *n = e - b;
+  if (!idx && !energies) return bude::ok();
+  if (cap < e - b) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: cap too small");
+  for (size_t i = b; i < e; i++) {
+    if (idx) idx[i - b] = c->lab_j[i];
+    if (energies) energies[i - b] = c->lab_energy[i];
   }
   return bude::ok();
 }

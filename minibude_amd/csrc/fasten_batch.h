@@ -55,4 +55,11 @@ PinnedGrid pinned_grid(int cus, size_t lds_per_cu, int wps);
 // work-items, each wave scoring tiles of 64 * rb poses.
 hipError_t launch_batch(const BatchArgs &a, int rb, int wgsize, int blocks, hipStream_t s);
 
+// The split form for short passes: a.tiles are (ligand, first pose) pairs of
+// 16 poses, one wave each, four lanes per pose; a.counter and a.lds_bytes
+// are not used.  Energies agree with launch_batch's to rounding (the sums
+// are grouped by ligand-atom slice), not bit for bit.
+constexpr int kSplitTile = 16;
+hipError_t launch_split(const BatchArgs &a, hipStream_t s);
+
 }  // namespace bude

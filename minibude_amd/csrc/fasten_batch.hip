@@ -76,6 +76,72 @@ __global__ __launch_bounds__(WG) void fasten_batch_kernel(
   }
 }
 
+// Split form for short passes (a few tiles per SIMD, where a wave scoring
+// 64 poses runs nearly alone on its SIMD and its scalar-load latency is not
+// hidden): a wave scores 16 poses, four lanes per pose, lane s of a pose
+// taking the ligand atoms il = s, s + 4, ...; the four partial energies are
+// then added across the lanes.  Four times the waves for the same work, so
+// the pass fills the SIMDs.  The pair coefficients differ per lane and sit
+// in VGPRs (the next group's are loaded while the current group runs);
+// protein atoms stay wave-uniform.  The sums are grouped differently from
+// the tiled kernels', so energies agree with theirs to rounding, not bit
+// for bit.  One wave per tile (no work queue); tiles are (ligand, first
+// pose) pairs of 16 poses.
+template <int FORM>
+__global__ __launch_bounds__(256) void fasten_split_kernel(
+    const float4 *__restrict__ protein, const int32_t *__restrict__ group_begin, const PairCoef *__restrict__ coef,
+    const float4 *__restrict__ ligands, const LigandMeta *__restrict__ meta, const int2 *__restrict__ tiles,
+    const float *__restrict__ poses, float *__restrict__ energies, size_t stride, int ngroups, int ntiles) {
+  const int t = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);  // wave-uniform
+  if (t >= ntiles) return;
+  const int lane = threadIdx.x & 63, p = lane >> 2, sl = lane & 3;
+  const int2 tile = tiles[t];
+  const LigandMeta m = meta[tile.x];
+  const int j = tile.y + p;
+  const Xform tr = make_xform(poses, stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
+  float e = 0.f;
+  for (int base = 0; base < m.natlig; base += 4) {  // uniform trip count
+    const int il = base + sl;
+    const bool on = il < m.natlig;
+    const int ia = on ? il : m.natlig - 1;  // (inactive lanes score a real atom and drop it)
+    const float4 la = ligands[m.lig_off + ia];
+    const float *mm = tr.m;
+    const float lx = mm[3] + la.x * mm[0] + la.y * mm[1] + la.z * mm[2];
+    const float ly = mm[7] + la.x * mm[4] + la.y * mm[5] + la.z * mm[6];
+    const float lz = mm[11] + la.x * mm[8] + la.y * mm[9] + la.z * mm[10];
+    const PairCoef *cl = coef + size_t(m.coef_off) + size_t(ia) * ngroups;
+    PairCoef c = cl[0];
+    float sum = 0.f;
+    for (int g = 0; g < ngroups; ++g) {
+      const PairCoef cn = cl[g + 1 < ngroups ? g + 1 : g];
+      float part = 0.f;
+      const int i1 = group_begin[g + 1];
+#pragma unroll 4
+      for (int i = group_begin[g]; i < i1; ++i) {
+        const float4 pa = protein[i];
+        const float dx = lx - pa.x, dy = ly - pa.y, dz = lz - pa.z;
+        const float d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+        if constexpr (FORM == kFormB) {
+          const float b = d - c.radij;
+          part = __builtin_fmaf(2.0f * kHardness, sat(b * c.neg_rr), part);
+          part = __builtin_fmaf(c.wq, sat(__builtin_fmaf(b, c.neg_e1, 1.0f)), part);
+          part = __builtin_fmaf(c.wd, sat(__builtin_fmaf(b, c.neg_rd, 1.0f)), part);
+        } else {
+          part = __builtin_fmaf(2.0f * kHardness, sat(__builtin_fmaf(d, c.neg_rr, 1.0f)), part);
+          part = __builtin_fmaf(c.wq, sat(__builtin_fmaf(d, c.neg_e1, c.c0)), part);
+          part = __builtin_fmaf(c.wd, sat(__builtin_fmaf(d, c.neg_rd, c.e0)), part);
+        }
+      }
+      sum += part;
+      c = cn;
+    }
+    if (on) e += sum;
+  }
+  e += __shfl_xor(e, 1);
+  e += __shfl_xor(e, 2);
+  if (sl == 0 && j < m.nposes) energies[size_t(m.pose_off) + size_t(j)] = e * 0.5f;
+}
+
 template <int RB, int WG>
 hipError_t launch_rb_wg(const BatchArgs &a, int blocks, hipStream_t s) {
   if (a.form == kFormD)
@@ -118,6 +184,20 @@ PinnedGrid pinned_grid(int cus, size_t lds_per_cu, int wps) {
   const size_t b = std::min<size_t>(per - 2048, 65536) & ~size_t(1023);
   if (size_t(wps) * (b + 2048) <= lds_per_cu && size_t(wps + 1) * b > lds_per_cu) g.lds_bytes = unsigned(b);
   return g;
+}
+
+hipError_t launch_split(const BatchArgs &a, hipStream_t s) {
+  if (a.ntiles <= 0) return hipSuccess;
+  const unsigned blocks = unsigned((a.ntiles + 3) / 4);
+  if (a.form == kFormD)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_split_kernel<kFormD>), dim3(blocks), dim3(256), 0, s, a.protein,
+                       a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.stride, a.ngroups,
+                       a.ntiles);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_split_kernel<kFormB>), dim3(blocks), dim3(256), 0, s, a.protein,
+                       a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.stride, a.ngroups,
+                       a.ntiles);
+  return hipGetLastError();
 }
 
 hipError_t launch_batch(const BatchArgs &a, int rb, int wgsize, int blocks, hipStream_t s) {

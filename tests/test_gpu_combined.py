@@ -5,6 +5,7 @@ for both surrogates, and per-pair fallback."""
 import numpy as np
 import pytest
 
+import oracle_lib as O
 from minibude_amd import combined as C
 from minibude_amd import deck as D
 from minibude_amd import screen as S
@@ -165,6 +166,34 @@ def test_selection_rescoring_and_recall(gpu_device, surrogate):
                 # and those equal the plain kernel's for one pair
                 want = plain_energies(protein, w.ligands[0], w.forcefield, w.poses[0], gpu_device)
                 assert np.array_equal(c.exact(0).view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("surrogate", ["field", "mlp"])
+def test_sampled_pose_energies(gpu_device, surrogate):
+    # the sampled-pose pass runs the split kernel (four lanes per pose): its
+    # energies agree with the tiled kernel's to rounding and with the FP64
+    # oracle to 16 eps of the cancellation magnitude
+    w = C.combined_workload(nproteins=1, nligands=6, nposes=3000, seed=13, natpro_range=(300, 900),
+                            natlig_range=(1, 30))
+    kw = dict(labels_per_pair=700, max_val_rmse=1e9)
+    if surrogate == "mlp":
+        kw.update(batch=256, epochs=1)
+    eps = 2.0 ** -24
+    with C.Combined(w.forcefield, w.ligands, w.poses, C.default_config(surrogate, **kw), gpu_device) as c:
+        c.screen(w.proteins[0])
+        c.exhaustive()
+        for l in range(len(w.ligands)):
+            idx, e = c.labels(l)
+            assert len(idx) == 700 and len(set(idx.tolist())) == 700 and idx.max() < 3000
+            sub = D.Deck(w.proteins[0], w.ligands[l], w.forcefield, np.ascontiguousarray(w.poses[l][:, idx]))
+            e64, mag = O.fasten_f64_mag(sub, 0, len(idx))
+            ok = mag > 0
+            assert np.all(e[~ok] == 0.0)
+            noise = np.abs(e.astype(np.float64) - e64)[ok] / mag[ok] / eps
+            tiled = np.abs(e.astype(np.float64) - c.exact(l)[idx])[ok] / mag[ok] / eps
+            print(f"{surrogate} pair {l} ({len(w.ligands[l])} atoms): split vs FP64 {noise.max():.2f} eps, "
+                  f"vs tiled {tiled.max():.2f} eps", flush=True)
+            assert noise.max() < 16.0 and tiled.max() < 16.0
 
 
 def test_set_keep_rescreens_with_the_new_fraction(gpu_device):
