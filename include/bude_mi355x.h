@@ -233,6 +233,8 @@ typedef struct BudeCombinedConfig {
   uint32_t epochs;           /* MLP: passes over the training samples */
   float lr;                  /* MLP: peak Adam learning rate */
   float grid_spacing;        /* FIELD: map grid spacing in Angstrom (0.05 .. 4) */
+  uint32_t pipeline_chunks;  /* FIELD: ligand chunks scored, selected and rescored as a pipeline (rescoring of
+                                chunk k beside the field scoring of chunk k + 1); 0 = automatic, at most 16 */
   uint64_t seed;             /* sampling, initialisation, shuffling */
 } BudeCombinedConfig;
 
@@ -242,8 +244,9 @@ typedef struct BudeCombinedResult {
   float label_ms;            /* exact scoring of the sampled poses */
   float train_ms;            /* MLP training, or building the maps */
   float score_ms;            /* surrogate over every pose + exhaustive scoring of fallback pairs */
-  float select_ms;           /* per-pair top-k */
-  float rescore_ms;          /* gather + exact rescoring of the selections */
+  float select_ms;           /* per-pair top-k and the rescoring tables */
+  float rescore_ms;          /* exact rescoring of the selections, first launch to last; in a field pipeline of
+                                several chunks it overlaps the scoring of the later chunks */
   uint64_t nposes;           /* poses screened (all pairs) */
   uint64_t nselected;        /* poses with exact energies returned */
   uint64_t labels;           /* sampled poses scored exactly */

@@ -29,7 +29,7 @@ class CombinedConfig(_c.Structure):
     _fields_ = [("surrogate", _c.c_uint32), ("labels_per_pair", _c.c_uint32), ("keep_fraction", _c.c_float),
                 ("max_val_rmse", _c.c_float), ("clip_quantile", _c.c_float), ("tail_quantile", _c.c_float),
                 ("tail_weight", _c.c_float), ("batch", _c.c_uint32), ("epochs", _c.c_uint32), ("lr", _c.c_float),
-                ("grid_spacing", _c.c_float), ("seed", _c.c_uint64)]
+                ("grid_spacing", _c.c_float), ("pipeline_chunks", _c.c_uint32), ("seed", _c.c_uint64)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -58,7 +58,7 @@ def default_config(surrogate: str = "field", **kw) -> CombinedConfig:
     (1/16 held out), the field only validates on 1,024."""
     c = dict(surrogate=SURROGATES[surrogate], labels_per_pair=16384 if surrogate == "mlp" else 1024,
              keep_fraction=0.02, max_val_rmse=0.8, clip_quantile=0.1, tail_quantile=0.05, tail_weight=8.0,
-             batch=8192, epochs=20, lr=4e-3, grid_spacing=0.5, seed=0)
+             batch=8192, epochs=20, lr=4e-3, grid_spacing=0.5, pipeline_chunks=0, seed=0)
     c.update(kw)
     return CombinedConfig(**c)
 

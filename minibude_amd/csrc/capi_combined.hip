@@ -40,7 +40,13 @@ using bude::DeviceBuffer;
 // for 5,243 tiles, 136.2 vs 137.3 ms for 65,536).
 constexpr int kBatchPpwi = 1;
 
-int batch_waves_per_simd(int ntiles, int cus) { return ntiles < 16 * 4 * cus ? 3 : 4; }
+int batch_waves_per_simd(int64_t ntiles, int cus) { return ntiles < 16 * 4 * int64_t(cus) ? 3 : 4; }
+
+// Field pipeline depth: at most 16 chunks, and (chosen automatically) each
+// expected to give at least 64 rescoring tiles per CU, so that every chunk's
+// rescoring launch still fills the machine; smaller screens run as one chunk.
+constexpr size_t kMaxChunks = 16;
+constexpr int64_t kChunkTilesPerCu = 64;
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -55,23 +61,48 @@ __global__ void gather_index_kernel(const float *__restrict__ src, size_t src_st
   for (int k = 0; k < 6; k++) dst[size_t(k) * dst_stride + j] = src[size_t(k) * src_stride + q];
 }
 
-// Row j of the gathered selection: ligand l with goff[l] <= j < goff[l+1],
-// pose sel[pose_off[l] + (j - goff[l])] of that ligand.
-__global__ void gather_selected_kernel(const float *__restrict__ src, size_t src_stride,
-                                       const uint32_t *__restrict__ sel, const int64_t *__restrict__ pose_off,
-                                       const int64_t *__restrict__ goff, int nlig, size_t n, float *__restrict__ dst,
-                                       size_t dst_stride) {
-  const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int lo = 0, hi = nlig - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (goff[mid] <= int64_t(j)) lo = mid;
-    else hi = mid - 1;
+// Rescoring work tables of the ligands order[0..n) (heaviest first) from
+// their selection counts, on the device right after the selection:
+// meta[l].nposes = sel_n[l], tiles (l, 64 i) for i < ceil(sel_n[l] / 64) in
+// list order from tiles[0], and the tile count in *ntiles.  One work-group
+// of 256; the host never waits for the counts.
+__global__ __launch_bounds__(256) void rescore_tables_kernel(const int32_t *__restrict__ order, int n,
+                                                             const uint32_t *__restrict__ sel_n,
+                                                             bude::LigandMeta *__restrict__ meta,
+                                                             int2 *__restrict__ tiles, int32_t *__restrict__ ntiles) {
+  __shared__ int32_t lig[256], cnt[256], off[256], wsum[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t base = 0;
+  for (int b = 0; b < n; b += 256) {
+    const int i = b + int(threadIdx.x);
+    int32_t c = 0, l = 0;
+    if (i < n) {
+      l = order[i];
+      const uint32_t k = sel_n[l];
+      meta[l].nposes = int32_t(k);
+      c = int32_t((k + 63) / 64);
+    }
+    int32_t x = c;  // inclusive scan over the wave, then over the four waves
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t y = __shfl_up(x, d);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int32_t wb = 0;
+    for (int q = 0; q < w; q++) wb += wsum[q];
+    lig[threadIdx.x] = l;
+    cnt[threadIdx.x] = c;
+    off[threadIdx.x] = base + wb + x - c;
+    const int32_t sum = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    const int m = min(256, n - b);
+    for (int q = 0; q < m; q++)
+      for (int t = int(threadIdx.x); t < cnt[q]; t += 256) tiles[off[q] + t] = make_int2(lig[q], 64 * t);
+    base += sum;
+    __syncthreads();  // lig/cnt/off/wsum are rewritten by the next batch
   }
-  const size_t q = size_t(pose_off[lo]) + sel[size_t(pose_off[lo]) + (j - size_t(goff[lo]))];
-#pragma unroll
-  for (int k = 0; k < 6; k++) dst[size_t(k) * dst_stride + j] = src[size_t(k) * src_stride + q];
+  if (threadIdx.x == 0) *ntiles = base;
 }
 
 unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
@@ -155,7 +186,19 @@ struct bude_combined {
   DeviceBuffer poses;                 // 6 rows of `total`
   DeviceBuffer pred, sel_idx, sel_n, exact_all;
   DeviceBuffer desc;                  // [nlig][kDesc]
-  bude::sel::Plan plan;
+  bude::sel::Plan plan;               // every pair
+  // The field screen runs as a pipeline over ligand ranges
+  // [chunk_lig[k], chunk_lig[k + 1]): the second stream scores, selects and
+  // builds the rescoring tables of chunk k while the first rescores chunk
+  // k - 1.  Per chunk: its selection plan, the first slot of its rescoring
+  // tiles (of Σ ceil(n / 64) over its pairs), the tiles its selection is
+  // expected to give, and two events (scores done, tables done).
+  std::vector<size_t> chunk_lig;
+  std::vector<std::unique_ptr<bude::sel::Plan>> chunk_plan;
+  std::vector<int64_t> chunk_tile;
+  std::vector<int64_t> chunk_expect;
+  std::vector<hipEvent_t> chunk_ev;
+  std::vector<int64_t> lig_expect;    // expected rescoring tiles of each pair: ceil(k / 64)
   bude::field::Grid grid{};
   // per protein
   bool screened = false;
@@ -165,7 +208,12 @@ struct bude_combined {
   // per-call work tables (each its own buffer; uploaded only while the stream is idle)
   DeviceBuffer meta_label, tiles_label, meta_lf, meta_fb, tiles_fb, meta_rs, tiles_rs, meta_all, tiles_all;
   DeviceBuffer tb_label, tb_all, lab_src, lab_poses, lab_e, lab_lig, lab_pred, norms, bias0;
-  DeviceBuffer goff, poff, gathered, exact_sel;
+  // rescoring: exact_sel[pose_off[l] + j] is the exact energy of pose
+  // sel_idx[pose_off[l] + j] of pair l (j < its selection count); rs_order
+  // lists each chunk's ligands heaviest first, rs_sel the non-fallback
+  // ligands when the whole set is rescored at once; rs_ntiles the tile
+  // counts (one per chunk, then the whole set's)
+  DeviceBuffer exact_sel, rs_order, rs_sel, rs_ntiles;
   // the last screen's sampled poses: pose index within the pair, energy, per-pair offsets
   std::vector<uint32_t> lab_j;
   std::vector<float> lab_energy;
@@ -174,7 +222,6 @@ struct bude_combined {
   bude::surrogate::Net net;
   // per pair results of the last protein
   std::vector<uint32_t> nsel;
-  std::vector<int64_t> gofs;
   std::vector<int32_t> fallback;
   std::vector<float> val_rmse, val_rmse_cal;
   // exact energies of every pair (exhaustive) and the protein they belong to
@@ -187,6 +234,8 @@ struct bude_combined {
 
   ~bude_combined() {
     for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto &e : chunk_ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
@@ -291,7 +340,7 @@ int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
 // Work tables of a field scoring of spans (as prepare_exact): meta and the
 // per-ligand tile prefix.  Returns the tile count in *ntiles.
 int prepare_field(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer &meta, DeviceBuffer &tile_begin,
-                  int64_t *ntiles) {
+                  int64_t *ntiles, std::vector<int64_t> *tb_host = nullptr) {
   *ntiles = 0;
   std::vector<bude::LigandMeta> m(c->nlig);
   std::vector<int64_t> tb(c->nlig + 1, 0);
@@ -304,6 +353,7 @@ int prepare_field(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer
     m[s.lig].pose_off = int64_t(s.off);
   }
   for (size_t l = 0; l < c->nlig; l++) tb[l + 1] = tb[l] + int64_t((size_t(m[l].nposes) + 63) / 64);
+  if (tb_host) *tb_host = tb;
   if (tb[c->nlig] == 0) return BUDE_OK;
   if (int st = meta.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
   if (int st = tile_begin.assign(tb.data(), tb.size() * sizeof(int64_t))) return st;
@@ -311,10 +361,11 @@ int prepare_field(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer
   return BUDE_OK;
 }
 
-// Interpolated field scores (out[off + j]) of the spans prepared by
-// prepare_field, from the current cells.
+// Interpolated field scores (out[off + j]) of tiles [tile0, tile0 + ntiles)
+// of the spans prepared by prepare_field, from the current cells.
 int launch_field(bude_combined *c, const float *poses, size_t stride, float *out, const DeviceBuffer &meta,
-                 const DeviceBuffer &tile_begin, int64_t ntiles, hipStream_t stream) {
+                 const DeviceBuffer &tile_begin, int64_t tile0, int64_t ntiles, hipStream_t stream,
+                 bool raise_priority = false) {
   if (ntiles == 0) return BUDE_OK;
   bude::field::ScoreArgs a{};
   a.cells = static_cast<const uint4 *>(c->cells.ptr);
@@ -326,7 +377,123 @@ int launch_field(bude_combined *c, const float *poses, size_t stride, float *out
   a.poses = poses;
   a.stride = stride;
   a.out = out;
+  a.tile0 = tile0;
+  a.raise_priority = raise_priority;
   return bude::hip_check(bude::field::score(a, ntiles, stream), "field score");
+}
+
+// Per-pair top-k of pred over the pairs of `plan` (pairs first .. first +
+// n - 1) and their rescoring tables from the ligand list `order` (n
+// entries) into tiles / *ntiles, on stream s.
+int select_and_tables(bude_combined *c, const bude::sel::Plan &plan, size_t first, const int32_t *order, int n,
+                      int2 *tiles, int32_t *ntiles, hipStream_t s) {
+  uint32_t *sel_n = static_cast<uint32_t *>(c->sel_n.ptr);
+  if (int st = bude::hip_check(bude::sel::select_lowest_segments(plan, static_cast<const float *>(c->pred.ptr),
+                                                                 static_cast<uint32_t *>(c->sel_idx.ptr),
+                                                                 sel_n + first, s),
+                               "select"))
+    return st;
+  if (n == 0) return BUDE_OK;
+  hipLaunchKernelGGL(rescore_tables_kernel, dim3(1), dim3(256), 0, s, order, n, sel_n,
+                     static_cast<bude::LigandMeta *>(c->meta_rs.ptr), tiles, ntiles);
+  return bude::hip_check(hipGetLastError(), "rescoring tables");
+}
+
+// Exact energies of the selected poses listed by tiles / *ntiles (written
+// on the device by select_and_tables) into exact_sel, on the first stream;
+// `expect` tiles set the grid depth.
+int launch_rescore(bude_combined *c, const int2 *tiles, const int32_t *ntiles, int64_t expect) {
+  bude::BatchArgs a{};
+  a.protein = static_cast<const float4 *>(c->protein.ptr);
+  a.group_begin = static_cast<const int32_t *>(c->group_begin.ptr);
+  a.coef = static_cast<const bude::PairCoef *>(c->coef.ptr);
+  a.ligands = static_cast<const float4 *>(c->ligands.ptr);
+  a.meta = static_cast<const bude::LigandMeta *>(c->meta_rs.ptr);
+  a.tiles = tiles;
+  a.poses = static_cast<const float *>(c->poses.ptr);
+  a.energies = static_cast<float *>(c->exact_sel.ptr);
+  a.counter = static_cast<unsigned int *>(c->counter.ptr);
+  a.stride = c->total;
+  a.ngroups = int(c->ngroups);
+  a.form = bude::plain_form();
+  a.pose_index = static_cast<const uint32_t *>(c->sel_idx.ptr);
+  a.ntiles_dev = ntiles;
+  const bude::PinnedGrid g = bude::pinned_grid(c->cus, c->lds_per_cu, batch_waves_per_simd(expect, c->cus));
+  a.lds_bytes = g.lds_bytes;
+  return bude::hip_check(bude::launch_batch_indexed(a, g.blocks, c->stream), "combined rescoring launch");
+}
+
+// The ligands [a, b) heaviest first (ties in index order).
+std::vector<int32_t> heaviest_first(const bude_combined *c, size_t a, size_t b, const std::vector<int32_t> *skip) {
+  std::vector<int32_t> o;
+  for (size_t l = a; l < b; l++)
+    if (!skip || !(*skip)[l]) o.push_back(int32_t(l));
+  std::stable_sort(o.begin(), o.end(), [&](int32_t x, int32_t y) { return c->lig[x].size() > c->lig[y].size(); });
+  return o;
+}
+
+// Selection segments (the lowest ceil(keep n) of each pair) of the whole set
+// and of each pipeline chunk, the chunks' rescoring tile slots and ligand
+// lists, and their events.  Both streams must be idle.
+int plan_selection(bude_combined *c, float keep) {
+  const size_t L = c->nlig;
+  std::vector<bude::sel::Segment> segs(L);
+  std::vector<int64_t> &kt = c->lig_expect, cap(L);
+  kt.assign(L, 0);
+  int64_t expect_all = 0;
+  double cost = 0;
+  for (size_t i = 0; i < L; i++) {
+    const double kf = std::ceil(double(keep) * double(c->nposes[i]));
+    const uint32_t k = uint32_t(std::min<double>(double(c->nposes[i]), std::max(1.0, kf)));
+    segs[i] = bude::sel::Segment{int64_t(c->pose_off[i]), uint32_t(c->nposes[i]), k};
+    kt[i] = int64_t((k + 63) / 64);
+    cap[i] = int64_t((c->nposes[i] + 63) / 64);
+    expect_all += kt[i];
+    cost += double(c->nposes[i]) * double(c->lig[i].size());
+  }
+  if (int st = c->plan.prepare(segs)) return st;
+  size_t K = 1;
+  if (c->cfg.surrogate == BUDE_SURROGATE_FIELD)
+    K = c->cfg.pipeline_chunks ? size_t(c->cfg.pipeline_chunks)
+                               : size_t(std::max<int64_t>(1, expect_all / (kChunkTilesPerCu * c->cus)));
+  K = std::min({K, L, kMaxChunks});
+  // boundaries at equal shares of the field work (poses x atoms)
+  c->chunk_lig.assign(1, 0);
+  double acc = 0;
+  for (size_t l = 0; l + 1 < L && c->chunk_lig.size() < K; l++) {
+    acc += double(c->nposes[l]) * double(c->lig[l].size());
+    if (acc >= cost * double(c->chunk_lig.size()) / double(K)) c->chunk_lig.push_back(l + 1);
+  }
+  c->chunk_lig.push_back(L);
+  K = c->chunk_lig.size() - 1;
+  c->chunk_plan.clear();
+  c->chunk_tile.assign(K + 1, 0);
+  c->chunk_expect.assign(K, 0);
+  std::vector<int32_t> order;
+  for (size_t k = 0; k < K; k++) {
+    const size_t a = c->chunk_lig[k], b = c->chunk_lig[k + 1];
+    c->chunk_plan.push_back(std::make_unique<bude::sel::Plan>());
+    const std::vector<bude::sel::Segment> cs(segs.begin() + long(a), segs.begin() + long(b));
+    if (int st = c->chunk_plan[k]->prepare(cs)) return st;
+    int64_t t = 0;
+    for (size_t l = a; l < b; l++) {
+      t += cap[l];
+      c->chunk_expect[k] += kt[l];
+    }
+    c->chunk_tile[k + 1] = c->chunk_tile[k] + t;
+    const std::vector<int32_t> o = heaviest_first(c, a, b, nullptr);
+    order.insert(order.end(), o.begin(), o.end());
+  }
+  if (c->chunk_tile[K] > 0x7fffffff) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: too many rescoring tiles");
+  if (int st = c->rs_order.assign(order.data(), order.size() * 4)) return st;
+  if (int st = c->tiles_rs.reserve(size_t(c->chunk_tile[K]) * sizeof(int2))) return st;
+  if (int st = c->rs_ntiles.reserve((K + 1) * 4)) return st;
+  while (c->chunk_ev.size() < 2 * K) {
+    hipEvent_t e = nullptr;
+    if (int st = bude::hip_check(hipEventCreate(&e), "hipEventCreate")) return st;
+    c->chunk_ev.push_back(e);
+  }
+  return BUDE_OK;
 }
 
 int sync(bude_combined *c) { return bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
@@ -335,11 +502,13 @@ int record(bude_combined *c, int i, hipStream_t s = nullptr) {
   return bude::hip_check(hipEventRecord(c->ev[i], s ? s : c->stream), "hipEventRecord");
 }
 
-float elapsed(bude_combined *c, int a, int b) {
+float elapsed_events(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, c->ev[a], c->ev[b]);
+  (void)hipEventElapsedTime(&ms, a, b);
   return ms;
 }
+
+float elapsed(bude_combined *c, int a, int b) { return elapsed_events(c->ev[a], c->ev[b]); }
 
 int check_config(const BudeCombinedConfig *cfg) {
   if (!cfg) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: config is NULL");
@@ -358,6 +527,8 @@ int check_config(const BudeCombinedConfig *cfg) {
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: MLP batch must be a multiple of 32 in [32, 2^20], lr > 0");
   if (cfg->surrogate == BUDE_SURROGATE_FIELD && !(cfg->grid_spacing >= 0.05f && cfg->grid_spacing <= 4.f))
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: grid_spacing must be in [0.05, 4]");
+  if (cfg->pipeline_chunks > kMaxChunks)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: pipeline_chunks must be in [0, 16]");
   return BUDE_OK;
 }
 
@@ -461,17 +632,9 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
   if (cfg->surrogate == BUDE_SURROGATE_FIELD)
     if (int st = make_grid(c.get(), poses)) return st;
 
-  std::vector<bude::sel::Segment> segs(nligands);
-  for (size_t i = 0; i < nligands; i++) {
-    const double kf = std::ceil(double(cfg->keep_fraction) * double(nposes[i]));
-    segs[i] = bude::sel::Segment{int64_t(c->pose_off[i]), uint32_t(nposes[i]),
-                                 uint32_t(std::min<double>(double(nposes[i]), std::max(1.0, kf)))};
-  }
-  std::vector<int64_t> poff(c->pose_off.begin(), c->pose_off.end());
-  int st = c->plan.prepare(segs);
+  int st = plan_selection(c.get(), cfg->keep_fraction);
   if (!st) st = c->ligands.upload(lig4.data(), lig4.size() * 4);
   if (!st) st = c->desc.upload(desc.data(), desc.size() * 4);
-  if (!st) st = c->poff.upload(poff.data(), poff.size() * sizeof(int64_t));
   if (!st) st = c->poses.alloc(6 * total * sizeof(float));
   for (size_t i = 0; i < nligands && !st; i++)
     for (int k = 0; k < 6 && !st; k++)
@@ -482,6 +645,7 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
   if (!st) st = c->pred.alloc(total * sizeof(float));
   if (!st) st = c->sel_idx.alloc(total * sizeof(uint32_t));
   if (!st) st = c->sel_n.alloc(nligands * sizeof(uint32_t));
+  if (!st) st = c->exact_sel.alloc(total * sizeof(float));
   if (!st) st = c->counter.alloc(256);
   if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking), "hipStreamCreate");
@@ -588,14 +752,24 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   //    issued while a kernel fills the device waits for it
   int nt_label = 0;
   int64_t ft_all = 0, ft_label = 0;
+  std::vector<int64_t> tb_all;
   if (int st = prepare_exact(c, lab_spans, c->meta_label, c->tiles_label, &nt_label, bude::kSplitTile)) return st;
   if (!mlp) {
     const size_t np = c->grid.points(), nslots = c->slot_type.size();
     if (int st = c->maps.reserve(np * nslots * 4)) return st;
     if (int st = c->cells.reserve(c->grid.cells() * nslots * 16)) return st;
-    if (int st = prepare_field(c, all, c->meta_all, c->tb_all, &ft_all)) return st;
+    if (int st = prepare_field(c, all, c->meta_all, c->tb_all, &ft_all, &tb_all)) return st;
     if (int st = prepare_field(c, lab_spans, c->meta_lf, c->tb_label, &ft_label)) return st;
   }
+  {  // rescoring meta: everything but the pose counts, which the selection writes on the device
+    std::vector<bude::LigandMeta> m(L);
+    for (size_t l = 0; l < L; l++)
+      m[l] = bude::LigandMeta{c->lig_off[l], int32_t(c->lig[l].size()), c->coef_off[l], 0, int64_t(c->pose_off[l]), 0};
+    if (int st = c->meta_rs.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
+  }
+  const size_t K = c->chunk_lig.size() - 1;
+  int2 *rs_tiles = static_cast<int2 *>(c->tiles_rs.ptr);
+  int32_t *rs_ntiles = static_cast<int32_t *>(c->rs_ntiles.ptr);
 
   // 4. exact energies of the sampled poses (one ragged batched launch) on the
   //    first stream, and, for the field, maps -> cells -> scores of every
@@ -625,7 +799,21 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
                                  "pack cells"))
       return st;
     if (int st = record(c, 3, c->stream2)) return st;
-    if (int st = launch_field(c, poses, c->total, pred, c->meta_all, c->tb_all, ft_all, c->stream2)) return st;
+    // chunk by chunk: scores, top-k and rescoring tables, all on the device;
+    // the scoring of chunk k > 0 runs beside the rescoring of chunk k - 1
+    // and issues ahead of it (chunk 0 runs beside the sampled-pose pass,
+    // which the host waits for)
+    for (size_t k = 0; k < K; k++) {
+      const size_t a = c->chunk_lig[k], b = c->chunk_lig[k + 1];
+      if (int st = launch_field(c, poses, c->total, pred, c->meta_all, c->tb_all, tb_all[a], tb_all[b] - tb_all[a],
+                                c->stream2, k > 0))
+        return st;
+      if (int st = bude::hip_check(hipEventRecord(c->chunk_ev[2 * k], c->stream2), "hipEventRecord")) return st;
+      if (int st = select_and_tables(c, *c->chunk_plan[k], a, static_cast<const int32_t *>(c->rs_order.ptr) + a,
+                                     int(b - a), rs_tiles + c->chunk_tile[k], rs_ntiles + k, c->stream2))
+        return st;
+      if (int st = bude::hip_check(hipEventRecord(c->chunk_ev[2 * k + 1], c->stream2), "hipEventRecord")) return st;
+    }
     if (int st = record(c, 4, c->stream2)) return st;
   }
   std::vector<float> le(nlab);
@@ -634,7 +822,7 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     return st;
   if (!mlp) {  // field scores of the sampled poses, once the cells exist
     if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->ev[3], 0), "hipStreamWaitEvent")) return st;
-    if (int st = launch_field(c, lpos, nlab, lp, c->meta_lf, c->tb_label, ft_label, c->stream)) return st;
+    if (int st = launch_field(c, lpos, nlab, lp, c->meta_lf, c->tb_label, 0, ft_label, c->stream)) return st;
   }
   if (int st = sync(c)) return st;
   res->label_ms = elapsed(c, 0, 1);
@@ -734,12 +922,24 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     res->fallback_pairs += uint32_t(c->fallback[l]);
   }
 
-  // 7. MLP scores of every pose of every non-fallback pair (the field's are
-  //    on stream2 already); exhaustive exact scores of the fallback pairs
-  //    overwrite theirs
+  // 7. Without fallback pairs the field screen is already selecting chunk
+  //    by chunk on the second stream.  Otherwise: MLP scores of every pose
+  //    of every non-fallback pair (the field's are on the second stream),
+  //    exhaustive exact scores of the fallback pairs overwriting theirs,
+  //    and the rescoring list of the non-fallback pairs
   std::vector<Span> fb;
   for (size_t l = 0; l < L; l++)
     if (c->fallback[l]) fb.push_back(all[l]);
+  const bool pipelined = !mlp && fb.empty();
+  int64_t expect = 0;
+  int nrs = 0;
+  if (!pipelined) {
+    std::vector<int32_t> o = heaviest_first(c, 0, L, &c->fallback);
+    for (int32_t l : o) expect += c->lig_expect[size_t(l)];
+    nrs = int(o.size());
+    if (o.empty()) o.push_back(0);  // every pair fell back: nothing to rescore
+    if (int st = c->rs_sel.assign(o.data(), o.size() * 4)) return st;
+  }
   if (mlp) {
     if (int st = record(c, 4)) return st;
     for (size_t l = 0; l < L; l++)
@@ -748,54 +948,48 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
                                               pred + c->pose_off[l], 2 * c->cus, c->stream,
                                               static_cast<const float *>(c->bias0.ptr) + l * bude::mlp::kHidden))
           return st;
-  } else if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->ev[4], 0), "hipStreamWaitEvent")) {
-    return st;
+  } else if (!pipelined) {
+    if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->ev[4], 0), "hipStreamWaitEvent")) return st;
   }
   if (int st = record(c, 5)) return st;
   if (int st = exact_ragged(c, fb, poses, c->total, pred, c->meta_fb, c->tiles_fb)) return st;
   if (int st = record(c, 6)) return st;
 
-  // 8. per-pair top-k of the scores
-  if (int st = bude::hip_check(bude::sel::select_lowest_segments(c->plan, pred, static_cast<uint32_t *>(c->sel_idx.ptr),
-                                                                 static_cast<uint32_t *>(c->sel_n.ptr), c->stream),
-                               "select"))
-    return st;
+  // 8-9. per-pair top-k of the scores and exact rescoring of every
+  //      non-fallback pair's selection: chunk by chunk behind the second
+  //      stream, or the whole set at once
+  if (pipelined) {
+    for (size_t k = 0; k < K; k++) {
+      if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->chunk_ev[2 * k + 1], 0), "hipStreamWaitEvent"))
+        return st;
+      if (k == 0)
+        if (int st = record(c, 7)) return st;
+      if (int st = launch_rescore(c, rs_tiles + c->chunk_tile[k], rs_ntiles + k, c->chunk_expect[k])) return st;
+    }
+  } else {
+    if (int st = select_and_tables(c, c->plan, 0, static_cast<const int32_t *>(c->rs_sel.ptr), nrs, rs_tiles,
+                                   rs_ntiles + K, c->stream))
+      return st;
+    if (int st = record(c, 7)) return st;
+    if (nrs)
+      if (int st = launch_rescore(c, rs_tiles, rs_ntiles + K, expect)) return st;
+  }
   c->nsel.assign(L, 0);
   if (int st = bude::hip_check(hipMemcpyAsync(c->nsel.data(), c->sel_n.ptr, L * 4, hipMemcpyDeviceToHost, c->stream),
                                "hipMemcpyAsync"))
     return st;
-  if (int st = record(c, 7)) return st;
-  if (int st = sync(c)) return st;
-
-  // 9. exact rescoring of every non-fallback pair's selection in one launch
-  c->gofs.assign(L + 1, 0);
-  for (size_t l = 0; l < L; l++) c->gofs[l + 1] = c->gofs[l] + (c->fallback[l] ? 0 : int64_t(c->nsel[l]));
-  const size_t G = size_t(c->gofs[L]);
-  if (G) {
-    if (int st = c->goff.reserve((L + 1) * sizeof(int64_t))) return st;
-    if (int st = bude::hip_check(hipMemcpy(c->goff.ptr, c->gofs.data(), (L + 1) * sizeof(int64_t),
-                                           hipMemcpyHostToDevice),
-                                 "hipMemcpy goff"))
-      return st;
-    if (int st = c->gathered.reserve(6 * G * 4)) return st;
-    if (int st = c->exact_sel.reserve(G * 4)) return st;
-    const size_t gstride = c->gathered.bytes / 24;
-    hipLaunchKernelGGL(gather_selected_kernel, dim3(blocks_for(G, 256)), dim3(256), 0, c->stream, poses, c->total,
-                       static_cast<const uint32_t *>(c->sel_idx.ptr), static_cast<const int64_t *>(c->poff.ptr),
-                       static_cast<const int64_t *>(c->goff.ptr), int(L), G, static_cast<float *>(c->gathered.ptr),
-                       gstride);
-    if (int st = bude::hip_check(hipGetLastError(), "gather selected")) return st;
-    std::vector<Span> rs;
-    for (size_t l = 0; l < L; l++)
-      if (!c->fallback[l]) rs.push_back(Span{l, size_t(c->gofs[l]), c->nsel[l]});
-    if (int st = exact_ragged(c, rs, static_cast<const float *>(c->gathered.ptr), gstride,
-                              static_cast<float *>(c->exact_sel.ptr), c->meta_rs, c->tiles_rs))
-      return st;
-  }
   if (int st = record(c, 8)) return st;
   if (int st = sync(c)) return st;
-  res->score_ms = elapsed(c, mlp ? 4 : 3, mlp ? 5 : 4) + elapsed(c, 5, 6);
-  res->select_ms = elapsed(c, 6, 7);
+  if (pipelined) {
+    res->score_ms = elapsed_events(c->ev[3], c->chunk_ev[0]);
+    for (size_t k = 0; k < K; k++) {
+      if (k) res->score_ms += elapsed_events(c->chunk_ev[2 * k - 1], c->chunk_ev[2 * k]);
+      res->select_ms += elapsed_events(c->chunk_ev[2 * k], c->chunk_ev[2 * k + 1]);
+    }
+  } else {
+    res->score_ms = elapsed(c, mlp ? 4 : 3, mlp ? 5 : 4) + elapsed(c, 5, 6);
+    res->select_ms = elapsed(c, 6, 7);
+  }
   res->rescore_ms = elapsed(c, 7, 8);
   res->total_ms = ms_since(t0);
   res->nposes = c->total;
@@ -811,13 +1005,8 @@ extern "C" int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fracti
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_set_keep: keep_fraction must be in (0, 1]");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return st;
-  std::vector<bude::sel::Segment> segs(c->nlig);
-  for (size_t i = 0; i < c->nlig; i++) {
-    const double kf = std::ceil(double(keep_fraction) * double(c->nposes[i]));
-    segs[i] = bude::sel::Segment{int64_t(c->pose_off[i]), uint32_t(c->nposes[i]),
-                                 uint32_t(std::min<double>(double(c->nposes[i]), std::max(1.0, kf)))};
-  }
-  if (int st = c->plan.prepare(segs)) return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(c->stream2), "hipStreamSynchronize")) return st;
+  if (int st = plan_selection(c, keep_fraction)) return st;
   c->cfg.keep_fraction = keep_fraction;
   return bude::ok();
 }
@@ -858,7 +1047,7 @@ extern "C" int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, ui
                                    "hipMemcpy"))
         return st;
       for (size_t j = 0; j < k; j++) ex[j] = all[id[j]];
-    } else if (int st = bude::hip_check(hipMemcpy(ex.data(), static_cast<float *>(c->exact_sel.ptr) + c->gofs[ligand],
+    } else if (int st = bude::hip_check(hipMemcpy(ex.data(), static_cast<float *>(c->exact_sel.ptr) + po,
                                                   k * 4, hipMemcpyDeviceToHost),
                                         "hipMemcpy")) {
       return st;

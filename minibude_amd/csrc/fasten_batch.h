@@ -34,6 +34,10 @@ struct BatchArgs {
   int form;                     // hinge form (plain_form())
   unsigned lds_bytes;           // dynamic LDS per work-group, unused by the kernel: sized so that exactly
                                 // n work-groups fit on a CU, it pins the waves per SIMD (0: no pinning)
+  // launch_batch_indexed only: row j of ligand l is pose meta[l].pose_off + pose_index[meta[l].pose_off + j]
+  // (energies still at meta[l].pose_off + j), and the tile count is read from device memory
+  const uint32_t *pose_index;
+  const int32_t *ntiles_dev;
 };
 
 bool batch_shape_supported(int rb, int wgsize);
@@ -54,6 +58,12 @@ PinnedGrid pinned_grid(int cus, size_t lds_per_cu, int wps);
 // Counter reset + one persistent launch of `blocks` work-groups of `wgsize`
 // work-items, each wave scoring tiles of 64 * rb poses.
 hipError_t launch_batch(const BatchArgs &a, int rb, int wgsize, int blocks, hipStream_t s);
+
+// launch_batch over rows that are a per-segment selection (pose_index) with
+// tiles and tile count written by an earlier kernel (ntiles_dev): one pose
+// per work-item (rb 1), work-groups of 256.  Energies are bit-equal to
+// launch_batch's for the same poses.
+hipError_t launch_batch_indexed(const BatchArgs &a, int blocks, hipStream_t s);
 
 // The split form for short passes: a.tiles are (ligand, first pose) pairs of
 // 16 poses, one wave each, four lanes per pose; a.counter and a.lds_bytes

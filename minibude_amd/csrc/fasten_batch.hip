@@ -30,13 +30,16 @@ __global__ void reset_counter_kernel(unsigned int *__restrict__ counter) {
   if (threadIdx.x == 0) counter[0] = 0u;
 }
 
-template <int RB, int WG, int FORM>
+// INDEXED: pose rows through pose_index, tile count from ntiles_dev (both
+// written by earlier kernels on the device).
+template <int RB, int WG, int FORM, bool INDEXED = false>
 __global__ __launch_bounds__(WG) void fasten_batch_kernel(
     const float4 *__restrict__ protein, const int32_t *__restrict__ group_begin, const PairCoef *__restrict__ coef,
     const float4 *__restrict__ ligands, const LigandMeta *__restrict__ meta, const int2 *__restrict__ tiles,
     const float *__restrict__ poses, float *__restrict__ energies, unsigned int *__restrict__ counter, size_t stride,
-    int ngroups, int ntiles) {
+    int ngroups, int ntiles, const uint32_t *__restrict__ pose_index, const int32_t *__restrict__ ntiles_dev) {
   const int lane = threadIdx.x & 63;
+  if constexpr (INDEXED) ntiles = *ntiles_dev;
   for (;;) {
     // One atomic per wave; every wave leaves once the list is exhausted.
     unsigned int t = 0;
@@ -51,7 +54,9 @@ __global__ __launch_bounds__(WG) void fasten_batch_kernel(
 #pragma unroll
     for (int k = 0; k < RB; ++k) {
       const int j = tile.y + lane + k * 64;
-      tr[k] = make_xform(poses, stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
+      size_t q = size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1);
+      if constexpr (INDEXED) q = size_t(m.pose_off) + pose_index[q];
+      tr[k] = make_xform(poses, stride, q);
       e[k] = 0.f;
     }
     for (int il = 0; il < m.natlig; ++il) {
@@ -147,11 +152,11 @@ hipError_t launch_rb_wg(const BatchArgs &a, int blocks, hipStream_t s) {
   if (a.form == kFormD)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormD>), dim3(unsigned(blocks)), dim3(WG),
                        a.lds_bytes, s, a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses,
-                       a.energies, a.counter, a.stride, a.ngroups, a.ntiles);
+                       a.energies, a.counter, a.stride, a.ngroups, a.ntiles, nullptr, nullptr);
   else
     hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<RB, WG, kFormB>), dim3(unsigned(blocks)), dim3(WG),
                        a.lds_bytes, s, a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses,
-                       a.energies, a.counter, a.stride, a.ngroups, a.ntiles);
+                       a.energies, a.counter, a.stride, a.ngroups, a.ntiles, nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -176,13 +181,13 @@ bool batch_shape_supported(int rb, int wgsize) {
 PinnedGrid pinned_grid(int cus, size_t lds_per_cu, int wps) {
   PinnedGrid g{256, std::max(1, cus * wps), 0u};
   if (wps < 1) return g;
-  // whole KiB, at most the 64 KiB a work-group may allocate, with 2 KiB of
-  // slack per work-group for the allocation granularity: wps work-groups fit
-  // and wps + 1 do not
-  const size_t per = lds_per_cu / size_t(wps);
-  if (per <= 3072) return g;
-  const size_t b = std::min<size_t>(per - 2048, 65536) & ~size_t(1023);
-  if (size_t(wps) * (b + 2048) <= lds_per_cu && size_t(wps + 1) * b > lds_per_cu) g.lds_bytes = unsigned(b);
+  // the least whole KiB (at most the 64 KiB a work-group may allocate) with
+  // which wps + 1 work-groups do not fit, while wps fit with 2 KiB of slack
+  // each for the allocation granularity; the rest of the CU's LDS stays free
+  // for kernels running beside the grid (the selection's histograms)
+  const size_t b = (lds_per_cu / size_t(wps + 1) + 1024) & ~size_t(1023);
+  if (b <= 65536 && size_t(wps) * (b + 2048) <= lds_per_cu && size_t(wps + 1) * b > lds_per_cu)
+    g.lds_bytes = unsigned(b);
   return g;
 }
 
@@ -197,6 +202,21 @@ hipError_t launch_split(const BatchArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_split_kernel<kFormB>), dim3(blocks), dim3(256), 0, s, a.protein,
                        a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.stride, a.ngroups,
                        a.ntiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_indexed(const BatchArgs &a, int blocks, hipStream_t s) {
+  if (!a.pose_index || !a.ntiles_dev || blocks < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, a.counter);
+  if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+  if (a.form == kFormD)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<1, 256, kFormD, true>), dim3(unsigned(blocks)), dim3(256),
+                       a.lds_bytes, s, a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses,
+                       a.energies, a.counter, a.stride, a.ngroups, 0, a.pose_index, a.ntiles_dev);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_batch_kernel<1, 256, kFormB, true>), dim3(unsigned(blocks)), dim3(256),
+                       a.lds_bytes, s, a.protein, a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses,
+                       a.energies, a.counter, a.stride, a.ngroups, 0, a.pose_index, a.ntiles_dev);
   return hipGetLastError();
 }
 

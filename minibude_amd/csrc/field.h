@@ -54,10 +54,13 @@ struct ScoreArgs {
   const float *poses;         // 6 rows of `stride` floats
   size_t stride;
   float *out;                 // out[meta.pose_off + j]
+  int64_t tile0;              // first tile scored
+  bool raise_priority;        // waves issue ahead of a concurrent kernel's
 };
 
-// Interpolated energy of every pose of every ligand: one wave per tile of 64
-// poses of one ligand; ntiles = tile_begin[nligands].
+// Interpolated energy of every pose of tiles [tile0, tile0 + ntiles): one
+// wave per tile of 64 poses of one ligand; every tile is tile0 = 0, ntiles =
+// tile_begin[nligands].
 hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s);
 
 }  // namespace field

@@ -69,10 +69,15 @@ __global__ __launch_bounds__(256) void score_kernel(const uint4 *__restrict__ ce
                                                     const LigandMeta *__restrict__ meta,
                                                     const int64_t *__restrict__ tile_begin, int nligands,
                                                     const float *__restrict__ poses, size_t stride,
-                                                    float *__restrict__ out, int64_t ntiles) {
+                                                    float *__restrict__ out, int64_t tile0, int64_t tend,
+                                                    int raise_priority) {
+  // Optional issue priority over a concurrent exact pass (the pipelined
+  // combined screen rescoring the previous chunk): these waves wait on cell
+  // reads most of the time and get the VALU the moment their data arrives.
+  if (raise_priority) __builtin_amdgcn_s_setprio(2);
   const int lane = threadIdx.x & 63;
-  const int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (t >= ntiles) return;
+  const int64_t t = tile0 + int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (t >= tend) return;
   // ligand of this tile: last l with tile_begin[l] <= t (wave-uniform search)
   int lo = 0, hi = nligands - 1;
   while (lo < hi) {
@@ -136,10 +141,12 @@ hipError_t pack_cells(const float *maps, int nslots, const Grid &g, uint4 *cells
 
 hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
-  if (a.nligands <= 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 || (ntiles + 3) / 4 > 0x7fffffff)
+  if (a.nligands <= 0 || a.tile0 < 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 ||
+      (ntiles + 3) / 4 > 0x7fffffff)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a.cells, a.grid, a.ligands,
-                     a.meta, a.tile_begin, a.nligands, a.poses, a.stride, a.out, ntiles);
+                     a.meta, a.tile_begin, a.nligands, a.poses, a.stride, a.out, a.tile0, a.tile0 + ntiles,
+                     int(a.raise_priority));
   return hipGetLastError();
 }
 

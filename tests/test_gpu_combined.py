@@ -328,3 +328,64 @@ def test_mlp_combined_learns(gpu_device):
         print(f"mlp {name}: recall {out[name][0]}, calibrated val_rmse {out[name][1]}", flush=True)
     assert np.all(out["trained"][1] < out["untrained"][1])
     assert out["trained"][0].mean() > max(0.15, 3 * out["untrained"][0].mean())
+
+
+@pytest.mark.parametrize("keep", [0.05, 1.0])
+def test_field_pipeline_chunks_give_the_same_screen(gpu_device, keep):
+    # ligand chunks scored, selected and rescored as a pipeline (2, 5 and the
+    # clamp of 16 to the 7 ligands) select the same poses as one chunk and
+    # return the same exact energies, bit for bit; ragged pose counts put
+    # chunk edges next to partial tiles
+    w = small_workload(nposes=3000, nligands=7, nproteins=2, seed=15)
+    poses = [w.poses[0][:, :1], w.poses[1][:, :63], w.poses[2][:, :65], w.poses[3], w.poses[4][:, :129],
+             w.poses[5][:, :64], w.poses[6]]
+    runs = {}
+    for chunks in (1, 2, 5, 16):
+        cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48, keep_fraction=keep,
+                               pipeline_chunks=chunks)
+        with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
+            out = []
+            for p, protein in enumerate(w.proteins):
+                print(f"pipeline_chunks={chunks} keep={keep} protein {p}", flush=True)
+                r = c.screen(protein)
+                assert r.fallback_pairs == 0
+                c.exhaustive()
+                sel = [c.selected(l) for l in range(7)]
+                for l, (idx, e) in enumerate(sel):
+                    exact = c.exact(l)
+                    assert np.array_equal(e.view(np.uint32), exact[idx].view(np.uint32)), (chunks, p, l)
+                    if keep == 1.0:
+                        assert len(idx) == poses[l].shape[1]
+                assert r.nselected == sum(len(i) for i, _ in sel)
+                out.append(sel)
+            runs[chunks] = out
+    for chunks in (2, 5, 16):
+        for p in range(2):
+            for l in range(7):
+                assert np.array_equal(runs[chunks][p][l][0], runs[1][p][l][0]), (chunks, p, l)
+                assert np.array_equal(runs[chunks][p][l][1].view(np.uint32), runs[1][p][l][1].view(np.uint32))
+
+
+def test_field_pipeline_with_fallback_pairs(gpu_device):
+    # a pair that falls back switches the screen to one selection over the
+    # whole set after the chunked scoring; the other pairs are still rescored
+    w = small_workload(nposes=2000, nligands=6, nproteins=1, seed=16)
+    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48, pipeline_chunks=3)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        c.screen(w.proteins[0])
+        cal = np.array([c.pair(l).val_rmse_calibrated for l in range(6)])
+    cfg = C.default_config("field", max_val_rmse=float(np.sort(cal)[3]), labels_per_pair=48, pipeline_chunks=3)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        r = c.screen(w.proteins[0])
+        c.exhaustive()
+        fb = [c.pair(l).fallback for l in range(6)]
+        assert r.fallback_pairs == sum(fb) == 2
+        for l in range(6):
+            exact = c.exact(l)
+            idx, e = c.selected(l)
+            score = c.scores(l)
+            k = kept(0.02, 2000)
+            assert set(idx.tolist()) == set(np.nonzero(score <= np.sort(score)[k - 1])[0].tolist()), l
+            assert np.array_equal(e.view(np.uint32), exact[idx].view(np.uint32)), l
+            if fb[l]:
+                assert np.array_equal(score.view(np.uint32), exact.view(np.uint32))
