@@ -1,11 +1,12 @@
 // capi_combined.hip -- C ABI of the combined screen: a fixed set of ligands,
 // each with its own pose set, screened against one protein after another.
 // Per protein: exact energies of a few sampled poses of every (protein,
-// ligand) pair (one ragged batched launch), a surrogate (the per-protein MLP
-// or the protein's single-atom energy maps) validated per pair on them, the
+// ligand) pair (one ragged launch), a surrogate (the per-protein MLP or the
+// protein's single-atom energy maps) validated per pair on them, the
 // surrogate over every pose of every pair, per-pair top-k of its scores, and
-// exact rescoring of every pair's selection in one ragged batched launch;
-// pairs whose validation error is too large are scored exhaustively instead.
+// exact rescoring of every pair's selection (with the maps, chunk by chunk of
+// ligands, each chunk's rescoring beside the next chunk's scoring); pairs
+// whose validation error is too large are scored exhaustively instead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -771,7 +772,7 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   int2 *rs_tiles = static_cast<int2 *>(c->tiles_rs.ptr);
   int32_t *rs_ntiles = static_cast<int32_t *>(c->rs_ntiles.ptr);
 
-  // 4. exact energies of the sampled poses (one ragged batched launch) on the
+  // 4. exact energies of the sampled poses (one ragged split launch) on the
   //    first stream, and, for the field, maps -> cells -> scores of every
   //    pose of every pair on the second stream, concurrently
   if (int st = record(c, 0)) return st;

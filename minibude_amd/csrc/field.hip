@@ -50,6 +50,9 @@ __device__ __forceinline__ _Float16 to_half(float v) { return _Float16(fminf(fma
 
 __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ maps, const Grid g,
                                                    uint4 *__restrict__ cells) {
+  // on the critical path like the maps: issue ahead of the concurrent
+  // sampled-pose pass (0.26 ms beside it, 0.03 ms alone)
+  __builtin_amdgcn_s_setprio(2);
   const size_t nc = g.cells(), np = g.points();
   const size_t c = size_t(blockIdx.x) * 256 + threadIdx.x;
   if (c >= nc) return;
