@@ -389,3 +389,27 @@ def test_field_pipeline_with_fallback_pairs(gpu_device):
             assert np.array_equal(e.view(np.uint32), exact[idx].view(np.uint32)), l
             if fb[l]:
                 assert np.array_equal(score.view(np.uint32), exact.view(np.uint32))
+
+
+@pytest.mark.parametrize("chunks", [1, 3])
+def test_tied_scores_are_all_rescored(gpu_device, chunks):
+    # every pose repeated three times: tied scores, so a pair keeps more than
+    # ceil(keep n) poses (all ties at the threshold); the device-built
+    # rescoring tables must cover every kept pose
+    w = small_workload(nposes=1200, nligands=4, nproteins=1, seed=17)
+    poses = [np.ascontiguousarray(np.repeat(q[:, :400], 3, axis=1)) for q in w.poses]
+    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48, keep_fraction=0.0085,
+                           pipeline_chunks=chunks)
+    with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
+        r = c.screen(w.proteins[0])
+        c.exhaustive()
+        for l in range(4):
+            idx, e = c.selected(l)
+            score = c.scores(l)
+            k = kept(0.0085, 1200)  # 11: the threshold falls inside a triple
+            thr = np.sort(score)[k - 1]
+            want = np.nonzero(score <= thr)[0]
+            assert len(idx) == len(want) and len(idx) % 3 == 0 and len(idx) > k, (l, len(idx), k)
+            assert np.array_equal(idx, want)
+            assert np.array_equal(e.view(np.uint32), c.exact(l)[idx].view(np.uint32)), l
+        assert r.nselected == sum(len(c.selected(l)[0]) for l in range(4))
