@@ -200,6 +200,7 @@ struct bude_combined {
   std::vector<int64_t> chunk_expect;
   std::vector<hipEvent_t> chunk_ev;
   std::vector<int64_t> lig_expect;    // expected rescoring tiles of each pair: ceil(k / 64)
+  bool plan_ok = false;               // the selection tables above are complete
   bude::field::Grid grid{};
   // per protein
   bool screened = false;
@@ -435,8 +436,10 @@ std::vector<int32_t> heaviest_first(const bude_combined *c, size_t a, size_t b, 
 
 // Selection segments (the lowest ceil(keep n) of each pair) of the whole set
 // and of each pipeline chunk, the chunks' rescoring tile slots and ligand
-// lists, and their events.  Both streams must be idle.
+// lists, and their events.  Both streams must be idle.  Until it succeeds
+// the object screens nothing (a failure part way leaves mixed tables).
 int plan_selection(bude_combined *c, float keep) {
+  c->plan_ok = false;
   const size_t L = c->nlig;
   std::vector<bude::sel::Segment> segs(L);
   std::vector<int64_t> &kt = c->lig_expect, cap(L);
@@ -494,6 +497,7 @@ int plan_selection(bude_combined *c, float keep) {
     if (int st = bude::hip_check(hipEventCreate(&e), "hipEventCreate")) return st;
     c->chunk_ev.push_back(e);
   }
+  c->plan_ok = true;
   return BUDE_OK;
 }
 
@@ -666,6 +670,10 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     return st;
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   *res = BudeCombinedResult{};
+  if (!c->plan_ok)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "bude_mi355x_combined_screen: the selection tables are incomplete (a failed set_keep); "
+                      "call bude_mi355x_combined_set_keep again");
   c->screened = false;
   {  // FNV-1a of the protein: exact energies stay valid while it is unchanged
     uint64_t hsum = 1469598103934665603ull ^ uint64_t(natpro);
@@ -1007,7 +1015,10 @@ extern "C" int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fracti
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(c->stream2), "hipStreamSynchronize")) return st;
-  if (int st = plan_selection(c, keep_fraction)) return st;
+  if (int st = plan_selection(c, keep_fraction)) {
+    c->screened = false;
+    return st;
+  }
   c->cfg.keep_fraction = keep_fraction;
   return bude::ok();
 }
