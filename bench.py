@@ -87,6 +87,9 @@ def cpu_baseline(deck: D.Deck, ref: np.ndarray) -> dict | None:
     """The reference omp model on the first CPU_SAMPLE_POSES poses of the deck."""
     flags = Path("/proc/cpuinfo").read_text() if Path("/proc/cpuinfo").exists() else ""
     v4 = " avx512f " in flags.replace("\n", " ")
+    cpu = re.search(r"^model name\s*:\s*(.+)$", flags, re.M)
+    cpu = cpu.group(1).strip() if cpu else "unknown CPU"
+    bind = os.environ.get("OMP_PROC_BIND", "unset")
     exe = ROOT / "oracle" / "_ref" / ("omp-bude-v4" if v4 else "omp-bude-v3")
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     if not exe.exists():
@@ -129,8 +132,8 @@ def cpu_baseline(deck: D.Deck, ref: np.ndarray) -> dict | None:
         "sample": (f"first {n} of {deck.nposes} poses of the {WORKLOAD} deck ({deck.natpro} protein / {deck.natlig} "
                    f"ligand atoms); reference omp model {exe.name} (src/main.cpp + src/omp/fasten.hpp, -O3 "
                    f"-ffast-math -fopenmp), ppwi=64, 2 warmup + {CPU_ITERS} timed passes timed by its own "
-                   f"raw_iterations, {wall:.1f}s wall; its validation against the strict reference kernel: "
-                   f"valid={valid}"),
+                   f"raw_iterations, {wall:.1f}s wall ({cores} threads of {os.cpu_count()} on {cpu}, "
+                   f"OMP_PROC_BIND={bind}); its validation against the strict reference kernel: valid={valid}"),
     }
 
 
