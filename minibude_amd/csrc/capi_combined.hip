@@ -108,6 +108,20 @@ __global__ __launch_bounds__(256) void rescore_tables_kernel(const int32_t *__re
 
 unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
 
+// Pair coefficients of every ligand atom against every protein group:
+// coef[a * ngroups + g] = slot_coef[slot(a) * ngroups + g], the atom's map
+// slot in the w bits of its float4 (a coefficient depends on the ligand
+// atom's type and the group's type only).
+__global__ __launch_bounds__(256) void expand_coef_kernel(const float4 *__restrict__ ligands, size_t n, int ngroups,
+                                                          const bude::PairCoef *__restrict__ slot_coef,
+                                                          bude::PairCoef *__restrict__ coef) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const size_t a = i / size_t(ngroups), g = i - a * size_t(ngroups);
+  const int slot = __float_as_int(ligands[a].w);
+  coef[i] = slot_coef[size_t(slot) * size_t(ngroups) + g];
+}
+
 // Eleven numbers describing a ligand to the per-protein MLP: atom count,
 // radius of gyration, the normalised second-moment tensor (6) in the
 // ligand's own frame (with the rotation-matrix inputs this orients the
@@ -686,27 +700,30 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   const BudeCombinedConfig &cfg = c->cfg;
   const bool mlp = cfg.surrogate == BUDE_SURROGATE_MLP;
 
-  // 1. protein tables: grouped atoms, pair coefficients of every ligand atom
-  //    (and of every map slot)
+  // 1. protein tables: grouped atoms and the pair coefficients of every map
+  //    slot's type against every protein group (a coefficient depends on
+  //    the two types only); the table of every ligand atom ([atom][group],
+  //    pair l's at coef_off[l] = lig_off[l] * ngroups) is expanded from it
+  //    on the device
   const bude::ProteinGroups groups = bude::group_protein(protein, natpro);
   c->ngroups = groups.ngroups();
-  std::vector<bude::PairCoef> coef;
+  const size_t natoms = size_t(c->lig_off.back()) + c->lig.back().size(), ncoef = natoms * c->ngroups;
+  if (ncoef > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: pair tables too large");
   c->coef_off.assign(L, 0);
-  for (size_t l = 0; l < L; l++) {
-    c->coef_off[l] = int32_t(coef.size());
-    const std::vector<bude::PairCoef> t = bude::coef_table(c->lig[l].data(), c->lig[l].size(), groups, c->ff.data());
-    coef.insert(coef.end(), t.begin(), t.end());
-    if (coef.size() > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: pair tables too large");
-  }
-  if (int st = c->protein.assign(groups.xyzw.data(), groups.xyzw.size() * 4)) return st;
-  if (int st = c->group_begin.assign(groups.begin.data(), groups.begin.size() * 4)) return st;
-  if (int st = c->coef.assign(coef.data(), coef.size() * sizeof(bude::PairCoef))) return st;
-  if (!mlp) {
+  for (size_t l = 0; l < L; l++) c->coef_off[l] = int32_t(size_t(c->lig_off[l]) * c->ngroups);
+  {
     std::vector<BudeAtom> probe(c->slot_type.size());
     for (size_t s = 0; s < probe.size(); s++) probe[s] = BudeAtom{0.f, 0.f, 0.f, c->slot_type[s]};
     const std::vector<bude::PairCoef> sc = bude::coef_table(probe.data(), probe.size(), groups, c->ff.data());
     if (int st = c->slot_coef.assign(sc.data(), sc.size() * sizeof(bude::PairCoef))) return st;
   }
+  if (int st = c->protein.assign(groups.xyzw.data(), groups.xyzw.size() * 4)) return st;
+  if (int st = c->group_begin.assign(groups.begin.data(), groups.begin.size() * 4)) return st;
+  if (int st = c->coef.reserve(ncoef * sizeof(bude::PairCoef))) return st;
+  hipLaunchKernelGGL(expand_coef_kernel, dim3(blocks_for(ncoef, 256)), dim3(256), 0, c->stream,
+                     static_cast<const float4 *>(c->ligands.ptr), ncoef, int(c->ngroups),
+                     static_cast<const bude::PairCoef *>(c->slot_coef.ptr), static_cast<bude::PairCoef *>(c->coef.ptr));
+  if (int st = bude::hip_check(hipGetLastError(), "expand pair tables")) return st;
   res->prepare_ms = float(ms_since(t0));
 
   // 2. sampled poses of every pair: first 15/16 train the MLP, the rest
