@@ -12,6 +12,8 @@ from minibude_amd import deck as D
 
 def test_struct_layouts_match_the_header():
     assert ctypes.sizeof(C.CombinedConfig) == 56
+    # pipeline_chunks sits in what was the padding before the 8-byte seed
+    assert C.CombinedConfig.pipeline_chunks.offset == 44 and C.CombinedConfig.seed.offset == 48
     assert ctypes.sizeof(C.CombinedResult) == 64
     assert ctypes.sizeof(C.PairResult) == 20
 
@@ -39,6 +41,8 @@ def test_create_checks_config_and_ligands_first():
         C.Combined(w.forcefield, w.ligands, w.poses, cfg)
     with pytest.raises(capi.BudeInvalidArgument, match="grid_spacing"):
         C.Combined(w.forcefield, w.ligands, w.poses, C.default_config("field", grid_spacing=0.0))
+    with pytest.raises(capi.BudeInvalidArgument, match="pipeline_chunks"):
+        C.Combined(w.forcefield, w.ligands, w.poses, C.default_config("field", pipeline_chunks=17))
     bad = [l.copy() for l in w.ligands]
     bad[1]["type"][0] = 99
     with pytest.raises(capi.BudeInvalidArgument, match="ligand 1"):
