@@ -413,3 +413,28 @@ def test_tied_scores_are_all_rescored(gpu_device, chunks):
             assert np.array_equal(idx, want)
             assert np.array_equal(e.view(np.uint32), c.exact(l)[idx].view(np.uint32)), l
         assert r.nselected == sum(len(c.selected(l)[0]) for l in range(4))
+
+
+def test_automatic_chunking_follows_set_keep(gpu_device):
+    # 8 pairs of 262,144 poses: keeping every pose expects 32,768 rescoring
+    # tiles, which the automatic choice splits into two pipeline chunks on a
+    # 256-CU device (one chunk at 2 %); the chunk tables are rebuilt by
+    # set_keep and the screens agree with the exhaustive energies either way
+    w = small_workload(nposes=262144, nligands=8, nproteins=1, seed=18)
+    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=64, keep_fraction=1.0)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        for keep in (1.0, 0.02, 1.0):
+            c.set_keep(keep)
+            print(f"keep {keep}", flush=True)
+            r = c.screen(w.proteins[0])
+            c.exhaustive()
+            total = 0
+            for l in range(8):
+                idx, e = c.selected(l)
+                exact = c.exact(l)
+                score = c.scores(l)
+                k = kept(keep, 262144)
+                assert set(idx.tolist()) == set(np.nonzero(score <= np.sort(score)[k - 1])[0].tolist()), (keep, l)
+                assert np.array_equal(e.view(np.uint32), exact[idx].view(np.uint32)), (keep, l)
+                total += len(idx)
+            assert r.nselected == total
