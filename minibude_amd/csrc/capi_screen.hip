@@ -53,11 +53,11 @@ struct bude_screen {
   float clip = 0.f, mu = 0.f, sigma = 1.f, tail = -3.0e38f, val_rmse = 0.f;
   // candidate poses
   size_t ncand = 0;
-  DeviceBuffer cand, pred, sel_idx, sel_n, gathered, exact;
-  // rescoring: the persistent batched kernel over tiles (0, 64 j) of the
-  // gathered poses, one ligand whose pose count is the selection's size
-  DeviceBuffer rs_tiles, rs_meta, rs_counter;
-  bude::LigandMeta rs_meta_host{};
+  DeviceBuffer cand, pred, sel_idx, sel_n, exact;
+  // rescoring: the persistent batched kernel over the selection (its rows
+  // through sel_idx), one ligand whose pose count and tiles (0, 64 j) the
+  // device writes from the selection's size
+  DeviceBuffer rs_tiles, rs_meta, rs_counter, rs_order, rs_ntiles;
   bude::sel::Plan plan;  // one segment: the loaded poses, k = ceil(keep_fraction * n)
   size_t plan_n = 0, plan_k = 0;
   size_t nsel = 0;
@@ -100,20 +100,19 @@ int upload_poses(DeviceBuffer &d, const float *const poses[6], size_t n) {
   return BUDE_OK;
 }
 
-// Exact energies of the nsel gathered poses into exact[0, nsel): the
-// persistent batched kernel (bit-identical to the plain kernel) on tiles of
-// 64 poses pulled from a queue by a grid pinned to 3 waves on every SIMD, so
-// the SIMDs finish together on a pass of a few thousand tiles.  The stream is idle
-// here (the selection's size was just read back), so the one-ligand table is
-// uploaded now.
-int rescore(bude_screen *s, size_t nsel) {
-  if (nsel == 0) return BUDE_OK;
-  s->rs_meta_host = bude::LigandMeta{};
-  s->rs_meta_host.natlig = int32_t(s->natlig);
-  s->rs_meta_host.nposes = int32_t(nsel);
-  if (int st = bude::hip_check(hipMemcpyAsync(s->rs_meta.ptr, &s->rs_meta_host, sizeof(bude::LigandMeta),
-                                              hipMemcpyHostToDevice, s->stream),
-                               "hipMemcpyAsync rescoring table"))
+// Exact energies of the selected poses (rows sel_idx[0, nsel) of the
+// candidates) into exact[0, nsel): the persistent batched kernel
+// (bit-identical to the plain kernel) on tiles of 64 poses pulled from a
+// queue by a grid pinned to 3 waves on every SIMD, so the SIMDs finish
+// together on a pass of a few thousand tiles.  The tables come from the
+// selection's count on the device; the host does not wait for it.
+int rescore(bude_screen *s) {
+  if (int st = bude::hip_check(bude::build_rescore_tables(static_cast<const int32_t *>(s->rs_order.ptr), 1,
+                                                          static_cast<const uint32_t *>(s->sel_n.ptr),
+                                                          static_cast<bude::LigandMeta *>(s->rs_meta.ptr),
+                                                          static_cast<int2 *>(s->rs_tiles.ptr),
+                                                          static_cast<int32_t *>(s->rs_ntiles.ptr), s->stream),
+                               "rescoring tables"))
     return st;
   bude::BatchArgs a{};
   a.protein = static_cast<const float4 *>(s->protein.ptr);
@@ -122,16 +121,17 @@ int rescore(bude_screen *s, size_t nsel) {
   a.ligands = static_cast<const float4 *>(s->ligand.ptr);
   a.meta = static_cast<const bude::LigandMeta *>(s->rs_meta.ptr);
   a.tiles = static_cast<const int2 *>(s->rs_tiles.ptr);
-  a.poses = static_cast<const float *>(s->gathered.ptr);
+  a.poses = static_cast<const float *>(s->cand.ptr);
   a.energies = static_cast<float *>(s->exact.ptr);
   a.counter = static_cast<unsigned int *>(s->rs_counter.ptr);
-  a.stride = nsel;
+  a.stride = s->ncand;
   a.ngroups = int(s->ngroups);
-  a.ntiles = int((nsel + 63) / 64);
   a.form = bude::plain_form();
+  a.pose_index = static_cast<const uint32_t *>(s->sel_idx.ptr);
+  a.ntiles_dev = static_cast<const int32_t *>(s->rs_ntiles.ptr);
   const bude::PinnedGrid g = bude::pinned_grid(s->cus, s->lds_per_cu, 3);
   a.lds_bytes = g.lds_bytes;
-  return bude::hip_check(bude::launch_batch(a, 1, g.wgsize, g.blocks, s->stream), "rescore launch");
+  return bude::hip_check(bude::launch_batch_indexed(a, g.blocks, s->stream), "rescore launch");
 }
 
 // Surrogate predictions (normalised target units) for n poses whose 6 rows
@@ -298,15 +298,18 @@ extern "C" int bude_mi355x_screen_load(bude_screen *s, const float *const poses[
   int st = upload_poses(s->cand, poses, n);
   if (!st) st = s->pred.alloc(n * 4);
   if (!st) st = s->sel_idx.alloc(n * 4);
-  if (!st) st = s->gathered.alloc(6 * n * 4);
   if (!st) st = s->exact.alloc(n * 4);
-  if (!st) {
-    std::vector<int2> t((n + 63) / 64);
-    for (size_t j = 0; j < t.size(); j++) t[j] = make_int2(0, int(64 * j));
-    st = s->rs_tiles.assign(t.data(), t.size() * sizeof(int2));
-  }
+  if (!st) st = s->rs_tiles.alloc((n + 63) / 64 * sizeof(int2));
   if (!st) st = s->rs_counter.alloc(sizeof(unsigned int));
-  if (!st) st = s->rs_meta.alloc(sizeof(bude::LigandMeta));
+  if (!st) st = s->rs_ntiles.alloc(sizeof(int32_t));
+  if (!st) {  // the one ligand's row of the rescoring meta (its pose count comes from the selection)
+    const bude::LigandMeta m{0, int32_t(s->natlig), 0, 0, 0, 0};
+    st = s->rs_meta.assign(&m, sizeof(m));
+  }
+  if (!st) {
+    const int32_t zero = 0;
+    st = s->rs_order.assign(&zero, sizeof(zero));
+  }
   return st;
 }
 
@@ -334,7 +337,8 @@ extern "C" int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res) {
     if (int st = bude::hip_check(hipEventRecord(s->ev[1], st_), "hipEventRecord")) return st;
   } else {
     if (int st = bude::hip_check(hipEventRecord(s->ev[0], st_), "hipEventRecord")) return st;
-    if (int st = predict_all(s, static_cast<const float *>(s->cand.ptr), n, n, static_cast<float *>(s->pred.ptr)))
+    if (int st = bude::surrogate::predict(s->net, static_cast<const float *>(s->cand.ptr), n, n,
+                                          static_cast<float *>(s->pred.ptr), 2 * s->cus, st_))
       return st;
     if (int st = bude::hip_check(hipEventRecord(s->ev[1], st_), "hipEventRecord")) return st;
   }
@@ -343,23 +347,15 @@ extern "C" int bude_mi355x_screen_run(bude_screen *s, BudeScreenResult *res) {
                                                                  static_cast<uint32_t *>(s->sel_n.ptr), st_),
                                "select"))
     return st;
+  if (int st = bude::hip_check(hipEventRecord(s->ev[2], st_), "hipEventRecord")) return st;
+  if (!s->fallback)  // (in fallback the exact energies are already in pred)
+    if (int st = rescore(s)) return st;
+  if (int st = bude::hip_check(hipEventRecord(s->ev[3], st_), "hipEventRecord")) return st;
   uint32_t nsel = 0;
   if (int st = bude::hip_check(hipMemcpyAsync(&nsel, s->sel_n.ptr, 4, hipMemcpyDeviceToHost, st_), "hipMemcpyAsync"))
     return st;
-  if (int st = bude::hip_check(hipEventRecord(s->ev[2], st_), "hipEventRecord")) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize")) return st;
   s->nsel = nsel;
-  if (!s->fallback) {  // (in fallback the exact energies are already in pred)
-    if (int st = bude::hip_check(bude::mlp::gather_poses(static_cast<const float *>(s->cand.ptr), n,
-                                                         static_cast<const uint32_t *>(s->sel_idx.ptr),
-                                                         static_cast<const uint32_t *>(s->sel_n.ptr), nsel,
-                                                         static_cast<float *>(s->gathered.ptr), nsel, st_),
-                                 "gather"))
-      return st;
-    if (int st = rescore(s, nsel)) return st;
-  }
-  if (int st = bude::hip_check(hipEventRecord(s->ev[3], st_), "hipEventRecord")) return st;
-  if (int st = bude::hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize")) return st;
   (void)hipEventElapsedTime(&ms01, s->ev[0], s->ev[1]);
   (void)hipEventElapsedTime(&ms12, s->ev[1], s->ev[2]);
   (void)hipEventElapsedTime(&ms23, s->ev[2], s->ev[3]);

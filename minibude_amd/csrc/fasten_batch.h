@@ -59,6 +59,15 @@ PinnedGrid pinned_grid(int cus, size_t lds_per_cu, int wps);
 // work-items, each wave scoring tiles of 64 * rb poses.
 hipError_t launch_batch(const BatchArgs &a, int rb, int wgsize, int blocks, hipStream_t s);
 
+// Work tables of a rescoring pass over per-segment selections, built on the
+// device from the selection counts: for the ligands order[0..n) (list
+// order, e.g. heaviest first) meta[l].nposes = sel_n[l] (every other field
+// of meta[l] set beforehand), tiles (l, 64 i) for i < ceil(sel_n[l] / 64)
+// from tiles[0] (room for the worst case is the caller's), and the tile
+// count in *ntiles; one work-group, on stream s after the selection.
+hipError_t build_rescore_tables(const int32_t *order, int n, const uint32_t *sel_n, LigandMeta *meta, int2 *tiles,
+                                int32_t *ntiles, hipStream_t s);
+
 // launch_batch over rows that are a per-segment selection (pose_index) with
 // tiles and tile count written by an earlier kernel (ntiles_dev): one pose
 // per work-item (rb 1), work-groups of 256.  Energies are bit-equal to

@@ -147,6 +147,50 @@ __global__ __launch_bounds__(256) void fasten_split_kernel(
   if (sl == 0 && j < m.nposes) energies[size_t(m.pose_off) + size_t(j)] = e * 0.5f;
 }
 
+// Rescoring work tables of the ligands order[0..n) from their selection
+// counts, on the device right after the selection:
+// meta[l].nposes = sel_n[l], tiles (l, 64 i) for i < ceil(sel_n[l] / 64) in
+// list order from tiles[0], and the tile count in *ntiles.  One work-group
+// of 256; the host never waits for the counts.
+__global__ __launch_bounds__(256) void rescore_tables_kernel(const int32_t *__restrict__ order, int n,
+                                                             const uint32_t *__restrict__ sel_n,
+                                                             LigandMeta *__restrict__ meta,
+                                                             int2 *__restrict__ tiles, int32_t *__restrict__ ntiles) {
+  __shared__ int32_t lig[256], cnt[256], off[256], wsum[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t base = 0;
+  for (int b = 0; b < n; b += 256) {
+    const int i = b + int(threadIdx.x);
+    int32_t c = 0, l = 0;
+    if (i < n) {
+      l = order[i];
+      const uint32_t k = sel_n[l];
+      meta[l].nposes = int32_t(k);
+      c = int32_t((k + 63) / 64);
+    }
+    int32_t x = c;  // inclusive scan over the wave, then over the four waves
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t y = __shfl_up(x, d);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int32_t wb = 0;
+    for (int q = 0; q < w; q++) wb += wsum[q];
+    lig[threadIdx.x] = l;
+    cnt[threadIdx.x] = c;
+    off[threadIdx.x] = base + wb + x - c;
+    const int32_t sum = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    const int m = min(256, n - b);
+    for (int q = 0; q < m; q++)
+      for (int t = int(threadIdx.x); t < cnt[q]; t += 256) tiles[off[q] + t] = make_int2(lig[q], 64 * t);
+    base += sum;
+    __syncthreads();  // lig/cnt/off/wsum are rewritten by the next batch
+  }
+  if (threadIdx.x == 0) *ntiles = base;
+}
+
 template <int RB, int WG>
 hipError_t launch_rb_wg(const BatchArgs &a, int blocks, hipStream_t s) {
   if (a.form == kFormD)
@@ -202,6 +246,13 @@ hipError_t launch_split(const BatchArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(fasten_split_kernel<kFormB>), dim3(blocks), dim3(256), 0, s, a.protein,
                        a.group_begin, a.coef, a.ligands, a.meta, a.tiles, a.poses, a.energies, a.stride, a.ngroups,
                        a.ntiles);
+  return hipGetLastError();
+}
+
+hipError_t build_rescore_tables(const int32_t *order, int n, const uint32_t *sel_n, LigandMeta *meta, int2 *tiles,
+                                int32_t *ntiles, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(rescore_tables_kernel, dim3(1), dim3(256), 0, s, order, n, sel_n, meta, tiles, ntiles);
   return hipGetLastError();
 }
 

@@ -121,9 +121,6 @@ hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hi
 hipError_t infer(const float *poses, size_t stride, size_t n, const bf16 *const wt[4], const bf16 *w3,
                  const float *const bias[4], float *out, int blocks, hipStream_t s);
 
-// Gather 6 pose rows for idx[0..n) into dst rows of stride dst_stride.
-hipError_t gather_poses(const float *src, size_t src_stride, const uint32_t *idx, const uint32_t *n_dev, size_t n_max,
-                        float *dst, size_t dst_stride, hipStream_t s);
 
 }  // namespace mlp
 }  // namespace bude

@@ -292,15 +292,6 @@ __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, 
   if (w16t) w16t[size_t(i % cols) * rows + i / cols] = h;
 }
 
-__global__ void gather_poses_kernel(const float *__restrict__ src, size_t src_stride, const uint32_t *__restrict__ idx,
-                                    const uint32_t *__restrict__ n_dev, float *__restrict__ dst, size_t dst_stride) {
-  const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= *n_dev) return;
-  const size_t q = idx[j];
-#pragma unroll
-  for (int k = 0; k < 6; k++) dst[size_t(k) * dst_stride + j] = src[size_t(k) * src_stride + q];
-}
-
 // ---- fused inference ---------------------------------------------------------
 // One work-group (8 waves) scores tiles of kTile poses through the whole
 // network with the activations in one LDS buffer (pose-major rows):
@@ -530,14 +521,6 @@ hipError_t infer(const float *poses, size_t stride, size_t n, const bf16 *const 
   const unsigned grid = unsigned(std::min<size_t>(tiles, size_t(blocks)));
   hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(kInferThreads), 0, s, poses, stride, n, wt[0], bias[0], wt[1], bias[1],
                      wt[2], bias[2], w3, bias[3], out);
-  return hipGetLastError();
-}
-
-hipError_t gather_poses(const float *src, size_t src_stride, const uint32_t *idx, const uint32_t *n_dev, size_t n_max,
-                        float *dst, size_t dst_stride, hipStream_t s) {
-  if (n_max == 0) return hipSuccess;
-  hipLaunchKernelGGL(gather_poses_kernel, dim3(blocks_for(n_max, 256)), dim3(256), 0, s, src, src_stride, idx, n_dev,
-                     dst, dst_stride);
   return hipGetLastError();
 }
 
