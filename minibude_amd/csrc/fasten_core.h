@@ -67,7 +67,10 @@ __device__ __forceinline__ void atom_pairs(const float4 p, const PairCoef &c, fl
     const float dx = lx[k] - p.x;
     const float dy = ly[k] - p.y;
     const float dz = lz[k] - p.z;
-    const float d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+    // the contraction written out (dx^2, then fma dy, then fma dz): left to
+    // the compiler it can pick another order in another kernel, and every
+    // kernel sharing this code must round the same way
+    const float d = __builtin_amdgcn_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
     if constexpr (FORM == kFormB) {
       const float b = d - c.radij;
       e[k] = __builtin_fmaf(2.0f * kHardness, sat(b * c.neg_rr), e[k]);

@@ -11,6 +11,69 @@ namespace {
 
 using namespace core;
 
+// Distance from a protein atom beyond which all three hinges of a pair with
+// coefficients c are exactly zero in either hinge form (the largest zero
+// crossing of the terms present, widened by a margin far above the
+// rounding of the distances compared with it).  Past it a pair adds
+// fma(w, 0, e) == e, so leaving it out changes no bit of the sum.
+__device__ __forceinline__ float pair_reach(const PairCoef &c) {
+  float r = fmaxf(c.radij, -1.0f / c.neg_rr);
+  if (c.wq != 0.f) r = fmaxf(r, fmaxf(c.c0 / -c.neg_e1, c.radij - 1.0f / c.neg_e1));
+  if (c.wd != 0.f && c.neg_rd != 0.f) r = fmaxf(r, fmaxf(c.e0 / -c.neg_rd, c.radij - 1.0f / c.neg_rd));
+  return r * 1.0001f + 1e-3f;
+}
+
+constexpr int kMapList = 512;  // culled atoms per wave held in LDS before they are scored
+
+// The pairs of one group's culled atoms (LDS, in atom order) with this
+// work-item's point, into part.
+template <int FORM, bool CHG, bool DSLV>
+__device__ __forceinline__ void list_pairs(const float4 *list, int n, const PairCoef c, float lx, float ly, float lz,
+                                           float &part) {
+  float c0 = 0.f, e0 = 0.f;
+  if constexpr (FORM == kFormD) {
+    if constexpr (CHG) c0 = vgpr_copy(c.c0);
+    if constexpr (DSLV) e0 = vgpr_copy(c.e0);
+  }
+  const float px[1] = {lx}, py[1] = {ly}, pz[1] = {lz};
+  float e[1] = {part};
+  int t = 0;
+  for (; t + 4 <= n; t += 4) {
+    const float4 a0 = list[t], a1 = list[t + 1], a2 = list[t + 2], a3 = list[t + 3];
+    atom_pairs<1, FORM, CHG, DSLV>(a0, c, c0, e0, px, py, pz, e);
+    atom_pairs<1, FORM, CHG, DSLV>(a1, c, c0, e0, px, py, pz, e);
+    atom_pairs<1, FORM, CHG, DSLV>(a2, c, c0, e0, px, py, pz, e);
+    atom_pairs<1, FORM, CHG, DSLV>(a3, c, c0, e0, px, py, pz, e);
+  }
+  for (; t < n; ++t) atom_pairs<1, FORM, CHG, DSLV>(list[t], c, c0, e0, px, py, pz, e);
+  part = e[0];
+}
+
+template <int FORM>
+__device__ __forceinline__ void list_dispatch(const float4 *list, int n, const PairCoef c, float lx, float ly,
+                                              float lz, float &part) {
+  const bool chg = c.wq != 0.f, dslv = c.wd != 0.f;  // wave-uniform
+  if (chg && dslv) list_pairs<FORM, true, true>(list, n, c, lx, ly, lz, part);
+  else if (chg) list_pairs<FORM, true, false>(list, n, c, lx, ly, lz, part);
+  else if (dslv) list_pairs<FORM, false, true>(list, n, c, lx, ly, lz, part);
+  else list_pairs<FORM, false, false>(list, n, c, lx, ly, lz, part);
+}
+
+// Makes a wave's LDS writes visible to its own later reads.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wave per 4 x 4 x 4 brick of grid points (lane = (x, y, z) within the
+// brick) and per map slot.  For each protein group the wave first keeps the
+// atoms within reach of its brick (64 atoms per step, compacted in order
+// into its LDS list), then scores only those: at 0.5 A spacing a brick
+// reaches about a tenth of a protein's atoms.  The pairs left out are
+// exactly zero and the kept ones are summed in the same order by the same
+// code as the plain kernel's, so every map value is the plain kernel's
+// one-atom energy bit for bit.
 template <int FORM>
 __global__ __launch_bounds__(256) void map_kernel(const float4 *__restrict__ protein,
                                                   const int32_t *__restrict__ group_begin,
@@ -20,18 +83,53 @@ __global__ __launch_bounds__(256) void map_kernel(const float4 *__restrict__ pro
   // critical path (maps -> cells -> field scores), the sampled poses are
   // not: raised issue priority lets the maps go first (2-4 % per screen).
   __builtin_amdgcn_s_setprio(2);
-  const size_t np = g.points();
-  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-  const size_t q = i < np ? i : np - 1;
-  const size_t n12 = size_t(g.n[1]) * size_t(g.n[2]);
-  const int ix = int(q / n12), iy = int((q / size_t(g.n[2])) % size_t(g.n[1])), iz = int(q % size_t(g.n[2]));
-  float lx[1] = {g.lo[0] + float(ix) * g.h};
-  float ly[1] = {g.lo[1] + float(iy) * g.h};
-  float lz[1] = {g.lo[2] + float(iz) * g.h};
-  float e[1] = {0.f};
+  __shared__ float4 lists[4][kMapList];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nb1 = (g.n[1] + 3) / 4, nb2 = (g.n[2] + 3) / 4;
+  const int64_t nbricks = int64_t((g.n[0] + 3) / 4) * nb1 * nb2;
+  const int64_t brick = int64_t(blockIdx.x) * 4 + wave;
+  if (brick >= nbricks) return;  // whole wave
+  const int bx = int(brick / (int64_t(nb1) * nb2)), by = int((brick / nb2) % nb1), bz = int(brick % nb2);
+  const int ix = 4 * bx + (lane >> 4), iy = 4 * by + ((lane >> 2) & 3), iz = 4 * bz + (lane & 3);
+  const bool valid = ix < g.n[0] && iy < g.n[1] && iz < g.n[2];
+  const float lx = g.lo[0] + float(min(ix, g.n[0] - 1)) * g.h;
+  const float ly = g.lo[1] + float(min(iy, g.n[1] - 1)) * g.h;
+  const float lz = g.lo[2] + float(min(iz, g.n[2] - 1)) * g.h;
+  // the brick's box (its points, clamped or not, lie inside it)
+  const float cx = g.lo[0] + (float(4 * bx) + 1.5f) * g.h, cy = g.lo[1] + (float(4 * by) + 1.5f) * g.h,
+              cz = g.lo[2] + (float(4 * bz) + 1.5f) * g.h;
+  const float half_diag = 1.5f * 1.7320508f * g.h * 1.0001f;
+  float4 *list = lists[wave];
   const PairCoef *c = coef + size_t(blockIdx.y) * size_t(ngroups);
-  for (int k = 0; k < ngroups; ++k) group_dispatch<1, FORM>(protein, group_begin[k], group_begin[k + 1], c[k], lx, ly, lz, e);
-  if (i < np) maps[size_t(blockIdx.y) * np + i] = e[0] * 0.5f;
+  float e = 0.f;
+  for (int k = 0; k < ngroups; ++k) {
+    const PairCoef ck = c[k];
+    const float reach = pair_reach(ck) + half_diag, reach2 = reach * reach;
+    const int i0 = group_begin[k], i1 = group_begin[k + 1];
+    float part = 0.f;
+    int n = 0;
+    for (int base = i0; base < i1; base += 64) {
+      const int j = base + lane;
+      float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+      bool keep = false;
+      if (j < i1) {
+        p = protein[j];
+        const float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+        keep = dx * dx + dy * dy + dz * dz < reach2;
+      }
+      const uint64_t mask = __ballot(keep);
+      if (keep) list[n + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u))] = p;
+      n += __popcll(mask);
+      if (n > kMapList - 64 || base + 64 >= i1) {  // (wave-uniform)
+        wave_sync();
+        list_dispatch<FORM>(list, n, ck, lx, ly, lz, part);
+        wave_sync();  // the list is rewritten next
+        n = 0;
+      }
+    }
+    e += part;
+  }
+  if (valid) maps[size_t(blockIdx.y) * g.points() + (size_t(ix) * size_t(g.n[1]) + size_t(iy)) * size_t(g.n[2]) + size_t(iz)] = e * 0.5f;
 }
 
 // Cell index and fraction along one axis, clamped to the grid.
@@ -126,7 +224,8 @@ hipError_t build_maps(const float4 *protein, const int32_t *group_begin, const P
   if (np == 0 || nslots <= 0 || nslots > 65535 || g.n[0] < 2 || g.n[1] < 2 || g.n[2] < 2 || np > 0x7fffffffu ||
       (np + 255) / 256 > 0x7fffffffu)
     return hipErrorInvalidValue;
-  const dim3 grid(unsigned((np + 255) / 256), unsigned(nslots));
+  const size_t nbricks = size_t((g.n[0] + 3) / 4) * size_t((g.n[1] + 3) / 4) * size_t((g.n[2] + 3) / 4);
+  const dim3 grid(unsigned((nbricks + 3) / 4), unsigned(nslots));
   if (form == kFormD)
     hipLaunchKernelGGL(map_kernel<kFormD>, grid, dim3(256), 0, s, protein, group_begin, coef, ngroups, g, maps);
   else
