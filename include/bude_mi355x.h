@@ -269,6 +269,21 @@ int bude_mi355x_combined_create(int device, const BudeFFParams *forcefield, size
                                 const size_t *nposes, const BudeCombinedConfig *cfg, bude_combined **out);
 /* Screens every ligand's poses against this protein. */
 int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *protein, size_t natpro, BudeCombinedResult *res);
+/* Screens proteins[0..nproteins) in order, protein i's sampled poses, maps
+ * and field scores running beside protein i - 1's exact rescoring (field
+ * screens without fallback pairs; otherwise one protein after another).
+ * res[i] is protein i's result; done(user, i), when not NULL, runs as soon
+ * as protein i is complete, with protein i as the protein screened last for
+ * the accessors below (_pair, _selected, _labels, _exhaustive, _exact,
+ * _recall), except _scores and _map, which then fail for every protein but
+ * the batch's last (its successor's scoring has begun).  The selections of
+ * every protein equal those of screening them one by one.  *wall_ms (may be
+ * NULL) is the host time of the whole call.  No other call on the object
+ * may run meanwhile, except the accessors from done. */
+typedef void (*bude_combined_done_fn)(void *user, size_t index);
+int bude_mi355x_combined_screen_batch(bude_combined *c, size_t nproteins, const BudeAtom *const *proteins,
+                                      const size_t *natpro, BudeCombinedResult *res, bude_combined_done_fn done,
+                                      void *user, double *wall_ms);
 /* Changes the per-pair fraction later screens rescore exactly. */
 int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fraction);
 int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePairResult *r);

@@ -52,6 +52,7 @@ EXPORTED = (
     "bude_mi355x_screen_set_keep",
     "bude_mi355x_combined_create",
     "bude_mi355x_combined_screen",
+    "bude_mi355x_combined_screen_batch",
     "bude_mi355x_combined_set_keep",
     "bude_mi355x_combined_pair",
     "bude_mi355x_combined_selected",

@@ -69,10 +69,13 @@ def _copy_config(cfg: CombinedConfig) -> CombinedConfig:
 
 
 _P, _S = _c.c_void_p, _c.c_size_t
+_DONE_FN = _c.CFUNCTYPE(None, _c.c_void_p, _c.c_size_t)
 _SIG = {
     "bude_mi355x_combined_create": [_c.c_int, _P, _S, _S, _c.POINTER(_P), _c.POINTER(_S), _c.POINTER(_P),
                                     _c.POINTER(_S), _P, _c.POINTER(_P)],
     "bude_mi355x_combined_screen": [_P, _P, _S, _P],
+    "bude_mi355x_combined_screen_batch": [_P, _S, _c.POINTER(_P), _c.POINTER(_S), _P, _P, _P,
+                                          _c.POINTER(_c.c_double)],
     "bude_mi355x_combined_set_keep": [_P, _c.c_float],
     "bude_mi355x_combined_pair": [_P, _S, _P],
     "bude_mi355x_combined_selected": [_P, _S, _P, _P, _S, _c.POINTER(_S)],
@@ -123,6 +126,35 @@ class Combined:
         r = CombinedResult()
         capi.check(_lib().bude_mi355x_combined_screen(self._h, capi.ptr(pro), len(pro), _c.byref(r)))
         return r
+
+    def screen_batch(self, proteins, done=None) -> tuple[list[CombinedResult], float]:
+        """Screens every protein in order, each one's scoring beside the
+        previous one's rescoring (bude_mi355x_combined_screen_batch).
+        done(i), if given, runs as soon as protein i is complete, with
+        protein i as the protein screened last for selected(), pair(),
+        labels(), exhaustive(), exact() and recall().  Returns the results
+        and the host time of the whole call in ms."""
+        pros = [np.ascontiguousarray(p) for p in proteins]
+        n = len(pros)
+        ptrs = (_P * n)(*[p.ctypes.data for p in pros])
+        counts = (_S * n)(*[len(p) for p in pros])
+        res = (CombinedResult * n)()
+        wall = _c.c_double(0.0)
+        errors = []
+
+        def trampoline(user, i):
+            try:
+                done(int(i))
+            except BaseException as exc:  # (cannot propagate through C)
+                errors.append(exc)
+
+        cb = _DONE_FN(trampoline) if done is not None else None
+        capi.check(_lib().bude_mi355x_combined_screen_batch(self._h, n, ptrs, counts, _c.cast(res, _P),
+                                                            _c.cast(cb, _P) if cb else None, None,
+                                                            _c.byref(wall)))
+        if errors:
+            raise errors[0]
+        return list(res), wall.value
 
     def set_keep(self, keep_fraction: float) -> None:
         capi.check(_lib().bude_mi355x_combined_set_keep(self._h, keep_fraction))

@@ -141,6 +141,47 @@ Norm norm_of(std::vector<float> e, float clip_q, float tail_q) {
 
 }  // namespace
 
+// One protein's screen: its tables, selection, exact energies of the
+// selection, per-pair results, sampled poses and events.
+struct ProteinSlot {
+  size_t ngroups = 0;
+  std::vector<int32_t> coef_off;
+  DeviceBuffer protein, group_begin, coef, slot_coef;
+  // exact_sel[pose_off[l] + j] is the exact energy of pose
+  // sel_idx[pose_off[l] + j] of pair l (j < nsel[l]); meta_rs / tiles_rs /
+  // rs_ntiles the rescoring tables (tiles of each chunk, counts per chunk
+  // then the whole set's); rs_sel the non-fallback ligands when the whole
+  // set is rescored at once
+  DeviceBuffer sel_idx, sel_n, exact_sel, meta_rs, tiles_rs, rs_ntiles, rs_sel;
+  std::vector<hipEvent_t> chunk_ev;  // 2 per chunk: scores done, tables done
+  hipEvent_t ev[12] = {};
+  std::vector<uint32_t> nsel;
+  std::vector<int32_t> fallback;
+  std::vector<float> val_rmse, val_rmse_cal;
+  // the sampled poses: pose index within the pair, exact energy, per-pair offsets
+  std::vector<uint32_t> lab_j;
+  std::vector<float> lab_energy;
+  std::vector<size_t> lab_offs;
+  uint64_t protein_sum = 0;
+  // begun and not yet ended; how its selection and rescoring run
+  bool busy = false, valid = false, pipelined = false, mlp = false;
+  uint32_t *nsel_host = nullptr;  // pinned: the counts' copy lands here without blocking the host
+  size_t K = 0;
+  BudeCombinedResult res{};
+  std::chrono::steady_clock::time_point t0;
+
+  ProteinSlot() = default;
+  ProteinSlot(const ProteinSlot &) = delete;
+  ProteinSlot &operator=(const ProteinSlot &) = delete;
+  ~ProteinSlot() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto &e : chunk_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (nsel_host) (void)hipHostFree(nsel_host);
+  }
+};
+
 struct bude_combined {
   int device = 0, cus = 0;
   size_t lds_per_cu = 0;
@@ -155,7 +196,7 @@ struct bude_combined {
   std::vector<int32_t> slot_type;     // map slot -> forcefield type
   DeviceBuffer ligands;               // float4: x, y, z, slot bits
   DeviceBuffer poses;                 // 6 rows of `total`
-  DeviceBuffer pred, sel_idx, sel_n, exact_all;
+  DeviceBuffer pred, exact_all;
   DeviceBuffer desc;                  // [nlig][kDesc]
   bude::sel::Plan plan;               // every pair
   // The field screen runs as a pipeline over ligand ranges
@@ -168,49 +209,39 @@ struct bude_combined {
   std::vector<std::unique_ptr<bude::sel::Plan>> chunk_plan;
   std::vector<int64_t> chunk_tile;
   std::vector<int64_t> chunk_expect;
-  std::vector<hipEvent_t> chunk_ev;
   std::vector<int64_t> lig_expect;    // expected rescoring tiles of each pair: ceil(k / 64)
   bool plan_ok = false;               // the selection tables above are complete
   bude::field::Grid grid{};
-  // per protein
-  bool screened = false;
-  size_t ngroups = 0, proteins_done = 0;
-  DeviceBuffer protein, group_begin, coef, slot_coef, maps, cells, counter;
-  std::vector<int32_t> coef_off;
-  // per-call work tables (each its own buffer; uploaded only while the stream is idle)
-  DeviceBuffer meta_label, tiles_label, meta_lf, meta_fb, tiles_fb, meta_rs, tiles_rs, meta_all, tiles_all;
+  // the screen's proteins: the one screened last is slots[cur]; a batch
+  // begins the next protein in the other slot while this one is rescored
+  ProteinSlot slots[2];
+  int cur = 0;
+  ProteinSlot &s() { return slots[cur]; }
+  const ProteinSlot &s() const { return slots[cur]; }
+  size_t proteins_begun = 0;  // seeds each protein's sampling
+  DeviceBuffer maps, cells, counter;
+  // per-call work tables (each its own buffer; uploaded only while no queued
+  // kernel reads it)
+  DeviceBuffer meta_label, tiles_label, meta_lf, meta_fb, tiles_fb, meta_all, meta_ex, tiles_all;
   DeviceBuffer tb_label, tb_all, lab_src, lab_poses, lab_e, lab_lig, lab_pred, norms, bias0;
-  // rescoring: exact_sel[pose_off[l] + j] is the exact energy of pose
-  // sel_idx[pose_off[l] + j] of pair l (j < its selection count); rs_order
-  // lists each chunk's ligands heaviest first, rs_sel the non-fallback
-  // ligands when the whole set is rescored at once; rs_ntiles the tile
-  // counts (one per chunk, then the whole set's)
-  DeviceBuffer exact_sel, rs_order, rs_sel, rs_ntiles;
-  // the last screen's sampled poses: pose index within the pair, energy, per-pair offsets
-  std::vector<uint32_t> lab_j;
-  std::vector<float> lab_energy;
-  std::vector<size_t> lab_offs;
+  std::vector<int64_t> tb_all_host;  // field tile prefix of every pair (fixed: the poses are)
+  bool field_tables = false;
+  // rs_order lists each pipeline chunk's ligands heaviest first
+  DeviceBuffer rs_order;
   // MLP
   bude::surrogate::Net net;
-  // per pair results of the last protein
-  std::vector<uint32_t> nsel;
-  std::vector<int32_t> fallback;
-  std::vector<float> val_rmse, val_rmse_cal;
   // exact energies of every pair (exhaustive) and the protein they belong to
-  uint64_t protein_sum = 0, exact_sum = 0;
+  uint64_t exact_sum = 0;
   bool have_exact = false;
-  // stream: exact passes, selection, MLP; stream2: maps and field scoring,
-  // overlapping the exact scoring of the sampled poses
-  hipStream_t stream = nullptr, stream2 = nullptr;
-  hipEvent_t ev[12] = {};
+  // stream: exact passes (work-queue counter), selection of the whole set,
+  // MLP; stream2: maps, field scores, chunk selections and tables; stream3:
+  // the sampled-pose pass and its field scores
+  hipStream_t stream = nullptr, stream2 = nullptr, stream3 = nullptr;
 
   ~bude_combined() {
-    for (auto &e : ev)
-      if (e) (void)hipEventDestroy(e);
-    for (auto &e : chunk_ev)
-      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
+    if (stream3) (void)hipStreamDestroy(stream3);
   }
 };
 
@@ -232,7 +263,7 @@ int prepare_exact(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer
   for (size_t l = 0; l < c->nlig; l++) {
     m[l].lig_off = c->lig_off[l];
     m[l].natlig = int32_t(c->lig[l].size());
-    m[l].coef_off = c->coef_off[l];
+    m[l].coef_off = c->s().coef_off[l];
   }
   std::vector<size_t> order;
   for (size_t i = 0; i < spans.size(); i++)
@@ -260,9 +291,9 @@ int launch_exact(bude_combined *c, const float *poses, size_t stride, float *ene
                  const DeviceBuffer &tiles, int ntiles) {
   if (ntiles == 0) return BUDE_OK;
   bude::BatchArgs a{};
-  a.protein = static_cast<const float4 *>(c->protein.ptr);
-  a.group_begin = static_cast<const int32_t *>(c->group_begin.ptr);
-  a.coef = static_cast<const bude::PairCoef *>(c->coef.ptr);
+  a.protein = static_cast<const float4 *>(c->s().protein.ptr);
+  a.group_begin = static_cast<const int32_t *>(c->s().group_begin.ptr);
+  a.coef = static_cast<const bude::PairCoef *>(c->s().coef.ptr);
   a.ligands = static_cast<const float4 *>(c->ligands.ptr);
   a.meta = static_cast<const bude::LigandMeta *>(meta.ptr);
   a.tiles = static_cast<const int2 *>(tiles.ptr);
@@ -270,7 +301,7 @@ int launch_exact(bude_combined *c, const float *poses, size_t stride, float *ene
   a.energies = energies;
   a.counter = static_cast<unsigned int *>(c->counter.ptr);
   a.stride = stride;
-  a.ngroups = int(c->ngroups);
+  a.ngroups = int(c->s().ngroups);
   a.ntiles = ntiles;
   a.form = bude::plain_form();
   const bude::PinnedGrid g = bude::pinned_grid(c->cus, c->lds_per_cu, batch_waves_per_simd(ntiles, c->cus));
@@ -284,22 +315,22 @@ int launch_exact(bude_combined *c, const float *poses, size_t stride, float *ene
 // train or validate the surrogate; every energy the screen returns comes
 // from the tiled kernel.
 int launch_sampled(bude_combined *c, const float *poses, size_t stride, float *energies, const DeviceBuffer &meta,
-                   const DeviceBuffer &tiles, int ntiles) {
+                   const DeviceBuffer &tiles, int ntiles, hipStream_t stream) {
   if (ntiles == 0) return BUDE_OK;
   bude::BatchArgs a{};
-  a.protein = static_cast<const float4 *>(c->protein.ptr);
-  a.group_begin = static_cast<const int32_t *>(c->group_begin.ptr);
-  a.coef = static_cast<const bude::PairCoef *>(c->coef.ptr);
+  a.protein = static_cast<const float4 *>(c->s().protein.ptr);
+  a.group_begin = static_cast<const int32_t *>(c->s().group_begin.ptr);
+  a.coef = static_cast<const bude::PairCoef *>(c->s().coef.ptr);
   a.ligands = static_cast<const float4 *>(c->ligands.ptr);
   a.meta = static_cast<const bude::LigandMeta *>(meta.ptr);
   a.tiles = static_cast<const int2 *>(tiles.ptr);
   a.poses = poses;
   a.energies = energies;
   a.stride = stride;
-  a.ngroups = int(c->ngroups);
+  a.ngroups = int(c->s().ngroups);
   a.ntiles = ntiles;
   a.form = bude::plain_form();
-  return bude::hip_check(bude::launch_split(a, c->stream), "combined sampled-pose launch");
+  return bude::hip_check(bude::launch_split(a, stream), "combined sampled-pose launch");
 }
 
 int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *poses, size_t stride,
@@ -359,14 +390,14 @@ int launch_field(bude_combined *c, const float *poses, size_t stride, float *out
 // entries) into tiles / *ntiles, on stream s.
 int select_and_tables(bude_combined *c, const bude::sel::Plan &plan, size_t first, const int32_t *order, int n,
                       int2 *tiles, int32_t *ntiles, hipStream_t s) {
-  uint32_t *sel_n = static_cast<uint32_t *>(c->sel_n.ptr);
+  uint32_t *sel_n = static_cast<uint32_t *>(c->s().sel_n.ptr);
   if (int st = bude::hip_check(bude::sel::select_lowest_segments(plan, static_cast<const float *>(c->pred.ptr),
-                                                                 static_cast<uint32_t *>(c->sel_idx.ptr),
+                                                                 static_cast<uint32_t *>(c->s().sel_idx.ptr),
                                                                  sel_n + first, s),
                                "select"))
     return st;
   return bude::hip_check(
-      bude::build_rescore_tables(order, n, sel_n, static_cast<bude::LigandMeta *>(c->meta_rs.ptr), tiles, ntiles, s),
+      bude::build_rescore_tables(order, n, sel_n, static_cast<bude::LigandMeta *>(c->s().meta_rs.ptr), tiles, ntiles, s),
       "rescoring tables");
 }
 
@@ -375,19 +406,19 @@ int select_and_tables(bude_combined *c, const bude::sel::Plan &plan, size_t firs
 // `expect` tiles set the grid depth.
 int launch_rescore(bude_combined *c, const int2 *tiles, const int32_t *ntiles, int64_t expect) {
   bude::BatchArgs a{};
-  a.protein = static_cast<const float4 *>(c->protein.ptr);
-  a.group_begin = static_cast<const int32_t *>(c->group_begin.ptr);
-  a.coef = static_cast<const bude::PairCoef *>(c->coef.ptr);
+  a.protein = static_cast<const float4 *>(c->s().protein.ptr);
+  a.group_begin = static_cast<const int32_t *>(c->s().group_begin.ptr);
+  a.coef = static_cast<const bude::PairCoef *>(c->s().coef.ptr);
   a.ligands = static_cast<const float4 *>(c->ligands.ptr);
-  a.meta = static_cast<const bude::LigandMeta *>(c->meta_rs.ptr);
+  a.meta = static_cast<const bude::LigandMeta *>(c->s().meta_rs.ptr);
   a.tiles = tiles;
   a.poses = static_cast<const float *>(c->poses.ptr);
-  a.energies = static_cast<float *>(c->exact_sel.ptr);
+  a.energies = static_cast<float *>(c->s().exact_sel.ptr);
   a.counter = static_cast<unsigned int *>(c->counter.ptr);
   a.stride = c->total;
-  a.ngroups = int(c->ngroups);
+  a.ngroups = int(c->s().ngroups);
   a.form = bude::plain_form();
-  a.pose_index = static_cast<const uint32_t *>(c->sel_idx.ptr);
+  a.pose_index = static_cast<const uint32_t *>(c->s().sel_idx.ptr);
   a.ntiles_dev = ntiles;
   const bude::PinnedGrid g = bude::pinned_grid(c->cus, c->lds_per_cu, batch_waves_per_simd(expect, c->cus));
   a.lds_bytes = g.lds_bytes;
@@ -459,12 +490,14 @@ int plan_selection(bude_combined *c, float keep) {
   }
   if (c->chunk_tile[K] > 0x7fffffff) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: too many rescoring tiles");
   if (int st = c->rs_order.assign(order.data(), order.size() * 4)) return st;
-  if (int st = c->tiles_rs.reserve(size_t(c->chunk_tile[K]) * sizeof(int2))) return st;
-  if (int st = c->rs_ntiles.reserve((K + 1) * 4)) return st;
-  while (c->chunk_ev.size() < 2 * K) {
-    hipEvent_t e = nullptr;
-    if (int st = bude::hip_check(hipEventCreate(&e), "hipEventCreate")) return st;
-    c->chunk_ev.push_back(e);
+  for (ProteinSlot &ps : c->slots) {
+    if (int st = ps.tiles_rs.reserve(size_t(c->chunk_tile[K]) * sizeof(int2))) return st;
+    if (int st = ps.rs_ntiles.reserve((K + 1) * 4)) return st;
+    while (ps.chunk_ev.size() < 2 * K) {
+      hipEvent_t e = nullptr;
+      if (int st = bude::hip_check(hipEventCreate(&e), "hipEventCreate")) return st;
+      ps.chunk_ev.push_back(e);
+    }
   }
   c->plan_ok = true;
   return BUDE_OK;
@@ -473,7 +506,7 @@ int plan_selection(bude_combined *c, float keep) {
 int sync(bude_combined *c) { return bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
 
 int record(bude_combined *c, int i, hipStream_t s = nullptr) {
-  return bude::hip_check(hipEventRecord(c->ev[i], s ? s : c->stream), "hipEventRecord");
+  return bude::hip_check(hipEventRecord(c->s().ev[i], s ? s : c->stream), "hipEventRecord");
 }
 
 float elapsed_events(hipEvent_t a, hipEvent_t b) {
@@ -482,7 +515,7 @@ float elapsed_events(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-float elapsed(bude_combined *c, int a, int b) { return elapsed_events(c->ev[a], c->ev[b]); }
+float elapsed(bude_combined *c, int a, int b) { return elapsed_events(c->s().ev[a], c->s().ev[b]); }
 
 int check_config(const BudeCombinedConfig *cfg) {
   if (!cfg) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: config is NULL");
@@ -617,40 +650,52 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
                                      hipMemcpyHostToDevice),
                            "hipMemcpy poses");
   if (!st) st = c->pred.alloc(total * sizeof(float));
-  if (!st) st = c->sel_idx.alloc(total * sizeof(uint32_t));
-  if (!st) st = c->sel_n.alloc(nligands * sizeof(uint32_t));
-  if (!st) st = c->exact_sel.alloc(total * sizeof(float));
+  for (ProteinSlot &ps : c->slots) {
+    if (!st) st = ps.sel_idx.alloc(total * sizeof(uint32_t));
+    if (!st) st = ps.sel_n.alloc(nligands * sizeof(uint32_t));
+    if (!st) st = ps.exact_sel.alloc(total * sizeof(float));
+    for (auto &e : ps.ev)
+      if (!st) st = bude::hip_check(hipEventCreate(&e), "hipEventCreate");
+    if (!st)
+      st = bude::hip_check(hipHostMalloc(reinterpret_cast<void **>(&ps.nsel_host), nligands * sizeof(uint32_t)),
+                           "hipHostMalloc");
+  }
   if (!st) st = c->counter.alloc(256);
   if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking), "hipStreamCreate");
-  for (auto &e : c->ev)
-    if (!st) st = bude::hip_check(hipEventCreate(&e), "hipEventCreate");
+  if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking), "hipStreamCreate");
   if (!st) st = bude::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   if (st) return st;
   *out = c.release();
   return bude::ok();
 }
 
-extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *protein, size_t natpro,
-                                           BudeCombinedResult *res) {
-  if (!c || !res) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_screen: NULL argument");
+namespace {
+
+// First half of one protein's screen, into slot c->s(): everything up to
+// the exact rescoring of the selection, which is left queued on the first
+// stream (with the copy of the selection counts) for screen_end.  The host
+// waits only for the sampled-pose pass.
+int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
   if (int st = bude::check_deck(protein, natpro, c->lig[0].data(), c->lig[0].size(), c->ff.data(), c->ff.size(),
                                 "bude_mi355x_combined_screen"))
     return st;
-  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
-  *res = BudeCombinedResult{};
   if (!c->plan_ok)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
                       "bude_mi355x_combined_screen: the selection tables are incomplete (a failed set_keep); "
                       "call bude_mi355x_combined_set_keep again");
-  c->screened = false;
+  BudeCombinedResult *res = &c->s().res;
+  *res = BudeCombinedResult{};
+  c->s().valid = false;
+  c->s().busy = true;
   {  // FNV-1a of the protein: exact energies stay valid while it is unchanged
     uint64_t hsum = 1469598103934665603ull ^ uint64_t(natpro);
     const auto *bytes = reinterpret_cast<const unsigned char *>(protein);
     for (size_t i = 0; i < natpro * sizeof(BudeAtom); i++) hsum = (hsum ^ bytes[i]) * 1099511628211ull;
-    c->protein_sum = hsum;
+    c->s().protein_sum = hsum;
   }
   const auto t0 = std::chrono::steady_clock::now();
+  c->s().t0 = t0;
   const size_t L = c->nlig;
   const BudeCombinedConfig &cfg = c->cfg;
   const bool mlp = cfg.surrogate == BUDE_SURROGATE_MLP;
@@ -661,29 +706,29 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   //    pair l's at coef_off[l] = lig_off[l] * ngroups) is expanded from it
   //    on the device
   const bude::ProteinGroups groups = bude::group_protein(protein, natpro);
-  c->ngroups = groups.ngroups();
-  const size_t natoms = size_t(c->lig_off.back()) + c->lig.back().size(), ncoef = natoms * c->ngroups;
+  c->s().ngroups = groups.ngroups();
+  const size_t natoms = size_t(c->lig_off.back()) + c->lig.back().size(), ncoef = natoms * c->s().ngroups;
   if (ncoef > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: pair tables too large");
-  c->coef_off.assign(L, 0);
-  for (size_t l = 0; l < L; l++) c->coef_off[l] = int32_t(size_t(c->lig_off[l]) * c->ngroups);
+  c->s().coef_off.assign(L, 0);
+  for (size_t l = 0; l < L; l++) c->s().coef_off[l] = int32_t(size_t(c->lig_off[l]) * c->s().ngroups);
   {
     std::vector<BudeAtom> probe(c->slot_type.size());
     for (size_t s = 0; s < probe.size(); s++) probe[s] = BudeAtom{0.f, 0.f, 0.f, c->slot_type[s]};
     const std::vector<bude::PairCoef> sc = bude::coef_table(probe.data(), probe.size(), groups, c->ff.data());
-    if (int st = c->slot_coef.assign(sc.data(), sc.size() * sizeof(bude::PairCoef))) return st;
+    if (int st = c->s().slot_coef.assign(sc.data(), sc.size() * sizeof(bude::PairCoef))) return st;
   }
-  if (int st = c->protein.assign(groups.xyzw.data(), groups.xyzw.size() * 4)) return st;
-  if (int st = c->group_begin.assign(groups.begin.data(), groups.begin.size() * 4)) return st;
-  if (int st = c->coef.reserve(ncoef * sizeof(bude::PairCoef))) return st;
-  hipLaunchKernelGGL(expand_coef_kernel, dim3(blocks_for(ncoef, 256)), dim3(256), 0, c->stream,
-                     static_cast<const float4 *>(c->ligands.ptr), ncoef, int(c->ngroups),
-                     static_cast<const bude::PairCoef *>(c->slot_coef.ptr), static_cast<bude::PairCoef *>(c->coef.ptr));
+  if (int st = c->s().protein.assign(groups.xyzw.data(), groups.xyzw.size() * 4)) return st;
+  if (int st = c->s().group_begin.assign(groups.begin.data(), groups.begin.size() * 4)) return st;
+  if (int st = c->s().coef.reserve(ncoef * sizeof(bude::PairCoef))) return st;
+  hipLaunchKernelGGL(expand_coef_kernel, dim3(blocks_for(ncoef, 256)), dim3(256), 0, c->stream3,
+                     static_cast<const float4 *>(c->ligands.ptr), ncoef, int(c->s().ngroups),
+                     static_cast<const bude::PairCoef *>(c->s().slot_coef.ptr), static_cast<bude::PairCoef *>(c->s().coef.ptr));
   if (int st = bude::hip_check(hipGetLastError(), "expand pair tables")) return st;
   res->prepare_ms = float(ms_since(t0));
 
   // 2. sampled poses of every pair: first 15/16 train the MLP, the rest
   //    validate (the field validates on all of them)
-  std::mt19937_64 rng(cfg.seed + 0x9e3779b97f4a7c15ull * (c->proteins_done + 1));
+  std::mt19937_64 rng(cfg.seed + 0x9e3779b97f4a7c15ull * (c->proteins_begun + 1));
   std::vector<uint32_t> src, lab_lig;
   std::vector<size_t> lab_off(L + 1, 0), lab_train(L, 0);
   for (size_t l = 0; l < L; l++) {
@@ -733,44 +778,49 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   //    issued while a kernel fills the device waits for it
   int nt_label = 0;
   int64_t ft_all = 0, ft_label = 0;
-  std::vector<int64_t> tb_all;
+  const std::vector<int64_t> &tb_all = c->tb_all_host;
   if (int st = prepare_exact(c, lab_spans, c->meta_label, c->tiles_label, &nt_label, bude::kSplitTile)) return st;
   if (!mlp) {
     const size_t np = c->grid.points(), nslots = c->slot_type.size();
     if (int st = c->maps.reserve(np * nslots * 4)) return st;
     if (int st = c->cells.reserve(c->grid.cells() * nslots * 16)) return st;
-    if (int st = prepare_field(c, all, c->meta_all, c->tb_all, &ft_all, &tb_all)) return st;
+    if (!c->field_tables) {  // (the pose counts never change: built once, never rewritten while read)
+      if (int st = prepare_field(c, all, c->meta_all, c->tb_all, &ft_all, &c->tb_all_host)) return st;
+      c->field_tables = true;
+    }
+    ft_all = c->tb_all_host[L];
     if (int st = prepare_field(c, lab_spans, c->meta_lf, c->tb_label, &ft_label)) return st;
   }
   {  // rescoring meta: everything but the pose counts, which the selection writes on the device
     std::vector<bude::LigandMeta> m(L);
     for (size_t l = 0; l < L; l++)
-      m[l] = bude::LigandMeta{c->lig_off[l], int32_t(c->lig[l].size()), c->coef_off[l], 0, int64_t(c->pose_off[l]), 0};
-    if (int st = c->meta_rs.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
+      m[l] = bude::LigandMeta{c->lig_off[l], int32_t(c->lig[l].size()), c->s().coef_off[l], 0, int64_t(c->pose_off[l]), 0};
+    if (int st = c->s().meta_rs.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
   }
   const size_t K = c->chunk_lig.size() - 1;
-  int2 *rs_tiles = static_cast<int2 *>(c->tiles_rs.ptr);
-  int32_t *rs_ntiles = static_cast<int32_t *>(c->rs_ntiles.ptr);
+  int2 *rs_tiles = static_cast<int2 *>(c->s().tiles_rs.ptr);
+  int32_t *rs_ntiles = static_cast<int32_t *>(c->s().rs_ntiles.ptr);
 
   // 4. exact energies of the sampled poses (one ragged split launch) on the
-  //    first stream, and, for the field, maps -> cells -> scores of every
-  //    pose of every pair on the second stream, concurrently
-  if (int st = record(c, 0)) return st;
-  hipLaunchKernelGGL(gather_index_kernel, dim3(blocks_for(nlab, 256)), dim3(256), 0, c->stream, poses, c->total,
+  //    third stream, and, for the field, maps -> cells -> scores of every
+  //    pose of every pair on the second stream, concurrently (and beside the
+  //    previous protein's rescoring on the first stream, in a batch)
+  if (int st = record(c, 0, c->stream3)) return st;
+  hipLaunchKernelGGL(gather_index_kernel, dim3(blocks_for(nlab, 256)), dim3(256), 0, c->stream3, poses, c->total,
                      static_cast<const uint32_t *>(c->lab_src.ptr), nlab, static_cast<float *>(c->lab_poses.ptr),
                      nlab);
   if (int st = bude::hip_check(hipGetLastError(), "gather labels")) return st;
   if (int st = launch_sampled(c, lpos, nlab, static_cast<float *>(c->lab_e.ptr), c->meta_label, c->tiles_label,
-                              nt_label))
+                              nt_label, c->stream3))
     return st;
-  if (int st = record(c, 1)) return st;
+  if (int st = record(c, 1, c->stream3)) return st;
   if (!mlp) {
     const size_t nslots = c->slot_type.size();
     if (int st = record(c, 2, c->stream2)) return st;
     if (int st = bude::hip_check(
-            bude::field::build_maps(static_cast<const float4 *>(c->protein.ptr),
-                                    static_cast<const int32_t *>(c->group_begin.ptr),
-                                    static_cast<const bude::PairCoef *>(c->slot_coef.ptr), int(c->ngroups),
+            bude::field::build_maps(static_cast<const float4 *>(c->s().protein.ptr),
+                                    static_cast<const int32_t *>(c->s().group_begin.ptr),
+                                    static_cast<const bude::PairCoef *>(c->s().slot_coef.ptr), int(c->s().ngroups),
                                     int(nslots), c->grid, bude::plain_form(), static_cast<float *>(c->maps.ptr),
                                     c->stream2),
             "build maps"))
@@ -789,29 +839,32 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
       if (int st = launch_field(c, poses, c->total, pred, c->meta_all, c->tb_all, tb_all[a], tb_all[b] - tb_all[a],
                                 c->stream2, k > 0))
         return st;
-      if (int st = bude::hip_check(hipEventRecord(c->chunk_ev[2 * k], c->stream2), "hipEventRecord")) return st;
+      if (int st = bude::hip_check(hipEventRecord(c->s().chunk_ev[2 * k], c->stream2), "hipEventRecord")) return st;
       if (int st = select_and_tables(c, *c->chunk_plan[k], a, static_cast<const int32_t *>(c->rs_order.ptr) + a,
                                      int(b - a), rs_tiles + c->chunk_tile[k], rs_ntiles + k, c->stream2))
         return st;
-      if (int st = bude::hip_check(hipEventRecord(c->chunk_ev[2 * k + 1], c->stream2), "hipEventRecord")) return st;
+      if (int st = bude::hip_check(hipEventRecord(c->s().chunk_ev[2 * k + 1], c->stream2), "hipEventRecord")) return st;
     }
     if (int st = record(c, 4, c->stream2)) return st;
   }
-  std::vector<float> le(nlab);
-  if (int st = bude::hip_check(hipMemcpyAsync(le.data(), c->lab_e.ptr, nlab * 4, hipMemcpyDeviceToHost, c->stream),
+  std::vector<float> le(nlab), lpred(nlab);
+  if (int st = bude::hip_check(hipMemcpyAsync(le.data(), c->lab_e.ptr, nlab * 4, hipMemcpyDeviceToHost, c->stream3),
                                "hipMemcpyAsync"))
     return st;
   if (!mlp) {  // field scores of the sampled poses, once the cells exist
-    if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->ev[3], 0), "hipStreamWaitEvent")) return st;
-    if (int st = launch_field(c, lpos, nlab, lp, c->meta_lf, c->tb_label, 0, ft_label, c->stream)) return st;
+    if (int st = bude::hip_check(hipStreamWaitEvent(c->stream3, c->s().ev[3], 0), "hipStreamWaitEvent")) return st;
+    if (int st = launch_field(c, lpos, nlab, lp, c->meta_lf, c->tb_label, 0, ft_label, c->stream3)) return st;
+    if (int st = bude::hip_check(hipMemcpyAsync(lpred.data(), lp, nlab * 4, hipMemcpyDeviceToHost, c->stream3),
+                                 "hipMemcpyAsync"))
+      return st;
   }
-  if (int st = sync(c)) return st;
+  if (int st = bude::hip_check(hipStreamSynchronize(c->stream3), "hipStreamSynchronize")) return st;
   res->label_ms = elapsed(c, 0, 1);
-  c->lab_energy = le;
-  c->lab_offs = lab_off;
-  c->lab_j.resize(nlab);
+  c->s().lab_energy = le;
+  c->s().lab_offs = lab_off;
+  c->s().lab_j.resize(nlab);
   for (size_t l = 0; l < L; l++)
-    for (size_t i = lab_off[l]; i < lab_off[l + 1]; i++) c->lab_j[i] = uint32_t(src[i] - c->pose_off[l]);
+    for (size_t i = lab_off[l]; i < lab_off[l + 1]; i++) c->s().lab_j[i] = uint32_t(src[i] - c->pose_off[l]);
 
   // per-pair target transform from the training part (all samples for the field)
   std::vector<Norm> norms(L);
@@ -828,7 +881,7 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     if (int st = record(c, 2)) return st;
     if (int st = c->norms.assign(tn.data(), tn.size() * sizeof(tn[0]))) return st;
     std::vector<std::vector<float>> w, b;
-    bude::surrogate::init_weights(cfg.seed + c->proteins_done, bude::mlp::kIn, w, b);
+    bude::surrogate::init_weights(cfg.seed + c->proteins_begun, bude::mlp::kIn, w, b);
     if (int st = bude::surrogate::set_weights(c->net, w, b, c->stream)) return st;
     std::vector<uint32_t> train;
     for (size_t l = 0; l < L; l++)
@@ -847,7 +900,7 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     tp.epochs = cfg.epochs;
     tp.lr = cfg.lr;
     tp.tail_weight = cfg.tail_weight;
-    tp.seed = cfg.seed + c->proteins_done;
+    tp.seed = cfg.seed + c->proteins_begun;
     if (int st = bude::surrogate::train(c->net, td, std::move(train), tp, c->stream, nullptr, 0, nullptr)) return st;
     if (int st = c->bias0.reserve(L * bude::mlp::kHidden * 4)) return st;
     if (int st = bude::hip_check(bude::mlp::fold_bias(static_cast<const float *>(c->net.b[0].ptr),
@@ -865,17 +918,17 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
         return st;
     }
   }
-  std::vector<float> lpred(nlab);
-  if (int st = bude::hip_check(hipMemcpyAsync(lpred.data(), lp, nlab * 4, hipMemcpyDeviceToHost, c->stream),
-                               "hipMemcpyAsync"))
-    return st;
-  if (int st = sync(c)) return st;
-  res->train_ms = elapsed(c, 2, 3);
+  if (mlp) {
+    if (int st = bude::hip_check(hipMemcpyAsync(lpred.data(), lp, nlab * 4, hipMemcpyDeviceToHost, c->stream),
+                                 "hipMemcpyAsync"))
+      return st;
+    if (int st = sync(c)) return st;
+  }
 
   // 6. validation per pair on the clipped, standardised scale of that pair
-  c->fallback.assign(L, 0);
-  c->val_rmse.assign(L, 0.f);
-  c->val_rmse_cal.assign(L, 0.f);
+  c->s().fallback.assign(L, 0);
+  c->s().val_rmse.assign(L, 0.f);
+  c->s().val_rmse_cal.assign(L, 0.f);
   for (size_t l = 0; l < L; l++) {
     const size_t b = lab_off[l] + lab_train[l], e = lab_off[l + 1];
     if (e <= b) continue;
@@ -895,12 +948,12 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
     const double vy = std::max(0.0, syy / n - (sy / n) * (sy / n));
     const double vp = std::max(0.0, spp / n - (sp / n) * (sp / n));
     const double cv = syp / n - (sy / n) * (sp / n);
-    c->val_rmse[l] = float(std::sqrt(se / n));
+    c->s().val_rmse[l] = float(std::sqrt(se / n));
     // residual of the best increasing affine map of the scores onto the
     // targets: the error that matters for a ranking
-    c->val_rmse_cal[l] = float(std::sqrt(vp > 0 && cv > 0 ? std::max(0.0, vy - cv * cv / vp) : vy));
-    c->fallback[l] = !(c->val_rmse_cal[l] <= cfg.max_val_rmse);
-    res->fallback_pairs += uint32_t(c->fallback[l]);
+    c->s().val_rmse_cal[l] = float(std::sqrt(vp > 0 && cv > 0 ? std::max(0.0, vy - cv * cv / vp) : vy));
+    c->s().fallback[l] = !(c->s().val_rmse_cal[l] <= cfg.max_val_rmse);
+    res->fallback_pairs += uint32_t(c->s().fallback[l]);
   }
 
   // 7. Without fallback pairs the field screen is already selecting chunk
@@ -910,27 +963,27 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   //    and the rescoring list of the non-fallback pairs
   std::vector<Span> fb;
   for (size_t l = 0; l < L; l++)
-    if (c->fallback[l]) fb.push_back(all[l]);
+    if (c->s().fallback[l]) fb.push_back(all[l]);
   const bool pipelined = !mlp && fb.empty();
   int64_t expect = 0;
   int nrs = 0;
   if (!pipelined) {
-    std::vector<int32_t> o = heaviest_first(c, 0, L, &c->fallback);
+    std::vector<int32_t> o = heaviest_first(c, 0, L, &c->s().fallback);
     for (int32_t l : o) expect += c->lig_expect[size_t(l)];
     nrs = int(o.size());
     if (o.empty()) o.push_back(0);  // every pair fell back: nothing to rescore
-    if (int st = c->rs_sel.assign(o.data(), o.size() * 4)) return st;
+    if (int st = c->s().rs_sel.assign(o.data(), o.size() * 4)) return st;
   }
   if (mlp) {
     if (int st = record(c, 4)) return st;
     for (size_t l = 0; l < L; l++)
-      if (!c->fallback[l])
+      if (!c->s().fallback[l])
         if (int st = bude::surrogate::predict(c->net, poses + c->pose_off[l], c->total, c->nposes[l],
                                               pred + c->pose_off[l], 2 * c->cus, c->stream,
                                               static_cast<const float *>(c->bias0.ptr) + l * bude::mlp::kHidden))
           return st;
   } else if (!pipelined) {
-    if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->ev[4], 0), "hipStreamWaitEvent")) return st;
+    if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->s().ev[4], 0), "hipStreamWaitEvent")) return st;
   }
   if (int st = record(c, 5)) return st;
   if (int st = exact_ragged(c, fb, poses, c->total, pred, c->meta_fb, c->tiles_fb)) return st;
@@ -941,54 +994,146 @@ extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *pro
   //      stream, or the whole set at once
   if (pipelined) {
     for (size_t k = 0; k < K; k++) {
-      if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->chunk_ev[2 * k + 1], 0), "hipStreamWaitEvent"))
+      if (int st = bude::hip_check(hipStreamWaitEvent(c->stream, c->s().chunk_ev[2 * k + 1], 0), "hipStreamWaitEvent"))
         return st;
       if (k == 0)
         if (int st = record(c, 7)) return st;
       if (int st = launch_rescore(c, rs_tiles + c->chunk_tile[k], rs_ntiles + k, c->chunk_expect[k])) return st;
     }
   } else {
-    if (int st = select_and_tables(c, c->plan, 0, static_cast<const int32_t *>(c->rs_sel.ptr), nrs, rs_tiles,
+    if (int st = select_and_tables(c, c->plan, 0, static_cast<const int32_t *>(c->s().rs_sel.ptr), nrs, rs_tiles,
                                    rs_ntiles + K, c->stream))
       return st;
     if (int st = record(c, 7)) return st;
     if (nrs)
       if (int st = launch_rescore(c, rs_tiles, rs_ntiles + K, expect)) return st;
   }
-  c->nsel.assign(L, 0);
-  if (int st = bude::hip_check(hipMemcpyAsync(c->nsel.data(), c->sel_n.ptr, L * 4, hipMemcpyDeviceToHost, c->stream),
+  // the counts land in pinned memory: the host does not wait here
+  if (int st = bude::hip_check(hipMemcpyAsync(c->s().nsel_host, c->s().sel_n.ptr, L * 4, hipMemcpyDeviceToHost,
+                                              c->stream),
                                "hipMemcpyAsync"))
     return st;
   if (int st = record(c, 8)) return st;
-  if (int st = sync(c)) return st;
-  if (pipelined) {
-    res->score_ms = elapsed_events(c->ev[3], c->chunk_ev[0]);
-    for (size_t k = 0; k < K; k++) {
-      if (k) res->score_ms += elapsed_events(c->chunk_ev[2 * k - 1], c->chunk_ev[2 * k]);
-      res->select_ms += elapsed_events(c->chunk_ev[2 * k], c->chunk_ev[2 * k + 1]);
+  c->s().pipelined = pipelined;
+  c->s().mlp = mlp;
+  c->s().K = K;
+  res->nposes = c->total;
+  res->labels = nlab;
+  c->proteins_begun++;
+  return BUDE_OK;
+}
+
+// Second half: waits for slot c->s()'s rescoring and fills its result.
+int screen_end(bude_combined *c) {
+  ProteinSlot &ps = c->s();
+  BudeCombinedResult *res = &ps.res;
+  ps.busy = false;
+  if (int st = bude::hip_check(hipEventSynchronize(ps.ev[8]), "hipEventSynchronize")) return st;
+  const size_t L = c->nlig;
+  ps.nsel.assign(ps.nsel_host, ps.nsel_host + L);
+  res->train_ms = elapsed(c, 2, 3);
+  if (ps.pipelined) {
+    res->score_ms = elapsed_events(ps.ev[3], ps.chunk_ev[0]);
+    for (size_t k = 0; k < ps.K; k++) {
+      if (k) res->score_ms += elapsed_events(ps.chunk_ev[2 * k - 1], ps.chunk_ev[2 * k]);
+      res->select_ms += elapsed_events(ps.chunk_ev[2 * k], ps.chunk_ev[2 * k + 1]);
     }
   } else {
-    res->score_ms = elapsed(c, mlp ? 4 : 3, mlp ? 5 : 4) + elapsed(c, 5, 6);
+    res->score_ms = elapsed(c, ps.mlp ? 4 : 3, ps.mlp ? 5 : 4) + elapsed(c, 5, 6);
     res->select_ms = elapsed(c, 6, 7);
   }
   res->rescore_ms = elapsed(c, 7, 8);
-  res->total_ms = ms_since(t0);
-  res->nposes = c->total;
-  for (size_t l = 0; l < L; l++) res->nselected += c->nsel[l];
-  res->labels = nlab;
-  c->screened = true;
-  c->proteins_done++;
+  res->total_ms = ms_since(ps.t0);
+  res->nselected = 0;
+  for (size_t l = 0; l < L; l++) res->nselected += ps.nsel[l];
+  ps.valid = true;
+  return BUDE_OK;
+}
+
+}  // namespace
+
+extern "C" int bude_mi355x_combined_screen(bude_combined *c, const BudeAtom *protein, size_t natpro,
+                                           BudeCombinedResult *res) {
+  if (!c || !res) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_screen: NULL argument");
+  if (c->slots[c->cur ^ 1].busy)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_screen: a batch is running");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  *res = BudeCombinedResult{};
+  int st = screen_begin(c, protein, natpro);
+  if (!st) st = screen_end(c);
+  if (st) {  // leave nothing in flight behind an error
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamSynchronize(c->stream3);
+    c->s().busy = false;
+    c->s().valid = false;
+    return st;
+  }
+  *res = c->s().res;
+  return bude::ok();
+}
+
+extern "C" int bude_mi355x_combined_screen_batch(bude_combined *c, size_t nproteins, const BudeAtom *const *proteins,
+                                                 const size_t *natpro, BudeCombinedResult *res,
+                                                 bude_combined_done_fn done, void *user, double *wall_ms) {
+  if (!c || !proteins || !natpro || (nproteins && !res))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_screen_batch: NULL argument");
+  if (c->slots[0].busy || c->slots[1].busy)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_screen_batch: a batch is running");
+  if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
+  const auto t0 = std::chrono::steady_clock::now();
+  // protein i begins in one slot while protein i - 1, in the other, is
+  // rescored; i - 1 then ends (and its callback runs with it as the protein
+  // screened last).  A protein whose selection ran outside the pipeline (the
+  // MLP, or fallback pairs) shares buffers with the next one's scoring and
+  // ends first.
+  int prev = -1, st = BUDE_OK;
+  auto finish = [&](int slot, size_t i) -> int {
+    c->cur = slot;
+    if (int e = screen_end(c)) return e;
+    res[i] = c->s().res;
+    if (done) done(user, i);
+    return BUDE_OK;
+  };
+  for (size_t i = 0; i < nproteins && !st; i++) {
+    if (prev >= 0 && !c->slots[prev].pipelined) {
+      st = finish(prev, i - 1);
+      if (st) break;
+      prev = -1;
+    }
+    const int slot = prev >= 0 ? prev ^ 1 : c->cur;
+    c->cur = slot;
+    st = screen_begin(c, proteins[i], natpro[i]);
+    if (st) break;
+    if (prev >= 0) st = finish(prev, i - 1);
+    c->cur = slot;
+    prev = slot;
+  }
+  if (!st && prev >= 0) st = finish(prev, nproteins - 1);
+  if (st) {  // leave nothing in flight behind an error
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamSynchronize(c->stream3);
+    for (ProteinSlot &ps : c->slots) {
+      ps.busy = false;
+      ps.valid = false;
+    }
+    return st;
+  }
+  if (wall_ms) *wall_ms = ms_since(t0);
   return bude::ok();
 }
 
 extern "C" int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fraction) {
   if (!c || !(keep_fraction > 0.f) || keep_fraction > 1.f)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_set_keep: keep_fraction must be in (0, 1]");
+  if (c->slots[0].busy || c->slots[1].busy)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_set_keep: not during a batch");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(c->stream2), "hipStreamSynchronize")) return st;
   if (int st = plan_selection(c, keep_fraction)) {
-    c->screened = false;
+    c->s().valid = false;
     return st;
   }
   c->cfg.keep_fraction = keep_fraction;
@@ -997,12 +1142,12 @@ extern "C" int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fracti
 
 extern "C" int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePairResult *r) {
   if (!c || !r || ligand >= c->nlig) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_pair: bad arguments");
-  if (!c->screened) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_pair: no protein screened yet");
+  if (!c->s().valid) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_pair: no protein screened yet");
   r->nposes = uint32_t(c->nposes[ligand]);
-  r->nselected = c->nsel[ligand];
-  r->fallback = c->fallback[ligand];
-  r->val_rmse = c->val_rmse[ligand];
-  r->val_rmse_calibrated = c->val_rmse_cal[ligand];
+  r->nselected = c->s().nsel[ligand];
+  r->fallback = c->s().fallback[ligand];
+  r->val_rmse = c->s().val_rmse[ligand];
+  r->val_rmse_calibrated = c->s().val_rmse_cal[ligand];
   return bude::ok();
 }
 
@@ -1010,8 +1155,8 @@ extern "C" int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, ui
                                              size_t *n) {
   if (!c || !n || ligand >= c->nlig)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_selected: bad arguments");
-  if (!c->screened) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_selected: no protein screened yet");
-  const size_t k = c->nsel[ligand];
+  if (!c->s().valid) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_selected: no protein screened yet");
+  const size_t k = c->s().nsel[ligand];
   *n = k;
   if (!idx && !energies) return bude::ok();
   if (cap < k) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_selected: buffer too small");
@@ -1020,18 +1165,18 @@ extern "C" int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, ui
   std::vector<float> ex(k);
   const size_t po = c->pose_off[ligand];
   if (k) {
-    if (int st = bude::hip_check(hipMemcpy(id.data(), static_cast<uint32_t *>(c->sel_idx.ptr) + po, k * 4,
+    if (int st = bude::hip_check(hipMemcpy(id.data(), static_cast<uint32_t *>(c->s().sel_idx.ptr) + po, k * 4,
                                            hipMemcpyDeviceToHost),
                                  "hipMemcpy"))
       return st;
-    if (c->fallback[ligand]) {  // the scores of a fallback pair are its exact energies
+    if (c->s().fallback[ligand]) {  // the scores of a fallback pair are its exact energies
       std::vector<float> all(c->nposes[ligand]);
       if (int st = bude::hip_check(hipMemcpy(all.data(), static_cast<float *>(c->pred.ptr) + po, all.size() * 4,
                                              hipMemcpyDeviceToHost),
                                    "hipMemcpy"))
         return st;
       for (size_t j = 0; j < k; j++) ex[j] = all[id[j]];
-    } else if (int st = bude::hip_check(hipMemcpy(ex.data(), static_cast<float *>(c->exact_sel.ptr) + po,
+    } else if (int st = bude::hip_check(hipMemcpy(ex.data(), static_cast<float *>(c->s().exact_sel.ptr) + po,
                                                   k * 4, hipMemcpyDeviceToHost),
                                         "hipMemcpy")) {
       return st;
@@ -1050,15 +1195,15 @@ extern "C" int bude_mi355x_combined_selected(bude_combined *c, size_t ligand, ui
 extern "C" int bude_mi355x_combined_labels(const bude_combined *c, size_t ligand, uint32_t *idx, float *energies,
                                            size_t cap, size_t *n) {
   if (!c || !n) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: NULL argument");
-  if (!c->screened) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: screen a protein first");
+  if (!c->s().valid) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: screen a protein first");
   if (ligand >= c->nlig) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: no such ligand");
-  const size_t b = c->lab_offs[ligand], e = c->lab_offs[ligand + 1];
+  const size_t b = c->s().lab_offs[ligand], e = c->s().lab_offs[ligand + 1];
   *n = e - b;
   if (!idx && !energies) return bude::ok();
   if (cap < e - b) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_labels: cap too small");
   for (size_t i = b; i < e; i++) {
-    if (idx) idx[i - b] = c->lab_j[i];
-    if (energies) energies[i - b] = c->lab_energy[i];
+    if (idx) idx[i - b] = c->s().lab_j[i];
+    if (energies) energies[i - b] = c->s().lab_energy[i];
   }
   return bude::ok();
 }
@@ -1066,7 +1211,9 @@ extern "C" int bude_mi355x_combined_labels(const bude_combined *c, size_t ligand
 extern "C" int bude_mi355x_combined_scores(bude_combined *c, size_t ligand, float *out) {
   if (!c || !out || ligand >= c->nlig)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_scores: bad arguments");
-  if (!c->screened) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_scores: no protein screened yet");
+  if (!c->s().valid) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_scores: no protein screened yet");
+  if (c->slots[c->cur ^ 1].busy)  // (the scores and maps are the next protein's by now)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_scores: not for an earlier protein of a batch");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   return bude::hip_check(hipMemcpy(out, static_cast<float *>(c->pred.ptr) + c->pose_off[ligand],
                                    c->nposes[ligand] * 4, hipMemcpyDeviceToHost),
@@ -1075,7 +1222,7 @@ extern "C" int bude_mi355x_combined_scores(bude_combined *c, size_t ligand, floa
 
 extern "C" int bude_mi355x_combined_exhaustive(bude_combined *c, double *device_ms) {
   if (!c) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exhaustive: NULL argument");
-  if (!c->screened)
+  if (!c->s().valid)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exhaustive: no protein screened yet");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
   if (int st = c->exact_all.reserve(c->total * 4)) return st;
@@ -1083,20 +1230,20 @@ extern "C" int bude_mi355x_combined_exhaustive(bude_combined *c, double *device_
   for (size_t l = 0; l < c->nlig; l++) all[l] = Span{l, c->pose_off[l], c->nposes[l]};
   if (int st = record(c, 0)) return st;
   if (int st = exact_ragged(c, all, static_cast<const float *>(c->poses.ptr), c->total,
-                            static_cast<float *>(c->exact_all.ptr), c->meta_all, c->tiles_all))
+                            static_cast<float *>(c->exact_all.ptr), c->meta_ex, c->tiles_all))
     return st;
   if (int st = record(c, 1)) return st;
   if (int st = sync(c)) return st;
   if (device_ms) *device_ms = elapsed(c, 0, 1);
   c->have_exact = true;
-  c->exact_sum = c->protein_sum;
+  c->exact_sum = c->s().protein_sum;
   return bude::ok();
 }
 
 extern "C" int bude_mi355x_combined_exact(bude_combined *c, size_t ligand, float *out) {
   if (!c || !out || ligand >= c->nlig)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_exact: bad arguments");
-  if (!c->have_exact || c->exact_sum != c->protein_sum)
+  if (!c->have_exact || c->exact_sum != c->s().protein_sum)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
                       "bude_mi355x_combined_exact: run bude_mi355x_combined_exhaustive for this protein first");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
@@ -1108,7 +1255,7 @@ extern "C" int bude_mi355x_combined_exact(bude_combined *c, size_t ligand, float
 extern "C" int bude_mi355x_combined_recall(bude_combined *c, float top_fraction, float *recall) {
   if (!c || !recall || !(top_fraction > 0.f) || top_fraction > 1.f)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_recall: bad arguments");
-  if (!c->have_exact || c->exact_sum != c->protein_sum)
+  if (!c->have_exact || c->exact_sum != c->s().protein_sum)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
                       "bude_mi355x_combined_recall: run bude_mi355x_combined_exhaustive for this protein first");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;
@@ -1136,12 +1283,12 @@ extern "C" int bude_mi355x_combined_recall(bude_combined *c, float top_fraction,
     return e;
   for (size_t l = 0; l < c->nlig; l++) {
     const size_t po = c->pose_off[l];
-    std::vector<uint32_t> t(tcount[l]), s(c->nsel[l]);
+    std::vector<uint32_t> t(tcount[l]), s(c->s().nsel[l]);
     if (int e = bude::hip_check(hipMemcpy(t.data(), static_cast<uint32_t *>(tidx.ptr) + po, t.size() * 4,
                                           hipMemcpyDeviceToHost),
                                 "hipMemcpy"))
       return e;
-    if (int e = bude::hip_check(hipMemcpy(s.data(), static_cast<uint32_t *>(c->sel_idx.ptr) + po, s.size() * 4,
+    if (int e = bude::hip_check(hipMemcpy(s.data(), static_cast<uint32_t *>(c->s().sel_idx.ptr) + po, s.size() * 4,
                                           hipMemcpyDeviceToHost),
                                 "hipMemcpy"))
       return e;
@@ -1169,8 +1316,10 @@ extern "C" int bude_mi355x_combined_grid(bude_combined *c, float lo[3], float *h
 extern "C" int bude_mi355x_combined_map(bude_combined *c, int32_t type, float *out) {
   if (!c || !out || type < 0 || size_t(type) >= c->slot_of_type.size())
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_map: bad arguments");
-  if (c->cfg.surrogate != BUDE_SURROGATE_FIELD || !c->screened)
+  if (c->cfg.surrogate != BUDE_SURROGATE_FIELD || !c->s().valid)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_map: no field screen run yet");
+  if (c->slots[c->cur ^ 1].busy)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_map: not for an earlier protein of a batch");
   const int32_t slot = c->slot_of_type[size_t(type)];
   if (slot < 0) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_map: no ligand atom has this type");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from minibude_amd import capi
 from minibude_amd import combined as C
 from minibude_amd import deck as D
 from minibude_amd import screen as S
@@ -438,3 +439,59 @@ def test_automatic_chunking_follows_set_keep(gpu_device):
                 assert np.array_equal(e.view(np.uint32), exact[idx].view(np.uint32)), (keep, l)
                 total += len(idx)
             assert r.nselected == total
+
+
+@pytest.mark.parametrize("kind", ["field1", "field3", "fallback", "mlp"])
+def test_batch_equals_one_by_one(gpu_device, kind):
+    # screen_batch overlaps each protein's scoring with the previous one's
+    # rescoring (or, with fallback pairs or the MLP, runs them one after
+    # another): every protein's selection, exact energies, sampled poses and
+    # counts equal those of screening the proteins one by one on a fresh
+    # object, and the callback sees each protein as the one screened last
+    w = small_workload(nposes=3000, nligands=5, nproteins=4, seed=19)
+    L = len(w.ligands)
+    kw = dict(labels_per_pair=48)
+    if kind == "mlp":
+        cfg = lambda: C.default_config("mlp", labels_per_pair=256, batch=64, epochs=1, max_val_rmse=1e9)  # noqa: E731
+    else:
+        kw.update(max_val_rmse=0.0 if kind == "fallback" else 1e9, pipeline_chunks=3 if kind == "field3" else 1)
+        cfg = lambda: C.default_config("field", **kw)  # noqa: E731
+
+    def snapshot(c):
+        return ([c.selected(l) for l in range(L)], [c.labels(l) for l in range(L)],
+                [c.pair(l).as_dict() for l in range(L)])
+
+    single, nsel = [], []
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg(), gpu_device) as c:
+        for p in w.proteins:
+            nsel.append(c.screen(p).nselected)
+            single.append(snapshot(c))
+        last_scores = [c.scores(l) for l in range(L)]
+    got, scores_err, exact_ok = {}, [], []
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg(), gpu_device) as c:
+        def done(i):
+            print(f"{kind}: protein {i} complete", flush=True)
+            got[i] = snapshot(c)
+            if i == 1:  # exhaustive energies of protein 1 from inside the batch
+                c.exhaustive()
+                exact_ok.append(all(np.array_equal(c.exact(l)[idx].view(np.uint32), e.view(np.uint32))
+                                    for l, (idx, e) in enumerate(got[i][0])))
+            if i < 3 and kind in ("field1", "field3"):
+                try:
+                    c.scores(0)
+                except capi.BudeError:
+                    scores_err.append(i)
+
+        res, wall = c.screen_batch(w.proteins, done)
+        assert sorted(got) == [0, 1, 2, 3] and wall > 0
+        assert exact_ok == [True]
+        if kind in ("field1", "field3"):
+            assert scores_err == [0, 1, 2]  # the next protein's scoring had begun
+        for i in range(4):
+            assert res[i].nselected == nsel[i]
+            for l in range(L):
+                for a, b in zip(got[i][0][l] + got[i][1][l], single[i][0][l] + single[i][1][l]):
+                    assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32)), (kind, i, l)
+                assert got[i][2][l] == single[i][2][l]
+        for l in range(L):  # after the batch the last protein is the one screened last, scores included
+            assert np.array_equal(c.scores(l).view(np.uint32), last_scores[l].view(np.uint32))
