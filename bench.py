@@ -350,8 +350,9 @@ def combined(full: bool, kinds=("field", "mlp"), nproteins: int = 0) -> dict:
                 sel += r.nselected
                 for k in parts:
                     parts[k] += getattr(r, k)
-            # recall from one-by-one screens of the same proteins (their
-            # selections equal the batch's), exact energies of every pair
+            # recall from one-by-one screens of the same proteins (a protein
+            # draws the same samples alone or in a batch, so their selections
+            # equal the batch's), exact energies of every pair
             for p, protein in enumerate(w.proteins):
                 if not full or p == 0:
                     one_by_one += c.screen(protein).total_ms

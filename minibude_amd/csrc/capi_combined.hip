@@ -218,7 +218,6 @@ struct bude_combined {
   int cur = 0;
   ProteinSlot &s() { return slots[cur]; }
   const ProteinSlot &s() const { return slots[cur]; }
-  size_t proteins_begun = 0;  // seeds each protein's sampling
   DeviceBuffer maps, cells, counter;
   // per-call work tables (each its own buffer; uploaded only while no queued
   // kernel reads it)
@@ -728,7 +727,10 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
 
   // 2. sampled poses of every pair: first 15/16 train the MLP, the rest
   //    validate (the field validates on all of them)
-  std::mt19937_64 rng(cfg.seed + 0x9e3779b97f4a7c15ull * (c->proteins_begun + 1));
+  // (seeded by the protein itself: a protein draws the same samples whatever
+  // was screened before it, alone or in a batch)
+  const uint64_t pseed = cfg.seed + 0x9e3779b97f4a7c15ull * c->s().protein_sum;
+  std::mt19937_64 rng(pseed);
   std::vector<uint32_t> src, lab_lig;
   std::vector<size_t> lab_off(L + 1, 0), lab_train(L, 0);
   for (size_t l = 0; l < L; l++) {
@@ -881,7 +883,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     if (int st = record(c, 2)) return st;
     if (int st = c->norms.assign(tn.data(), tn.size() * sizeof(tn[0]))) return st;
     std::vector<std::vector<float>> w, b;
-    bude::surrogate::init_weights(cfg.seed + c->proteins_begun, bude::mlp::kIn, w, b);
+    bude::surrogate::init_weights(pseed, bude::mlp::kIn, w, b);
     if (int st = bude::surrogate::set_weights(c->net, w, b, c->stream)) return st;
     std::vector<uint32_t> train;
     for (size_t l = 0; l < L; l++)
@@ -900,7 +902,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     tp.epochs = cfg.epochs;
     tp.lr = cfg.lr;
     tp.tail_weight = cfg.tail_weight;
-    tp.seed = cfg.seed + c->proteins_begun;
+    tp.seed = pseed;
     if (int st = bude::surrogate::train(c->net, td, std::move(train), tp, c->stream, nullptr, 0, nullptr)) return st;
     if (int st = c->bias0.reserve(L * bude::mlp::kHidden * 4)) return st;
     if (int st = bude::hip_check(bude::mlp::fold_bias(static_cast<const float *>(c->net.b[0].ptr),
@@ -1019,7 +1021,6 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
   c->s().K = K;
   res->nposes = c->total;
   res->labels = nlab;
-  c->proteins_begun++;
   return BUDE_OK;
 }
 
