@@ -6,7 +6,9 @@
 // surrogate over every pose of every pair, per-pair top-k of its scores, and
 // exact rescoring of every pair's selection (with the maps, chunk by chunk of
 // ligands, each chunk's rescoring beside the next chunk's scoring); pairs
-// whose validation error is too large are scored exhaustively instead.
+// whose validation error is too large are scored exhaustively instead.  A
+// batch of proteins runs each protein's scoring beside the previous one's
+// rescoring, in two protein slots.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
