@@ -168,6 +168,10 @@ struct ProteinSlot {
   // begun and not yet ended; how its selection and rescoring run
   bool busy = false, valid = false, pipelined = false, mlp = false;
   uint32_t *nsel_host = nullptr;  // pinned: the counts' copy lands here without blocking the host
+  // pinned staging of the protein's uploads (a synchronous copy would wait
+  // for the previous protein's kernels on the other streams)
+  char *stage = nullptr;
+  size_t stage_cap = 0, stage_off = 0;
   size_t K = 0;
   BudeCombinedResult res{};
   std::chrono::steady_clock::time_point t0;
@@ -181,6 +185,7 @@ struct ProteinSlot {
     for (auto &e : chunk_ev)
       if (e) (void)hipEventDestroy(e);
     if (nsel_host) (void)hipHostFree(nsel_host);
+    if (stage) (void)hipHostFree(stage);
   }
 };
 
@@ -252,13 +257,57 @@ struct Span {
   size_t lig, off, n;  // ligand, first pose in the pose array, poses
 };
 
+// Bytes one protein of natpro atoms uploads (its tables, the rescoring meta,
+// the sampled poses' tables), with room for the 256-byte alignment.
+size_t stage_bound(const bude_combined *c, size_t natpro) {
+  const size_t L = c->nlig, lpp = std::min<size_t>(c->cfg.labels_per_pair, c->total);
+  return natpro * 16 + (natpro + 2) * 4 + c->slot_type.size() * c->ff.size() * sizeof(bude::PairCoef) +
+         3 * L * sizeof(bude::LigandMeta) + 2 * L * lpp * 4 + L * (lpp / bude::kSplitTile + 1) * 8 + (L + 1) * 8 +
+         16 * 256;
+}
+
+// Room for `bytes` of this protein's uploads in the slot's staging buffer
+// (grown with headroom, before any upload of the protein is queued: the
+// slot's earlier uploads finished with its previous protein).
+int stage_reserve(ProteinSlot &ps, size_t bytes) {
+  ps.stage_off = 0;
+  if (bytes <= ps.stage_cap) return BUDE_OK;
+  if (ps.stage) (void)hipHostFree(ps.stage);
+  ps.stage = nullptr;
+  ps.stage_cap = 0;
+  const size_t cap = bytes + bytes / 2;
+  if (int st = bude::hip_check(hipHostMalloc(reinterpret_cast<void **>(&ps.stage), cap), "hipHostMalloc")) return st;
+  ps.stage_cap = cap;
+  return BUDE_OK;
+}
+
+// Queues the upload of n bytes into dst on stream s through the staging
+// buffer; the host does not wait for it.
+int stage_upload(ProteinSlot &ps, DeviceBuffer &dst, const void *src, size_t n, hipStream_t s) {
+  if (int st = dst.reserve(n)) return st;
+  if (n == 0) return BUDE_OK;
+  const size_t end = ps.stage_off + ((n + 255) & ~size_t(255));
+  if (end > ps.stage_cap) return bude::fail(BUDE_ERR_RUNTIME, "combined: staging buffer too small");
+  std::memcpy(ps.stage + ps.stage_off, src, n);
+  if (int st = bude::hip_check(hipMemcpyAsync(dst.ptr, ps.stage + ps.stage_off, n, hipMemcpyHostToDevice, s),
+                               "hipMemcpyAsync H2D"))
+    return st;
+  ps.stage_off = end;
+  return BUDE_OK;
+}
+
+// Synchronous upload, or staged on stream s when s is not null.
+int put(bude_combined *c, DeviceBuffer &dst, const void *src, size_t n, hipStream_t s) {
+  return s ? stage_upload(c->s(), dst, src, n, s) : dst.assign(src, n);
+}
+
 // Work tables of one persistent batched launch scoring spans[i].n poses of
 // ligand spans[i].lig starting at row spans[i].off of the pose array:
 // per-ligand meta and the tile list (heaviest ligands first), uploaded into
 // this launch's own buffers (synchronously: nothing may be reading them).
 // Returns the tile count in *ntiles (0: nothing to score).
 int prepare_exact(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer &meta, DeviceBuffer &tiles,
-                  int *ntiles, size_t tile_poses = 64 * kBatchPpwi) {
+                  int *ntiles, size_t tile_poses = 64 * kBatchPpwi, hipStream_t stage_on = nullptr) {
   *ntiles = 0;
   std::vector<bude::LigandMeta> m(c->nlig);
   for (size_t l = 0; l < c->nlig; l++) {
@@ -280,8 +329,8 @@ int prepare_exact(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer
   for (size_t i : order)
     for (size_t j = 0; j < spans[i].n; j += tile_poses) t.push_back(make_int2(int(spans[i].lig), int(j)));
   if (t.size() > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: too many tiles");
-  if (int st = meta.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
-  if (int st = tiles.assign(t.data(), t.size() * sizeof(int2))) return st;
+  if (int st = put(c, meta, m.data(), m.size() * sizeof(bude::LigandMeta), stage_on)) return st;
+  if (int st = put(c, tiles, t.data(), t.size() * sizeof(int2), stage_on)) return st;
   *ntiles = int(t.size());
   return BUDE_OK;
 }
@@ -344,7 +393,7 @@ int exact_ragged(bude_combined *c, const std::vector<Span> &spans, const float *
 // Work tables of a field scoring of spans (as prepare_exact): meta and the
 // per-ligand tile prefix.  Returns the tile count in *ntiles.
 int prepare_field(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer &meta, DeviceBuffer &tile_begin,
-                  int64_t *ntiles, std::vector<int64_t> *tb_host = nullptr) {
+                  int64_t *ntiles, std::vector<int64_t> *tb_host = nullptr, hipStream_t stage_on = nullptr) {
   *ntiles = 0;
   std::vector<bude::LigandMeta> m(c->nlig);
   std::vector<int64_t> tb(c->nlig + 1, 0);
@@ -359,8 +408,8 @@ int prepare_field(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer
   for (size_t l = 0; l < c->nlig; l++) tb[l + 1] = tb[l] + int64_t((size_t(m[l].nposes) + 63) / 64);
   if (tb_host) *tb_host = tb;
   if (tb[c->nlig] == 0) return BUDE_OK;
-  if (int st = meta.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
-  if (int st = tile_begin.assign(tb.data(), tb.size() * sizeof(int64_t))) return st;
+  if (int st = put(c, meta, m.data(), m.size() * sizeof(bude::LigandMeta), stage_on)) return st;
+  if (int st = put(c, tile_begin, tb.data(), tb.size() * sizeof(int64_t), stage_on)) return st;
   *ntiles = tb[c->nlig];
   return BUDE_OK;
 }
@@ -660,6 +709,16 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
     if (!st)
       st = bude::hip_check(hipHostMalloc(reinterpret_cast<void **>(&ps.nsel_host), nligands * sizeof(uint32_t)),
                            "hipHostMalloc");
+    // each slot's per-protein buffers sized now for proteins of up to 2^16
+    // atoms (a batch then allocates nothing: an allocation can wait for the
+    // kernels in flight)
+    const size_t natoms = lig4.size() / 4, ntypes_ = c->ff.size(), pro_max = size_t(1) << 16;
+    if (!st) st = ps.protein.reserve(pro_max * 16);
+    if (!st) st = ps.group_begin.reserve((ntypes_ + 1) * 4);
+    if (!st) st = ps.slot_coef.reserve(c->slot_type.size() * ntypes_ * sizeof(bude::PairCoef));
+    if (!st) st = ps.meta_rs.reserve(nligands * sizeof(bude::LigandMeta));
+    if (!st) st = ps.coef.reserve(natoms * ntypes_ * sizeof(bude::PairCoef));
+    if (!st) st = stage_reserve(ps, stage_bound(c.get(), pro_max));
   }
   if (!st) st = c->counter.alloc(256);
   if (!st) st = bude::hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -712,14 +771,28 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
   if (ncoef > 0x7fffffffu) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: pair tables too large");
   c->s().coef_off.assign(L, 0);
   for (size_t l = 0; l < L; l++) c->s().coef_off[l] = int32_t(size_t(c->lig_off[l]) * c->s().ngroups);
-  {
+  // every upload of this protein goes through the slot's pinned staging
+  if (int st = stage_reserve(c->s(), stage_bound(c, natpro))) return st;
+  {  // protein tables and the rescoring meta on the third stream (idle here:
+     // the previous protein's sampled poses are done; the second stream may
+     // still be scoring it); the second stream and, through the third, the
+     // first wait for them
     std::vector<BudeAtom> probe(c->slot_type.size());
     for (size_t s = 0; s < probe.size(); s++) probe[s] = BudeAtom{0.f, 0.f, 0.f, c->slot_type[s]};
     const std::vector<bude::PairCoef> sc = bude::coef_table(probe.data(), probe.size(), groups, c->ff.data());
-    if (int st = c->s().slot_coef.assign(sc.data(), sc.size() * sizeof(bude::PairCoef))) return st;
+    if (int st = stage_upload(c->s(), c->s().slot_coef, sc.data(), sc.size() * sizeof(bude::PairCoef), c->stream3))
+      return st;
+    std::vector<bude::LigandMeta> m(L);  // everything but the pose counts, which the selection writes
+    for (size_t l = 0; l < L; l++)
+      m[l] = bude::LigandMeta{c->lig_off[l], int32_t(c->lig[l].size()), c->s().coef_off[l], 0, int64_t(c->pose_off[l]), 0};
+    if (int st = stage_upload(c->s(), c->s().meta_rs, m.data(), m.size() * sizeof(bude::LigandMeta), c->stream3))
+      return st;
   }
-  if (int st = c->s().protein.assign(groups.xyzw.data(), groups.xyzw.size() * 4)) return st;
-  if (int st = c->s().group_begin.assign(groups.begin.data(), groups.begin.size() * 4)) return st;
+  if (int st = stage_upload(c->s(), c->s().protein, groups.xyzw.data(), groups.xyzw.size() * 4, c->stream3)) return st;
+  if (int st = stage_upload(c->s(), c->s().group_begin, groups.begin.data(), groups.begin.size() * 4, c->stream3))
+    return st;
+  if (int st = record(c, 9, c->stream3)) return st;
+  if (int st = bude::hip_check(hipStreamWaitEvent(c->stream2, c->s().ev[9], 0), "hipStreamWaitEvent")) return st;
   if (int st = c->s().coef.reserve(ncoef * sizeof(bude::PairCoef))) return st;
   hipLaunchKernelGGL(expand_coef_kernel, dim3(blocks_for(ncoef, 256)), dim3(256), 0, c->stream3,
                      static_cast<const float4 *>(c->ligands.ptr), ncoef, int(c->s().ngroups),
@@ -763,8 +836,8 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     lab_train[l] = mlp ? m - std::max<size_t>(1, m / 16) : 0;
   }
   const size_t nlab = src.size();
-  if (int st = c->lab_src.assign(src.data(), nlab * 4)) return st;
-  if (int st = c->lab_lig.assign(lab_lig.data(), nlab * 4)) return st;
+  if (int st = stage_upload(c->s(), c->lab_src, src.data(), nlab * 4, c->stream3)) return st;
+  if (int st = stage_upload(c->s(), c->lab_lig, lab_lig.data(), nlab * 4, c->stream3)) return st;
   if (int st = c->lab_poses.reserve(6 * nlab * 4)) return st;
   if (int st = c->lab_e.reserve(nlab * 4)) return st;
   if (int st = c->lab_pred.reserve(nlab * 4)) return st;
@@ -783,7 +856,8 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
   int nt_label = 0;
   int64_t ft_all = 0, ft_label = 0;
   const std::vector<int64_t> &tb_all = c->tb_all_host;
-  if (int st = prepare_exact(c, lab_spans, c->meta_label, c->tiles_label, &nt_label, bude::kSplitTile)) return st;
+  if (int st = prepare_exact(c, lab_spans, c->meta_label, c->tiles_label, &nt_label, bude::kSplitTile, c->stream3))
+    return st;
   if (!mlp) {
     const size_t np = c->grid.points(), nslots = c->slot_type.size();
     if (int st = c->maps.reserve(np * nslots * 4)) return st;
@@ -793,13 +867,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
       c->field_tables = true;
     }
     ft_all = c->tb_all_host[L];
-    if (int st = prepare_field(c, lab_spans, c->meta_lf, c->tb_label, &ft_label)) return st;
-  }
-  {  // rescoring meta: everything but the pose counts, which the selection writes on the device
-    std::vector<bude::LigandMeta> m(L);
-    for (size_t l = 0; l < L; l++)
-      m[l] = bude::LigandMeta{c->lig_off[l], int32_t(c->lig[l].size()), c->s().coef_off[l], 0, int64_t(c->pose_off[l]), 0};
-    if (int st = c->s().meta_rs.assign(m.data(), m.size() * sizeof(bude::LigandMeta))) return st;
+    if (int st = prepare_field(c, lab_spans, c->meta_lf, c->tb_label, &ft_label, nullptr, c->stream3)) return st;
   }
   const size_t K = c->chunk_lig.size() - 1;
   int2 *rs_tiles = static_cast<int2 *>(c->s().tiles_rs.ptr);
@@ -837,11 +905,13 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     // chunk by chunk: scores, top-k and rescoring tables, all on the device;
     // the scoring of chunk k > 0 runs beside the rescoring of chunk k - 1
     // and issues ahead of it (chunk 0 runs beside the sampled-pose pass,
-    // which the host waits for)
+    // which the host waits for, and, in a batch, beside the previous
+    // protein's rescoring, which it issues ahead of too)
+    const bool overlap = c->slots[c->cur ^ 1].busy;
     for (size_t k = 0; k < K; k++) {
       const size_t a = c->chunk_lig[k], b = c->chunk_lig[k + 1];
       if (int st = launch_field(c, poses, c->total, pred, c->meta_all, c->tb_all, tb_all[a], tb_all[b] - tb_all[a],
-                                c->stream2, k > 0))
+                                c->stream2, k > 0 || overlap))
         return st;
       if (int st = bude::hip_check(hipEventRecord(c->s().chunk_ev[2 * k], c->stream2), "hipEventRecord")) return st;
       if (int st = select_and_tables(c, *c->chunk_plan[k], a, static_cast<const int32_t *>(c->rs_order.ptr) + a,

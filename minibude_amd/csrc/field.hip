@@ -23,7 +23,9 @@ __device__ __forceinline__ float pair_reach(const PairCoef &c) {
   return r * 1.0001f + 1e-3f;
 }
 
-constexpr int kMapList = 512;  // culled atoms per wave held in LDS before they are scored
+// culled atoms per wave held in LDS before they are scored: 8 KiB per
+// work-group, so the maps still fit beside a pinned rescoring grid
+constexpr int kMapList = 128;
 
 // The pairs of one group's culled atoms (LDS, in atom order) with this
 // work-item's point, into part.
