@@ -495,3 +495,27 @@ def test_batch_equals_one_by_one(gpu_device, kind):
                 assert got[i][2][l] == single[i][2][l]
         for l in range(L):  # after the batch the last protein is the one screened last, scores included
             assert np.array_equal(c.scores(l).view(np.uint32), last_scores[l].view(np.uint32))
+
+
+def test_batch_edges_and_errors(gpu_device):
+    # an empty batch, a batch of one, and a batch whose second protein is
+    # malformed: the error is reported, nothing stays in flight, and the
+    # object screens normally afterwards
+    w = small_workload(nposes=2000, nligands=3, nproteins=2, seed=20)
+    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48)
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        assert c.screen_batch([])[0] == []
+        seen = []
+        res, _ = c.screen_batch(w.proteins[:1], seen.append)
+        assert seen == [0] and res[0].nposes == 6000
+        one = [c.selected(l) for l in range(3)]
+        bad = w.proteins[1].copy()
+        bad["type"][0] = 999
+        with pytest.raises(capi.BudeInvalidArgument):
+            c.screen_batch([w.proteins[0], bad])
+        with pytest.raises(capi.BudeError, match="no protein screened"):
+            c.selected(0)
+        c.screen(w.proteins[0])
+        for l in range(3):
+            idx, e = c.selected(l)
+            assert np.array_equal(idx, one[l][0]) and np.array_equal(e.view(np.uint32), one[l][1].view(np.uint32))
