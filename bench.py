@@ -428,7 +428,7 @@ def main(argv: list[str] | None = None) -> int:
         return 0
 
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only: the plain kernel is data-parallel over poses and this build runs "
+        print(f"bench.py measures one GPU only: the plain kernel is data-parallel over poses and the library runs "
               f"on a single MI355X (--gpus {a.gpus} requested). Run with --gpus 1.", file=sys.stderr)
         return 2
     if device_count() < 1:

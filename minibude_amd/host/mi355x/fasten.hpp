@@ -2,7 +2,8 @@
 // libbude_mi355x through its C ABI (include/bude_mi355x.h).
 //
 // Placed at src/mi355x/fasten.hpp of a miniBUDE tree it is a model like
-// src/hip/fasten.hpp (selected with -DMI355X); this
+// src/hip/fasten.hpp (selected with -DMI355X: an `#elif defined(MI355X)`
+// include in src/main.cpp and a register_model line in CMakeLists.txt); this
 // project's own driver includes it the same way, next to its own bude.h.
 //
 // fasten() keeps HipBude::fasten's contract (src/hip/fasten.hpp:212-267):
