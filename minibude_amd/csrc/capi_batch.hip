@@ -128,11 +128,9 @@ int graph_for(bude_batch *b, const BudeBatchConfig *cfg, hipGraphExec_t *exec) {
                                  "hipStreamBeginCapture"))
       return st;
     const int lst = launch_once(b, cfg);
-    hipGraph_t graph = nullptr;
-    const hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+    const hipError_t ce = hipStreamEndCapture(b->stream, &g->graph);  // g owns (and frees) it on every path
     if (lst) return lst;
     if (int st = bude::hip_check(ce, "hipStreamEndCapture")) return st;
-    g->graph = graph;
     if (int st = bude::hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate"))
       return st;
     it = b->graphs.emplace(key, std::move(g)).first;

@@ -12,7 +12,8 @@
 // Behaviours kept on purpose (src/main.cpp line numbers):
 //   * PPWI is one of 1,2,4,8,16,32,64,128 (:48, :221-231); default ppwi 1 and
 //     wgsize 1 (:282-283) -- wgsize 1 is not a whole wavefront, so the model
-//     reports it incompatible and the run stops with the library's message;
+//     reports it incompatible: the combination is skipped with the library's
+//     reason before any device buffer is made, and the exit status is failure;
 //   * combos with nposes < ppwi*wgsize or nposes % (ppwi*wgsize) != 0 are
 //     skipped with a warning (:520-530);
 //   * validation: signed |ref - e| / ref, both |.| < 1 skipped, fail above
@@ -339,11 +340,19 @@ template <size_t N> void registerPpwi(std::map<size_t, KernelFn> &k) {
   k[N] = [](const Params &p, size_t wgsize, size_t device) {
     Mi355xBude<N> model;
     const auto copy = std::make_unique<Params>(p);
-    if (!model.compatible(*copy, wgsize, device))
-      std::cerr << "Selected device is not compatible with this implementation, results may not be correct!"
-                << std::endl;
     return model.fasten(*copy, wgsize, device);
   };
+}
+
+// Bude<PPWI>::compatible for a runtime ppwi; the reason goes to `why`.
+// The reference driver only warns here (src/main.cpp:492-495) and calls
+// fasten anyway; this driver skips the combination instead, before any
+// device buffer is made, since the library cannot run it.
+bool compatible(size_t ppwi, size_t wgsize, size_t device, std::string &why) {
+  int ok = 0;
+  if (bude_mi355x_compatible(int(device), ppwi, wgsize, &ok) == BUDE_OK && ok == 1) return true;
+  why = bude_mi355x_last_error();
+  return false;
 }
 
 bool run(const Cli &c) {
@@ -373,9 +382,16 @@ bool run(const Cli &c) {
   if (!p.csv) std::cout << "device: { index: " << dev.first << ", " << " name: \"" << dev.second << "\" }" << std::endl;
 
   std::vector<Result> results;
-  bool dumped = false;
+  bool dumped = false, skippedIncompatible = false;
   for (size_t ppwi : c.ppwis) {
     for (size_t wgsize : c.wgsizes) {
+      std::string why;
+      if (!compatible(ppwi, wgsize, size_t(dev.first), why)) {
+        std::cerr << "Selected device is not compatible with this implementation (ppwi " << ppwi << ", wgsize "
+                  << wgsize << ": " << why << "), skipping" << std::endl;
+        skippedIncompatible = true;
+        continue;
+      }
       if (p.nposes() < ppwi * wgsize) {
         std::cout << " # WARNING: pose count " << p.nposes() << " <= (" << wgsize << " (wgsize) * " << ppwi
                   << " (ppwi)), skipping" << std::endl;
@@ -403,14 +419,15 @@ bool run(const Cli &c) {
   }
   if (results.empty()) {
     std::cout << (p.csv ? "# " : "") << "best: ~" << std::endl;
-    return true;
+    return !skippedIncompatible;
   }
   const auto best = std::min_element(results.begin(), results.end(),
                                      [](const Result &a, const Result &b) { return a.ms.sum < b.ms.sum; });
   std::cout << (p.csv ? "# " : "") << "best: { min_ms: " << best->ms.min << ", max_ms: " << best->ms.max
             << ", sum_ms: " << best->ms.sum << ", avg_ms: " << best->ms.mean << ", ppwi: " << best->sample.ppwi
             << ", wgsize: " << best->sample.wgsize << " }\n";
-  return std::all_of(results.begin(), results.end(), [](const Result &r) { return r.valid; });
+  return !skippedIncompatible &&
+         std::all_of(results.begin(), results.end(), [](const Result &r) { return r.valid; });
 }
 
 }  // namespace
@@ -441,5 +458,10 @@ int main(int argc, char *argv[]) {
                 << "  wgsize:       [" << join(c.wgsizes) << "]" << std::endl;
     }
   }
-  return run(c) ? EXIT_SUCCESS : EXIT_FAILURE;
+  try {
+    return run(c) ? EXIT_SUCCESS : EXIT_FAILURE;
+  } catch (const std::exception &e) {
+    std::cerr << "error: " << e.what() << std::endl;
+    return EXIT_FAILURE;
+  }
 }

@@ -44,7 +44,13 @@ def test_csv_and_energy_dump(gpu_device, exe, tmp_path):
 
 
 def test_default_wgsize_is_incompatible_not_wrong(gpu_device):
-    # the reference default wgsize=1 is not a whole wavefront on this model
+    # the reference default wgsize=1 is not a whole wavefront on this model:
+    # the driver skips it before any device buffer is made and exits with
+    # failure status (not an abort), reporting no result
     r = run(DRIVER, "--deck", str(GOLDEN / "bm1"), "-n", "64", "-i", "1")
-    assert r.returncode != 0
-    assert "not compatible" in r.stderr and "wgsize 1" in r.stderr
+    assert r.returncode == 1, (r.returncode, r.stderr[-2000:])
+    assert "not compatible" in r.stderr and "wgsize 1" in r.stderr and "skipping" in r.stderr
+    assert "best: ~" in r.stdout and "valid:" not in r.stdout
+    # a mix: the incompatible point is skipped, the whole-wavefront one runs and validates
+    r = run(DRIVER, "--deck", str(GOLDEN / "bm1"), "-n", "256", "-i", "1", "-w", "1,64")
+    assert r.returncode == 1 and r.stdout.count("valid: true") == 1, r.stdout[-2000:] + r.stderr[-2000:]
