@@ -97,6 +97,34 @@ int bude_mi355x_context_ms(const bude_ctx *ctx, double *ms);
 /* Frees every device buffer of the context (src/hip/fasten.hpp:253-264). */
 int bude_mi355x_destroy(bude_ctx *ctx);
 
+/* ---- plain fasten_main without allocation.  No counterpart in the
+ *      reference (whose model allocates and frees in every fasten call); it
+ *      is for callers that cannot allocate device memory: code under stream
+ *      capture, tight serving loops, harnesses that forbid hipMalloc. ----
+ * One pass of the plain kernel over caller device memory, with the
+ * energies bude_mi355x_create + bude_mi355x_fasten give for the same deck,
+ * bit for bit.  Every pointer is a device pointer: protein[natpro],
+ * ligand[natlig] and forcefield[ntypes] deck records, `poses` six rows of
+ * pose_stride floats (row k at poses + k * pose_stride, the first nposes of
+ * each used), energies[nposes] written.  The deck's derived tables (atoms
+ * grouped by type, pair coefficients) go to a fixed array of
+ * BUDE_MI355X_NOALLOC_SCRATCH_BYTES per device, held until the process ends;
+ * a deck that needs more (about 16 B per protein atom + 32 B per ligand atom
+ * and forcefield type) or more than BUDE_MI355X_NOALLOC_MAX_TYPES types is
+ * rejected with BUDE_ERR_INVALID_ARGUMENT before any device work.  Nothing is
+ * allocated, on the device or by the library on the host.  Work runs on
+ * `stream` (a hipStream_t; NULL: the null stream) and the call returns when
+ * the energies are written.  An atom type outside the forcefield is reported
+ * as BUDE_ERR_INVALID_ARGUMENT and leaves the energies untouched.  Calls on
+ * one device share its array: calls that overlap in time on one device race
+ * and must be serialised by the caller. */
+#define BUDE_MI355X_NOALLOC_SCRATCH_BYTES (4u << 20)
+#define BUDE_MI355X_NOALLOC_MAX_TYPES 256
+int bude_mi355x_fasten_noalloc(int device, const BudeAtom *protein, size_t natpro, const BudeAtom *ligand,
+                               size_t natlig, const BudeFFParams *forcefield, size_t ntypes, const float *poses,
+                               size_t pose_stride, size_t nposes, size_t ppwi, size_t wgsize, float *energies,
+                               void *stream);
+
 /* ---- batched ragged fasten_main: one protein, many ligands with their own
  *      atom and pose counts, in ONE persistent launch.  No counterpart in the
  *      reference, whose driver runs one (ligand, pose set) per

@@ -42,6 +42,7 @@ LIB_SOURCES = [
     "deck_io.cpp",
     "fasten_plain.hip",
     "capi_plain.hip",
+    "capi_noalloc.hip",
     "fasten_batch.hip",
     "capi_batch.hip",
     "mlp.hip",

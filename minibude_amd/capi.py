@@ -38,6 +38,7 @@ EXPORTED = (
     "bude_mi355x_fasten",
     "bude_mi355x_context_ms",
     "bude_mi355x_destroy",
+    "bude_mi355x_fasten_noalloc",
     "bude_mi355x_batch_create",
     "bude_mi355x_batch_run",
     "bude_mi355x_batch_run_separate",
@@ -131,6 +132,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "bude_mi355x_fasten": (_I, [_P, _S, _S, _S, _S, _P, _P, _P]),
         "bude_mi355x_context_ms": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
         "bude_mi355x_destroy": (_I, [_P]),
+        "bude_mi355x_fasten_noalloc": (_I, [_I, _P, _S, _P, _S, _P, _S, _P, _S, _S, _S, _S, _P, _P]),
         "bude_mi355x_batch_create": (_I, [_I, _P, _S, _P, _S, _S, ctypes.POINTER(_P), ctypes.POINTER(_S),
                                           ctypes.POINTER(_P), ctypes.POINTER(_S), ctypes.POINTER(_P)]),
         "bude_mi355x_batch_run": (_I, [_P, _P, _S, _S, _P]),
