@@ -63,17 +63,12 @@ METRIC = "G-interactions/s plain vs batched; screened poses/s at \u22650.99 top-
 WORKLOAD = "c1"
 DEFAULT_PPWI = 1
 DEFAULT_WGSIZE = 256
-# persistent ragged-batch configuration chosen by bude_mi355x_batch_autotune on
-# the 256-ligand workload: ppwi, wgsize, waves per SIMD, heaviest-first, graph
-BATCH_CONFIG = (1, 64, 8, 0, 1)
 # screen operating points: the workload's 2 % first, then larger fractions
 # until the exact top-1 % recall reaches 0.99
 SCREEN_KEEP = (0.02, 0.03, 0.04, 0.05, 0.06, 0.08)
-# combined screen sizes: the default keeps the run to seconds with recall on
-# every pair; the full size is the workload's own
+# combined screen sizes: the workload's own (default), and a reduced one
 COMBINED_DEFAULT = dict(nproteins=16, nligands=64, nposes=1 << 18)
 COMBINED_FULL = dict(nproteins=16, nligands=512, nposes=1 << 20)
-COMBINED_FIELD_KEEPS = (0.01, 0.0125, 0.015)
 TRAFFIC_SUMMARY = ROOT / "profiles" / "fasten_plain_c1_traffic.json"
 CPU_SAMPLE_POSES = 65536
 CPU_ITERS = 2
@@ -83,25 +78,24 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(deck: D.Deck, ref: np.ndarray) -> dict | None:
-    """The reference omp model on the first CPU_SAMPLE_POSES poses of the deck."""
-    flags = Path("/proc/cpuinfo").read_text() if Path("/proc/cpuinfo").exists() else ""
-    v4 = " avx512f " in flags.replace("\n", " ")
-    cpu = re.search(r"^model name\s*:\s*(.+)$", flags, re.M)
-    cpu = cpu.group(1).strip() if cpu else "unknown CPU"
-    bind = os.environ.get("OMP_PROC_BIND", "unset")
-    exe = ROOT / "oracle" / "_ref" / ("omp-bude-v4" if v4 else "omp-bude-v3")
-    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    if not exe.exists():
-        log(f"cpu baseline skipped: {exe} not built")
-        return None
-    n = min(CPU_SAMPLE_POSES, deck.nposes)
-    sub = deck.head(n)
-    sub.ref_energies = ref[:n]
+def _host() -> dict:
+    info = Path("/proc/cpuinfo").read_text() if Path("/proc/cpuinfo").exists() else ""
+    cpu = re.search(r"^model name\s*:\s*(.+)$", info, re.M)
+    return {"v4": " avx512f " in info.replace("\n", " "), "model": cpu.group(1).strip() if cpu else "unknown CPU",
+            "bind": os.environ.get("OMP_PROC_BIND", "unset"),
+            "cores": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1), "nproc": os.cpu_count()}
+
+
+def run_reference_omp(exe: Path, deck: D.Deck, ppwi: int, iters: int, cores: int) -> dict | None:
+    """The reference driver's own omp model on a deck written to a temporary
+    directory: per-iteration wall times from its raw_iterations (2 warmups
+    first, src/main.cpp:160), its validation outcome, G-interactions/s of the
+    timed iterations."""
+    n = deck.nposes
     with tempfile.TemporaryDirectory() as tmp:
-        D.write_deck(tmp, sub)
-        cmd = [str(exe), "--deck", tmp, "-n", str(n), "-i", str(CPU_ITERS), "-p", "64"]
-        log("cpu baseline: " + " ".join(cmd[:1] + cmd[2:2] + cmd[3:]))
+        D.write_deck(tmp, deck)
+        cmd = [str(exe), "--deck", tmp, "-n", str(n), "-i", str(iters), "-p", str(ppwi)]
+        log("cpu baseline: " + " ".join([exe.name] + cmd[3:]))
         t0 = time.time()
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=300,
                            env=dict(os.environ, OMP_NUM_THREADS=str(cores)))
@@ -109,9 +103,6 @@ def cpu_baseline(deck: D.Deck, ref: np.ndarray) -> dict | None:
     if r.returncode not in (0, 1):
         log(f"cpu baseline failed ({r.returncode}): {r.stderr[-400:]}")
         return None
-    # time from the reference's own per-iteration wall clock (raw_iterations,
-    # warmups first): its CSV reports zero rates for a result that fails its
-    # validation, which the reference's FP32 does on this ill-conditioned deck
     raw = re.search(r"raw_iterations:\s*\[([^\]]*)\]", r.stdout)
     outcome = re.search(r"valid: (\w+), max_diff_%: ([0-9.]+)", r.stdout)
     if not raw:
@@ -121,19 +112,55 @@ def cpu_baseline(deck: D.Deck, ref: np.ndarray) -> dict | None:
     timed = its[2:] if len(its) > 2 else its
     ms = sum(timed) / len(timed)
     ips = n * deck.natlig * deck.natpro / (ms * 1e-3)
-    valid = outcome.group(1) if outcome else "?"
-    log(f"cpu baseline: raw_iterations {its} ms -> {ips / 1e9:.2f} G-interactions/s (reference outcome "
-        f"{outcome.group(0) if outcome else '?'})")
+    out = {"ppwi": ppwi, "iterations": len(timed), "ms_per_iteration": round(ms, 3),
+           "g_interactions_per_s": round(ips / 1e9, 4), "wall_s": round(wall, 1),
+           "valid": outcome.group(1) == "true" if outcome else None,
+           "max_diff_pct": float(outcome.group(2)) if outcome else None}
+    log(f"cpu baseline: {out}")
+    return out
+
+
+def cpu_baseline(c1: D.Deck, c1_ref: np.ndarray) -> dict | None:
+    """The reference's own omp model (src/main.cpp + src/omp/fasten.hpp, built
+    from the reference sources into oracle/_ref) on the host cores: the
+    CPU workload as configured (bm1-shaped synthetic deck c0: 938 protein /
+    26 ligand atoms, 65,536 poses, seed 0; ppwi 1, 8 timed iterations after
+    the driver's 2 warmups), the same deck at its best ppwi (64), and the
+    first 65,536 poses of the GPU deck c1 at ppwi 64."""
+    h = _host()
+    exe = ROOT / "oracle" / "_ref" / ("omp-bude-v4" if h["v4"] else "omp-bude-v3")
+    if not exe.exists():
+        log(f"cpu baseline skipped: {exe} not built")
+        return None
+    c0 = D.workload_deck("c0")
+    fx = ROOT / "tests" / "golden" / "c0" / "ref_energies.f32"
+    c0.ref_energies = np.fromfile(fx, "<f4") if fx.exists() else None
+    if c0.ref_energies is None:
+        log("cpu baseline skipped: no c0 reference energies")
+        return None
+    conf = run_reference_omp(exe, c0, 1, 8, h["cores"])
+    best = run_reference_omp(exe, c0, 64, 8, h["cores"])
+    n = min(CPU_SAMPLE_POSES, c1.nposes)
+    sub = c1.head(n)
+    sub.ref_energies = c1_ref[:n]
+    c1s = run_reference_omp(exe, sub, 64, CPU_ITERS, h["cores"])
+    if conf is None:
+        return None
     return {
-        "value": round(ips / 1e9, 4),
+        "value": conf["g_interactions_per_s"],
         "unit": "G-interactions/s",
-        "cores": cores,
+        "cores": h["cores"],
         "kind": "reference",
-        "sample": (f"first {n} of {deck.nposes} poses of the {WORKLOAD} deck ({deck.natpro} protein / {deck.natlig} "
-                   f"ligand atoms); reference omp model {exe.name} (src/main.cpp + src/omp/fasten.hpp, -O3 "
-                   f"-ffast-math -fopenmp), ppwi=64, 2 warmup + {CPU_ITERS} timed passes timed by its own "
-                   f"raw_iterations, {wall:.1f}s wall ({cores} threads of {os.cpu_count()} on {cpu}, "
-                   f"OMP_PROC_BIND={bind}); its validation against the strict reference kernel: valid={valid}"),
+        "sample": (f"c0: bm1-shaped synthetic deck (938 protein / 26 ligand atoms, 34 types, 65,536 poses, seed 0), "
+                   f"the reference's omp model {exe.name} (src/main.cpp + src/omp/fasten.hpp, -O3 -ffast-math "
+                   f"-fopenmp) at ppwi 1 and wgsize 1, 2 warmup + 8 timed iterations timed by its own "
+                   f"raw_iterations ({conf['wall_s']} s wall), valid={conf['valid']} "
+                   f"(max_diff_% {conf['max_diff_pct']}); {h['cores']} threads of {h['nproc']} on {h['model']}, "
+                   f"OMP_PROC_BIND={h['bind']}"),
+        "configured": conf,
+        "best_ppwi": best,
+        "c1_sample": None if c1s is None else dict(c1s, deck=f"first {n} poses of the c1 deck (4,096 / 64 atoms)"),
+        "host": {"nproc": h["nproc"], "model": h["model"], "omp_num_threads": h["cores"], "omp_proc_bind": h["bind"]},
     }
 
 
@@ -192,11 +219,14 @@ def sweep(ctx: Context, ref: np.ndarray | None, warmup: int, steps: int) -> dict
 
 def batched() -> dict:
     from minibude_amd.batch import Batch, ragged_workload
-    from minibude_amd.capi import BatchConfig
     ls = ragged_workload()
     log(f"batched: {len(ls.ligands)} ligands, {ls.total_poses} poses, {ls.interactions:.3e} interactions")
-    cfg = BatchConfig(*BATCH_CONFIG)
     with Batch(ls, 0) as b:
+        # the launch shape is the project's autotuner's choice on this batch
+        t0 = time.time()
+        cfg, tune_ms = b.autotune()
+        tune_s = time.time() - t0
+        log(f"batched: autotuned in {tune_s:.1f} s: {cfg.as_dict()} ({tune_ms:.2f} ms)")
         ms = b.run(cfg, warmup=1, iterations=3)[1:]
         sep = b.run_separate(1, 256, warmup=1, iterations=1)[1:]
     t, ts = float(np.median(ms)), float(np.median(sep))
@@ -209,74 +239,116 @@ def batched() -> dict:
         "separate_g_interactions_per_s": round(ls.interactions / ts / 1e6, 2),
         "speedup_vs_separate": round(ts / t, 2),
         "config": cfg.as_dict(),
+        "autotune": {"seconds": round(tune_s, 2), "best_ms": round(tune_ms, 3),
+                     "search": ("bude_mi355x_batch_autotune on this batch: 24 candidates (tile width 64/128/256 "
+                                "poses, work-group 64/256, 4/8 waves per SIMD, heaviest-first or ligand order; "
+                                "hipGraph replays), each timed once, the 4 fastest by the median of 3")},
     }
     log(f"batched: {out}")
     return out
 
 
+def breakeven(setup_ms: float, exhaustive_ms: float, screen_ms: float, n: int) -> float | None:
+    """Poses beyond which a surrogate screen (setup_ms of labels and training,
+    then screen_ms per n poses) beats exhaustive scoring (exhaustive_ms per n)."""
+    per = (exhaustive_ms - screen_ms) / n
+    return round(setup_ms / per) if per > 0 else None
+
+
 def screen(epochs: int = 0) -> dict:
+    """The learned-prescreen workload (4,096-atom protein, 26-atom ligand,
+    16.8M fresh poses): the pose MLP as configured (12 pose inputs) and with
+    the 9 rotation-matrix entries added (21), each trained on 8.4M plain-
+    kernel energies; the learned per-atom field and the exact single-atom
+    maps through the combined screen on the same pair."""
+    from minibude_amd import combined as C
     from minibude_amd import screen as S
     w = S.screen_workload()
-    cfg = S.default_config(**({"epochs": epochs} if epochs else {}))
     n = w.screen_poses.shape[1]
-    log(f"screen: training on {w.train_poses.shape[1]} poses, screening {n}")
-    points = []
-    with S.Screen(w.protein, w.ligand, w.forcefield, cfg, 0) as sc:
-        st, _ = sc.train(w.train_poses)
-        sc.load(w.screen_poses)
-        exact, ex_ms = sc.exhaustive()
-        # the workload's operating point (2 % rescored), then the smallest
-        # fraction whose recall of the exact top 1 % reaches 0.99
-        for keep in SCREEN_KEEP:
-            sc.set_keep(keep)
-            r = min((sc.run() for _ in range(3)), key=lambda x: x.total_ms)
-            idx, _ = sc.selected()
-            rec = S.recall(idx, exact, 0.01)
-            points.append({
-                "keep_fraction": keep,
-                "poses_per_s": round(n / (r.total_ms * 1e-3), 1),
-                "screen_ms": round(r.total_ms, 3),
-                "surrogate_ms": round(r.score_ms, 3),
-                "select_ms": round(r.select_ms, 3),
-                "rescore_ms": round(r.rescore_ms, 3),
-                "speedup_vs_exhaustive": round(ex_ms / r.total_ms, 2),
-                "recall_top1pct": round(rec, 4),
-                "recall_top0.1pct": round(S.recall(idx, exact, 0.001), 4),
-                "fallback": bool(r.fallback),
-            })
-            log(f"screen: {points[-1]}")
-            if keep != SCREEN_KEEP[0] and rec >= 0.99:
-                break
-    # the same workload through the combined screen's field surrogate (one pair)
-    from minibude_amd import combined as C
-    with C.Combined(w.forcefield, [w.ligand], [w.screen_poses], C.default_config("field"), 0) as c:
-        runs = [c.screen(w.protein) for _ in range(3)]
-        r = min(runs, key=lambda x: x.total_ms)
-        idx, _ = c.selected(0)
-    field = {
-        "keep_fraction": 0.02,
-        "poses_per_s": round(n / (r.total_ms * 1e-3), 1),
-        "screen_ms": round(r.total_ms, 3),
-        "speedup_vs_exhaustive": round(ex_ms / r.total_ms, 2),
-        "recall_top1pct": round(S.recall(idx, exact, 0.01), 4),
-        "recall_top0.1pct": round(S.recall(idx, exact, 0.001), 4),
-        "fallback": bool(r.fallback_pairs),
-        "phases_ms": {k: round(getattr(r, k), 3) for k in ("label_ms", "train_ms", "score_ms", "select_ms",
-                                                           "rescore_ms")},
-    }
-    log(f"screen (field surrogate): {field}")
-    out = {
-        "workload": (f"c3: 4,096-atom protein, 26-atom ligand; surrogate trained on {w.train_poses.shape[1]} plain-"
+    exact, ex_ms = None, None
+    nets = {}
+    for feats in (12, 21):
+        cfg = S.default_config(features=feats, **({"epochs": epochs} if epochs else {}))
+        log(f"screen: {feats}-input MLP, training on {w.train_poses.shape[1]} poses, screening {n}")
+        points = []
+        with S.Screen(w.protein, w.ligand, w.forcefield, cfg, 0) as sc:
+            st, _ = sc.train(w.train_poses)
+            sc.load(w.screen_poses)
+            if exact is None:
+                exact, ex_ms = sc.exhaustive()
+            # the workload's operating point (2 % rescored), then the smallest
+            # fraction whose recall of the exact top 1 % reaches 0.99
+            for keep in SCREEN_KEEP:
+                sc.set_keep(keep)
+                r = min((sc.run() for _ in range(3)), key=lambda x: x.total_ms)
+                idx, _ = sc.selected()
+                rec = S.recall(idx, exact, 0.01)
+                points.append({
+                    "keep_fraction": keep,
+                    "poses_per_s": round(n / (r.total_ms * 1e-3), 1),
+                    "screen_ms": round(r.total_ms, 3),
+                    "surrogate_ms": round(r.score_ms, 3),
+                    "select_ms": round(r.select_ms, 3),
+                    "rescore_ms": round(r.rescore_ms, 3),
+                    "speedup_vs_exhaustive": round(ex_ms / r.total_ms, 2),
+                    "recall_top1pct": round(rec, 4),
+                    "recall_top0.1pct": round(S.recall(idx, exact, 0.001), 4),
+                    "fallback": bool(r.fallback),
+                })
+                log(f"screen ({feats} inputs): {points[-1]}")
+                if keep != SCREEN_KEEP[0] and rec >= 0.99:
+                    break
+        setup = st.label_ms + st.train_ms
+        nets[feats] = {
+            "inputs": feats,
+            "val_rmse_standardised": round(st.val_rmse, 4),
+            "val_rmse_energy_units": round(st.val_rmse * st.sigma, 3),
+            "target": f"min(E, {st.clip:.2f}) standardised by mu {st.mu:.2f}, sigma {st.sigma:.2f}",
+            "epochs": int(cfg.epochs),
+            "labels_and_training_s": round(setup * 1e-3, 2),
+            "points": points,
+            "end_to_end_ms_at_2pct": round(setup + points[0]["screen_ms"], 1),
+            "breakeven_poses_at_2pct": breakeven(setup, ex_ms, points[0]["screen_ms"], n),
+        }
+    # the learned per-atom field and the exact maps on the same pair (the
+    # combined screen with one protein and one ligand); every screen trains
+    # (learned field) or builds the maps (field) for its protein
+    others = {}
+    for kind in ("mlp", "field"):
+        with C.Combined(w.forcefield, [w.ligand], [w.screen_poses], C.default_config(kind), 0) as c:
+            runs = [c.screen(w.protein) for _ in range(3)]
+            r = min(runs, key=lambda x: x.total_ms)
+            idx, _ = c.selected(0)
+            pair = c.pair(0)
+        setup = r.label_ms + r.train_ms
+        scr = r.score_ms + r.select_ms + r.rescore_ms
+        others[kind] = {
+            "keep_fraction": 0.02,
+            "screen_ms_excluding_setup": round(scr, 3),
+            "poses_per_s_excluding_setup": round(n / (scr * 1e-3), 1),
+            "speedup_vs_exhaustive_excluding_setup": round(ex_ms / scr, 2),
+            "end_to_end_ms": round(r.total_ms, 3),
+            "speedup_vs_exhaustive_end_to_end": round(ex_ms / r.total_ms, 2),
+            "breakeven_poses": breakeven(setup, ex_ms, scr, n),
+            "recall_top1pct": round(S.recall(idx, exact, 0.01), 4),
+            "recall_top0.1pct": round(S.recall(idx, exact, 0.001), 4),
+            "fallback": bool(pair.fallback),
+            "gate_z": round(pair.gate_z, 2),
+            "one_atom_rmse_energy_units": round(r.surrogate_rmse, 3) if kind == "mlp" else None,
+            "phases_ms": {k: round(getattr(r, k), 3) for k in ("label_ms", "train_ms", "score_ms", "select_ms",
+                                                               "rescore_ms")},
+        }
+        log(f"screen ({kind} surrogate): {others[kind]}")
+    return {
+        "workload": (f"c3: 4,096-atom protein, 26-atom ligand; pose MLP trained on {w.train_poses.shape[1]} plain-"
                      f"kernel energies, {n} fresh poses screened, lowest keep_fraction rescored exactly"),
         "exhaustive_ms": round(ex_ms, 3),
         "exhaustive_poses_per_s": round(n / (ex_ms * 1e-3), 1),
-        "val_rmse": round(st.val_rmse, 4),
-        "epochs": int(cfg.epochs),
-        "train_s": round((st.label_ms + st.train_ms) * 1e-3, 2),
-        "points": points,
-        "field_surrogate": field,
+        "pose_mlp_configured_12_inputs": nets[12],
+        "pose_mlp_21_inputs": nets[21],
+        "learned_field": others["mlp"],
+        "field_surrogate": others["field"],
     }
-    return out
 
 
 def copy_calibration(nbytes: int) -> None:
@@ -300,102 +372,111 @@ def copy_calibration(nbytes: int) -> None:
     log(f"copied {nbytes} bytes device to device")
 
 
-def traffic() -> float | None:
+def counter_summary() -> dict:
+    """The committed rocprofv3 --pmc summary of the plain kernel on c1."""
     if TRAFFIC_SUMMARY.exists():
         try:
-            return float(json.loads(TRAFFIC_SUMMARY.read_text())["hbm_bytes_per_launch"])
-        except Exception:
-            return None
-    return None
+            return json.loads(TRAFFIC_SUMMARY.read_text())
+        except ValueError:
+            return {}
+    return {}
 
 
-def combined(full: bool, kinds=("field", "mlp"), nproteins: int = 0) -> dict:
-    """Both surrogates of the combined screen on the same workload: every
-    protein screened (timed), exact energies of every pair for recall (all
-    proteins at the default size, the first one at full size, where the
-    exhaustive time of the rest is extrapolated from its interaction rate)."""
+def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_proteins: int = 1) -> dict:
+    """The combined screen with each surrogate on the same workload: every
+    protein screened in one batch call (timed), then exact energies of every
+    pair of the first `recall_proteins` proteins (one exhaustive batched pass
+    each, measured once and shared by the surrogates) for per-pair recall; the
+    exhaustive time of the rest is extrapolated from those proteins'
+    interaction rate."""
     from minibude_amd import combined as C
-    shape = dict(COMBINED_FULL if full else COMBINED_DEFAULT)
+    from minibude_amd import screen as S
+    shape = dict(COMBINED_DEFAULT if reduced else COMBINED_FULL)
     if nproteins:
         shape["nproteins"] = nproteins
     t0 = time.time()
     w = C.combined_workload(**shape)
+    L = len(w.ligands)
     per_protein = [len(p) * sum(len(l) * q.shape[1] for l, q in zip(w.ligands, w.poses)) for p in w.proteins]
-    log(f"combined: {len(w.proteins)} proteins x {len(w.ligands)} ligands x {shape['nposes']} poses "
+    log(f"combined: {len(w.proteins)} proteins x {L} ligands x {shape['nposes']} poses "
         f"({w.total_poses} pair-poses, {w.interactions:.3e} interactions), generated in {time.time() - t0:.1f} s")
+    full = not reduced and not nproteins
     out = {
-        "workload": (f"c4: {len(w.proteins)} proteins of 900-6000 atoms x {len(w.ligands)} ligands of 8-96 atoms x "
+        "workload": (f"c4: {len(w.proteins)} proteins of 900-6000 atoms x {L} ligands of 8-96 atoms x "
                      f"{shape['nposes']} poses per pair" +
-                     ("" if full and not nproteins else " (reduced from 16 proteins x 512 ligands x 2^20 poses)")),
+                     ("" if full else " (reduced from 16 proteins x 512 ligands x 2^20 poses)")),
         "pairs": w.pairs,
         "pair_poses": w.total_poses,
         "interactions": w.interactions,
     }
+    exact: dict[int, list[np.ndarray]] = {}
+    ex_ms: dict[int, float] = {}
+    nrec = min(recall_proteins, len(w.proteins))
     for kind in kinds:
         cfg = C.default_config(kind)
-        screen_ms, ex_ms, ex_inter, rec, fb, labels, sel = 0.0, 0.0, 0, [], 0, 0, 0
         parts = {"label_ms": 0.0, "train_ms": 0.0, "score_ms": 0.0, "select_ms": 0.0, "rescore_ms": 0.0}
-        sweep = {k: [0.0, []] for k in (COMBINED_FIELD_KEEPS if kind == "field" else ())}
-        keep0 = float(cfg.keep_fraction)
-        one_by_one = 0.0
+        fb, labels, sel, rmse = 0, 0, 0, []
+        rec, trusted, gate_z, one_ms = [], [], [], 0.0
         with C.Combined(w.forcefield, w.ligands, w.poses, cfg, 0) as c:
-            # the timed screen: every protein in one batch call (each one's
-            # scoring beside the previous one's rescoring), after one untimed
-            # screen of the first protein that sizes the object's buffers
+            # one untimed screen of the first protein sizes the object's
+            # buffers, then every protein in one batch call (each one's
+            # scoring beside the previous one's rescoring)
             c.screen(w.proteins[0])
             results, screen_ms = c.screen_batch(w.proteins)
             for r in results:
                 fb += r.fallback_pairs
                 labels += r.labels
                 sel += r.nselected
+                rmse.append(r.surrogate_rmse)
                 for k in parts:
                     parts[k] += getattr(r, k)
-            # recall from one-by-one screens of the same proteins (a protein
-            # draws the same samples alone or in a batch, so their selections
-            # equal the batch's), exact energies of every pair
-            for p, protein in enumerate(w.proteins):
-                if not full or p == 0:
-                    one_by_one += c.screen(protein).total_ms
-                    ex_ms += c.exhaustive()
-                    ex_inter += per_protein[p]
-                    rec.extend(c.recall(0.01).tolist())
-                    # smaller rescored fractions on the same protein
-                    for keep, acc in sweep.items():
-                        c.set_keep(keep)
-                        acc[0] += c.screen(protein).total_ms
-                        acc[1].extend(c.recall(0.01).tolist())
-                    if sweep:
-                        c.set_keep(keep0)
+            # recall of every pair of the first proteins (a protein draws the
+            # same samples and trains the same network alone or in a batch,
+            # so its selection equals the batch's)
+            for p in range(nrec):
+                one_ms += c.screen(w.proteins[p]).total_ms
+                if p not in exact:
+                    ex_ms[p] = c.exhaustive()
+                    exact[p] = [c.exact(l) for l in range(L)]
+                for l in range(L):
+                    pr = c.pair(l)
+                    r_l = S.recall(c.selected(l)[0], exact[p][l], 0.01)
+                    rec.append(r_l)
+                    gate_z.append(pr.gate_z)
+                    if not pr.fallback:
+                        trusted.append(r_l)
         rec = np.array(rec)
-        ex_total_ms = ex_ms * w.interactions / ex_inter
+        ex_inter = sum(per_protein[p] for p in range(nrec))
+        ex_total_ms = sum(ex_ms[p] for p in range(nrec)) * w.interactions / ex_inter
         res = {
             "poses_per_s": round(w.total_poses / (screen_ms * 1e-3), 1),
             "screen_ms": round(screen_ms, 2),
-            "screen_ms_one_by_one": None if full else round(one_by_one, 2),
+            "screen_ms_one_by_one_recall_proteins": round(one_ms, 2),
             "exhaustive_ms": round(ex_total_ms, 2),
             "exhaustive_poses_per_s": round(w.total_poses / (ex_total_ms * 1e-3), 1),
             "speedup_vs_exhaustive": round(ex_total_ms / screen_ms, 2),
             "recall_top1pct_mean": round(float(rec.mean()), 4),
             "recall_top1pct_min": round(float(rec.min()), 4),
             "recall_top1pct_p05": round(float(np.percentile(rec, 5)), 4),
+            "trusted_recall_top1pct_min": round(float(min(trusted)), 4) if trusted else None,
             "pairs_recall_ge_0.99": round(float((rec >= 0.99).mean()), 4),
             "recall_pairs_measured": int(len(rec)),
             "fallback_pairs": fb,
+            "fallback_fraction": round(fb / w.pairs, 4),
+            "gate_z_min_measured": round(float(np.min(gate_z)), 2),
             "labels": labels,
             "rescored": sel,
             "phases_ms": {k: round(v, 2) for k, v in parts.items()},
-            "config": {**{k: (round(v, 6) if isinstance(v, float) else v) for k, v in cfg.as_dict().items()},
-                       "keep_fraction": keep0},
+            "config": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in cfg.as_dict().items()},
+            "exhaustive_note": (f"exhaustive batched scoring measured on the first {nrec} protein(s) "
+                                f"({ex_inter:.3e} interactions), extrapolated to all by interactions"),
         }
-        if sweep:
-            ex_meas = ex_ms  # the sweep ran on the proteins whose exhaustive time was measured
-            res["keep_sweep"] = [{"keep_fraction": k, "speedup_vs_exhaustive": round(ex_meas / v[0], 2),
-                                  "recall_top1pct_mean": round(float(np.mean(v[1])), 4),
-                                  "recall_top1pct_min": round(float(np.min(v[1])), 4),
-                                  "pairs_recall_ge_0.99": round(float((np.array(v[1]) >= 0.99).mean()), 4)}
-                                 for k, v in sweep.items()]
-        if full:
-            res["exhaustive_note"] = "measured on the first protein's pairs, extrapolated by interactions"
+        if kind == "mlp":
+            res["surrogate"] = ("learned per-atom field: per protein a learned 1 A grid of 8 features, trilinear, "
+                                "then a 2 x 32 ReLU MLP with the atom type one-hot (BF16 MFMA), summed over the "
+                                "ligand's atoms; trained by this library's kernels on 2^21 exact one-atom "
+                                "energies (the plain kernel's pair loop) for 2,000 Adam steps of 16,384")
+            res["one_atom_rmse_energy_units"] = [round(x, 3) for x in rmse]
         log(f"combined {kind}: {res}")
         out[kind] = res
     return out
@@ -412,13 +493,15 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-screen", action="store_true")
     ap.add_argument("--no-combined", action="store_true")
-    ap.add_argument("--sweep", action="store_true",
-                    help="also run the ppwi x wgsize autotune sweep of the plain kernel (40 points)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the ppwi x wgsize autotune sweep of the plain kernel (40 points)")
     ap.add_argument("--screen-epochs", type=int, default=0, help="training epochs of the learned prescreen "
                     "(default: its configuration's 60; fewer only to keep a kernel trace short)")
-    ap.add_argument("--combined-full", action="store_true",
-                    help="combined screen at 16 proteins x 512 ligands x 2^20 poses (recall on the first protein)")
-    ap.add_argument("--combined-surrogates", default="field,mlp", help="comma list of field, mlp")
+    ap.add_argument("--combined-reduced", action="store_true",
+                    help="combined screen at 16 proteins x 64 ligands x 2^18 poses instead of the full 16 x 512 x 2^20")
+    ap.add_argument("--combined-surrogates", default="field,mlp", help="comma list of field, mlp, pose_mlp")
+    ap.add_argument("--combined-recall-proteins", type=int, default=1,
+                    help="proteins whose pairs get exhaustive energies for recall")
     ap.add_argument("--combined-proteins", type=int, default=0, help="screen only the first N proteins")
     ap.add_argument("--copy-calibration", type=int, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
@@ -445,7 +528,7 @@ def main(argv: list[str] | None = None) -> int:
         log(f"uploaded in {ctx.context_ms:.1f} ms; {a.warmup} warmup + {a.steps} timed steps "
             f"(ppwi={a.ppwi}, wgsize={a.wgsize})")
         s = ctx.fasten(a.ppwi, a.wgsize, warmup=a.warmup, iterations=a.steps, want_energies=True)
-        sweep_out = sweep(ctx, ref, 1, 2) if a.sweep else None
+        sweep_out = None if a.no_sweep else sweep(ctx, ref, 1, 1)
     host = np.array(s.kernel_ms[a.warmup:])
     dev = np.array(s.device_ms[a.warmup:])
     ms_per_step = float(host.mean())
@@ -454,6 +537,13 @@ def main(argv: list[str] | None = None) -> int:
     flops = metrics.flops_per_pass(deck.nposes, deck.natlig, deck.natpro, a.ppwi)
     achieved = flops / (float(dev.mean()) * 1e-3) / 1e12
     peak = metrics.FP32_VECTOR_PEAK_TFLOPS
+    # the same launch under two other accountings: without the per-pair
+    # parameter derivation (10/PPWI) and ligand-transform share (2/PPWI)
+    # that this kernel does once per (ligand atom, protein type) on the host,
+    # and the FLOP the hardware counted (committed counter summary)
+    flops30 = float(deck.nposes) * (28 + deck.natlig * 18 + deck.natlig * deck.natpro * 30)
+    cs = counter_summary()
+    counted = float(cs.get("counted_flop_per_interaction", 0.0)) * inter
     log(f"{ms_per_step:.3f} ms/step host, {dev.mean():.3f} ms/launch device, {value:.1f} G-interactions/s, "
         f"{achieved:.2f} TFLOP/s ({achieved / peak:.3f} of FP32 vector peak)")
 
@@ -493,9 +583,22 @@ def main(argv: list[str] | None = None) -> int:
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": traffic(),
+            "traffic": cs.get("hbm_bytes_per_launch"),
+            "traffic_source": ("committed rocprofv3 --pmc summary profiles/fasten_plain_c1_traffic.json (FETCH_SIZE "
+                               "x 2.0 + WRITE_SIZE x 1.0, factors from a 512 MiB device copy), not measured in "
+                               "this run"),
             "kernel_ms": round(float(dev.mean()), 4),
             "flop_accounting": "src/main.cpp:358-360 (30 + 10/PPWI per interaction)",
+            "frac_without_pair_setup": round(flops30 / (float(dev.mean()) * 1e-3) / 1e12 / peak, 4),
+            "frac_counted": round(counted / (float(dev.mean()) * 1e-3) / 1e12 / peak, 4) if counted else None,
+            "frac_counted_of_unpacked_rate": (round(counted / (float(dev.mean()) * 1e-3) / 1e12 / (peak / 2), 4)
+                                              if counted else None),
+            "accountings": ("frac: the reference's 30 + 10/PPWI FLOP per interaction; frac_without_pair_setup: "
+                            "30 per interaction (no per-pair parameter derivation, which this kernel does per "
+                            "ligand atom and protein type on the host); frac_counted: the "
+                            f"{cs.get('counted_flop_per_interaction')} FLOP per interaction the hardware counters "
+                            "record (committed summary), against the packed-FP32 peak and against half of it "
+                            "(packed FP32 is not used)"),
         },
         "cpu_baseline": cpu,
         "valid": valid,
@@ -503,8 +606,8 @@ def main(argv: list[str] | None = None) -> int:
         "sweep": sweep_out,
         "batched": None if a.no_batched else batched(),
         "screen": None if a.no_screen else screen(a.screen_epochs),
-        "combined": None if a.no_combined else combined(a.combined_full, tuple(a.combined_surrogates.split(",")),
-                                                         a.combined_proteins),
+        "combined": None if a.no_combined else combined(a.combined_reduced, tuple(a.combined_surrogates.split(",")),
+                                                         a.combined_proteins, a.combined_recall_proteins),
         "device": device_name(0),
     }
     print(json.dumps(out), flush=True)

@@ -164,9 +164,10 @@ int bude_mi355x_batch_energies(bude_batch *b, size_t ligand, float *out);
 int bude_mi355x_batch_clear(bude_batch *b);
 int bude_mi355x_batch_destroy(bude_batch *b);
 
-/* ---- learned prescreen: an MLP pose-energy surrogate (21 pose features:
- *      the 6 pose parameters, sin and cos of the 3 angles, and the 9 entries
- *      of the rotation matrix they determine -> 256 -> 256 -> 256 -> 1, GELU,
+/* ---- learned prescreen: an MLP pose-energy surrogate (12 or 21 pose
+ *      features: the 6 pose parameters, sin and cos of the 3 angles, and
+ *      optionally the 9 entries of the rotation matrix they determine ->
+ *      256 -> 256 -> 256 -> 1, GELU,
  *      BF16 weights, FP32 accumulation on MFMA),
  *      trained on the GPU by this library's own forward/backward kernels and
  *      Adam on energies the plain kernel computes; screening runs the
@@ -186,6 +187,8 @@ typedef struct BudeScreenConfig {
   float tail_quantile;  /* training energies below this quantile get loss weight tail_weight (0: none) */
   float tail_weight;
   uint64_t seed;        /* weight initialisation and shuffling */
+  uint32_t features;    /* pose inputs: 12 (the 6 pose parameters and the sines and cosines of the angles) or 21
+                           (those and the 9 entries of the rotation matrix) */
 } BudeScreenConfig;
 
 typedef struct BudeScreenTrainStats {
@@ -235,35 +238,58 @@ int bude_mi355x_screen_destroy(bude_screen *s);
  *      energies of labels_per_pair sampled poses of every (protein, ligand)
  *      pair, a surrogate validated per pair on them, the surrogate over
  *      every pose of every pair, and exact rescoring of each pair's lowest
- *      keep_fraction in one ragged batched launch; a pair whose calibrated
- *      validation RMSE (below) exceeds max_val_rmse is scored exhaustively
- *      instead.
+ *      keep_fraction in one ragged batched launch; a pair that fails
+ *      validation is scored exhaustively instead.  Validation (every
+ *      surrogate): on the pair's samples, the gap between the exact energies
+ *      at the keep_fraction and gate_top_fraction quantiles, over the
+ *      surrogate's error deviation on the lowest quarter of the samples (the
+ *      residual of a least-squares line of score on exact energy there, in
+ *      energy units), must reach gate_z -- with Gaussian errors and exact
+ *      energies spread evenly over the gap the selection then loses a
+ *      fraction G(z) / z, G(z) = phi(z) - z Phi(-z), of the exact top
+ *      gate_top_fraction (0.0043 at z = 2) -- and the calibrated validation
+ *      RMSE (below) must stay within max_val_rmse.
  *      Surrogates: BUDE_SURROGATE_MLP, the learned prescreen's network with
  *      11 ligand descriptor inputs, one per protein, trained on 15/16 of the
  *      samples; BUDE_SURROGATE_FIELD, single-ligand-atom energy maps of the
  *      protein on a grid (one per ligand atom type, computed by the pair loop
  *      of fasten_main) summed over the ligand's atoms by trilinear
- *      interpolation.  No counterpart in the reference; every energy
- *      returned is fasten_main's (src/serial/fasten.hpp:18-132). ---- */
+ *      interpolation; BUDE_SURROGATE_ATOM_MLP, the learned per-atom field:
+ *      one network per protein (a learned grid of BUDE_ATOMNET_FEATURES
+ *      features at grid_spacing, trilinear, then a ReLU MLP of two
+ *      BUDE_ATOMNET_HIDDEN-wide layers with the atom's forcefield type as a
+ *      one-hot input) giving one ligand atom's energy at a position, summed
+ *      over the ligand's atoms; trained by BF16 MFMA kernels and Adam for
+ *      `steps` steps of `batch` points on train_points exact one-atom
+ *      energies (the plain kernel's pair loop) at positions the screened
+ *      poses put ligand atoms at.  No counterpart in the reference; every
+ *      energy returned is fasten_main's (src/serial/fasten.hpp:18-132). ---- */
 typedef struct bude_combined bude_combined;
 
-enum { BUDE_SURROGATE_MLP = 0, BUDE_SURROGATE_FIELD = 1 };
+enum { BUDE_SURROGATE_MLP = 0, BUDE_SURROGATE_FIELD = 1, BUDE_SURROGATE_ATOM_MLP = 2 };
+#define BUDE_ATOMNET_FEATURES 8
+#define BUDE_ATOMNET_HIDDEN 32
 
 typedef struct BudeCombinedConfig {
-  uint32_t surrogate;        /* BUDE_SURROGATE_MLP or BUDE_SURROGATE_FIELD */
+  uint32_t surrogate;        /* BUDE_SURROGATE_MLP, BUDE_SURROGATE_FIELD or BUDE_SURROGATE_ATOM_MLP */
   uint32_t labels_per_pair;  /* sampled poses per pair scored exactly (2 .. 2^24) */
   float keep_fraction;       /* per pair: the lowest fraction of surrogate scores is rescored exactly */
-  float max_val_rmse;        /* bound on a pair's calibrated validation RMSE (standardised units) */
+  float max_val_rmse;        /* bound on a pair's calibrated validation RMSE (standardised units; +inf: none) */
   float clip_quantile;       /* per-pair target transform, as the learned prescreen */
   float tail_quantile;
   float tail_weight;
-  uint32_t batch;            /* MLP: training mini-batch, a multiple of 32 */
+  uint32_t batch;            /* MLP: training mini-batch, a multiple of 32; ATOM_MLP: points, a multiple of 64 */
   uint32_t epochs;           /* MLP: passes over the training samples */
-  float lr;                  /* MLP: peak Adam learning rate */
-  float grid_spacing;        /* FIELD: map grid spacing in Angstrom (0.05 .. 4) */
-  uint32_t pipeline_chunks;  /* FIELD: ligand chunks scored, selected and rescored as a pipeline (rescoring of
-                                chunk k beside the field scoring of chunk k + 1); 0 = automatic, at most 16 */
+  float lr;                  /* MLP, ATOM_MLP: peak Adam learning rate of the weights */
+  float grid_spacing;        /* FIELD, ATOM_MLP: grid spacing in Angstrom (0.05 .. 4) */
+  uint32_t pipeline_chunks;  /* FIELD, ATOM_MLP: ligand chunks scored, selected and rescored as a pipeline
+                                (rescoring of chunk k beside the scoring of chunk k + 1); 0 = automatic, <= 16 */
   uint64_t seed;             /* sampling, initialisation, shuffling */
+  uint32_t train_points;     /* ATOM_MLP: training points per protein (a multiple of batch) */
+  uint32_t steps;            /* ATOM_MLP: training steps */
+  float grid_lr;             /* ATOM_MLP: peak Adam learning rate of the grid features */
+  float gate_z;              /* validation: required (quantile gap) / (tail error deviation), see above */
+  float gate_top_fraction;   /* validation: the exact top fraction the selection must keep (e.g. 0.01) */
 } BudeCombinedConfig;
 
 typedef struct BudeCombinedResult {
@@ -279,6 +305,7 @@ typedef struct BudeCombinedResult {
   uint64_t nselected;        /* poses with exact energies returned */
   uint64_t labels;           /* sampled poses scored exactly */
   uint32_t fallback_pairs;
+  float surrogate_rmse;      /* ATOM_MLP: one-atom energy RMSE on held-out points (energy units) */
 } BudeCombinedResult;
 
 typedef struct BudePairResult {
@@ -286,7 +313,9 @@ typedef struct BudePairResult {
   int32_t fallback;          /* 1: scored exhaustively */
   float val_rmse;            /* surrogate vs exact on the pair's validation samples (clipped, standardised) */
   float val_rmse_calibrated; /* the same after the least-squares increasing affine map of the scores onto the
-                                targets (the part of the error that changes the ranking); fallback criterion */
+                                targets (the part of the error that changes the ranking); a fallback criterion */
+  float gate_z;              /* validation gap / tail error deviation (energy units); fallback below cfg gate_z */
+  float sample_recall;       /* of the samples' exact lowest 5 %, the fraction among the surrogate's lowest 10 % */
 } BudePairResult;
 
 /* ligands[i] has natlig[i] atoms and poses[i] 6 rows of nposes[i] floats (as
@@ -354,6 +383,35 @@ int bude_mi355x_mlp_forward_backward(int device, size_t batch, const float *feat
  * floats) with caller weights as above: predictions per pose. */
 int bude_mi355x_mlp_predict(int device, const float *const poses[6], size_t n, const float *const weights[4],
                             const float *const biases[4], float *pred);
+
+/* Exact energy of one ligand atom of forcefield type types[i] at
+ * points[3i .. 3i+2] against the protein: fasten_main's sum for a one-atom
+ * ligand there (src/serial/fasten.hpp:56-131), computed by the plain
+ * kernel's pair loop (what the learned per-atom field trains on). */
+int bude_mi355x_atom_energies(int device, const BudeAtom *protein, size_t natpro, const BudeFFParams *forcefield,
+                              size_t ntypes, const float *points, const int32_t *types, size_t n, float *out);
+/* One forward + backward pass of the learned per-atom field's training
+ * kernels on `batch` points (a multiple of 64; slots[i] < nslots) against
+ * standardised targets, with caller parameters: the feature grid
+ * [n0 * n1 * n2][BUDE_ATOMNET_FEATURES] (node (i, j, k) at grid_lo + h (i, j,
+ * k), row-major), W0 [32 * ceil((BUDE_ATOMNET_FEATURES + nslots) / 32)]
+ * [BUDE_ATOMNET_HIDDEN] (feature rows, then one row per slot, then zeros),
+ * W1 [BUDE_ATOMNET_HIDDEN][BUDE_ATOMNET_HIDDEN], W2 [BUDE_ATOMNET_HIDDEN][1]
+ * and the biases; predictions, mean squared error and every FP32 gradient.
+ * Exposes the training kernels to gradient checks. */
+int bude_mi355x_atomnet_forward_backward(int device, size_t batch, const float *points, const int32_t *slots,
+                                         const float *target, const float grid_lo[3], float grid_h,
+                                         const int32_t grid_n[3], const float *grid, size_t nslots,
+                                         const float *const weights[3], const float *const biases[3], float *pred,
+                                         float *loss, float *grad_grid, float *const grad_w[3],
+                                         float *const grad_b[3]);
+/* The fused scorer the combined screen uses, on the poses of one ligand
+ * (atom type fields hold slots < nslots) with caller parameters as above:
+ * scores[j] = sum over the ligand's atoms of sigma * y + mu. */
+int bude_mi355x_atomnet_score(int device, const BudeAtom *ligand, size_t natlig, const float *const poses[6],
+                              size_t nposes, const float grid_lo[3], float grid_h, const int32_t grid_n[3],
+                              const float *grid, size_t nslots, const float *const weights[3],
+                              const float *const biases[3], float sigma, float mu, float *scores);
 
 /* ---- deck tooling: the synthetic generator behind the benchmark decks
  *      (the role makedeck/main.cpp:49-234 plays for bm1/bm2).  Every value

@@ -50,6 +50,8 @@ LIB_SOURCES = [
     "capi_screen.hip",
     "select.hip",
     "field.hip",
+    "atomnet.hip",
+    "capi_atomnet.hip",
     "capi_combined.hip",
 ]
 

@@ -70,6 +70,9 @@ EXPORTED = (
     "bude_mi355x_screen_destroy",
     "bude_mi355x_mlp_forward_backward",
     "bude_mi355x_mlp_predict",
+    "bude_mi355x_atom_energies",
+    "bude_mi355x_atomnet_forward_backward",
+    "bude_mi355x_atomnet_score",
     "bude_synth_forcefield",
     "bude_synth_protein",
     "bude_synth_ligand",
@@ -133,6 +136,13 @@ def _declare(lib: ctypes.CDLL) -> None:
         "bude_mi355x_context_ms": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
         "bude_mi355x_destroy": (_I, [_P]),
         "bude_mi355x_fasten_noalloc": (_I, [_I, _P, _S, _P, _S, _P, _S, _P, _S, _S, _S, _S, _P, _P]),
+        "bude_mi355x_atom_energies": (_I, [_I, _P, _S, _P, _S, _P, _P, _S, _P]),
+        "bude_mi355x_atomnet_forward_backward": (_I, [_I, _S, _P, _P, _P, _P, ctypes.c_float, _P, _P, _S,
+                                                      ctypes.POINTER(_P), ctypes.POINTER(_P), _P, _P, _P,
+                                                      ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+        "bude_mi355x_atomnet_score": (_I, [_I, _P, _S, ctypes.POINTER(_P), _S, _P, ctypes.c_float, _P, _P, _S,
+                                           ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.c_float, ctypes.c_float,
+                                           _P]),
         "bude_mi355x_batch_create": (_I, [_I, _P, _S, _P, _S, _S, ctypes.POINTER(_P), ctypes.POINTER(_S),
                                           ctypes.POINTER(_P), ctypes.POINTER(_S), ctypes.POINTER(_P)]),
         "bude_mi355x_batch_run": (_I, [_P, _P, _S, _S, _P]),

@@ -1,0 +1,849 @@
+// atomnet.hip -- the learned per-atom field (see atomnet.h): training
+// points and their exact labels, the training step (BF16 MFMA GEMMs of
+// mlp::gemm, deterministic fixed-point grid gradients, Adam), and the fused
+// scorer that sums the network over a ligand's atoms for every pose.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <random>
+#include <vector>
+
+#include "atomnet.h"
+#include "fasten_core.h"
+
+namespace bude {
+namespace atomnet {
+
+namespace {
+
+using namespace core;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float floatx2_t __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16 (round to nearest even), two values per instruction.
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(floatx2_t{lo, hi}, bf16x2_t));
+}
+__device__ __forceinline__ float lo_bf16(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float hi_bf16(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Cell index and fraction along one axis, clamped to the grid (as the
+// field scorer's).
+__device__ __forceinline__ void cell(float w, float lo, float inv_h, int n, int &i0, float &f) {
+  const float u = (w - lo) * inv_h;
+  i0 = min(max(int(floorf(u)), 0), n - 2);
+  f = __builtin_amdgcn_fmed3f(u - float(i0), 0.0f, 1.0f);
+}
+
+// Trilinear corners of position (x, y, z): node index of corner c = (dx, dy,
+// dz) bits (x high, z low) and its weight.
+struct Corners {
+  int node[8];
+  float w[8];
+};
+
+__device__ __forceinline__ Corners corners(float x, float y, float z, const field::Grid &g, float inv_h) {
+  int ix, iy, iz;
+  float fx, fy, fz;
+  cell(x, g.lo[0], inv_h, g.n[0], ix, fx);
+  cell(y, g.lo[1], inv_h, g.n[1], iy, fy);
+  cell(z, g.lo[2], inv_h, g.n[2], iz, fz);
+  const int s1 = g.n[2], s0 = g.n[1] * g.n[2];
+  const int base = ix * s0 + iy * s1 + iz;
+  Corners c;
+  const float gx[2] = {1.f - fx, fx}, gy[2] = {1.f - fy, fy}, gz[2] = {1.f - fz, fz};
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int dx = (k >> 2) & 1, dy = (k >> 1) & 1, dz = k & 1;
+    c.node[k] = base + dx * s0 + dy * s1 + dz;
+    c.w[k] = gx[dx] * gy[dy] * gz[dz];
+  }
+  return c;
+}
+
+// ---- training points and labels ---------------------------------------------
+
+__global__ void sample_kernel(const float4 *__restrict__ ligands, const LigandMeta *__restrict__ meta, int nligands,
+                              const float *__restrict__ poses, size_t stride, int nslots, uint64_t seed, size_t n,
+                              float4 *__restrict__ out) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t h1 = mix64(seed ^ (uint64_t(i) * 0xd1342543de82ef95ull));
+  const uint64_t h2 = mix64(h1 ^ 0x632be59bd9b4e019ull);
+  const LigandMeta m = meta[h1 % uint64_t(nligands)];
+  const size_t j = size_t((h2 >> 32) % uint64_t(m.nposes));
+  const int a = int((h2 & 0xffffffffull) % uint64_t(m.natlig));
+  const Xform x = make_xform(poses, stride, size_t(m.pose_off) + j);
+  const float4 la = ligands[m.lig_off + a];
+  const float px = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
+  const float py = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
+  const float pz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
+  out[i] = make_float4(px, py, pz, __int_as_float(int((i / 64) % size_t(nslots))));
+}
+
+// One wave per 64 points of one slot: the plain kernel's pair loop for a
+// single ligand atom at each point (lx = the point, exactly what a pose of
+// zero angles and translation p gives an atom at the origin).
+template <int FORM>
+__global__ __launch_bounds__(256) void label_kernel(const float4 *__restrict__ protein,
+                                                    const int32_t *__restrict__ group_begin,
+                                                    const PairCoef *__restrict__ slot_coef, int ngroups,
+                                                    const float4 *__restrict__ points, size_t n,
+                                                    float *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const size_t base = (size_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64;
+  if (base >= n) return;
+  const float4 p = points[base + lane];
+  const int slot = __builtin_amdgcn_readfirstlane(__float_as_int(p.w));
+  const PairCoef *c = slot_coef + size_t(slot) * size_t(ngroups);
+  const float lx[1] = {p.x}, ly[1] = {p.y}, lz[1] = {p.z};
+  float e[1] = {0.f};
+  for (int g = 0; g < ngroups; ++g) group_dispatch<1, FORM>(protein, group_begin[g], group_begin[g + 1], c[g], lx, ly, lz, e);
+  out[base + lane] = e[0] * 0.5f;
+}
+
+// Partial sums (sum, sum of squares, in FP64) of labels[0, n) per block.
+__global__ __launch_bounds__(256) void stats_kernel(const float *__restrict__ labels, size_t n,
+                                                    double *__restrict__ part) {
+  __shared__ double s1[256], s2[256];
+  double a = 0, b = 0;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    const double v = labels[i];
+    a += v;
+    b += v * v;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
+// ---- training kernels -------------------------------------------------------
+
+// Input rows of points idx[b]: [bf16(phi), onehot(slot), 0 ...] (k0 columns)
+// and the transposed copy x0t [k0][B], from the FP32 grid.
+__global__ void features_kernel(const float4 *__restrict__ points, const uint32_t *__restrict__ idx, size_t B,
+                                const float *__restrict__ grid, const field::Grid g, int k0, bf16 *__restrict__ x0,
+                                bf16 *__restrict__ x0t) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const float4 p = points[idx ? idx[b] : b];
+  const int slot = __float_as_int(p.w);
+  const Corners c = corners(p.x, p.y, p.z, g, 1.0f / g.h);
+  float phi[kF];
+#pragma unroll
+  for (int k = 0; k < kF; k++) phi[k] = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const float4 *v = reinterpret_cast<const float4 *>(grid + size_t(c.node[q]) * kF);
+#pragma unroll
+    for (int h = 0; h < kF / 4; h++) {
+      const float4 u = v[h];
+      phi[4 * h + 0] = __builtin_fmaf(c.w[q], u.x, phi[4 * h + 0]);
+      phi[4 * h + 1] = __builtin_fmaf(c.w[q], u.y, phi[4 * h + 1]);
+      phi[4 * h + 2] = __builtin_fmaf(c.w[q], u.z, phi[4 * h + 2]);
+      phi[4 * h + 3] = __builtin_fmaf(c.w[q], u.w, phi[4 * h + 3]);
+    }
+  }
+  uint32_t *row = reinterpret_cast<uint32_t *>(x0 + b * size_t(k0));
+#pragma unroll
+  for (int k = 0; k < kF; k += 2) {
+    const uint32_t u = pack_bf16(phi[k], phi[k + 1]);
+    row[k / 2] = u;
+    x0t[size_t(k) * B + b] = bf16(u & 0xffffu);
+    x0t[size_t(k + 1) * B + b] = bf16(u >> 16);
+  }
+  for (int k = kF; k < k0; k += 2) {  // one-hot slot columns (BF16 1.0 = 0x3f80)
+    const bf16 h0 = k == kF + slot ? bf16(0x3f80) : bf16(0), h1 = k + 1 == kF + slot ? bf16(0x3f80) : bf16(0);
+    row[k / 2] = uint32_t(h0) | (uint32_t(h1) << 16);
+    x0t[size_t(k) * B + b] = h0;
+    x0t[size_t(k + 1) * B + b] = h1;
+  }
+}
+
+// Grid gradient: every point's dL/dphi (the first kF columns of dx0
+// [B][32]) spread to its 8 corners with the trilinear weights, accumulated
+// as 2^40 fixed point in 64-bit integer atomics (order-independent, so the
+// training is deterministic).
+__global__ void grid_grad_kernel(const float4 *__restrict__ points, const uint32_t *__restrict__ idx, size_t B,
+                                 const field::Grid g, const float *__restrict__ dx0,
+                                 unsigned long long *__restrict__ acc) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const float4 p = points[idx ? idx[b] : b];
+  const Corners c = corners(p.x, p.y, p.z, g, 1.0f / g.h);
+  float d[kF];
+#pragma unroll
+  for (int k = 0; k < kF; k++) d[k] = dx0[b * 32 + k];
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int k = 0; k < kF; k++) {
+      const long long v = llrint(double(c.w[q] * d[k]) * kGradScale);
+      atomicAdd(acc + size_t(c.node[q]) * kF + k, static_cast<unsigned long long>(v));
+    }
+}
+
+// Adam on the grid features from the fixed-point accumulators (which it
+// clears), refreshing the BF16 copy.
+__global__ void grid_adam_kernel(float *__restrict__ w, unsigned long long *__restrict__ acc, float *__restrict__ m,
+                                 float *__restrict__ v, bf16 *__restrict__ w16, size_t n, float lr, float b1, float b2,
+                                 float eps, float bc1, float bc2) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = float(double(static_cast<long long>(acc[i])) / kGradScale);
+  acc[i] = 0ull;
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  w[i] = wi;
+  w16[i] = bf16(pack_bf16(wi, 0.f) & 0xffffu);
+}
+
+// Fixed-point accumulators -> FP32 gradient (forward_backward for tests).
+__global__ void grid_grad_out_kernel(unsigned long long *__restrict__ acc, float *__restrict__ out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = float(double(static_cast<long long>(acc[i])) / kGradScale);
+  acc[i] = 0ull;
+}
+
+// FP32 grid -> BF16 copy.
+__global__ void grid_bf16_kernel(const float *__restrict__ w, bf16 *__restrict__ w16, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) w16[i] = bf16(pack_bf16(w[i], 0.f) & 0xffffu);
+}
+
+// Squared error of sigma * pred + mu against min(label, clip), summed in FP64 per block.
+__global__ __launch_bounds__(256) void sq_err_kernel(const float *__restrict__ pred, const float *__restrict__ labels,
+                                                     size_t n, float sigma, float mu, float clip,
+                                                     double *__restrict__ part) {
+  __shared__ double s[256];
+  double a = 0;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    const double d = double(sigma) * pred[i] + mu - double(fminf(labels[i], clip));
+    a += d * d;
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// A standard normal deviate from a hash of (seed, i) (Box-Muller).
+__device__ __forceinline__ float normal(uint64_t seed, uint64_t i) {
+  const uint64_t h = mix64(seed ^ (i * 0x9e3779b97f4a7c15ull));
+  const float u1 = (float(uint32_t(h >> 40)) + 1.0f) * (1.0f / 16777217.0f);  // (0, 1]
+  const float u2 = float(uint32_t(h) >> 8) * (1.0f / 16777216.0f);
+  return sqrtf(-2.0f * logf(u1)) * cosf(6.2831853f * u2);
+}
+
+// w[r][c] = sd * N(0, 1) for r < rows, c < cols of a [.][ld] matrix, 0
+// elsewhere (n entries in all); m, v (Adam moments) cleared.
+__global__ void init_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v, size_t n, int ld,
+                            int rows, int cols, float sd, uint64_t seed) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = int(i / size_t(ld)), c = int(i % size_t(ld));
+  w[i] = (r < rows && c < cols) ? sd * normal(seed, i) : 0.f;
+  m[i] = 0.f;
+  v[i] = 0.f;
+}
+
+// slot_bias[s][j] = b0[j] + W0[kF + s][j] (W0's BF16 copy, row-major [k0][kH]).
+__global__ void slot_bias_kernel(const float *__restrict__ b0, const bf16 *__restrict__ w16, int nslots,
+                                 float *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots * kH) return;
+  const int s = i / kH, j = i % kH;
+  out[i] = b0[j] + __uint_as_float(uint32_t(w16[size_t(kF + s) * kH + j]) << 16);
+}
+
+// ---- fused scorer -------------------------------------------------------------
+// One wave per 64 poses of one ligand (lane = pose).  Per ligand atom: the
+// transformed position, the interpolated features (FP32 from the BF16 grid,
+// 16 bytes per corner), then the decoder on MFMA, as the transpose
+// D[neuron][pose] = W^T . x^T so the weights are the (constant, register-
+// held) A operand and the wave's 64 poses four 16-pose B operands read from
+// the wave's own LDS rows: layer 1 (K = the kF features, the rest of the
+// K = 32 step zero; the slot's bias accumulated onto), ReLU, layer 2, ReLU,
+// and the output neuron as a dot product with W2 over the lane's neurons,
+// reduced over the four lane groups that share a pose.
+constexpr int kRow = kH + 8;  // bf16 per LDS row (padding staggers banks)
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bf16x8 lds8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+__global__ __launch_bounds__(256) void score_kernel(const ScoreArgs a, int64_t tend) {
+  __shared__ bf16 act[4][64 * kRow];
+  __shared__ float w2s[kH], b1s[kH];
+  const int tid = threadIdx.x;
+  if (tid < kH) {
+    w2s[tid] = __uint_as_float(uint32_t(a.w2[size_t(tid) * kOut]) << 16);
+    b1s[tid] = a.b1[tid];
+  }
+  __syncthreads();
+  if (a.raise_priority) __builtin_amdgcn_s_setprio(2);
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t t = a.tile0 + int64_t(blockIdx.x) * 4 + wave;
+  if (t >= tend) return;
+  int lo = 0, hi = a.nligands - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.tile_begin[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const LigandMeta m = a.meta[lo];
+  const int j = int(t - a.tile_begin[lo]) * 64 + lane;
+  const Xform x = make_xform(a.poses, a.stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
+  const int r = lane & 15, grp = lane >> 4;
+  // register-held A fragments: W0^T rows 16 jn + r, k 8 grp .. (only grp 0
+  // holds features), W1^T rows 16 jn + r, k 8 grp ..
+  const bf16x8 zero8 = bf16x8{};
+  bf16x8 fw0[kH / 16], fw1[kH / 16];
+#pragma unroll
+  for (int jn = 0; jn < kH / 16; jn++) {
+    fw0[jn] = grp == 0 ? *reinterpret_cast<const bf16x8 *>(a.w0t + size_t(16 * jn + r) * a.k0) : zero8;
+    fw1[jn] = *reinterpret_cast<const bf16x8 *>(a.w1t + size_t(16 * jn + r) * kH + 8 * grp);
+  }
+  float w2l[kH / 16][4], b1l[kH / 16][4];
+#pragma unroll
+  for (int jn = 0; jn < kH / 16; jn++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      w2l[jn][q] = w2s[16 * jn + 4 * grp + q];
+      b1l[jn][q] = b1s[16 * jn + 4 * grp + q];
+    }
+  bf16 *buf = act[wave];
+  const float inv_h = 1.0f / a.grid.h;
+  const uint4 *g16 = reinterpret_cast<const uint4 *>(a.grid16);
+  float sy = 0.f;
+  for (int il = 0; il < m.natlig; ++il) {
+    const float4 la = a.ligands[m.lig_off + il];
+    const int slot = __builtin_amdgcn_readfirstlane(__float_as_int(la.w));
+    const float px = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
+    const float py = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
+    const float pz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
+    const Corners c = corners(px, py, pz, a.grid, inv_h);
+    uint4 v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = g16[c.node[q]];
+    float phi[kF];
+#pragma unroll
+    for (int k = 0; k < kF; k++) phi[k] = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint32_t u[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        phi[2 * h] = __builtin_fmaf(c.w[q], lo_bf16(u[h]), phi[2 * h]);
+        phi[2 * h + 1] = __builtin_fmaf(c.w[q], hi_bf16(u[h]), phi[2 * h + 1]);
+      }
+    }
+    *reinterpret_cast<uint4 *>(buf + lane * kRow) =
+        make_uint4(pack_bf16(phi[0], phi[1]), pack_bf16(phi[2], phi[3]), pack_bf16(phi[4], phi[5]),
+                   pack_bf16(phi[6], phi[7]));
+    wave_sync();
+    // layer 1 onto the slot's bias
+    floatx4 acc[kH / 16][4];
+#pragma unroll
+    for (int jn = 0; jn < kH / 16; jn++) {
+      const float4 sb = *reinterpret_cast<const float4 *>(a.slot_bias + size_t(slot) * kH + 16 * jn + 4 * grp);
+#pragma unroll
+      for (int i = 0; i < 4; i++) acc[jn][i] = floatx4{sb.x, sb.y, sb.z, sb.w};
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const bf16x8 fb = grp == 0 ? lds8(buf + (16 * i + r) * kRow) : zero8;
+#pragma unroll
+      for (int jn = 0; jn < kH / 16; jn++)
+        acc[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw0[jn], fb, acc[jn][i], 0, 0, 0);
+    }
+    wave_sync();
+    // ReLU -> BF16 rows [pose][neuron]
+#pragma unroll
+    for (int jn = 0; jn < kH / 16; jn++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const floatx4 z = acc[jn][i];
+        *reinterpret_cast<uint2 *>(buf + (16 * i + r) * kRow + 16 * jn + 4 * grp) =
+            make_uint2(pack_bf16(fmaxf(z[0], 0.f), fmaxf(z[1], 0.f)), pack_bf16(fmaxf(z[2], 0.f), fmaxf(z[3], 0.f)));
+      }
+    wave_sync();
+    // layer 2 onto b1, ReLU, dot with W2 over this lane's neurons
+    float part[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const bf16x8 fb = lds8(buf + (16 * i + r) * kRow + 8 * grp);
+      float s = 0.f;
+#pragma unroll
+      for (int jn = 0; jn < kH / 16; jn++) {
+        floatx4 z = floatx4{b1l[jn][0], b1l[jn][1], b1l[jn][2], b1l[jn][3]};
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw1[jn], fb, z, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) s = __builtin_fmaf(w2l[jn][q], fmaxf(z[q], 0.f), s);
+      }
+      part[i] = s;
+    }
+    wave_sync();  // the rows are rewritten by the next atom
+    // sum over the four lane groups of pose block i, kept by group i:
+    // groups {g, g ^ 2} first (each keeps its half of the blocks), then {g, g ^ 1}
+    const bool upper = grp >= 2;
+    const float s0 = upper ? part[0] : part[2], s1 = upper ? part[1] : part[3];
+    const float m0 = (upper ? part[2] : part[0]) + __shfl_xor(s0, 32);
+    const float m1 = (upper ? part[3] : part[1]) + __shfl_xor(s1, 32);
+    const bool odd = grp & 1;
+    sy += (odd ? m1 : m0) + __shfl_xor(odd ? m0 : m1, 16);
+  }
+  if (j < m.nposes)
+    a.out[size_t(m.pose_off) + size_t(j)] = a.sigma * (sy + float(m.natlig) * a.b2) + float(m.natlig) * a.mu;
+}
+
+unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
+
+uint64_t mix_seed(uint64_t seed, uint64_t k) { return (seed ^ (k * 0xd6e8feb86659fd93ull)) * 0x9e3779b97f4a7c15ull; }
+
+mlp::GemmArgs gemm_args(const void *a, const void *b, int M, int N, int K, int lda, int ldb, int epi) {
+  mlp::GemmArgs g{};
+  g.a = static_cast<const bf16 *>(a);
+  g.b = static_cast<const bf16 *>(b);
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.lda = lda;
+  g.ldb = ldb;
+  g.epi = epi;
+  return g;
+}
+
+using Acts = Workspace;
+
+// Forward of B rows whose inputs are in A.x0; training keeps z, a, a^T.
+int forward(const Net &net, Acts &A, size_t B, bool training, hipStream_t s) {
+  const int R = int(B);
+  const void *in = A.x0.ptr;
+  int in_ld = net.k0;
+  for (int l = 0; l < 2; l++) {
+    auto g = gemm_args(in, net.w16t[l].ptr, R, kH, layer_k(net, l), in_ld, layer_k(net, l), mlp::kEpiHiddenRelu);
+    g.bias = static_cast<const float *>(net.b[l].ptr);
+    g.out_f32 = training ? static_cast<float *>(A.z[l].ptr) : nullptr;
+    g.out_bf16 = static_cast<bf16 *>(A.a[l].ptr);
+    g.out_t = training ? static_cast<bf16 *>(A.at[l].ptr) : nullptr;
+    g.ldo = kH;
+    g.ldt = R;
+    if (int st = hip_check(mlp::gemm(g, s), "atomnet gemm forward")) return st;
+    in = g.out_bf16;
+    in_ld = kH;
+  }
+  auto g = gemm_args(in, net.w16t[2].ptr, R, kOut, kH, kH, kH, mlp::kEpiOut);
+  g.bias = static_cast<const float *>(net.b[2].ptr);
+  g.out_f32 = static_cast<float *>(A.pred.ptr);
+  return hip_check(mlp::gemm(g, s), "atomnet gemm output");
+}
+
+// Backward from dz3 (written by the loss kernel): weight and bias gradients
+// into net.gw / gb and dL/dx0 (first 32 columns) into A.dx0.
+int backward(Net &net, Acts &A, size_t B, hipStream_t s) {
+  const int R = int(B);
+  mlp::BiasGradBatch bg{};
+  bg.count = 3;
+  bg.B = R;
+  bg.ldt = R;
+  const void *dzt_next = A.dz3t.ptr, *dz_next = A.dz3.ptr;
+  for (int l = 2; l >= 0; l--) {
+    const int K = layer_k(net, l), N = layer_n(l);
+    const void *prev_t = l == 0 ? A.x0t.ptr : A.at[l - 1].ptr;
+    auto gw = gemm_args(prev_t, dzt_next, K, N, R, R, R, mlp::kEpiF32);
+    gw.out_f32 = static_cast<float *>(net.gw[l].ptr);
+    gw.ldo = N;
+    gw.ksplit = mlp::auto_ksplit(K, N, R);
+    gw.work = static_cast<float *>(A.work.ptr);
+    if (int st = hip_check(mlp::gemm(gw, s), "atomnet gemm dW")) return st;
+    bg.dz_t[l] = static_cast<const bf16 *>(dzt_next);
+    bg.db[l] = static_cast<float *>(net.gb[l].ptr);
+    bg.N[l] = N;
+    if (l == 0) {
+      // dL/dx0 for the grid: dz1 W0^T over the first 32 rows of W0
+      auto gx = gemm_args(dz_next, net.w16[0].ptr, R, 32, kH, kH, kH, mlp::kEpiF32);
+      gx.out_f32 = static_cast<float *>(A.dx0.ptr);
+      gx.ldo = 32;
+      if (int st = hip_check(mlp::gemm(gx, s), "atomnet gemm dx0")) return st;
+      break;
+    }
+    auto ga = gemm_args(dz_next, net.w16[l].ptr, R, kH, N, N, N, mlp::kEpiBackRelu);
+    ga.zprev = static_cast<const float *>(A.z[l - 1].ptr);
+    ga.out_bf16 = static_cast<bf16 *>(A.dz[l - 1].ptr);
+    ga.out_t = static_cast<bf16 *>(A.dzt[l - 1].ptr);
+    ga.ldo = kH;
+    ga.ldt = R;
+    if (int st = hip_check(mlp::gemm(ga, s), "atomnet gemm dA")) return st;
+    dz_next = A.dz[l - 1].ptr;
+    dzt_next = A.dzt[l - 1].ptr;
+  }
+  return hip_check(mlp::bias_grad_multi(bg, s), "atomnet bias grads");
+}
+
+int adam_step(Net &net, float lr, float grid_lr, hipStream_t s) {
+  net.step++;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float bc1 = 1.f - std::pow(b1, float(net.step)), bc2 = 1.f - std::pow(b2, float(net.step));
+  mlp::AdamBatch ab{};
+  for (int l = 0; l < 3; l++) {
+    const int K = layer_k(net, l), N = layer_n(l);
+    ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(net.gw[l].ptr),
+                   static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
+                   static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), K, N};
+    ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(net.gb[l].ptr),
+                       static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), nullptr, nullptr, 1,
+                       N};
+  }
+  ab.count = 6;
+  if (int st = hip_check(mlp::adam_multi(ab, lr, b1, b2, eps, bc1, bc2, s), "atomnet adam")) return st;
+  const size_t n = net.nodes * kF;
+  hipLaunchKernelGGL(grid_adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, static_cast<float *>(net.g.ptr),
+                     static_cast<unsigned long long *>(net.gacc.ptr), static_cast<float *>(net.gm.ptr),
+                     static_cast<float *>(net.gv.ptr), static_cast<bf16 *>(net.g16.ptr), n, grid_lr, b1, b2, eps, bc1,
+                     bc2);
+  return hip_check(hipGetLastError(), "atomnet grid adam");
+}
+
+// Forward + loss + backward of one batch (grid gradients accumulated in net.gacc).
+int step_grads(Net &net, Acts &A, const float4 *points, const uint32_t *idx, const float *labels, size_t B,
+               mlp::TargetNorm norm, hipStream_t s) {
+  hipLaunchKernelGGL(features_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, s, points, idx, B,
+                     static_cast<const float *>(net.g.ptr), net.grid, net.k0, static_cast<bf16 *>(A.x0.ptr),
+                     static_cast<bf16 *>(A.x0t.ptr));
+  if (int st = hip_check(hipGetLastError(), "atomnet features")) return st;
+  if (int st = forward(net, A, B, true, s)) return st;
+  if (int st = hip_check(mlp::loss_grad(static_cast<const float *>(A.pred.ptr), labels, idx, B, norm, nullptr, nullptr,
+                                        1.f, static_cast<bf16 *>(A.dz3.ptr), static_cast<bf16 *>(A.dz3t.ptr), int(B),
+                                        static_cast<float *>(A.loss.ptr), s),
+                         "atomnet loss"))
+    return st;
+  if (int st = backward(net, A, B, s)) return st;
+  hipLaunchKernelGGL(grid_grad_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, s, points, idx, B, net.grid,
+                     static_cast<const float *>(A.dx0.ptr), static_cast<unsigned long long *>(net.gacc.ptr));
+  return hip_check(hipGetLastError(), "atomnet grid grad");
+}
+
+int alloc_zero(DeviceBuffer &d, size_t bytes) {
+  if (int st = d.alloc(bytes)) return st;
+  return hip_check(hipMemset(d.ptr, 0, bytes), "hipMemset");
+}
+
+}  // namespace
+
+int reserve(const Net &net, Workspace &A, size_t B, size_t nval) {
+  if (A.rows >= B && A.k0 == net.k0 && A.vpred.bytes >= std::max<size_t>(nval, 1) * 4) return BUDE_OK;
+  A.rows = B;
+  A.k0 = net.k0;
+  int st = A.x0.reserve(B * net.k0 * 2);
+  if (!st) st = A.x0t.reserve(B * net.k0 * 2);
+  for (int l = 0; l < 2 && !st; l++) {
+    st = A.z[l].reserve(B * kH * 4);
+    if (!st) st = A.a[l].reserve(B * kH * 2);
+    if (!st) st = A.at[l].reserve(B * kH * 2);
+    if (!st) st = A.dz[l].reserve(B * kH * 2);
+    if (!st) st = A.dzt[l].reserve(B * kH * 2);
+  }
+  if (!st) st = A.pred.reserve(B * 4);
+  if (!st) st = A.dz3.reserve(B * kOut * 2);
+  if (!st) st = A.dz3t.reserve(B * kOut * 2);
+  if (!st) st = A.dx0.reserve(B * 32 * 4);
+  if (!st) st = A.loss.reserve(256);
+  if (!st) st = A.part.reserve(1024 * 2 * 8);
+  if (!st) st = A.vpred.reserve(std::max<size_t>(nval, 1) * 4);
+  size_t wmax = 0;
+  for (int l = 0; l < 3; l++) {
+    const int K = layer_k(net, l), N = layer_n(l);
+    wmax = std::max(wmax, size_t(mlp::auto_ksplit(K, N, int(B))) * K * N);
+  }
+  if (!st) st = A.work.reserve(wmax * 4);
+  return st;
+}
+
+int set_params(Net &net, const float *grid, const float *const w[3], const float *const b[3], hipStream_t s) {
+  const size_t ng = net.nodes * kF;
+  int st = net.g.upload(grid, ng * 4);
+  if (!st) st = alloc_zero(net.gm, ng * 4);
+  if (!st) st = alloc_zero(net.gv, ng * 4);
+  if (!st) st = alloc_zero(net.gacc, ng * 8);
+  if (!st) st = net.g16.alloc(ng * 2);
+  if (!st) {
+    hipLaunchKernelGGL(grid_bf16_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, s, static_cast<const float *>(net.g.ptr),
+                       static_cast<bf16 *>(net.g16.ptr), ng);
+    st = hip_check(hipGetLastError(), "atomnet grid bf16");
+  }
+  for (int l = 0; l < 3 && !st; l++) {
+    const int K = layer_k(net, l), N = layer_n(l);
+    const size_t nw = size_t(K) * N;
+    st = net.w[l].upload(w[l], nw * 4);
+    if (!st) st = net.b[l].upload(b[l], size_t(N) * 4);
+    for (DeviceBuffer *d : {&net.mw[l], &net.vw[l], &net.gw[l]})
+      if (!st) st = alloc_zero(*d, nw * 4);
+    for (DeviceBuffer *d : {&net.mb[l], &net.vb[l], &net.gb[l]})
+      if (!st) st = alloc_zero(*d, size_t(N) * 4);
+    if (!st) st = net.w16[l].alloc(nw * 2);
+    if (!st) st = net.w16t[l].alloc(nw * 2);
+    if (!st)
+      st = hip_check(mlp::to_bf16(static_cast<const float *>(net.w[l].ptr), K, N, static_cast<bf16 *>(net.w16[l].ptr),
+                                  static_cast<bf16 *>(net.w16t[l].ptr), s),
+                     "to_bf16");
+  }
+  if (!st) st = net.slot_bias.alloc(size_t(net.nslots) * kH * 4);
+  net.step = 0;
+  if (!st) st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  return st;
+}
+
+int init(Net &net, const field::Grid &grid, int nslots, uint64_t seed, hipStream_t s) {
+  if (nslots <= 0 || nslots > kMaxSlots)
+    return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: " + std::to_string(nslots) + " forcefield slots, at most " +
+                                               std::to_string(kMaxSlots));
+  net.grid = grid;
+  net.nslots = nslots;
+  net.k0 = input_width(nslots);
+  net.nodes = grid.points();
+  net.step = 0;
+  // buffers kept from call to call (an allocation can wait for the kernels
+  // in flight); every value written on the stream below
+  const size_t ng = net.nodes * kF;
+  int st = net.g.reserve(ng * 4);
+  if (!st) st = net.gm.reserve(ng * 4);
+  if (!st) st = net.gv.reserve(ng * 4);
+  if (!st) st = net.gacc.reserve(ng * 8);
+  if (!st) st = net.g16.reserve(ng * 2);
+  for (int l = 0; l < 3 && !st; l++) {
+    const size_t nw = size_t(layer_k(net, l)) * layer_n(l), nb = size_t(layer_n(l));
+    for (DeviceBuffer *d : {&net.w[l], &net.mw[l], &net.vw[l], &net.gw[l]})
+      if (!st) st = d->reserve(nw * 4);
+    for (DeviceBuffer *d : {&net.b[l], &net.mb[l], &net.vb[l], &net.gb[l]})
+      if (!st) st = d->reserve(nb * 4);
+    if (!st) st = net.w16[l].reserve(nw * 2);
+    if (!st) st = net.w16t[l].reserve(nw * 2);
+  }
+  if (!st) st = net.slot_bias.reserve(size_t(nslots) * kH * 4);
+  if (st) return st;
+  // grid features N(0, 0.1^2); He-normal hidden weights (first layer: fan-in
+  // of the features and one hot input), 1 / sqrt(fan_in) output column;
+  // zero biases and Adam state
+  hipLaunchKernelGGL(init_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, s, static_cast<float *>(net.g.ptr),
+                     static_cast<float *>(net.gm.ptr), static_cast<float *>(net.gv.ptr), ng, kF, int(net.nodes), kF,
+                     0.1f, seed);
+  if (int e = hip_check(hipGetLastError(), "atomnet init")) return e;
+  if (int e = hip_check(hipMemsetAsync(net.gacc.ptr, 0, ng * 8, s), "hipMemsetAsync")) return e;
+  hipLaunchKernelGGL(grid_bf16_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, s, static_cast<const float *>(net.g.ptr),
+                     static_cast<bf16 *>(net.g16.ptr), ng);
+  if (int e = hip_check(hipGetLastError(), "atomnet grid bf16")) return e;
+  for (int l = 0; l < 3; l++) {
+    const int K = layer_k(net, l), N = layer_n(l);
+    const size_t nw = size_t(K) * N;
+    const int rows = l == 0 ? kF + nslots : K, cols = l == 2 ? 1 : N;
+    const float fan_in = l == 0 ? float(kF + 1) : float(K);
+    const float sd = l == 2 ? std::sqrt(1.f / fan_in) : std::sqrt(2.f / fan_in);
+    hipLaunchKernelGGL(init_kernel, dim3(blocks_for(nw, 256)), dim3(256), 0, s, static_cast<float *>(net.w[l].ptr),
+                       static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr), nw, N, rows, cols, sd,
+                       mix_seed(seed, uint64_t(l) + 1));
+    if (int e = hip_check(hipGetLastError(), "atomnet init")) return e;
+    hipLaunchKernelGGL(init_kernel, dim3(1), dim3(256), 0, s, static_cast<float *>(net.b[l].ptr),
+                       static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), size_t(N), N, 0, 0,
+                       0.f, seed);
+    if (int e = hip_check(hipGetLastError(), "atomnet init")) return e;
+    if (int e = hip_check(mlp::to_bf16(static_cast<const float *>(net.w[l].ptr), K, N,
+                                       static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), s),
+                          "to_bf16"))
+      return e;
+  }
+  return BUDE_OK;
+}
+
+hipError_t sample_points(const float4 *ligands, const LigandMeta *meta, int nligands, const float *poses,
+                         size_t stride, int nslots, uint64_t seed, size_t n, float4 *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (nligands <= 0 || nslots <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, ligands, meta, nligands, poses, stride,
+                     nslots, seed, n, out);
+  return hipGetLastError();
+}
+
+hipError_t label_points(const float4 *protein, const int32_t *group_begin, const PairCoef *slot_coef, int ngroups,
+                        const float4 *points, size_t n, int form, float *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (n % 64) return hipErrorInvalidValue;
+  const unsigned blocks = blocks_for(n / 64, 4);
+  if (form == kFormD)
+    hipLaunchKernelGGL(label_kernel<kFormD>, dim3(blocks), dim3(256), 0, s, protein, group_begin, slot_coef, ngroups,
+                       points, n, out);
+  else
+    hipLaunchKernelGGL(label_kernel<kFormB>, dim3(blocks), dim3(256), 0, s, protein, group_begin, slot_coef, ngroups,
+                       points, n, out);
+  return hipGetLastError();
+}
+
+int train(Net &net, Workspace &A, const float4 *points, const float *labels, const uint32_t *perm, size_t ntrain,
+          size_t nval, const TrainParams &p, hipStream_t s, TrainReport *rep) {
+  const size_t B = p.batch;
+  if (B == 0 || B % 64 || ntrain < B || ntrain % B || nval % 64)
+    return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet train: the training points must be whole batches "
+                                           "(a multiple of 64) and the held-out points a multiple of 64");
+  if (int st = reserve(net, A, B, nval)) return st;
+  // target transform from the training labels
+  {
+    hipLaunchKernelGGL(stats_kernel, dim3(1024), dim3(256), 0, s, labels, ntrain, static_cast<double *>(A.part.ptr));
+    if (int st = hip_check(hipGetLastError(), "atomnet stats")) return st;
+    std::vector<double> part(2048);
+    if (int st = hip_check(hipMemcpyAsync(part.data(), A.part.ptr, part.size() * 8, hipMemcpyDeviceToHost, s),
+                           "hipMemcpyAsync"))
+      return st;
+    if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
+    double s1 = 0, s2 = 0;
+    for (size_t i = 0; i < 1024; i++) {
+      s1 += part[2 * i];
+      s2 += part[2 * i + 1];
+    }
+    const double mean = s1 / double(ntrain), var = std::max(1e-12, s2 / double(ntrain) - mean * mean);
+    net.mu = float(mean);
+    net.sigma = float(std::sqrt(var));
+    net.clip = float(mean + 8.0 * std::sqrt(var));
+  }
+  const mlp::TargetNorm norm{net.clip, net.mu, 1.0f / net.sigma, -INFINITY};
+  const size_t steps = std::max<uint32_t>(1, p.steps);
+  const size_t warm = std::min<size_t>(100, steps / 10 + 1);
+  const size_t tail = std::max<size_t>(1, steps / 100);
+  for (size_t step = 0; step < steps; step++) {
+    if (step + tail == steps)
+      if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, s), "hipMemsetAsync")) return st;
+    const uint32_t *idx = perm + (step * B) % ntrain;
+    if (int st = step_grads(net, A, points, idx, labels, B, norm, s)) return st;
+    const float warmf = step < warm ? float(step + 1) / float(warm) : 1.f;
+    const float prog = float(step) / float(std::max<size_t>(1, steps - 1));
+    const float f = warmf * (0.05f + 0.95f * 0.5f * (1.f + std::cos(3.14159265f * prog)));
+    if (int st = adam_step(net, p.lr * f, p.grid_lr * f, s)) return st;
+  }
+  TrainReport r{};
+  r.steps = uint32_t(steps);
+  float lsum = 0.f;
+  if (int st = hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return st;
+  // held-out RMSE in energy units (forward of the training kernels)
+  double se = 0;
+  if (nval) {
+    DeviceBuffer &vpred = A.vpred;
+    for (size_t v0 = 0; v0 < nval; v0 += B) {
+      const size_t nb = std::min(B, nval - v0);
+      hipLaunchKernelGGL(features_kernel, dim3(blocks_for(nb, 256)), dim3(256), 0, s, points + ntrain + v0, nullptr, nb,
+                         static_cast<const float *>(net.g.ptr), net.grid, net.k0, static_cast<bf16 *>(A.x0.ptr),
+                         static_cast<bf16 *>(A.x0t.ptr));
+      if (int st = hip_check(hipGetLastError(), "atomnet features")) return st;
+      if (int st = forward(net, A, nb, false, s)) return st;
+      if (int st = hip_check(hipMemcpyAsync(static_cast<float *>(vpred.ptr) + v0, A.pred.ptr, nb * 4,
+                                            hipMemcpyDeviceToDevice, s),
+                             "hipMemcpyAsync"))
+        return st;
+    }
+    hipLaunchKernelGGL(sq_err_kernel, dim3(256), dim3(256), 0, s, static_cast<const float *>(vpred.ptr),
+                       labels + ntrain, nval, net.sigma, net.mu, net.clip, static_cast<double *>(A.part.ptr));
+    if (int st = hip_check(hipGetLastError(), "atomnet val")) return st;
+    std::vector<double> part(256);
+    if (int st = hip_check(hipMemcpyAsync(part.data(), A.part.ptr, 256 * 8, hipMemcpyDeviceToHost, s),
+                           "hipMemcpyAsync"))
+      return st;
+    if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
+    for (double v : part) se += v;
+  } else if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) {
+    return st;
+  }
+  r.val_rmse = nval ? float(std::sqrt(se / double(nval))) : 0.f;
+  r.final_loss = lsum / float(tail * B);
+  if (rep) *rep = r;
+  return finalize(net, s);
+}
+
+int forward_backward(Net &net, const float4 *points, const float *target, size_t batch, float *pred, float *loss,
+                     float *grad_grid, hipStream_t s) {
+  if (batch == 0 || batch % 64) return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: batch must be a multiple of 64");
+  Acts A;
+  if (int st = reserve(net, A, batch, 0)) return st;
+  if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, s), "hipMemsetAsync")) return st;
+  const mlp::TargetNorm norm{INFINITY, 0.f, 1.f, -INFINITY};
+  if (int st = step_grads(net, A, points, nullptr, target, batch, norm, s)) return st;
+  const size_t ng = net.nodes * kF;
+  hipLaunchKernelGGL(grid_grad_out_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, s,
+                     static_cast<unsigned long long *>(net.gacc.ptr), grad_grid, ng);
+  if (int st = hip_check(hipGetLastError(), "atomnet grid grad out")) return st;
+  if (int st = hip_check(hipMemcpyAsync(pred, A.pred.ptr, batch * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+    return st;
+  float l = 0.f;
+  if (int st = hip_check(hipMemcpyAsync(&l, A.loss.ptr, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return st;
+  if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
+  *loss = l / float(batch);
+  return BUDE_OK;
+}
+
+int finalize(Net &net, hipStream_t s) {
+  hipLaunchKernelGGL(slot_bias_kernel, dim3(blocks_for(size_t(net.nslots) * kH, 256)), dim3(256), 0, s,
+                     static_cast<const float *>(net.b[0].ptr), static_cast<const bf16 *>(net.w16[0].ptr), net.nslots,
+                     static_cast<float *>(net.slot_bias.ptr));
+  if (int st = hip_check(hipGetLastError(), "atomnet slot bias")) return st;
+  if (int st = hip_check(hipMemcpyAsync(&net.b2, net.b[2].ptr, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync"))
+    return st;
+  return hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+ScoreArgs score_args(const Net &net) {
+  ScoreArgs a{};
+  a.grid16 = static_cast<const bf16 *>(net.g16.ptr);
+  a.grid = net.grid;
+  a.w0t = static_cast<const bf16 *>(net.w16t[0].ptr);
+  a.k0 = net.k0;
+  a.slot_bias = static_cast<const float *>(net.slot_bias.ptr);
+  a.w1t = static_cast<const bf16 *>(net.w16t[1].ptr);
+  a.b1 = static_cast<const float *>(net.b[1].ptr);
+  a.w2 = static_cast<const bf16 *>(net.w16[2].ptr);
+  a.b2 = net.b2;
+  a.sigma = net.sigma;
+  a.mu = net.mu;
+  return a;
+}
+
+hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  if (a.nligands <= 0 || a.tile0 < 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 ||
+      (ntiles + 3) / 4 > 0x7fffffff)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a, a.tile0 + ntiles);
+  return hipGetLastError();
+}
+
+}  // namespace atomnet
+}  // namespace bude

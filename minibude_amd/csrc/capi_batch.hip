@@ -282,23 +282,33 @@ extern "C" int bude_mi355x_batch_run_separate(bude_batch *b, size_t ppwi, size_t
 
 extern "C" int bude_mi355x_batch_autotune(bude_batch *b, BudeBatchConfig *best, double *best_ms) {
   if (!b || !best) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_batch_autotune: NULL argument");
-  // Candidate tile widths, work-group sizes and persistent-grid depths; the
-  // graph of each is built once and timed as the median of three replays.
-  double best_t = 1e300;
-  BudeBatchConfig winner{};
+  // Candidate tile widths, work-group sizes, persistent-grid depths and tile
+  // orders, by successive halving: every candidate's graph is built and
+  // replayed twice (the second replay timed), then the four fastest are
+  // timed as the median of three more replays each.
+  std::vector<std::pair<double, BudeBatchConfig>> round1;
   for (int ppwi : {1, 2, 4})
     for (int wg : {64, 256})
       for (int wps : {4, 8})
         for (int order : {0, 1}) {
           BudeBatchConfig c{ppwi, wg, wps, order, 1};
-          double ms[4] = {0, 0, 0, 0};
-          if (int st = bude_mi355x_batch_run(b, &c, 1, 3, ms)) return st;
-          std::sort(ms + 1, ms + 4);
-          if (ms[2] < best_t) {
-            best_t = ms[2];
-            winner = c;
-          }
+          double ms[2] = {0, 0};
+          if (int st = bude_mi355x_batch_run(b, &c, 1, 1, ms)) return st;
+          round1.emplace_back(ms[1], c);
         }
+  std::stable_sort(round1.begin(), round1.end(),
+                   [](const auto &x, const auto &y) { return x.first < y.first; });
+  double best_t = 1e300;
+  BudeBatchConfig winner{};
+  for (size_t i = 0; i < round1.size() && i < 4; i++) {
+    double ms[3] = {0, 0, 0};
+    if (int st = bude_mi355x_batch_run(b, &round1[i].second, 0, 3, ms)) return st;
+    std::sort(ms, ms + 3);
+    if (ms[1] < best_t) {
+      best_t = ms[1];
+      winner = round1[i].second;
+    }
+  }
   *best = winner;
   if (best_ms) *best_ms = best_t;
   return bude::ok();

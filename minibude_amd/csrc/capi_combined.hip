@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "atomnet.h"
 #include "bude_internal.h"
 #include "fasten_batch.h"
 #include "field.h"
@@ -159,7 +160,8 @@ struct ProteinSlot {
   hipEvent_t ev[12] = {};
   std::vector<uint32_t> nsel;
   std::vector<int32_t> fallback;
-  std::vector<float> val_rmse, val_rmse_cal;
+  std::vector<float> val_rmse, val_rmse_cal, gate_z, sample_recall;
+  float atom_val_rmse = 0.f;  // learned per-atom field: one-atom RMSE on held-out points (energy units)
   // the sampled poses: pose index within the pair, exact energy, per-pair offsets
   std::vector<uint32_t> lab_j;
   std::vector<float> lab_energy;
@@ -236,6 +238,14 @@ struct bude_combined {
   DeviceBuffer rs_order;
   // MLP
   bude::surrogate::Net net;
+  // learned per-atom field: the network and its training workspace, the
+  // training and held-out points (positions of ligand atoms under the
+  // resident poses, fixed for the object's life), their exact labels
+  // (per protein) and the order the batches read the training points in
+  bude::atomnet::Net anet;
+  bude::atomnet::Workspace aws;
+  DeviceBuffer apoints, alabels, aperm, ameta;
+  size_t antrain = 0, anval = 0;
   // exact energies of every pair (exhaustive) and the protein they belong to
   uint64_t exact_sum = 0;
   bool have_exact = false;
@@ -435,6 +445,120 @@ int launch_field(bude_combined *c, const float *poses, size_t stride, float *out
   return bude::hip_check(bude::field::score(a, ntiles, stream), "field score");
 }
 
+// Surrogate scores of the per-atom kinds (the field's maps or the learned
+// per-atom field) of tiles [tile0, tile0 + ntiles) of the spans prepared by
+// prepare_field.
+int launch_surrogate(bude_combined *c, const float *poses, size_t stride, float *out, const DeviceBuffer &meta,
+                     const DeviceBuffer &tile_begin, int64_t tile0, int64_t ntiles, hipStream_t stream,
+                     bool raise_priority = false) {
+  if (c->cfg.surrogate != BUDE_SURROGATE_ATOM_MLP)
+    return launch_field(c, poses, stride, out, meta, tile_begin, tile0, ntiles, stream, raise_priority);
+  if (ntiles == 0) return BUDE_OK;
+  bude::atomnet::ScoreArgs a = bude::atomnet::score_args(c->anet);
+  a.ligands = static_cast<const float4 *>(c->ligands.ptr);
+  a.meta = static_cast<const bude::LigandMeta *>(meta.ptr);
+  a.tile_begin = static_cast<const int64_t *>(tile_begin.ptr);
+  a.nligands = int(c->nlig);
+  a.poses = poses;
+  a.stride = stride;
+  a.out = out;
+  a.tile0 = tile0;
+  a.raise_priority = raise_priority;
+  return bude::hip_check(bude::atomnet::score(a, ntiles, stream), "atomnet score");
+}
+
+// Fallback statistics of one pair from its validation samples: exact
+// energies e and surrogate scores p (energy units).  z = (Q(keep) -
+// Q(top)) / sd, with Q the samples' exact energy quantiles and sd the
+// surrogate's error in energy units over the lowest tail_q of the samples
+// (by exact energy): the residual deviation of the least-squares line
+// p = a e + b there, over a (a ranking is blind to a and b).  A pose of
+// exact energy e is selected unless its error exceeds the selection
+// threshold's by Q(keep) - e; with Gaussian errors of deviation sd and the
+// exact energies spread evenly over the gap, the exact top fraction loses
+// G(z) / z of its poses, G(z) = phi(z) - z Phi(-z): 0.0093 at z = 1.75,
+// 0.0043 at z = 2.  recall: of the samples' exact lowest 5 %, the fraction
+// among the surrogate's lowest 10 %.
+struct Gate {
+  float z, recall;
+};
+
+Gate gate_stats(const std::vector<double> &e, const std::vector<double> &p, double keep, double top, double tail_q) {
+  const size_t n = e.size();
+  if (n < 2) return Gate{0.f, 0.f};
+  std::vector<size_t> oe(n), op(n);
+  std::iota(oe.begin(), oe.end(), size_t(0));
+  std::iota(op.begin(), op.end(), size_t(0));
+  std::stable_sort(oe.begin(), oe.end(), [&](size_t a, size_t b) { return e[a] < e[b]; });
+  std::stable_sort(op.begin(), op.end(), [&](size_t a, size_t b) { return p[a] < p[b]; });
+  auto q = [&](double f) {
+    const double x = std::min(1.0, std::max(0.0, f)) * double(n - 1);
+    const size_t i = size_t(x);
+    const double t = x - double(i);
+    return i + 1 < n ? e[oe[i]] * (1 - t) + e[oe[i + 1]] * t : e[oe[n - 1]];
+  };
+  const double gap = q(keep) - q(top);
+  const size_t nt = std::min(n, std::max<size_t>(8, size_t(std::ceil(tail_q * double(n)))));
+  double se = 0, sp = 0, see = 0, sep = 0, spp = 0;
+  for (size_t k = 0; k < nt; k++) {
+    const double x = e[oe[k]], y = p[oe[k]];
+    se += x;
+    sp += y;
+    see += x * x;
+    sep += x * y;
+    spp += y * y;
+  }
+  const double nn = double(nt), ve = see / nn - (se / nn) * (se / nn), vp = spp / nn - (sp / nn) * (sp / nn),
+               cv = sep / nn - (se / nn) * (sp / nn);
+  // slope of p on e and the residual deviation, in energy units; a flat or
+  // inverted relation gets no room
+  const double a = ve > 0 ? cv / ve : 0.0;
+  const double sd = a > 0 ? std::sqrt(std::max(0.0, vp - a * cv)) / a : INFINITY;
+  Gate g{};
+  g.z = sd > 0 ? float(gap / sd) : (gap > 0 ? INFINITY : 0.f);
+  const size_t kt = std::max<size_t>(1, size_t(std::lround(0.05 * double(n))));
+  const size_t ks = std::min(n, std::max(kt, size_t(std::lround(0.10 * double(n)))));
+  std::vector<char> sel(n, 0);
+  for (size_t k = 0; k < ks; k++) sel[op[k]] = 1;
+  size_t hits = 0;
+  for (size_t k = 0; k < kt; k++) hits += size_t(sel[oe[k]]);
+  g.recall = float(double(hits) / double(kt));
+  return g;
+}
+
+// Learned per-atom field for the protein in slot c->s() on the second
+// stream, after its tables: exact labels of the training and held-out
+// points, a fresh network (seeded by the protein), training.  The host waits
+// for it.
+int train_atom(bude_combined *c, uint64_t pseed) {
+  ProteinSlot &ps = c->s();
+  const size_t n = c->antrain + c->anval;
+  if (int st = bude::hip_check(bude::atomnet::label_points(static_cast<const float4 *>(ps.protein.ptr),
+                                                           static_cast<const int32_t *>(ps.group_begin.ptr),
+                                                           static_cast<const bude::PairCoef *>(ps.slot_coef.ptr),
+                                                           int(ps.ngroups), static_cast<const float4 *>(c->apoints.ptr),
+                                                           n, bude::plain_form(), static_cast<float *>(c->alabels.ptr),
+                                                           c->stream2),
+                               "label points"))
+    return st;
+  if (int st = bude::atomnet::init(c->anet, c->grid, int(c->slot_type.size()), pseed, c->stream2)) return st;
+  bude::atomnet::TrainParams tp;
+  tp.batch = c->cfg.batch;
+  tp.steps = c->cfg.steps;
+  tp.lr = c->cfg.lr;
+  tp.grid_lr = c->cfg.grid_lr;
+  tp.seed = pseed;
+  bude::atomnet::TrainReport rep{};
+  if (int st = bude::atomnet::train(c->anet, c->aws, static_cast<const float4 *>(c->apoints.ptr),
+                                    static_cast<const float *>(c->alabels.ptr),
+                                    static_cast<const uint32_t *>(c->aperm.ptr), c->antrain, c->anval, tp, c->stream2,
+                                    &rep))
+    return st;
+  ps.atom_val_rmse = rep.val_rmse;
+  ps.res.surrogate_rmse = rep.val_rmse;
+  return BUDE_OK;
+}
+
 // Per-pair top-k of pred over the pairs of `plan` (pairs first .. first +
 // n - 1) and their rescoring tables from the ligand list `order` (n
 // entries) into tiles / *ntiles, on stream s.
@@ -507,7 +631,7 @@ int plan_selection(bude_combined *c, float keep) {
   }
   if (int st = c->plan.prepare(segs)) return st;
   size_t K = 1;
-  if (c->cfg.surrogate == BUDE_SURROGATE_FIELD)
+  if (c->cfg.surrogate != BUDE_SURROGATE_MLP)
     K = c->cfg.pipeline_chunks ? size_t(c->cfg.pipeline_chunks)
                                : size_t(std::max<int64_t>(1, expect_all / (kChunkTilesPerCu * c->cus)));
   K = std::min({K, L, kMaxChunks});
@@ -569,8 +693,12 @@ float elapsed(bude_combined *c, int a, int b) { return elapsed_events(c->s().ev[
 
 int check_config(const BudeCombinedConfig *cfg) {
   if (!cfg) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: config is NULL");
-  if (cfg->surrogate != BUDE_SURROGATE_MLP && cfg->surrogate != BUDE_SURROGATE_FIELD)
-    return bude::fail(BUDE_ERR_UNSUPPORTED, "combined: surrogate must be BUDE_SURROGATE_MLP or BUDE_SURROGATE_FIELD");
+  if (cfg->surrogate != BUDE_SURROGATE_MLP && cfg->surrogate != BUDE_SURROGATE_FIELD &&
+      cfg->surrogate != BUDE_SURROGATE_ATOM_MLP)
+    return bude::fail(BUDE_ERR_UNSUPPORTED, "combined: surrogate must be BUDE_SURROGATE_MLP, BUDE_SURROGATE_FIELD or "
+                                            "BUDE_SURROGATE_ATOM_MLP");
+  if (!(cfg->gate_z >= 0.f) || !(cfg->gate_top_fraction > 0.f) || cfg->gate_top_fraction > cfg->keep_fraction)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: gate_z >= 0 and gate_top_fraction in (0, keep_fraction]");
   if (!(cfg->keep_fraction > 0.f) || cfg->keep_fraction > 1.f || !(cfg->clip_quantile > 0.f) ||
       cfg->clip_quantile > 1.f || cfg->tail_quantile < 0.f || cfg->tail_quantile > 1.f || !(cfg->tail_weight > 0.f) ||
       cfg->max_val_rmse < 0.f)
@@ -582,8 +710,15 @@ int check_config(const BudeCombinedConfig *cfg) {
   if (cfg->surrogate == BUDE_SURROGATE_MLP &&
       (cfg->batch < 32 || cfg->batch % 32 || cfg->batch > (1u << 20) || !(cfg->lr > 0.f)))
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: MLP batch must be a multiple of 32 in [32, 2^20], lr > 0");
-  if (cfg->surrogate == BUDE_SURROGATE_FIELD && !(cfg->grid_spacing >= 0.05f && cfg->grid_spacing <= 4.f))
+  if (cfg->surrogate != BUDE_SURROGATE_MLP && !(cfg->grid_spacing >= 0.05f && cfg->grid_spacing <= 4.f))
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: grid_spacing must be in [0.05, 4]");
+  if (cfg->surrogate == BUDE_SURROGATE_ATOM_MLP &&
+      (cfg->batch < 64 || cfg->batch % 64 || cfg->batch > (1u << 20) || cfg->train_points < cfg->batch ||
+       cfg->train_points % cfg->batch || cfg->train_points > (1u << 28) || cfg->steps == 0 || !(cfg->lr > 0.f) ||
+       !(cfg->grid_lr > 0.f)))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "combined: ATOM_MLP batch must be a multiple of 64 in [64, 2^20], train_points a multiple of "
+                      "batch up to 2^28, steps > 0, lr > 0, grid_lr > 0");
   if (cfg->pipeline_chunks > kMaxChunks)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: pipeline_chunks must be in [0, 16]");
   return BUDE_OK;
@@ -622,6 +757,41 @@ int make_grid(bude_combined *c, const float *const *poses) {
   if (points > 0x7fffffffu || points * c->slot_type.size() * 4 > (size_t(32) << 30))
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: maps would exceed 32 GiB (raise grid_spacing)");
   return BUDE_OK;
+}
+
+// Learned per-atom field: its training and held-out points (positions of
+// random atoms of random ligands under random resident poses, one forcefield
+// slot per 64 consecutive points) and the order training batches read them
+// in.  They depend on the ligand set only, so every protein trains on the
+// same positions (with its own exact labels).
+int prepare_atom_points(bude_combined *c) {
+  const size_t L = c->nlig, nslots = c->slot_type.size();
+  if (nslots > size_t(bude::atomnet::kMaxSlots))
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: the ligands use " + std::to_string(nslots) +
+                                                     " forcefield types, the learned field at most " +
+                                                     std::to_string(bude::atomnet::kMaxSlots));
+  std::vector<bude::LigandMeta> m(L);
+  for (size_t l = 0; l < L; l++)
+    m[l] = bude::LigandMeta{c->lig_off[l], int32_t(c->lig[l].size()), 0, int32_t(c->nposes[l]),
+                            int64_t(c->pose_off[l]), 0};
+  if (int st = c->ameta.upload(m.data(), L * sizeof(bude::LigandMeta))) return st;
+  c->antrain = c->cfg.train_points;
+  c->anval = 65536;
+  const size_t n = c->antrain + c->anval;
+  if (int st = c->apoints.alloc(n * sizeof(float4))) return st;
+  if (int st = c->alabels.alloc(n * 4)) return st;
+  if (int st = bude::hip_check(bude::atomnet::sample_points(static_cast<const float4 *>(c->ligands.ptr),
+                                                            static_cast<const bude::LigandMeta *>(c->ameta.ptr), int(L),
+                                                            static_cast<const float *>(c->poses.ptr), c->total,
+                                                            int(nslots), c->cfg.seed ^ 0x70017ull, n,
+                                                            static_cast<float4 *>(c->apoints.ptr), nullptr),
+                               "sample points"))
+    return st;
+  std::vector<uint32_t> perm(c->antrain);
+  std::iota(perm.begin(), perm.end(), 0u);
+  std::mt19937_64 rng(c->cfg.seed ^ 0x9e37ull);
+  std::shuffle(perm.begin(), perm.end(), rng);
+  return c->aperm.upload(perm.data(), perm.size() * 4);
 }
 
 }  // namespace
@@ -686,7 +856,7 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
     c->pose_off.push_back(off);
     off += nposes[i];
   }
-  if (cfg->surrogate == BUDE_SURROGATE_FIELD)
+  if (cfg->surrogate != BUDE_SURROGATE_MLP)
     if (int st = make_grid(c.get(), poses)) return st;
 
   int st = plan_selection(c.get(), cfg->keep_fraction);
@@ -700,6 +870,7 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
                                      hipMemcpyHostToDevice),
                            "hipMemcpy poses");
   if (!st) st = c->pred.alloc(total * sizeof(float));
+  if (!st && cfg->surrogate == BUDE_SURROGATE_ATOM_MLP) st = prepare_atom_points(c.get());
   for (ProteinSlot &ps : c->slots) {
     if (!st) st = ps.sel_idx.alloc(total * sizeof(uint32_t));
     if (!st) st = ps.sel_n.alloc(nligands * sizeof(uint32_t));
@@ -759,6 +930,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
   const size_t L = c->nlig;
   const BudeCombinedConfig &cfg = c->cfg;
   const bool mlp = cfg.surrogate == BUDE_SURROGATE_MLP;
+  const bool atom = cfg.surrogate == BUDE_SURROGATE_ATOM_MLP;
 
   // 1. protein tables: grouped atoms and the pair coefficients of every map
   //    slot's type against every protein group (a coefficient depends on
@@ -860,8 +1032,10 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     return st;
   if (!mlp) {
     const size_t np = c->grid.points(), nslots = c->slot_type.size();
-    if (int st = c->maps.reserve(np * nslots * 4)) return st;
-    if (int st = c->cells.reserve(c->grid.cells() * nslots * 16)) return st;
+    if (!atom) {
+      if (int st = c->maps.reserve(np * nslots * 4)) return st;
+      if (int st = c->cells.reserve(c->grid.cells() * nslots * 16)) return st;
+    }
     if (!c->field_tables) {  // (the pose counts never change: built once, never rewritten while read)
       if (int st = prepare_field(c, all, c->meta_all, c->tb_all, &ft_all, &c->tb_all_host)) return st;
       c->field_tables = true;
@@ -889,18 +1063,22 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
   if (!mlp) {
     const size_t nslots = c->slot_type.size();
     if (int st = record(c, 2, c->stream2)) return st;
-    if (int st = bude::hip_check(
-            bude::field::build_maps(static_cast<const float4 *>(c->s().protein.ptr),
-                                    static_cast<const int32_t *>(c->s().group_begin.ptr),
-                                    static_cast<const bude::PairCoef *>(c->s().slot_coef.ptr), int(c->s().ngroups),
-                                    int(nslots), c->grid, bude::plain_form(), static_cast<float *>(c->maps.ptr),
-                                    c->stream2),
-            "build maps"))
-      return st;
-    if (int st = bude::hip_check(bude::field::pack_cells(static_cast<const float *>(c->maps.ptr), int(nslots),
-                                                         c->grid, static_cast<uint4 *>(c->cells.ptr), c->stream2),
-                                 "pack cells"))
-      return st;
+    if (atom) {
+      if (int st = train_atom(c, pseed)) return st;
+    } else {
+      if (int st = bude::hip_check(
+              bude::field::build_maps(static_cast<const float4 *>(c->s().protein.ptr),
+                                      static_cast<const int32_t *>(c->s().group_begin.ptr),
+                                      static_cast<const bude::PairCoef *>(c->s().slot_coef.ptr), int(c->s().ngroups),
+                                      int(nslots), c->grid, bude::plain_form(), static_cast<float *>(c->maps.ptr),
+                                      c->stream2),
+              "build maps"))
+        return st;
+      if (int st = bude::hip_check(bude::field::pack_cells(static_cast<const float *>(c->maps.ptr), int(nslots),
+                                                           c->grid, static_cast<uint4 *>(c->cells.ptr), c->stream2),
+                                   "pack cells"))
+        return st;
+    }
     if (int st = record(c, 3, c->stream2)) return st;
     // chunk by chunk: scores, top-k and rescoring tables, all on the device;
     // the scoring of chunk k > 0 runs beside the rescoring of chunk k - 1
@@ -910,8 +1088,8 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     const bool overlap = c->slots[c->cur ^ 1].busy;
     for (size_t k = 0; k < K; k++) {
       const size_t a = c->chunk_lig[k], b = c->chunk_lig[k + 1];
-      if (int st = launch_field(c, poses, c->total, pred, c->meta_all, c->tb_all, tb_all[a], tb_all[b] - tb_all[a],
-                                c->stream2, k > 0 || overlap))
+      if (int st = launch_surrogate(c, poses, c->total, pred, c->meta_all, c->tb_all, tb_all[a], tb_all[b] - tb_all[a],
+                                    c->stream2, k > 0 || overlap))
         return st;
       if (int st = bude::hip_check(hipEventRecord(c->s().chunk_ev[2 * k], c->stream2), "hipEventRecord")) return st;
       if (int st = select_and_tables(c, *c->chunk_plan[k], a, static_cast<const int32_t *>(c->rs_order.ptr) + a,
@@ -927,7 +1105,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     return st;
   if (!mlp) {  // field scores of the sampled poses, once the cells exist
     if (int st = bude::hip_check(hipStreamWaitEvent(c->stream3, c->s().ev[3], 0), "hipStreamWaitEvent")) return st;
-    if (int st = launch_field(c, lpos, nlab, lp, c->meta_lf, c->tb_label, 0, ft_label, c->stream3)) return st;
+    if (int st = launch_surrogate(c, lpos, nlab, lp, c->meta_lf, c->tb_label, 0, ft_label, c->stream3)) return st;
     if (int st = bude::hip_check(hipMemcpyAsync(lpred.data(), lp, nlab * 4, hipMemcpyDeviceToHost, c->stream3),
                                  "hipMemcpyAsync"))
       return st;
@@ -999,13 +1177,28 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     if (int st = sync(c)) return st;
   }
 
-  // 6. validation per pair on the clipped, standardised scale of that pair
+  // 6. validation per pair: RMSE on the clipped, standardised scale of that
+  //    pair, and the gate statistics in energy units (the MLP's scores are
+  //    standardised: mapped back, they are energies below the clip)
   c->s().fallback.assign(L, 0);
   c->s().val_rmse.assign(L, 0.f);
   c->s().val_rmse_cal.assign(L, 0.f);
+  c->s().gate_z.assign(L, 0.f);
+  c->s().sample_recall.assign(L, 0.f);
+  const double tail_q = mlp ? std::min(0.25, double(cfg.clip_quantile)) : 0.25;
   for (size_t l = 0; l < L; l++) {
     const size_t b = lab_off[l] + lab_train[l], e = lab_off[l + 1];
     if (e <= b) continue;
+    {
+      std::vector<double> ev, pv;
+      for (size_t i = b; i < e; i++) {
+        ev.push_back(double(le[i]));
+        pv.push_back(mlp ? double(lpred[i]) * norms[l].sigma + norms[l].mu : double(lpred[i]));
+      }
+      const Gate g = gate_stats(ev, pv, double(cfg.keep_fraction), double(cfg.gate_top_fraction), tail_q);
+      c->s().gate_z[l] = g.z;
+      c->s().sample_recall[l] = g.recall;
+    }
     double se = 0, sy = 0, sp = 0, syy = 0, spp = 0, syp = 0;
     for (size_t i = b; i < e; i++) {
       const double y = (double(std::min(le[i], norms[l].clip)) - norms[l].mu) / norms[l].sigma;
@@ -1026,7 +1219,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     // residual of the best increasing affine map of the scores onto the
     // targets: the error that matters for a ranking
     c->s().val_rmse_cal[l] = float(std::sqrt(vp > 0 && cv > 0 ? std::max(0.0, vy - cv * cv / vp) : vy));
-    c->s().fallback[l] = !(c->s().val_rmse_cal[l] <= cfg.max_val_rmse);
+    c->s().fallback[l] = !(c->s().gate_z[l] >= cfg.gate_z) || !(c->s().val_rmse_cal[l] <= cfg.max_val_rmse);
     res->fallback_pairs += uint32_t(c->s().fallback[l]);
   }
 
@@ -1221,6 +1414,8 @@ extern "C" int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePa
   r->fallback = c->s().fallback[ligand];
   r->val_rmse = c->s().val_rmse[ligand];
   r->val_rmse_calibrated = c->s().val_rmse_cal[ligand];
+  r->gate_z = c->s().gate_z[ligand];
+  r->sample_recall = c->s().sample_recall[ligand];
   return bude::ok();
 }
 
@@ -1376,8 +1571,8 @@ extern "C" int bude_mi355x_combined_recall(bude_combined *c, float top_fraction,
 
 extern "C" int bude_mi355x_combined_grid(bude_combined *c, float lo[3], float *h, int32_t n[3]) {
   if (!c || !lo || !h || !n) return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_grid: NULL argument");
-  if (c->cfg.surrogate != BUDE_SURROGATE_FIELD)
-    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_grid: not a field screen");
+  if (c->cfg.surrogate == BUDE_SURROGATE_MLP)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_grid: not a field or learned-field screen");
   for (int k = 0; k < 3; k++) {
     lo[k] = c->grid.lo[k];
     n[k] = c->grid.n[k];

@@ -153,10 +153,11 @@ extern "C" int bude_mi355x_screen_create(int device, const BudeAtom *protein, si
     return st;
   if (cfg->batch < 32 || cfg->batch % 32 || cfg->batch > (1u << 20) || !(cfg->keep_fraction > 0.f) ||
       cfg->keep_fraction > 1.f || !(cfg->lr > 0.f) || !(cfg->clip_quantile > 0.f) || cfg->clip_quantile > 1.f ||
-      cfg->tail_quantile < 0.f || cfg->tail_quantile > 1.f || !(cfg->tail_weight > 0.f))
+      cfg->tail_quantile < 0.f || cfg->tail_quantile > 1.f || !(cfg->tail_weight > 0.f) ||
+      (cfg->features != 12 && cfg->features != uint32_t(bude::mlp::kFeat)))
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
                       "bude_mi355x_screen_create: batch must be a multiple of 32 in [32, 2^20], keep_fraction and "
-                      "clip_quantile in (0, 1], tail_quantile in [0, 1], lr > 0, tail_weight > 0");
+                      "clip_quantile in (0, 1], tail_quantile in [0, 1], lr > 0, tail_weight > 0, features 12 or 21");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || device < 0 || device >= ndev)
     return bude::fail(BUDE_ERR_NO_DEVICE, "bude_mi355x_screen_create: no such HIP device");
@@ -188,7 +189,9 @@ extern "C" int bude_mi355x_screen_create(int device, const BudeAtom *protein, si
   for (auto &e : s->ev)
     if (!st) st = bude::hip_check(hipEventCreate(&e), "hipEventCreate");
   std::vector<std::vector<float>> w, b;
-  bude::surrogate::init_weights(cfg->seed, bude::mlp::kFeat, w, b);
+  // (with 12 inputs the first-layer rows of the rotation entries start at
+  // zero and, their inputs being zero in training, stay there)
+  bude::surrogate::init_weights(cfg->seed, int(cfg->features), w, b);
   if (!st) st = bude::surrogate::set_weights(s->net, w, b, s->stream);
   if (st) return st;
   *out = s.release();
@@ -250,6 +253,7 @@ extern "C" int bude_mi355x_screen_train(bude_screen *s, const float *const poses
   td.stride = n;
   td.energy = static_cast<const float *>(te.ptr);
   td.norm = bude::mlp::TargetNorm{s->clip, s->mu, 1.0f / s->sigma, s->tail};
+  td.nfeat = int(s->cfg.features);
   bude::surrogate::TrainParams tpar;
   tpar.batch = uint32_t(B);
   tpar.epochs = s->cfg.epochs;

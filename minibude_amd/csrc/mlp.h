@@ -32,6 +32,8 @@ enum Epilogue : int {
   kEpiOut = 1,     // pred[m] = acc[m][0] + b[0]
   kEpiBackGelu = 2,  // dz = acc * gelu'(zprev[m][n]); store dz (bf16) and dz^T (bf16)
   kEpiF32 = 3,     // store acc (f32), e.g. a weight gradient
+  kEpiHiddenRelu = 4,  // as kEpiHidden with a = max(z, 0)
+  kEpiBackRelu = 5,    // as kEpiBackGelu with the ReLU derivative (zprev > 0)
 };
 
 struct GemmArgs {
@@ -61,9 +63,13 @@ int auto_ksplit(int M, int N, int K);
 // Features of poses idx[i] (or i when idx is null) from 6 pose rows of
 // `stride` floats: row i of out [n][kIn] (bf16), and out_t [kIn][ldt] if set.
 // With desc [nlig][kDesc] and lig (ligand of each pose row), columns kFeat..
-// hold desc[lig[q]]; otherwise they are zero.
+// hold desc[lig[q]]; otherwise they are zero.  Feature columns from nfeat
+// up are zero (nfeat = 12: the six pose parameters and the sines and
+// cosines of the angles only; a network whose first-layer rows for the
+// other inputs start at zero then keeps them at zero).
 hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t,
-                    int ldt, hipStream_t s, const float *desc = nullptr, const uint32_t *lig = nullptr);
+                    int ldt, hipStream_t s, const float *desc = nullptr, const uint32_t *lig = nullptr,
+                    int nfeat = kFeat);
 
 // Weighted MSE on the transformed target of pose q = idx[b] (or b):
 // y = (min(E, clip) - mu) * inv_sigma with the TargetNorm table[lig[q]]

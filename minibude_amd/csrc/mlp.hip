@@ -125,13 +125,13 @@ __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
       for (int q = 0; q < 4; q++) {
         const size_t o = size_t(mb + q) * g.ldo + n;
         const float c = acc[i][j][q];
-        if (g.epi == kEpiHidden) {
+        if (g.epi == kEpiHidden || g.epi == kEpiHiddenRelu) {
           const float z = c + g.bias[n];
           if (g.out_f32) g.out_f32[o] = z;
-          t[q] = to_bf16(gelu(z));
+          t[q] = to_bf16(g.epi == kEpiHidden ? gelu(z) : fmaxf(z, 0.f));
           g.out_bf16[o] = t[q];
-        } else if (g.epi == kEpiBackGelu) {
-          t[q] = to_bf16(c * gelu_grad(g.zprev[o]));
+        } else if (g.epi == kEpiBackGelu || g.epi == kEpiBackRelu) {
+          t[q] = to_bf16(g.epi == kEpiBackGelu ? c * gelu_grad(g.zprev[o]) : (g.zprev[o] > 0.f ? c : 0.f));
           g.out_bf16[o] = t[q];
         } else if (g.ksplit > 1) {
           g.work[size_t(blockIdx.z) * size_t(g.M) * g.ldo + o] = c;
@@ -159,12 +159,15 @@ __global__ void ksplit_reduce_kernel(const float *__restrict__ work, size_t n, i
 
 __global__ void features_kernel(const float *__restrict__ poses, size_t stride, const uint32_t *__restrict__ idx,
                                 size_t n, bf16 *__restrict__ out, bf16 *__restrict__ out_t, int ldt,
-                                const float *__restrict__ desc, const uint32_t *__restrict__ lig) {
+                                const float *__restrict__ desc, const uint32_t *__restrict__ lig, int nfeat) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const size_t q = idx ? idx[i] : i;
   float f[kIn];
   pose_features(poses, stride, q, f);
+#pragma unroll
+  for (int c = 0; c < kFeat; c++)
+    if (c >= nfeat) f[c] = 0.f;
   if (desc) {
     const float *d = desc + size_t(lig[q]) * kDesc;
 #pragma unroll
@@ -458,11 +461,11 @@ int auto_ksplit(int M, int N, int K) {
 }
 
 hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t, int ldt,
-                    hipStream_t s, const float *desc, const uint32_t *lig) {
+                    hipStream_t s, const float *desc, const uint32_t *lig, int nfeat) {
   if (n == 0) return hipSuccess;
-  if (desc && !lig) return hipErrorInvalidValue;
+  if ((desc && !lig) || nfeat < 0 || nfeat > kFeat) return hipErrorInvalidValue;
   hipLaunchKernelGGL(features_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, poses, stride, idx, n, out, out_t,
-                     ldt, desc, lig);
+                     ldt, desc, lig, nfeat);
   return hipGetLastError();
 }
 

@@ -56,6 +56,7 @@ struct TrainData {
   const mlp::TargetNorm *table = nullptr;
   const uint32_t *lig = nullptr;
   const float *desc = nullptr;
+  int nfeat = mlp::kFeat;  // pose inputs used (the rest zero): 12 or 21
 };
 
 struct TrainParams {

@@ -202,7 +202,7 @@ int train(Net &net, const TrainData &d, std::vector<uint32_t> perm, const TrainP
     for (size_t it = 0; it < steps_per_epoch; it++, step++) {
       const uint32_t *idx = static_cast<const uint32_t *>(dperm.ptr) + it * B;
       if (int st = hip_check(mlp::features(d.poses, d.stride, idx, B, static_cast<bf16 *>(A.feat.ptr),
-                                           static_cast<bf16 *>(A.feat_t.ptr), int(B), s, d.desc, d.lig),
+                                           static_cast<bf16 *>(A.feat_t.ptr), int(B), s, d.desc, d.lig, d.nfeat),
                              "features"))
         return st;
       if (int st = forward(net, A, B, true, s)) return st;

@@ -18,7 +18,10 @@ _c = ctypes
 class ScreenConfig(_c.Structure):
     _fields_ = [("batch", _c.c_uint32), ("epochs", _c.c_uint32), ("lr", _c.c_float), ("clip_quantile", _c.c_float),
                 ("keep_fraction", _c.c_float), ("max_val_rmse", _c.c_float), ("tail_quantile", _c.c_float),
-                ("tail_weight", _c.c_float), ("seed", _c.c_uint64)]
+                ("tail_weight", _c.c_float), ("seed", _c.c_uint64), ("features", _c.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class TrainStats(_c.Structure):
@@ -41,7 +44,7 @@ class ScreenResult(_c.Structure):
 
 def default_config(**kw) -> ScreenConfig:
     c = dict(batch=8192, epochs=60, lr=4e-3, clip_quantile=0.1, keep_fraction=0.02, max_val_rmse=0.8,
-             tail_quantile=0.05, tail_weight=8.0, seed=0)
+             tail_quantile=0.05, tail_weight=8.0, seed=0, features=21)
     c.update(kw)
     return ScreenConfig(**c)
 

@@ -1,21 +1,46 @@
 """Host side of the combined screen: C-struct layouts, the workload
 generator, and argument checks that run before any device work."""
 import ctypes
+import subprocess
+import tempfile
+from pathlib import Path
 
 import numpy as np
 import pytest
 
+from conftest import ROOT
+
 from minibude_amd import capi
 from minibude_amd import combined as C
 from minibude_amd import deck as D
+from minibude_amd import screen as S
 
 
-def test_struct_layouts_match_the_header():
-    assert ctypes.sizeof(C.CombinedConfig) == 56
-    # pipeline_chunks sits in what was the padding before the 8-byte seed
-    assert C.CombinedConfig.pipeline_chunks.offset == 44 and C.CombinedConfig.seed.offset == 48
-    assert ctypes.sizeof(C.CombinedResult) == 64
-    assert ctypes.sizeof(C.PairResult) == 20
+def c_layout(struct: str, fields) -> dict:
+    """sizeof and offsetof of a header struct, from the C compiler."""
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"bude_mi355x.h\"\nint main(void) {\n"
+    src += f'  printf("size %zu\\n", sizeof({struct}));\n'
+    for f in fields:
+        src += f'  printf("{f} %zu\\n", offsetof({struct}, {f}));\n'
+    src += "  return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        (Path(d) / "l.c").write_text(src)
+        subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(Path(d) / "l"), str(Path(d) / "l.c")],
+                       check=True)
+        out = subprocess.run([str(Path(d) / "l")], capture_output=True, text=True, check=True).stdout
+    return {k: int(v) for k, v in (ln.split() for ln in out.splitlines())}
+
+
+@pytest.mark.parametrize("cls,struct", [(C.CombinedConfig, "BudeCombinedConfig"),
+                                        (C.CombinedResult, "BudeCombinedResult"),
+                                        (C.PairResult, "BudePairResult"),
+                                        (S.ScreenConfig, "BudeScreenConfig")])
+def test_struct_layouts_match_the_header(cls, struct):
+    names = [f for f, _ in cls._fields_]
+    want = c_layout(struct, names)
+    assert ctypes.sizeof(cls) == want["size"]
+    for f in names:
+        assert getattr(cls, f).offset == want[f], f
 
 
 def test_workload_is_seeded_and_in_range():

@@ -1,0 +1,151 @@
+"""The learned per-atom field on the GPU: its labels are the plain kernel's
+one-atom energies bit for bit, its training kernels' gradients and its fused
+scorer agree with the CPU oracle, and in the combined screen it keeps the
+exact top 1 % of every pair it does not fall back on, where the same screen
+without training falls back on every pair."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from minibude_amd import atomnet as AN
+from minibude_amd import combined as C
+from minibude_amd import deck as D
+from minibude_amd.model import Context
+
+sys.path.insert(0, str(ROOT / "oracle"))
+import atomnet_oracle as A  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+F, H = AN.FEATURES, AN.HIDDEN
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def random_net(nslots, n, seed):
+    rng = np.random.default_rng(seed)
+    k0 = AN.input_width(nslots)
+    grid = rng.normal(0, 0.5, (int(np.prod(n)), F))
+    W = [np.zeros((k0, H)), rng.normal(0, 0.3, (H, H)), rng.normal(0, 0.3, (H, 1))]
+    W[0][:F + nslots] = rng.normal(0, 0.4, (F + nslots, H))
+    b = [rng.normal(0, 0.1, H), rng.normal(0, 0.1, H), rng.normal(0, 0.1, 1)]
+    return grid, W, b, k0
+
+
+def test_labels_are_plain_kernel_one_atom_energies(gpu_device):
+    ff = D.synth_forcefield(31, 34)
+    protein = D.synth_protein(31, 1500, 34)
+    rng = np.random.default_rng(0)
+    n = 700  # not a multiple of 64, several types
+    pts = rng.uniform(-15, 15, (n, 3)).astype(np.float32)
+    types = rng.integers(0, 34, n).astype(np.int32)
+    got = AN.atom_energies(protein, ff, pts, types, gpu_device)
+    for t in np.unique(types)[:6]:
+        sel = np.nonzero(types == t)[0]
+        poses = np.zeros((6, len(sel)), np.float32)
+        poses[3:] = pts[sel].T
+        atom = np.array([(0.0, 0.0, 0.0, t)], dtype=protein.dtype)
+        with Context(D.Deck(protein, atom, ff, poses), gpu_device) as c:
+            want = c.fasten(1, 64).energies
+        assert np.array_equal(got[sel].view(np.uint32), want.view(np.uint32)), t
+
+
+def test_forward_backward_matches_oracle(gpu_device):
+    n, lo, h, nslots = (6, 5, 7), np.array([-3.0, -2.5, -3.5]), 1.0, 5
+    grid, W, b, k0 = random_net(nslots, n, seed=3)
+    rng = np.random.default_rng(4)
+    B = 256
+    pts = lo + rng.uniform(0, 1, (B, 3)) * (np.array(n) - 1) * h
+    slots = rng.integers(0, nslots, B)
+    target = rng.normal(size=B)
+    pred, loss, gG, gW, gb = AN.forward_backward(pts, slots, target, lo, h, n, grid, nslots, W, b, gpu_device)
+    rp, rl, rgG, rgW, rgb = A.forward_backward(pts.astype(np.float32).astype(np.float64), slots, target,
+                                               grid.astype(np.float32), lo, h, n, [w.astype(np.float32) for w in W],
+                                               [x.astype(np.float32) for x in b], k0, bf16=True)
+    # BF16 storage points are emulated by the oracle; the rest is FP32 (GPU)
+    # vs FP64 accumulation, plus rare one-ulp BF16 flips; the grid gradient
+    # passes through 2^-40 fixed point
+    assert rel_err(pred, rp) < 2e-3
+    assert loss == pytest.approx(rl, rel=5e-3)
+    assert rel_err(gG, rgG) < 2e-2
+    for l in range(3):
+        assert rel_err(gW[l], rgW[l]) < 2e-2, f"dW{l}"
+        assert rel_err(gb[l], rgb[l]) < 2e-2, f"db{l}"
+    # the FP64 network without BF16 rounding is close too
+    fp = A.forward_backward(pts, slots, target, grid, lo, h, n, W, b, k0, bf16=False)
+    assert rel_err(pred, fp[0]) < 3e-2
+
+
+def test_fused_scorer_matches_oracle(gpu_device):
+    n, lo, h, nslots = (12, 12, 12), np.full(3, -11.0), 2.0, 4
+    grid, W, b, k0 = random_net(nslots, n, seed=5)
+    lig = D.synth_ligand(6, 23, nslots, radius=5.0)
+    poses = D.synth_poses(7, 1000, max_trans=4.0)  # ragged: 15 full tiles and a partial one
+    got = AN.score(lig, poses, lo, h, n, grid, nslots, W, b, sigma=2.5, mu=-0.75, device=gpu_device)
+    xyz = np.stack([lig["x"], lig["y"], lig["z"]], 1).astype(np.float64)
+    want = A.score_poses(xyz, lig["type"], poses, grid.astype(np.float32), lo, h, n,
+                         [w.astype(np.float32) for w in W], [x.astype(np.float32) for x in b], k0, 2.5, -0.75)
+    # FP32 interpolation and sums against FP64: where the two land on
+    # opposite sides of a BF16 rounding boundary a feature or activation
+    # differs by one BF16 ulp (2^-8 relative), so a pose's sum differs by
+    # about 1e-3 of its size, without bias
+    err = got - want
+    assert np.max(np.abs(err)) < 2e-3 * np.max(np.abs(want))
+    assert abs(np.mean(err)) < 2e-4 * np.max(np.abs(want))
+    assert np.corrcoef(got, want)[0, 1] > 0.9999
+
+
+def c4_like(nproteins=2, nligands=12, nposes=65536, seed=21):
+    # the combined workload's distributions (proteins of 900-6000 atoms,
+    # ligands of 8-96) at a size a test screens in seconds
+    return C.combined_workload(nproteins=nproteins, nligands=nligands, nposes=nposes, seed=seed)
+
+
+def test_learned_field_keeps_the_top_percent_of_every_trusted_pair(gpu_device):
+    w = c4_like()
+    cfg = C.default_config("mlp")
+    with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
+        trusted, recalls, fallbacks = [], [], 0
+        for p, protein in enumerate(w.proteins):
+            r = c.screen(protein)
+            c.exhaustive()
+            rec = c.recall(0.01)
+            pairs = [c.pair(l) for l in range(len(w.ligands))]
+            fallbacks += r.fallback_pairs
+            print(f"protein {p} ({len(protein)} atoms): one-atom rmse {r.surrogate_rmse:.2f}, fallbacks "
+                  f"{r.fallback_pairs}, recall {np.round(rec, 4).tolist()}, gate z "
+                  f"{[round(x.gate_z, 2) for x in pairs]}", flush=True)
+            for l, pr in enumerate(pairs):
+                recalls.append(rec[l])
+                if not pr.fallback:
+                    trusted.append(rec[l])
+    # every pair the gate trusts keeps >= 99 % of its exact top 1 % in its
+    # lowest 2 %; fallback pairs are exhaustive (recall 1)
+    assert min(trusted) >= 0.99, sorted(trusted)[:5]
+    assert min(recalls) >= 0.99
+    assert fallbacks <= 0.1 * len(recalls)
+
+
+def test_untrained_field_falls_back_everywhere(gpu_device):
+    # the same screen without training (one step at a vanishing learning
+    # rate): the gate finds no room for its error on any pair, so every pair
+    # is scored exhaustively -- what the trained network buys is the 98 % of
+    # each pair's poses it leaves unscored
+    w = c4_like(nproteins=1, nligands=6, nposes=32768, seed=22)
+    out = {}
+    for name, kw in (("trained", {}), ("untrained", dict(steps=1, lr=1e-30, grid_lr=1e-30))):
+        with C.Combined(w.forcefield, w.ligands, w.poses, C.default_config("mlp", **kw), gpu_device) as c:
+            r = c.screen(w.proteins[0])
+            c.exhaustive()
+            z = [c.pair(l).gate_z for l in range(6)]
+            rs = [c.pair(l).sample_recall for l in range(6)]
+            out[name] = (r.fallback_pairs, np.array(z), c.recall(0.01), np.array(rs))
+        print(f"{name}: fallbacks {out[name][0]}, gate z {np.round(out[name][1], 2)}, sample recall "
+              f"{out[name][3]}, recall {out[name][2]}", flush=True)
+    assert out["untrained"][0] == 6 and np.all(out["untrained"][2] == 1.0)
+    assert out["trained"][0] <= 1
+    assert np.all(out["trained"][1] > 4 * np.maximum(out["untrained"][1], 0.1))

@@ -55,7 +55,7 @@ def trilinear(maps, lo, h, n, pos):
 
 def test_field_maps_are_single_atom_plain_energies(gpu_device):
     w = small_workload(nposes=2048, nligands=3, nproteins=1)
-    cfg = C.default_config("field", grid_spacing=1.0, max_val_rmse=1e9)
+    cfg = C.default_config("field", grid_spacing=1.0, max_val_rmse=1e9, gate_z=0.0)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
         c.screen(w.proteins[0])
         lo, h, n = c.grid()
@@ -74,7 +74,7 @@ def test_field_maps_are_single_atom_plain_energies(gpu_device):
 
 def test_field_score_is_trilinear_sum_over_atoms(gpu_device):
     w = small_workload(nposes=4096, nligands=3, nproteins=1)
-    cfg = C.default_config("field", grid_spacing=0.5, max_val_rmse=1e9)
+    cfg = C.default_config("field", grid_spacing=0.5, max_val_rmse=1e9, gate_z=0.0)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
         c.screen(w.proteins[0])
         lo, h, n = c.grid()
@@ -114,7 +114,7 @@ def test_field_is_exact_when_atoms_sit_on_grid_points(gpu_device):
     n = 1024
     poses = np.zeros((6, n), np.float32)
     poses[3:] = rng.integers(-10, 11, size=(3, n)) * 0.5
-    cfg = C.default_config("field", grid_spacing=0.5, max_val_rmse=1e9, labels_per_pair=64)
+    cfg = C.default_config("field", grid_spacing=0.5, max_val_rmse=1e9, gate_z=0.0, labels_per_pair=64)
     with C.Combined(ff, [lig], [poses], cfg, gpu_device) as c:
         c.screen(protein)
         got = c.scores(0)
@@ -138,12 +138,16 @@ def test_field_is_exact_when_atoms_sit_on_grid_points(gpu_device):
     assert np.max(np.abs(got - exact) / mag) < 2.0 ** -11 + 1e-5
 
 
-@pytest.mark.parametrize("surrogate", ["field", "mlp"])
+# small training runs of the learned kinds (the mechanics, not the accuracy)
+QUICK = {"mlp": dict(train_points=65536, batch=4096, steps=100),
+         "pose_mlp": dict(labels_per_pair=2048, batch=1024, epochs=4)}
+
+
+@pytest.mark.parametrize("surrogate", ["field", "mlp", "pose_mlp"])
 def test_selection_rescoring_and_recall(gpu_device, surrogate):
     w = small_workload()
-    kw = dict(keep_fraction=0.05, max_val_rmse=1e9)
-    if surrogate == "mlp":
-        kw.update(labels_per_pair=2048, batch=1024, epochs=4)
+    kw = dict(keep_fraction=0.05, max_val_rmse=1e9, gate_z=0.0)
+    kw.update(QUICK.get(surrogate, {}))
     cfg = C.default_config(surrogate, **kw)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
         for p, protein in enumerate(w.proteins):
@@ -169,16 +173,16 @@ def test_selection_rescoring_and_recall(gpu_device, surrogate):
                 assert np.array_equal(c.exact(0).view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("surrogate", ["field", "mlp"])
+@pytest.mark.parametrize("surrogate", ["field", "mlp", "pose_mlp"])
 def test_sampled_pose_energies(gpu_device, surrogate):
     # the sampled-pose pass runs the split kernel (four lanes per pose): its
     # energies agree with the tiled kernel's to rounding and with the FP64
     # oracle to 16 eps of the cancellation magnitude
     w = C.combined_workload(nproteins=1, nligands=6, nposes=3000, seed=13, natpro_range=(300, 900),
                             natlig_range=(1, 30))
-    kw = dict(labels_per_pair=700, max_val_rmse=1e9)
-    if surrogate == "mlp":
-        kw.update(batch=256, epochs=1)
+    kw = dict(labels_per_pair=700, max_val_rmse=1e9, gate_z=0.0)
+    kw.update({"mlp": dict(train_points=65536, batch=4096, steps=10),
+               "pose_mlp": dict(batch=256, epochs=1)}.get(surrogate, {}))
     eps = 2.0 ** -24
     with C.Combined(w.forcefield, w.ligands, w.poses, C.default_config(surrogate, **kw), gpu_device) as c:
         c.screen(w.proteins[0])
@@ -199,7 +203,7 @@ def test_sampled_pose_energies(gpu_device, surrogate):
 
 def test_set_keep_rescreens_with_the_new_fraction(gpu_device):
     w = small_workload(nposes=4096, nligands=3, nproteins=2)
-    cfg = C.default_config("field", max_val_rmse=1e9)
+    cfg = C.default_config("field", max_val_rmse=1e9, gate_z=0.0)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
         c.screen(w.proteins[0])
         c.exhaustive()
@@ -245,14 +249,14 @@ def test_mixed_fallback_and_ragged_edges(gpu_device):
     # keep fraction of 1 on the same ligand set
     w = small_workload(nposes=3000, nligands=5, nproteins=1, seed=14)
     poses = [w.poses[0][:, :1], w.poses[1][:, :63], w.poses[2][:, :64], w.poses[3][:, :65], w.poses[4]]
-    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48, keep_fraction=0.05)
+    cfg = C.default_config("field", max_val_rmse=1e9, gate_z=0.0, labels_per_pair=48, keep_fraction=0.05)
     with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
         c.screen(w.proteins[0])
         cal = np.array([c.pair(l).val_rmse_calibrated for l in range(5)])
     thr = float(np.sort(cal)[2])  # the two largest fall back
     for keep in (0.05, 1.0):
         # a fresh object draws the same samples, so the same pairs fall back
-        cfg = C.default_config("field", max_val_rmse=thr, labels_per_pair=48, keep_fraction=keep)
+        cfg = C.default_config("field", max_val_rmse=thr, gate_z=0.0, labels_per_pair=48, keep_fraction=keep)
         with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
             r = c.screen(w.proteins[0])
             c.exhaustive()
@@ -296,9 +300,9 @@ def test_errors_are_reported_not_faked(gpu_device):
             c.map(unused)
         with pytest.raises(ValueError):
             c.pair(2)
-    cfg = C.default_config("mlp", labels_per_pair=64, batch=64, epochs=1)
+    cfg = C.default_config("pose_mlp", labels_per_pair=64, batch=64, epochs=1)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
-        with pytest.raises(ValueError, match="not a field screen"):
+        with pytest.raises(ValueError, match="not a field or learned-field screen"):
             c.grid()
 
 
@@ -312,21 +316,24 @@ def test_field_screen_recall(gpu_device):
         rec = c.recall(0.01)
     print(f"field recall@1% (2% kept): mean {rec.mean():.4f} min {rec.min():.4f}; {r.as_dict()}", flush=True)
     assert r.fallback_pairs == 0
-    assert rec.min() >= 0.97 and rec.mean() >= 0.99
+    assert rec.min() >= 0.99
 
 
-def test_mlp_combined_learns(gpu_device):
-    # the per-protein MLP against the same screen untrained (one step at a
-    # vanishing learning rate): training has to buy recall
+def test_pose_mlp_combined_learns(gpu_device):
+    # the per-protein pose MLP against the same screen untrained (one step at
+    # a vanishing learning rate): training has to buy recall (its recall stays
+    # far below what a screen needs: the gate sends such pairs to exhaustive
+    # scoring; the learned per-atom field is the learned surrogate that
+    # passes, tests/test_gpu_atomnet.py)
     w = C.combined_workload(nproteins=1, nligands=4, nposes=32768, seed=13, natpro_range=(1500, 2500))
     out = {}
     for name, kw in (("trained", dict(epochs=12, lr=4e-3)), ("untrained", dict(epochs=1, lr=1e-30))):
-        cfg = C.default_config("mlp", labels_per_pair=8192, batch=1024, max_val_rmse=1e9, **kw)
+        cfg = C.default_config("pose_mlp", labels_per_pair=8192, batch=1024, max_val_rmse=1e9, gate_z=0.0, **kw)
         with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
             c.screen(w.proteins[0])
             c.exhaustive()
             out[name] = (c.recall(0.01), np.array([c.pair(l).val_rmse_calibrated for l in range(4)]))
-        print(f"mlp {name}: recall {out[name][0]}, calibrated val_rmse {out[name][1]}", flush=True)
+        print(f"pose mlp {name}: recall {out[name][0]}, calibrated val_rmse {out[name][1]}", flush=True)
     assert np.all(out["trained"][1] < out["untrained"][1])
     assert out["trained"][0].mean() > max(0.15, 3 * out["untrained"][0].mean())
 
@@ -342,7 +349,7 @@ def test_field_pipeline_chunks_give_the_same_screen(gpu_device, keep):
              w.poses[5][:, :64], w.poses[6]]
     runs = {}
     for chunks in (1, 2, 5, 16):
-        cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48, keep_fraction=keep,
+        cfg = C.default_config("field", max_val_rmse=1e9, gate_z=0.0, labels_per_pair=48, keep_fraction=keep,
                                pipeline_chunks=chunks)
         with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
             out = []
@@ -371,11 +378,12 @@ def test_field_pipeline_with_fallback_pairs(gpu_device):
     # a pair that falls back switches the screen to one selection over the
     # whole set after the chunked scoring; the other pairs are still rescored
     w = small_workload(nposes=2000, nligands=6, nproteins=1, seed=16)
-    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48, pipeline_chunks=3)
+    cfg = C.default_config("field", max_val_rmse=1e9, gate_z=0.0, labels_per_pair=48, pipeline_chunks=3)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
         c.screen(w.proteins[0])
         cal = np.array([c.pair(l).val_rmse_calibrated for l in range(6)])
-    cfg = C.default_config("field", max_val_rmse=float(np.sort(cal)[3]), labels_per_pair=48, pipeline_chunks=3)
+    cfg = C.default_config("field", max_val_rmse=float(np.sort(cal)[3]), gate_z=0.0, labels_per_pair=48,
+                           pipeline_chunks=3)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
         r = c.screen(w.proteins[0])
         c.exhaustive()
@@ -399,7 +407,7 @@ def test_tied_scores_are_all_rescored(gpu_device, chunks):
     # rescoring tables must cover every kept pose
     w = small_workload(nposes=1200, nligands=4, nproteins=1, seed=17)
     poses = [np.ascontiguousarray(np.repeat(q[:, :400], 3, axis=1)) for q in w.poses]
-    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48, keep_fraction=0.0085,
+    cfg = C.default_config("field", max_val_rmse=1e9, gate_z=0.0, labels_per_pair=48, keep_fraction=0.0085,
                            pipeline_chunks=chunks)
     with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
         r = c.screen(w.proteins[0])
@@ -422,7 +430,7 @@ def test_automatic_chunking_follows_set_keep(gpu_device):
     # 256-CU device (one chunk at 2 %); the chunk tables are rebuilt by
     # set_keep and the screens agree with the exhaustive energies either way
     w = small_workload(nposes=262144, nligands=8, nproteins=1, seed=18)
-    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=64, keep_fraction=1.0)
+    cfg = C.default_config("field", max_val_rmse=1e9, gate_z=0.0, labels_per_pair=64, keep_fraction=1.0)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
         for keep in (1.0, 0.02, 1.0):
             c.set_keep(keep)
@@ -441,7 +449,7 @@ def test_automatic_chunking_follows_set_keep(gpu_device):
             assert r.nselected == total
 
 
-@pytest.mark.parametrize("kind", ["field1", "field3", "fallback", "mlp"])
+@pytest.mark.parametrize("kind", ["field1", "field3", "fallback", "mlp", "pose_mlp"])
 def test_batch_equals_one_by_one(gpu_device, kind):
     # screen_batch overlaps each protein's scoring with the previous one's
     # rescoring (or, with fallback pairs or the MLP, runs them one after
@@ -451,8 +459,12 @@ def test_batch_equals_one_by_one(gpu_device, kind):
     w = small_workload(nposes=3000, nligands=5, nproteins=4, seed=19)
     L = len(w.ligands)
     kw = dict(labels_per_pair=48)
-    if kind == "mlp":
-        cfg = lambda: C.default_config("mlp", labels_per_pair=256, batch=64, epochs=1, max_val_rmse=1e9)  # noqa: E731
+    if kind == "pose_mlp":
+        cfg = lambda: C.default_config("pose_mlp", labels_per_pair=256, batch=64, epochs=1, max_val_rmse=1e9,  # noqa: E731
+                                       gate_z=0.0)
+    elif kind == "mlp":
+        cfg = lambda: C.default_config("mlp", labels_per_pair=256, train_points=65536, batch=4096,  # noqa: E731
+                                       steps=50, gate_z=0.0)
     else:
         kw.update(max_val_rmse=0.0 if kind == "fallback" else 1e9, pipeline_chunks=3 if kind == "field3" else 1)
         cfg = lambda: C.default_config("field", **kw)  # noqa: E731
@@ -502,7 +514,7 @@ def test_batch_edges_and_errors(gpu_device):
     # malformed: the error is reported, nothing stays in flight, and the
     # object screens normally afterwards
     w = small_workload(nposes=2000, nligands=3, nproteins=2, seed=20)
-    cfg = C.default_config("field", max_val_rmse=1e9, labels_per_pair=48)
+    cfg = C.default_config("field", max_val_rmse=1e9, gate_z=0.0, labels_per_pair=48)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
         assert c.screen_batch([])[0] == []
         seen = []
