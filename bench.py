@@ -14,29 +14,42 @@ in HBM.  Prints ONE JSON line:
   roofline      FP32 VALU roofline of the kernel: the reference's FLOP
                 accounting (src/main.cpp:358-360) per launch / average
                 hipEvent duration of the launch on the library's stream, vs
-                the MI355X FP32 vector peak (157.3 TFLOP/s); `traffic` comes
-                from the committed rocprofv3 --pmc summary when present
+                the MI355X FP32 vector peak (157.3 TFLOP/s); beside it the
+                same launch at 30 FLOP per interaction and at the FLOP the
+                hardware counters record; `traffic` is the committed
+                rocprofv3 --pmc summary's (named in `traffic_source`)
   cpu_baseline  the reference's own omp model (src/main.cpp + src/omp/,
                 built from the reference sources into oracle/_ref) on the
-                host cores, on the first 65,536 poses of the same deck
+                host cores: the CPU workload as configured (deck c0, ppwi 1,
+                8 iterations), the same deck at ppwi 64, and the first
+                65,536 poses of c1
   valid         energies of the last timed step against the reference
                 energies of the deck: the reference metric (max_diff_% <
                 0.025) on every pose whose FP32 energy the reference itself
                 resolves (|E| >= 0.002 * M, M = 0.5 * sum |term| from the
                 committed FP64 fixture), and |E - E_fp64| / M within 16 FP32
                 epsilons on every pose (the reference's own FP32: 8.8)
+  sweep         the 40-point ppwi x wgsize sweep of the plain kernel
   batched       the ragged screen (256 ligands of 8-96 atoms, 4,096-65,536
                 poses each, one 4,096-atom protein) in one persistent
-                hipGraph-replayed launch vs 256 separate plain launches
-  screen        the learned prescreen (surrogate trained on 8M plain-kernel
-                energies, 16.8M fresh poses screened, lowest 2 % rescored
-                exactly) vs exhaustive plain scoring: poses/s and top-1 %
-                recall, at 2 % rescored and at the fraction reaching 0.99
-  combined      16 proteins x 64 ligands x 262,144 poses per pair (512 x
-                1,048,576 with --combined-full): both surrogates (per-protein
-                MLP, single-atom energy maps), per-pair lowest 2 % rescored in
-                one ragged batched launch per protein, vs exhaustive batched
-                scoring of every pair: poses/s and per-pair top-1 % recall
+                hipGraph-replayed launch at the launch shape the batched
+                autotuner picks on it, vs 256 separate plain launches
+  screen        the learned prescreen (4,096-atom protein, 26-atom ligand,
+                16.8M fresh poses, lowest 2 % rescored exactly): the pose MLP
+                with the 12 configured inputs and with 21, each trained on
+                8.4M plain-kernel energies (poses/s and top-1 % recall at
+                2 % and at the fraction reaching 0.99, end-to-end time with
+                labels and training, break-even pose count), then the
+                learned per-atom field and the exact single-atom maps on the
+                same pair
+  combined      16 proteins x 512 ligands x 1,048,576 poses per pair
+                (16 x 64 x 262,144 with --combined-reduced): the exact
+                single-atom maps and the learned per-atom field, per-pair
+                lowest 2 % rescored in one ragged batched launch per
+                protein, all proteins in one batch call, vs exhaustive
+                batched scoring (measured on the first protein, extrapolated
+                by interactions): poses/s, fallbacks and per-pair top-1 %
+                recall of the first protein's 512 pairs
 """
 from __future__ import annotations
 
@@ -472,10 +485,11 @@ def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_p
                                 f"({ex_inter:.3e} interactions), extrapolated to all by interactions"),
         }
         if kind == "mlp":
-            res["surrogate"] = ("learned per-atom field: per protein a learned 1 A grid of 8 features, trilinear, "
-                                "then a 2 x 32 ReLU MLP with the atom type one-hot (BF16 MFMA), summed over the "
-                                "ligand's atoms; trained by this library's kernels on 2^21 exact one-atom "
-                                "energies (the plain kernel's pair loop) for 2,000 Adam steps of 16,384")
+            res["surrogate"] = (f"learned per-atom field: per protein a learned {cfg.grid_spacing:g} A grid of 8 "
+                                "features, trilinear, then a 2 x 32 ReLU MLP with the atom type one-hot (BF16 "
+                                "MFMA), summed over the ligand's atoms; trained by this library's kernels on "
+                                f"{cfg.train_points} exact one-atom energies (the plain kernel's pair loop) for "
+                                f"{cfg.steps} Adam steps of {cfg.batch}")
             res["one_atom_rmse_energy_units"] = [round(x, 3) for x in rmse]
         log(f"combined {kind}: {res}")
         out[kind] = res

@@ -64,18 +64,19 @@ class PairResult(_c.Structure):
 
 def default_config(surrogate: str = "field", **kw) -> CombinedConfig:
     """Defaults per surrogate: the learned per-atom field ("mlp") trains for
-    2,000 steps of 16,384 on 2^21 exact one-atom energies with a 1 A feature
-    grid and validates on 4,096 sampled poses per pair; the field's maps are
-    0.5 A and validate on 1,024; the pose MLP trains on 16,384 sampled poses
-    per pair (1/16 held out).  Every kind falls back where its validation gap
-    is under 2 standard deviations of its tail error (top 1 %, keep 2 %: an
-    expected loss of 0.4 % of the top 1 %); the pose MLP also where its
-    calibrated validation RMSE exceeds 0.8."""
+    3,000 Adam steps of 32,768 on 2^22 exact one-atom energies with a 0.75 A
+    feature grid and validates on 4,096 sampled poses per pair; the field's
+    maps are 0.5 A and validate on 1,024; the pose MLP trains on 16,384
+    sampled poses per pair (1/16 held out).  Every kind falls back where its
+    validation gap is under 2 standard deviations of its tail error (top 1 %,
+    keep 2 %: an expected loss of 0.4 % of the top 1 %); the pose MLP also
+    where its calibrated validation RMSE exceeds 0.8."""
+    mlp = surrogate == "mlp"
     c = dict(surrogate=SURROGATES[surrogate], labels_per_pair={"pose_mlp": 16384, "mlp": 4096}.get(surrogate, 1024),
-             keep_fraction=0.02, max_val_rmse=0.8 if surrogate == "pose_mlp" else float("inf"), clip_quantile=0.1, tail_quantile=0.05, tail_weight=8.0,
-             batch=16384 if surrogate == "mlp" else 8192, epochs=20, lr=2e-3 if surrogate == "mlp" else 4e-3,
-             grid_spacing={"mlp": 1.0}.get(surrogate, 0.5), pipeline_chunks=0, seed=0, train_points=1 << 21,
-             steps=2000, grid_lr=1e-2, gate_z=2.0, gate_top_fraction=0.01)
+             keep_fraction=0.02, max_val_rmse=0.8 if surrogate == "pose_mlp" else float("inf"), clip_quantile=0.1,
+             tail_quantile=0.05, tail_weight=8.0, batch=32768 if mlp else 8192, epochs=20, lr=3e-3 if mlp else 4e-3,
+             grid_spacing=0.75 if mlp else 0.5, pipeline_chunks=0, seed=0, train_points=(1 << 22) if mlp else 1 << 21,
+             steps=3000, grid_lr=1.5e-2, gate_z=2.0, gate_top_fraction=0.01)
     c.update(kw)
     return CombinedConfig(**c)
 

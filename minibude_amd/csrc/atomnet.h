@@ -20,7 +20,7 @@
 // trained by Adam (grid and weights) on exact one-atom energies from the
 // plain kernel's pair loop at positions the screened poses put ligand atoms
 // at.  Inference folds the one-hot row of W0 into a per-slot first-layer
-// bias and reads a BF16 copy of the grid.
+// bias and reads the grid as FP16 pairs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,13 +49,18 @@ struct Net {
   field::Grid grid{};
   int nslots = 0, k0 = 0;
   size_t nodes = 0;
-  // grid features: FP32 master [nodes][kF], BF16 copy, Adam moments, int64
-  // fixed-point gradient accumulators (cleared by the Adam step)
-  DeviceBuffer g, g16, gm, gv, gacc;
+  // grid features: FP32 master [nodes][kF], Adam moments, int64 fixed-point
+  // gradient accumulators (cleared by the Adam step); for inference the
+  // z-pair table gzp [n0][n1][n2 - 1][kF] of FP16 pairs (feature k of node
+  // (i, j, l) and of node (i, j, l + 1)), one packed dot product per pair
+  DeviceBuffer g, gm, gv, gacc, gzp;
   // layers l = 0..2: W_l [K_l][N_l] (FP32 master, Adam moments, gradient,
   // BF16 row-major and transposed copies), b_l [N_l]
   DeviceBuffer w[3], b[3], mw[3], vw[3], mb[3], vb[3], gw[3], gb[3], w16[3], w16t[3];
-  DeviceBuffer slot_bias;  // [nslots][kH]: b0 + W0[kF + s] (inference)
+  // [nslots][kH] first-layer bias of each slot, b0 + W0[kF + s], as the BF16
+  // pair (hi, lo) with hi + lo = the FP32 bias to 2^-16: inference feeds it
+  // to the layer's MFMA as two extra inputs of value 1
+  DeviceBuffer slot_bias;
   float mu = 0.f, sigma = 1.f, clip = 0.f;
   float b2 = 0.f;  // b_2[0], read back by finalize
   int step = 0;
@@ -98,7 +103,7 @@ struct TrainParams {
 
 struct TrainReport {
   float val_rmse = 0.f;    // one-atom energy RMSE on the held-out points (energy units)
-  float final_loss = 0.f;  // mean squared error of the last 1 % of the steps (standardised units)
+  float final_loss = 0.f;  // mean squared error over all training steps (standardised units)
   uint32_t steps = 0;
 };
 
@@ -107,10 +112,34 @@ struct TrainReport {
 struct Workspace {
   size_t rows = 0;
   int k0 = 0;
-  DeviceBuffer x0, x0t, z[2], a[2], at[2], pred, dz3, dz3t, dz[2], dzt[2], dx0, loss, work, part, vpred;
+  DeviceBuffer x0, x0t, z[2], a[2], at[2], pred, dz3, dz3t, dz[2], dzt[2], dx0, loss, part, vpred;
+  // weight and bias gradients of layer l as splits[l] K slices (summed in
+  // order by the Adam step)
+  DeviceBuffer gpart[3], bpart[3];
+  int splits[3] = {1, 1, 1};
+  // per-step state in device memory (step count, learning rates, Adam bias
+  // corrections) and the step's batch indices, so that one captured step
+  // replays as every step
+  DeviceBuffer state, idx;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  Workspace() = default;
+  Workspace(const Workspace &) = delete;
+  Workspace &operator=(const Workspace &) = delete;
+  ~Workspace() { drop_graph(); }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
 };
 int reserve(const Net &net, Workspace &ws, size_t batch, size_t nval);
 
+// Training runs as one captured hipGraph of a whole step replayed `steps`
+// times (every per-step value is computed on the device from the step
+// count), or, with BUDE_MI355X_TRAIN_GRAPH=0 in the environment, as the same
+// kernels launched step by step; both give the same network bit for bit.
 // Trains on points[0, ntrain) with labels (energy units), validates on
 // points[ntrain, ntrain + nval); sets mu / sigma / clip from the training
 // labels (target y = (min(E, clip) - mu) / sigma, clip = mu + 8 sigma).
@@ -130,11 +159,12 @@ int forward_backward(Net &net, const float4 *points, const float *target, size_t
 int finalize(Net &net, hipStream_t s);
 
 struct ScoreArgs {
-  const bf16 *grid16;         // [nodes][kF]
+  const uint32_t *gzp;        // z-pair table (Net::gzp)
   field::Grid grid;
   const bf16 *w0t;            // W0^T [kH][k0]: columns 0..kF-1 read
   int k0;
-  const float *slot_bias;     // [nslots][kH]
+  const uint32_t *slot_bias;  // [nslots][kH] BF16 (hi, lo) pairs
+  int nslots;
   const bf16 *w1t;            // W1^T [kH][kH]
   const float *b1;            // [kH]
   const bf16 *w2;             // W2 [kH][kOut], column 0

@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
+#include <cstdlib>
 #include <random>
 #include <vector>
 
@@ -186,31 +188,49 @@ __global__ void features_kernel(const float4 *__restrict__ points, const uint32_
 // [B][32]) spread to its 8 corners with the trilinear weights, accumulated
 // as 2^40 fixed point in 64-bit integer atomics (order-independent, so the
 // training is deterministic).
+// One work-item per (point, corner): 8 work-items per point spread the
+// atomics over 8x the waves.
 __global__ void grid_grad_kernel(const float4 *__restrict__ points, const uint32_t *__restrict__ idx, size_t B,
                                  const field::Grid g, const float *__restrict__ dx0,
                                  unsigned long long *__restrict__ acc) {
-  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t b = t >> 3;
+  const int q = int(t & 7);
   if (b >= B) return;
   const float4 p = points[idx ? idx[b] : b];
   const Corners c = corners(p.x, p.y, p.z, g, 1.0f / g.h);
-  float d[kF];
+  int node = c.node[0];
+  float w = c.w[0];
 #pragma unroll
-  for (int k = 0; k < kF; k++) d[k] = dx0[b * 32 + k];
-#pragma unroll
-  for (int q = 0; q < 8; q++)
-#pragma unroll
-    for (int k = 0; k < kF; k++) {
-      const long long v = llrint(double(c.w[q] * d[k]) * kGradScale);
-      atomicAdd(acc + size_t(c.node[q]) * kF + k, static_cast<unsigned long long>(v));
+  for (int k = 1; k < 8; k++)
+    if (q == k) {
+      node = c.node[k];
+      w = c.w[k];
     }
+  const float4 d0 = *reinterpret_cast<const float4 *>(dx0 + b * 32);
+  const float4 d1 = *reinterpret_cast<const float4 *>(dx0 + b * 32 + 4);
+  const float d[kF] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+  for (int k = 0; k < kF; k++) {
+    const long long v = llrint(double(w * d[k]) * kGradScale);
+    atomicAdd(acc + size_t(node) * kF + k, static_cast<unsigned long long>(v));
+  }
 }
 
 // Adam on the grid features from the fixed-point accumulators (which it
 // clears), refreshing the BF16 copy.
+// With `state` set, this step's rates come from it, and the last kernel of
+// the step advances its step count.
 __global__ void grid_adam_kernel(float *__restrict__ w, unsigned long long *__restrict__ acc, float *__restrict__ m,
                                  float *__restrict__ v, bf16 *__restrict__ w16, size_t n, float lr, float b1, float b2,
-                                 float eps, float bc1, float bc2) {
+                                 float eps, float bc1, float bc2, mlp::StepState *__restrict__ state) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (state) {
+    lr = state->hyper[1];
+    bc1 = state->hyper[2];
+    bc2 = state->hyper[3];
+    if (i == 0) state->step += 1;  // (no kernel of this step reads it after begin_step)
+  }
   if (i >= n) return;
   const float gi = float(double(static_cast<long long>(acc[i])) / kGradScale);
   acc[i] = 0ull;
@@ -220,7 +240,7 @@ __global__ void grid_adam_kernel(float *__restrict__ w, unsigned long long *__re
   v[i] = vi;
   const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
   w[i] = wi;
-  w16[i] = bf16(pack_bf16(wi, 0.f) & 0xffffu);
+  if (w16) w16[i] = bf16(pack_bf16(wi, 0.f) & 0xffffu);
 }
 
 // Fixed-point accumulators -> FP32 gradient (forward_backward for tests).
@@ -231,10 +251,23 @@ __global__ void grid_grad_out_kernel(unsigned long long *__restrict__ acc, float
   acc[i] = 0ull;
 }
 
-// FP32 grid -> BF16 copy.
-__global__ void grid_bf16_kernel(const float *__restrict__ w, bf16 *__restrict__ w16, size_t n) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) w16[i] = bf16(pack_bf16(w[i], 0.f) & 0xffffu);
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack_half2(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, half2_t{_Float16(lo), _Float16(hi)});
+}
+
+// FP32 grid -> z-pair table: out[((i n1 + j)(n2 - 1) + l) kF + k] = FP16
+// (g[i][j][l][k], g[i][j][l + 1][k]).
+__global__ void zpair_kernel(const float *__restrict__ g, const field::Grid grid, uint32_t *__restrict__ out) {
+  const size_t n2m = size_t(grid.n[2] - 1);
+  const size_t n = size_t(grid.n[0]) * size_t(grid.n[1]) * n2m * kF;
+  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int k = int(t % kF);
+  const size_t e = t / kF, l = e % n2m, ij = e / n2m;
+  const size_t node = ij * size_t(grid.n[2]) + l;
+  out[t] = pack_half2(g[node * kF + k], g[(node + 1) * kF + k]);
 }
 
 // Squared error of sigma * pred + mu against min(label, clip), summed in FP64 per block.
@@ -276,25 +309,30 @@ __global__ void init_kernel(float *__restrict__ w, float *__restrict__ m, float 
   v[i] = 0.f;
 }
 
-// slot_bias[s][j] = b0[j] + W0[kF + s][j] (W0's BF16 copy, row-major [k0][kH]).
+// slot_bias[s][j] = b0[j] + W0[kF + s][j] (W0's BF16 copy, row-major [k0][kH])
+// as the BF16 pair hi = bf16(b), lo = bf16(b - hi) packed (hi low).
 __global__ void slot_bias_kernel(const float *__restrict__ b0, const bf16 *__restrict__ w16, int nslots,
-                                 float *__restrict__ out) {
+                                 uint32_t *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots * kH) return;
   const int s = i / kH, j = i % kH;
-  out[i] = b0[j] + __uint_as_float(uint32_t(w16[size_t(kF + s) * kH + j]) << 16);
+  const float b = b0[j] + __uint_as_float(uint32_t(w16[size_t(kF + s) * kH + j]) << 16);
+  const float hi = __uint_as_float(pack_bf16(b, 0.f) << 16);
+  out[i] = (pack_bf16(b, 0.f) & 0xffffu) | (pack_bf16(b - hi, 0.f) << 16);
 }
 
 // ---- fused scorer -------------------------------------------------------------
 // One wave per 64 poses of one ligand (lane = pose).  Per ligand atom: the
-// transformed position, the interpolated features (FP32 from the BF16 grid,
-// 16 bytes per corner), then the decoder on MFMA, as the transpose
-// D[neuron][pose] = W^T . x^T so the weights are the (constant, register-
-// held) A operand and the wave's 64 poses four 16-pose B operands read from
-// the wave's own LDS rows: layer 1 (K = the kF features, the rest of the
-// K = 32 step zero; the slot's bias accumulated onto), ReLU, layer 2, ReLU,
-// and the output neuron as a dot product with W2 over the lane's neurons,
-// reduced over the four lane groups that share a pose.
+// transformed position, the interpolated features -- four z-pairs of FP16
+// features (32 bytes each), one packed FP16 dot product per pair and feature
+// against the pair's two trilinear weights, FP32 accumulation -- then the
+// decoder on MFMA, as the transpose D[neuron][pose] = W^T . x^T so the
+// weights are the (register-held) A operand and the wave's 64 poses four
+// 16-pose B operands read from the wave's own LDS rows: layer 1 over K = 32
+// (the kF features, then the slot's bias as a BF16 (hi, lo) pair against two
+// inputs of value 1, then zeros), ReLU, layer 2 onto b1, ReLU, and the output
+// neuron as a dot product with W2 over the lane's neurons, reduced over the
+// four lane groups that share a pose.
 constexpr int kRow = kH + 8;  // bf16 per LDS row (padding staggers banks)
 
 __device__ __forceinline__ void wave_sync() {
@@ -305,14 +343,54 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ bf16x8 lds8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 
-__global__ __launch_bounds__(256) void score_kernel(const ScoreArgs a, int64_t tend) {
+__device__ __forceinline__ float fdot2(uint32_t a, uint32_t b, float c) {
+  return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, a), __builtin_bit_cast(half2_t, b), c, false);
+}
+
+constexpr int kStage = 128;  // ligand atoms staged in LDS per wave at a time
+
+// The four z-pairs around one transformed ligand atom and its cell
+// fractions: the gathers of the next atom are issued before the current
+// one is computed, so their latency hides behind its network.
+struct AtomFetch {
+  uint4 v[4][2];
+  float fx, fy, fz;
+  int slot;
+};
+
+__device__ __forceinline__ void fetch_atom(const ScoreArgs &a, const Xform &x, const float4 la, float inv_h,
+                                           AtomFetch &f) {
+  f.slot = __float_as_int(la.w);  // (the same in every lane)
+  const float px = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
+  const float py = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
+  const float pz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
+  int ix, iy, iz;
+  cell(px, a.grid.lo[0], inv_h, a.grid.n[0], ix, f.fx);
+  cell(py, a.grid.lo[1], inv_h, a.grid.n[1], iy, f.fy);
+  cell(pz, a.grid.lo[2], inv_h, a.grid.n[2], iz, f.fz);
+  const size_t n1 = size_t(a.grid.n[1]), n2m = size_t(a.grid.n[2] - 1);
+  // the four z-pairs (dx, dy), 8 FP16 pairs = 32 bytes each
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int dx = q >> 1, dy = q & 1;
+    const uint4 *e =
+        reinterpret_cast<const uint4 *>(a.gzp + ((size_t(ix + dx) * n1 + size_t(iy + dy)) * n2m + size_t(iz)) * kF);
+    f.v[q][0] = e[0];
+    f.v[q][1] = e[1];
+  }
+}
+
+__global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_t tend) {
   __shared__ bf16 act[4][64 * kRow];
   __shared__ float w2s[kH], b1s[kH];
+  __shared__ uint32_t sbias[kMaxSlots * kH];
+  __shared__ float4 latoms[4][kStage];
   const int tid = threadIdx.x;
   if (tid < kH) {
     w2s[tid] = __uint_as_float(uint32_t(a.w2[size_t(tid) * kOut]) << 16);
     b1s[tid] = a.b1[tid];
   }
+  for (int i = tid; i < a.nslots * kH; i += 256) sbias[i] = a.slot_bias[i];
   __syncthreads();
   if (a.raise_priority) __builtin_amdgcn_s_setprio(2);
   const int lane = tid & 63, wave = tid >> 6;
@@ -328,8 +406,8 @@ __global__ __launch_bounds__(256) void score_kernel(const ScoreArgs a, int64_t t
   const int j = int(t - a.tile_begin[lo]) * 64 + lane;
   const Xform x = make_xform(a.poses, a.stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
   const int r = lane & 15, grp = lane >> 4;
-  // register-held A fragments: W0^T rows 16 jn + r, k 8 grp .. (only grp 0
-  // holds features), W1^T rows 16 jn + r, k 8 grp ..
+  // register-held A fragments of W1^T (rows 16 jn + r, k 8 grp ..) and of
+  // the feature rows of W0^T (k 0 .. kF - 1: lane group 0 only)
   const bf16x8 zero8 = bf16x8{};
   bf16x8 fw0[kH / 16], fw1[kH / 16];
 #pragma unroll
@@ -337,58 +415,63 @@ __global__ __launch_bounds__(256) void score_kernel(const ScoreArgs a, int64_t t
     fw0[jn] = grp == 0 ? *reinterpret_cast<const bf16x8 *>(a.w0t + size_t(16 * jn + r) * a.k0) : zero8;
     fw1[jn] = *reinterpret_cast<const bf16x8 *>(a.w1t + size_t(16 * jn + r) * kH + 8 * grp);
   }
-  float w2l[kH / 16][4], b1l[kH / 16][4];
-#pragma unroll
-  for (int jn = 0; jn < kH / 16; jn++)
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      w2l[jn][q] = w2s[16 * jn + 4 * grp + q];
-      b1l[jn][q] = b1s[16 * jn + 4 * grp + q];
-    }
+  // B operand of lane group 1 in layer 1: the two unit inputs the slot's
+  // bias pair multiplies (k = kF, kF + 1)
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const bf16x8 ones2 = __builtin_bit_cast(bf16x8, u32x4{0x3f803f80u, 0u, 0u, 0u});
   bf16 *buf = act[wave];
   const float inv_h = 1.0f / a.grid.h;
-  const uint4 *g16 = reinterpret_cast<const uint4 *>(a.grid16);
-  float sy = 0.f;
-  for (int il = 0; il < m.natlig; ++il) {
-    const float4 la = a.ligands[m.lig_off + il];
-    const int slot = __builtin_amdgcn_readfirstlane(__float_as_int(la.w));
-    const float px = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
-    const float py = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
-    const float pz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
-    const Corners c = corners(px, py, pz, a.grid, inv_h);
-    uint4 v[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) v[q] = g16[c.node[q]];
+  float part[4] = {0.f, 0.f, 0.f, 0.f};  // this lane's share of pose block i, summed over atoms
+  // the ligand's atoms, staged kStage at a time in the wave's LDS rows (read
+  // as broadcasts: no global load, and so no wait, inside the atom loop)
+  const int natlig = __builtin_amdgcn_readfirstlane(m.natlig);
+  const int lig_off = __builtin_amdgcn_readfirstlane(m.lig_off);
+  float4 *ls = latoms[wave];
+  auto stage = [&](int base) {
+    wave_sync();  // (earlier reads of the rows are done)
+    for (int i = lane; i < kStage && base + i < natlig; i += 64) ls[i] = a.ligands[lig_off + base + i];
+    wave_sync();
+  };
+  // atom i's gathers; the first atom of a new stage reloads the rows
+  auto fetch = [&](int i, AtomFetch &f) {
+    if (i > 0 && i % kStage == 0) stage(i);
+    fetch_atom(a, x, ls[i % kStage], inv_h, f);
+  };
+  // one atom through the network, its output added to part[]
+  auto eval = [&](const AtomFetch &cur) {
+    const int slot = __builtin_amdgcn_readfirstlane(cur.slot);
+    const float fz = cur.fz;
+    const float wx[2] = {1.f - cur.fx, cur.fx}, wy[2] = {1.f - cur.fy, cur.fy};
     float phi[kF];
 #pragma unroll
     for (int k = 0; k < kF; k++) phi[k] = 0.f;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const uint32_t u[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+    for (int q = 0; q < 4; q++) {
+      const float wxy = wx[q >> 1] * wy[q & 1];
+      const uint32_t w2 = pack_half2(wxy * (1.f - fz), wxy * fz);
+      const uint32_t u[8] = {cur.v[q][0].x, cur.v[q][0].y, cur.v[q][0].z, cur.v[q][0].w,
+                             cur.v[q][1].x, cur.v[q][1].y, cur.v[q][1].z, cur.v[q][1].w};
 #pragma unroll
-      for (int h = 0; h < 4; h++) {
-        phi[2 * h] = __builtin_fmaf(c.w[q], lo_bf16(u[h]), phi[2 * h]);
-        phi[2 * h + 1] = __builtin_fmaf(c.w[q], hi_bf16(u[h]), phi[2 * h + 1]);
-      }
+      for (int k = 0; k < kF; k++) phi[k] = fdot2(w2, u[k], phi[k]);
     }
     *reinterpret_cast<uint4 *>(buf + lane * kRow) =
         make_uint4(pack_bf16(phi[0], phi[1]), pack_bf16(phi[2], phi[3]), pack_bf16(phi[4], phi[5]),
                    pack_bf16(phi[6], phi[7]));
     wave_sync();
-    // layer 1 onto the slot's bias
-    floatx4 acc[kH / 16][4];
+    // layer 1: features and the slot's bias on MFMA
+    bf16x8 fa[kH / 16];
 #pragma unroll
     for (int jn = 0; jn < kH / 16; jn++) {
-      const float4 sb = *reinterpret_cast<const float4 *>(a.slot_bias + size_t(slot) * kH + 16 * jn + 4 * grp);
-#pragma unroll
-      for (int i = 0; i < 4; i++) acc[jn][i] = floatx4{sb.x, sb.y, sb.z, sb.w};
+      const uint32_t b = sbias[slot * kH + 16 * jn + r];
+      fa[jn] = grp == 0 ? fw0[jn] : (grp == 1 ? __builtin_bit_cast(bf16x8, u32x4{b, 0u, 0u, 0u}) : zero8);
     }
+    floatx4 acc[kH / 16][4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const bf16x8 fb = grp == 0 ? lds8(buf + (16 * i + r) * kRow) : zero8;
+      const bf16x8 fb = grp == 0 ? lds8(buf + (16 * i + r) * kRow) : (grp == 1 ? ones2 : zero8);
 #pragma unroll
       for (int jn = 0; jn < kH / 16; jn++)
-        acc[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw0[jn], fb, acc[jn][i], 0, 0, 0);
+        acc[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[jn], fb, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     }
     wave_sync();
     // ReLU -> BF16 rows [pose][neuron]
@@ -402,30 +485,50 @@ __global__ __launch_bounds__(256) void score_kernel(const ScoreArgs a, int64_t t
       }
     wave_sync();
     // layer 2 onto b1, ReLU, dot with W2 over this lane's neurons
-    float part[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const bf16x8 fb = lds8(buf + (16 * i + r) * kRow + 8 * grp);
-      float s = 0.f;
+      float sum = part[i];
 #pragma unroll
       for (int jn = 0; jn < kH / 16; jn++) {
-        floatx4 z = floatx4{b1l[jn][0], b1l[jn][1], b1l[jn][2], b1l[jn][3]};
+        const float4 bb = *reinterpret_cast<const float4 *>(b1s + 16 * jn + 4 * grp);
+        const float4 ww = *reinterpret_cast<const float4 *>(w2s + 16 * jn + 4 * grp);
+        floatx4 z = floatx4{bb.x, bb.y, bb.z, bb.w};
         z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw1[jn], fb, z, 0, 0, 0);
-#pragma unroll
-        for (int q = 0; q < 4; q++) s = __builtin_fmaf(w2l[jn][q], fmaxf(z[q], 0.f), s);
+        sum = __builtin_fmaf(ww.x, fmaxf(z[0], 0.f), sum);
+        sum = __builtin_fmaf(ww.y, fmaxf(z[1], 0.f), sum);
+        sum = __builtin_fmaf(ww.z, fmaxf(z[2], 0.f), sum);
+        sum = __builtin_fmaf(ww.w, fmaxf(z[3], 0.f), sum);
       }
-      part[i] = s;
+      part[i] = sum;
     }
     wave_sync();  // the rows are rewritten by the next atom
-    // sum over the four lane groups of pose block i, kept by group i:
-    // groups {g, g ^ 2} first (each keeps its half of the blocks), then {g, g ^ 1}
-    const bool upper = grp >= 2;
-    const float s0 = upper ? part[0] : part[2], s1 = upper ? part[1] : part[3];
-    const float m0 = (upper ? part[2] : part[0]) + __shfl_xor(s0, 32);
-    const float m1 = (upper ? part[3] : part[1]) + __shfl_xor(s1, 32);
-    const bool odd = grp & 1;
-    sy += (odd ? m1 : m0) + __shfl_xor(odd ? m0 : m1, 16);
+  };
+  // two fetch buffers in turn: atom i + 1's gathers are in flight while
+  // atom i is evaluated.  The fetches are unconditional (past the last atom
+  // they fetch it again: in bounds, unused), so the loads' waits are placed
+  // for that order on every path.
+  if (natlig == 0) {
+    if (j < m.nposes) a.out[size_t(m.pose_off) + size_t(j)] = 0.f;
+    return;
   }
+  AtomFetch f0{}, f1{};
+  stage(0);
+  fetch(0, f0);
+  for (int il = 0; il < natlig; il += 2) {
+    fetch(min(il + 1, natlig - 1), f1);
+    eval(f0);
+    fetch(min(il + 2, natlig - 1), f0);
+    if (il + 1 < natlig) eval(f1);
+  }
+  // sum over the four lane groups of pose block i, kept by group i:
+  // groups {g, g ^ 2} first (each keeps its half of the blocks), then {g, g ^ 1}
+  const bool upper = grp >= 2;
+  const float s0 = upper ? part[0] : part[2], s1 = upper ? part[1] : part[3];
+  const float m0 = (upper ? part[2] : part[0]) + __shfl_xor(s0, 32);
+  const float m1 = (upper ? part[3] : part[1]) + __shfl_xor(s1, 32);
+  const bool odd = grp & 1;
+  const float sy = (odd ? m1 : m0) + __shfl_xor(odd ? m0 : m1, 16);
   if (j < m.nposes)
     a.out[size_t(m.pose_off) + size_t(j)] = a.sigma * (sy + float(m.natlig) * a.b2) + float(m.natlig) * a.mu;
 }
@@ -473,26 +576,22 @@ int forward(const Net &net, Acts &A, size_t B, bool training, hipStream_t s) {
 }
 
 // Backward from dz3 (written by the loss kernel): weight and bias gradients
-// into net.gw / gb and dL/dx0 (first 32 columns) into A.dx0.
+// as K slices into A.gpart / bpart, dL/dx0 (first 32 columns) into A.dx0.
 int backward(Net &net, Acts &A, size_t B, hipStream_t s) {
   const int R = int(B);
-  mlp::BiasGradBatch bg{};
-  bg.count = 3;
-  bg.B = R;
-  bg.ldt = R;
   const void *dzt_next = A.dz3t.ptr, *dz_next = A.dz3.ptr;
   for (int l = 2; l >= 0; l--) {
     const int K = layer_k(net, l), N = layer_n(l);
     const void *prev_t = l == 0 ? A.x0t.ptr : A.at[l - 1].ptr;
+    // dW_l = A_{l-1}^T dZ_l and db_l = column sums of dZ_l, per K slice
     auto gw = gemm_args(prev_t, dzt_next, K, N, R, R, R, mlp::kEpiF32);
-    gw.out_f32 = static_cast<float *>(net.gw[l].ptr);
     gw.ldo = N;
     gw.ksplit = mlp::auto_ksplit(K, N, R);
-    gw.work = static_cast<float *>(A.work.ptr);
+    gw.work = gw.out_f32 = static_cast<float *>(A.gpart[l].ptr);
+    gw.bias_part = static_cast<float *>(A.bpart[l].ptr);
+    gw.defer = 1;
+    A.splits[l] = gw.ksplit;
     if (int st = hip_check(mlp::gemm(gw, s), "atomnet gemm dW")) return st;
-    bg.dz_t[l] = static_cast<const bf16 *>(dzt_next);
-    bg.db[l] = static_cast<float *>(net.gb[l].ptr);
-    bg.N[l] = N;
     if (l == 0) {
       // dL/dx0 for the grid: dz1 W0^T over the first 32 rows of W0
       auto gx = gemm_args(dz_next, net.w16[0].ptr, R, 32, kH, kH, kH, mlp::kEpiF32);
@@ -511,30 +610,31 @@ int backward(Net &net, Acts &A, size_t B, hipStream_t s) {
     dz_next = A.dz[l - 1].ptr;
     dzt_next = A.dzt[l - 1].ptr;
   }
-  return hip_check(mlp::bias_grad_multi(bg, s), "atomnet bias grads");
+  return BUDE_OK;
 }
 
-int adam_step(Net &net, float lr, float grid_lr, hipStream_t s) {
-  net.step++;
+// Adam on every parameter with this step's values from the device state,
+// which it then advances to the next step.
+int adam_step_dev(Net &net, const Acts &A, mlp::StepState *state, hipStream_t s) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float bc1 = 1.f - std::pow(b1, float(net.step)), bc2 = 1.f - std::pow(b2, float(net.step));
   mlp::AdamBatch ab{};
   for (int l = 0; l < 3; l++) {
     const int K = layer_k(net, l), N = layer_n(l);
-    ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(net.gw[l].ptr),
+    ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(A.gpart[l].ptr),
                    static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
-                   static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), K, N};
-    ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(net.gb[l].ptr),
+                   static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), K, N,
+                   A.splits[l], size_t(K) * N};
+    ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(A.bpart[l].ptr),
                        static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), nullptr, nullptr, 1,
-                       N};
+                       N, A.splits[l], size_t(N)};
   }
   ab.count = 6;
-  if (int st = hip_check(mlp::adam_multi(ab, lr, b1, b2, eps, bc1, bc2, s), "atomnet adam")) return st;
+  if (int st = hip_check(mlp::adam_multi_dev(ab, mlp::step_hyper(state), b1, b2, eps, s), "atomnet adam")) return st;
   const size_t n = net.nodes * kF;
   hipLaunchKernelGGL(grid_adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, static_cast<float *>(net.g.ptr),
                      static_cast<unsigned long long *>(net.gacc.ptr), static_cast<float *>(net.gm.ptr),
-                     static_cast<float *>(net.gv.ptr), static_cast<bf16 *>(net.g16.ptr), n, grid_lr, b1, b2, eps, bc1,
-                     bc2);
+                     static_cast<float *>(net.gv.ptr), static_cast<bf16 *>(nullptr), n, 0.f, b1, b2, eps, 1.f, 1.f,
+                     state);
   return hip_check(hipGetLastError(), "atomnet grid adam");
 }
 
@@ -548,11 +648,11 @@ int step_grads(Net &net, Acts &A, const float4 *points, const uint32_t *idx, con
   if (int st = forward(net, A, B, true, s)) return st;
   if (int st = hip_check(mlp::loss_grad(static_cast<const float *>(A.pred.ptr), labels, idx, B, norm, nullptr, nullptr,
                                         1.f, static_cast<bf16 *>(A.dz3.ptr), static_cast<bf16 *>(A.dz3t.ptr), int(B),
-                                        static_cast<float *>(A.loss.ptr), s),
+                                        static_cast<unsigned long long *>(A.loss.ptr), s),
                          "atomnet loss"))
     return st;
   if (int st = backward(net, A, B, s)) return st;
-  hipLaunchKernelGGL(grid_grad_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, s, points, idx, B, net.grid,
+  hipLaunchKernelGGL(grid_grad_kernel, dim3(blocks_for(B * 8, 256)), dim3(256), 0, s, points, idx, B, net.grid,
                      static_cast<const float *>(A.dx0.ptr), static_cast<unsigned long long *>(net.gacc.ptr));
   return hip_check(hipGetLastError(), "atomnet grid grad");
 }
@@ -584,12 +684,14 @@ int reserve(const Net &net, Workspace &A, size_t B, size_t nval) {
   if (!st) st = A.loss.reserve(256);
   if (!st) st = A.part.reserve(1024 * 2 * 8);
   if (!st) st = A.vpred.reserve(std::max<size_t>(nval, 1) * 4);
-  size_t wmax = 0;
-  for (int l = 0; l < 3; l++) {
+  if (!st) st = A.state.reserve(sizeof(mlp::StepState));
+  if (!st) st = A.idx.reserve(B * 4);
+  for (int l = 0; l < 3 && !st; l++) {
     const int K = layer_k(net, l), N = layer_n(l);
-    wmax = std::max(wmax, size_t(mlp::auto_ksplit(K, N, int(B))) * K * N);
+    const size_t cap = mlp::max_ksplit(K, N);
+    st = A.gpart[l].reserve(cap * K * N * 4);
+    if (!st) st = A.bpart[l].reserve(cap * N * 4);
   }
-  if (!st) st = A.work.reserve(wmax * 4);
   return st;
 }
 
@@ -599,12 +701,7 @@ int set_params(Net &net, const float *grid, const float *const w[3], const float
   if (!st) st = alloc_zero(net.gm, ng * 4);
   if (!st) st = alloc_zero(net.gv, ng * 4);
   if (!st) st = alloc_zero(net.gacc, ng * 8);
-  if (!st) st = net.g16.alloc(ng * 2);
-  if (!st) {
-    hipLaunchKernelGGL(grid_bf16_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, s, static_cast<const float *>(net.g.ptr),
-                       static_cast<bf16 *>(net.g16.ptr), ng);
-    st = hip_check(hipGetLastError(), "atomnet grid bf16");
-  }
+  if (!st) st = net.gzp.alloc(ng * 4);
   for (int l = 0; l < 3 && !st; l++) {
     const int K = layer_k(net, l), N = layer_n(l);
     const size_t nw = size_t(K) * N;
@@ -621,7 +718,7 @@ int set_params(Net &net, const float *grid, const float *const w[3], const float
                                   static_cast<bf16 *>(net.w16t[l].ptr), s),
                      "to_bf16");
   }
-  if (!st) st = net.slot_bias.alloc(size_t(net.nslots) * kH * 4);
+  if (!st) st = net.slot_bias.alloc(size_t(net.nslots) * kH * 4);  // BF16 pairs
   net.step = 0;
   if (!st) st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   return st;
@@ -643,7 +740,7 @@ int init(Net &net, const field::Grid &grid, int nslots, uint64_t seed, hipStream
   if (!st) st = net.gm.reserve(ng * 4);
   if (!st) st = net.gv.reserve(ng * 4);
   if (!st) st = net.gacc.reserve(ng * 8);
-  if (!st) st = net.g16.reserve(ng * 2);
+  if (!st) st = net.gzp.reserve(ng * 4);
   for (int l = 0; l < 3 && !st; l++) {
     const size_t nw = size_t(layer_k(net, l)) * layer_n(l), nb = size_t(layer_n(l));
     for (DeviceBuffer *d : {&net.w[l], &net.mw[l], &net.vw[l], &net.gw[l]})
@@ -663,9 +760,6 @@ int init(Net &net, const field::Grid &grid, int nslots, uint64_t seed, hipStream
                      0.1f, seed);
   if (int e = hip_check(hipGetLastError(), "atomnet init")) return e;
   if (int e = hip_check(hipMemsetAsync(net.gacc.ptr, 0, ng * 8, s), "hipMemsetAsync")) return e;
-  hipLaunchKernelGGL(grid_bf16_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, s, static_cast<const float *>(net.g.ptr),
-                     static_cast<bf16 *>(net.g16.ptr), ng);
-  if (int e = hip_check(hipGetLastError(), "atomnet grid bf16")) return e;
   for (int l = 0; l < 3; l++) {
     const int K = layer_k(net, l), N = layer_n(l);
     const size_t nw = size_t(K) * N;
@@ -738,23 +832,44 @@ int train(Net &net, Workspace &A, const float4 *points, const float *labels, con
     net.clip = float(mean + 8.0 * std::sqrt(var));
   }
   const mlp::TargetNorm norm{net.clip, net.mu, 1.0f / net.sigma, -INFINITY};
-  const size_t steps = std::max<uint32_t>(1, p.steps);
-  const size_t warm = std::min<size_t>(100, steps / 10 + 1);
-  const size_t tail = std::max<size_t>(1, steps / 100);
-  for (size_t step = 0; step < steps; step++) {
-    if (step + tail == steps)
-      if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, s), "hipMemsetAsync")) return st;
-    const uint32_t *idx = perm + (step * B) % ntrain;
+  const uint32_t steps = std::max<uint32_t>(1, p.steps);
+  const uint32_t warm = std::min<uint32_t>(100, steps / 10 + 1);
+  const uint32_t step0 = uint32_t(net.step);
+  auto *state = static_cast<mlp::StepState *>(A.state.ptr);
+  uint32_t *idx = static_cast<uint32_t *>(A.idx.ptr);
+  // one training step, every per-step value taken from the device state
+  auto one_step = [&]() -> int {
+    if (int st = hip_check(mlp::begin_step(state, perm, uint32_t(ntrain / B), uint32_t(B), idx, steps, warm, p.lr,
+                                           p.grid_lr, 0.9f, 0.999f, step0, s),
+                           "atomnet begin step"))
+      return st;
     if (int st = step_grads(net, A, points, idx, labels, B, norm, s)) return st;
-    const float warmf = step < warm ? float(step + 1) / float(warm) : 1.f;
-    const float prog = float(step) / float(std::max<size_t>(1, steps - 1));
-    const float f = warmf * (0.05f + 0.95f * 0.5f * (1.f + std::cos(3.14159265f * prog)));
-    if (int st = adam_step(net, p.lr * f, p.grid_lr * f, s)) return st;
+    return adam_step_dev(net, A, state, s);
+  };
+  if (int st = hip_check(hipMemsetAsync(A.state.ptr, 0, sizeof(mlp::StepState), s), "hipMemsetAsync")) return st;
+  if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 8, s), "hipMemsetAsync")) return st;
+  const char *env = std::getenv("BUDE_MI355X_TRAIN_GRAPH");
+  if (env && env[0] == '0') {
+    for (uint32_t step = 0; step < steps; step++)
+      if (int st = one_step()) return st;
+  } else {
+    A.drop_graph();
+    if (int st = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+      return st;
+    const int cst = one_step();
+    const hipError_t ce = hipStreamEndCapture(s, &A.graph);
+    if (cst) return cst;
+    if (int st = hip_check(ce, "hipStreamEndCapture")) return st;
+    if (int st = hip_check(hipGraphInstantiate(&A.exec, A.graph, nullptr, nullptr, 0), "hipGraphInstantiate"))
+      return st;
+    for (uint32_t step = 0; step < steps; step++)
+      if (int st = hip_check(hipGraphLaunch(A.exec, s), "hipGraphLaunch")) return st;
   }
+  net.step += int(steps);
   TrainReport r{};
-  r.steps = uint32_t(steps);
-  float lsum = 0.f;
-  if (int st = hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return st;
+  r.steps = steps;
+  unsigned long long lsum = 0;
+  if (int st = hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return st;
   // held-out RMSE in energy units (forward of the training kernels)
   double se = 0;
   if (nval) {
@@ -784,7 +899,7 @@ int train(Net &net, Workspace &A, const float4 *points, const float *labels, con
     return st;
   }
   r.val_rmse = nval ? float(std::sqrt(se / double(nval))) : 0.f;
-  r.final_loss = lsum / float(tail * B);
+  r.final_loss = float(double(lsum) / mlp::kLossScale / (double(steps) * double(B)));
   if (rep) *rep = r;
   return finalize(net, s);
 }
@@ -794,26 +909,42 @@ int forward_backward(Net &net, const float4 *points, const float *target, size_t
   if (batch == 0 || batch % 64) return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: batch must be a multiple of 64");
   Acts A;
   if (int st = reserve(net, A, batch, 0)) return st;
-  if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, s), "hipMemsetAsync")) return st;
+  if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 8, s), "hipMemsetAsync")) return st;
   const mlp::TargetNorm norm{INFINITY, 0.f, 1.f, -INFINITY};
   if (int st = step_grads(net, A, points, nullptr, target, batch, norm, s)) return st;
+  // the K slices of the weight and bias gradients, summed in order into net.gw / gb
+  for (int l = 0; l < 3; l++) {
+    const size_t K = size_t(layer_k(net, l)), N = size_t(layer_n(l));
+    if (int st = hip_check(mlp::sum_slices(static_cast<const float *>(A.gpart[l].ptr), A.splits[l], K * N,
+                                           static_cast<float *>(net.gw[l].ptr), s),
+                           "atomnet dW"))
+      return st;
+    if (int st = hip_check(mlp::sum_slices(static_cast<const float *>(A.bpart[l].ptr), A.splits[l], N,
+                                           static_cast<float *>(net.gb[l].ptr), s),
+                           "atomnet db"))
+      return st;
+  }
   const size_t ng = net.nodes * kF;
   hipLaunchKernelGGL(grid_grad_out_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, s,
                      static_cast<unsigned long long *>(net.gacc.ptr), grad_grid, ng);
   if (int st = hip_check(hipGetLastError(), "atomnet grid grad out")) return st;
   if (int st = hip_check(hipMemcpyAsync(pred, A.pred.ptr, batch * 4, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
     return st;
-  float l = 0.f;
-  if (int st = hip_check(hipMemcpyAsync(&l, A.loss.ptr, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return st;
+  unsigned long long l = 0;
+  if (int st = hip_check(hipMemcpyAsync(&l, A.loss.ptr, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return st;
   if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
-  *loss = l / float(batch);
+  *loss = float(double(l) / mlp::kLossScale / double(batch));
   return BUDE_OK;
 }
 
 int finalize(Net &net, hipStream_t s) {
+  const size_t nz = size_t(net.grid.n[0]) * size_t(net.grid.n[1]) * size_t(net.grid.n[2] - 1) * kF;
+  hipLaunchKernelGGL(zpair_kernel, dim3(blocks_for(nz, 256)), dim3(256), 0, s, static_cast<const float *>(net.g.ptr),
+                     net.grid, static_cast<uint32_t *>(net.gzp.ptr));
+  if (int st = hip_check(hipGetLastError(), "atomnet z-pairs")) return st;
   hipLaunchKernelGGL(slot_bias_kernel, dim3(blocks_for(size_t(net.nslots) * kH, 256)), dim3(256), 0, s,
                      static_cast<const float *>(net.b[0].ptr), static_cast<const bf16 *>(net.w16[0].ptr), net.nslots,
-                     static_cast<float *>(net.slot_bias.ptr));
+                     static_cast<uint32_t *>(net.slot_bias.ptr));
   if (int st = hip_check(hipGetLastError(), "atomnet slot bias")) return st;
   if (int st = hip_check(hipMemcpyAsync(&net.b2, net.b[2].ptr, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync"))
     return st;
@@ -822,11 +953,12 @@ int finalize(Net &net, hipStream_t s) {
 
 ScoreArgs score_args(const Net &net) {
   ScoreArgs a{};
-  a.grid16 = static_cast<const bf16 *>(net.g16.ptr);
+  a.gzp = static_cast<const uint32_t *>(net.gzp.ptr);
   a.grid = net.grid;
   a.w0t = static_cast<const bf16 *>(net.w16t[0].ptr);
   a.k0 = net.k0;
-  a.slot_bias = static_cast<const float *>(net.slot_bias.ptr);
+  a.slot_bias = static_cast<const uint32_t *>(net.slot_bias.ptr);
+  a.nslots = net.nslots;
   a.w1t = static_cast<const bf16 *>(net.w16t[1].ptr);
   a.b1 = static_cast<const float *>(net.b[1].ptr);
   a.w2 = static_cast<const bf16 *>(net.w16[2].ptr);
@@ -839,7 +971,7 @@ ScoreArgs score_args(const Net &net) {
 hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (a.nligands <= 0 || a.tile0 < 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 ||
-      (ntiles + 3) / 4 > 0x7fffffff)
+      a.nslots < 1 || a.nslots > kMaxSlots || (ntiles + 3) / 4 > 0x7fffffff)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a, a.tile0 + ntiles);
   return hipGetLastError();

@@ -496,20 +496,22 @@ extern "C" int bude_mi355x_mlp_forward_backward(int device, size_t batch, const 
                                "to_bf16"))
     return st;
   if (int st = bude::surrogate::forward(net, A, batch, true, s)) return st;
-  if (int st = bude::hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, s), "hipMemsetAsync")) return st;
+  if (int st = bude::hip_check(hipMemsetAsync(A.loss.ptr, 0, 8, s), "hipMemsetAsync")) return st;
   if (int st = bude::hip_check(
           bude::mlp::loss_grad(static_cast<const float *>(A.pred.ptr), static_cast<const float *>(tgt.ptr), nullptr,
                                batch, bude::mlp::TargetNorm{3.0e38f, 0.f, 1.f, -3.0e38f}, nullptr, nullptr, 1.f,
                                static_cast<bf16 *>(A.dz3.ptr),
-                               static_cast<bf16 *>(A.dz3_t.ptr), int(batch), static_cast<float *>(A.loss.ptr), s),
+                               static_cast<bf16 *>(A.dz3_t.ptr), int(batch),
+                               static_cast<unsigned long long *>(A.loss.ptr), s),
           "loss"))
     return st;
   if (int st = bude::surrogate::backward(net, A, batch, s)) return st;
+  if (int st = bude::surrogate::sum_grads(net, A, s)) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
   if (int st = bude::hip_check(hipMemcpy(pred, A.pred.ptr, batch * 4, hipMemcpyDeviceToHost), "hipMemcpy")) return st;
-  float lsum = 0.f;
-  if (int st = bude::hip_check(hipMemcpy(&lsum, A.loss.ptr, 4, hipMemcpyDeviceToHost), "hipMemcpy")) return st;
-  *loss = lsum / float(batch);
+  unsigned long long lsum = 0;
+  if (int st = bude::hip_check(hipMemcpy(&lsum, A.loss.ptr, 8, hipMemcpyDeviceToHost), "hipMemcpy")) return st;
+  *loss = float(double(lsum) / bude::mlp::kLossScale / double(batch));
   for (int l = 0; l < 4; l++) {
     std::vector<float> gw(size_t(kK[l]) * kN[l]);
     std::vector<float> gb(static_cast<size_t>(kN[l]));

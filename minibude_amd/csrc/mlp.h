@@ -49,16 +49,27 @@ struct GemmArgs {
   int ldo, ldt;
   int ksplit;          // kEpiF32 only: > 1 splits K over ksplit work-groups per tile
   float *work;         // [ksplit][M][ldo] partial sums when ksplit > 1
+  int defer;           // kEpiF32: leave the K slices in `work` (slice 0 of ksplit = 1 too) for
+                       // the consumer to sum (Adam over AdamTensor slices), no reduction pass
+  float *bias_part;    // kEpiF32, optional: [ksplit][N] row sums of B over each K slice (the
+                       // bias gradient when B is dZ^T)
 };
 
-// C = A * B^T on 32x32 tiles (M, N, K multiples of 32).  With ksplit > 1
-// (kEpiF32) every tile's K range is split over ksplit work-groups writing
-// partial sums to `work`, then summed in a fixed order (deterministic).
+// C = A * B^T on 32x32 tiles (M, N, K multiples of 32), each wave loading
+// the next K step's fragments while it multiplies the current ones.  With
+// ksplit > 1 (kEpiF32) every tile's K range is split over ksplit
+// work-groups writing partial sums to `work`, then summed in a fixed order
+// (deterministic) by a second pass, or by the consumer (`defer`).
 hipError_t gemm(const GemmArgs &g, hipStream_t s);
+
+// out[i] = sum over z < splits of work[z * n + i], z ascending.
+hipError_t sum_slices(const float *work, int splits, size_t n, float *out, hipStream_t s);
 
 // Split count that gives a kEpiF32 GEMM about 1,024 waves (K split in
 // multiples of 256).
 int auto_ksplit(int M, int N, int K);
+// Largest split auto_ksplit returns for an M x N product (any K).
+int max_ksplit(int M, int N);
 
 // Features of poses idx[i] (or i when idx is null) from 6 pose rows of
 // `stride` floats: row i of out [n][kIn] (bf16), and out_t [kIn][ldt] if set.
@@ -76,47 +87,58 @@ hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size
 // (device memory) when table is set, else `norm`; weight tail_weight when
 // E < tail else 1.
 // dz[b][0] = 2 w (pred - y) / n, other columns 0 (also transposed);
-// loss_acc += sum (pred - y)^2 (f32 atomics, for monitoring).
+// loss_acc += sum (pred - y)^2 in fixed point (kLossScale units per 1,
+// integer atomics: the same sum whatever order the waves land in; for
+// monitoring).
+constexpr double kLossScale = 16777216.0;
 hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, TargetNorm norm,
                      const TargetNorm *table, const uint32_t *lig, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt,
-                     float *loss_acc, hipStream_t s);
+                     unsigned long long *loss_acc, hipStream_t s);
 
 // Per-ligand first-layer bias with the descriptor inputs folded in, for the
 // fused inference kernel (whose descriptor columns are zero):
 // out[l][j] = b[j] + sum_c bf16(desc[l][c]) * w16[kFeat + c][j], w16 = W_0 [kIn][kHidden].
 hipError_t fold_bias(const float *b, const bf16 *w16, const float *desc, int nlig, float *out, hipStream_t s);
 
-// db[n] = sum_b dz_t[n][b] for n < N rows of length B.
-hipError_t bias_grad(const bf16 *dz_t, int N, int B, int ldt, float *db, hipStream_t s);
-
-// Adam step on n parameters; refreshes the BF16 copy w16 [rows][cols] and
-// its transpose w16t [cols][rows] (either may be null).
-hipError_t adam(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
-                float bc1, float bc2, bf16 *w16, bf16 *w16t, int rows, int cols, hipStream_t s);
+// Device-side state of a training loop, so that one captured hipGraph of a
+// step replays as every step: the step count, and this step's two learning
+// rates (weights, and a second parameter group) and Adam bias corrections.
+struct StepState {
+  uint32_t step, pad;
+  float hyper[4];  // lr0, lr1, 1 - b1^t, 1 - b2^t
+};
+inline const float *step_hyper(const StepState *st) {
+  return reinterpret_cast<const float *>(reinterpret_cast<const char *>(st) + 8);
+}
 
 // One Adam step over up to 8 parameter tensors in a single launch.
 struct AdamTensor {
   float *w;
-  const float *g;
+  const float *g;    // gradient: `splits` slices `stride` floats apart, summed in order
   float *m, *v;
   bf16 *w16, *w16t;  // optional BF16 copies (row-major and transposed)
   int rows, cols;    // w is rows x cols row-major
+  int splits;        // 0 or 1: g is the gradient itself
+  size_t stride;
 };
 struct AdamBatch {
   AdamTensor t[8];
   int count;
 };
-hipError_t adam_multi(const AdamBatch &b, float lr, float b1, float b2, float eps, float bc1, float bc2,
-                      hipStream_t s);
+// The learning rate and the bias corrections are read from device memory
+// when the kernel runs: hyper[0] = lr, hyper[2] = bc1, hyper[3] = bc2 (so a
+// captured graph of a training step can be replayed step after step).  With
+// `advance` set the launch also ends the step: advance->step += 1.
+hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, float b2, float eps, hipStream_t s,
+                          StepState *advance = nullptr);
 
-// Bias gradients of up to 4 layers in one launch: db_l[n] = sum_b dz_t_l[n][b].
-struct BiasGradBatch {
-  const bf16 *dz_t[4];
-  float *db[4];
-  int N[4];
-  int count, B, ldt;
-};
-hipError_t bias_grad_multi(const BiasGradBatch &b, hipStream_t s);
+// Start of step s = st->step: idx[i] = perm[(s % per_epoch) * B + i] for
+// i < B (the batch), and the step's rates: linear warm-up over `warm` steps,
+// then cosine decay to 5 % of lr0 / lr1 at the last of `steps`; t = step0 +
+// s + 1 for the bias corrections.
+hipError_t begin_step(StepState *st, const uint32_t *perm, uint32_t per_epoch, uint32_t B, uint32_t *idx,
+                      uint32_t steps, uint32_t warm, float lr0, float lr1, float b1, float b2, uint32_t step0,
+                      hipStream_t s);
 
 // FP32 -> BF16 of a rows x cols matrix and its transpose.
 hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s);

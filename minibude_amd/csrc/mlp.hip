@@ -3,7 +3,8 @@
 // GEMMs run on gfx950 BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
 // accumulate): one wavefront owns a 32x32 tile of C = A * B^T as 2x2 MFMA
 // blocks, both operands k-contiguous, so every fragment is one 16-byte load
-// per lane.  MFMA 16x16x32 lane layout (lane l, r = l % 16, g = l / 16):
+// per lane, and the next K step's four fragments are in flight while the
+// current four multiply.  MFMA 16x16x32 lane layout (lane l, r = l % 16, g = l / 16):
 //   A fragment  A[r][8g .. 8g+7]       B fragment  B^T[8g .. 8g+7][r]
 //   accumulator C[4g .. 4g+3][r]
 #include <hip/hip_runtime.h>
@@ -98,14 +99,45 @@ __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // row sums of B over this K slice (the bias gradient when B is dZ^T),
+  // by the waves of the first row of tiles: lane (r, grp) sums its own
+  // fragment columns, the four lane groups are added at the end
+  const bool bsum = g.bias_part && blockIdx.y == 0;
+  float bs0 = 0.f, bs1 = 0.f;
 
+  bf16x8 fa0 = load8(a0 + kbeg), fa1 = load8(a1 + kbeg);
+  bf16x8 fb0 = load8(b0 + kbeg), fb1 = load8(b1 + kbeg);
   for (int k = kbeg; k < kend; k += 32) {
-    const bf16x8 fa0 = load8(a0 + k), fa1 = load8(a1 + k);
-    const bf16x8 fb0 = load8(b0 + k), fb1 = load8(b1 + k);
+    const int kn = k + 32 < kend ? k + 32 : k;  // (the last step reloads its own fragments)
+    const bf16x8 na0 = load8(a0 + kn), na1 = load8(a1 + kn);
+    const bf16x8 nb0 = load8(b0 + kn), nb1 = load8(b1 + kn);
     acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
     acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb1, acc[0][1], 0, 0, 0);
     acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb0, acc[1][0], 0, 0, 0);
     acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
+    if (bsum) {
+      const uint4 u0 = __builtin_bit_cast(uint4, fb0), u1 = __builtin_bit_cast(uint4, fb1);
+      const uint32_t w0[4] = {u0.x, u0.y, u0.z, u0.w}, w1[4] = {u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        bs0 += __uint_as_float(w0[e] << 16) + __uint_as_float(w0[e] & 0xffff0000u);
+        bs1 += __uint_as_float(w1[e] << 16) + __uint_as_float(w1[e] & 0xffff0000u);
+      }
+    }
+    fa0 = na0;
+    fa1 = na1;
+    fb0 = nb0;
+    fb1 = nb1;
+  }
+  if (bsum) {
+    bs0 += __shfl_xor(bs0, 16);
+    bs0 += __shfl_xor(bs0, 32);
+    bs1 += __shfl_xor(bs1, 16);
+    bs1 += __shfl_xor(bs1, 32);
+    if (grp == 0) {
+      g.bias_part[size_t(blockIdx.z) * g.N + n0 + r] = bs0;
+      g.bias_part[size_t(blockIdx.z) * g.N + n0 + 16 + r] = bs1;
+    }
   }
 
 #pragma unroll
@@ -133,7 +165,7 @@ __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
         } else if (g.epi == kEpiBackGelu || g.epi == kEpiBackRelu) {
           t[q] = to_bf16(g.epi == kEpiBackGelu ? c * gelu_grad(g.zprev[o]) : (g.zprev[o] > 0.f ? c : 0.f));
           g.out_bf16[o] = t[q];
-        } else if (g.ksplit > 1) {
+        } else if (g.ksplit > 1 || g.defer) {
           g.work[size_t(blockIdx.z) * size_t(g.M) * g.ldo + o] = c;
         } else {
           g.out_f32[o] = c;
@@ -192,7 +224,7 @@ __global__ void loss_grad_kernel(const float *__restrict__ pred, const float *__
                                  const uint32_t *__restrict__ idx, size_t n, TargetNorm norm,
                                  const TargetNorm *__restrict__ table, const uint32_t *__restrict__ lig,
                                  float tail_weight, bf16 *__restrict__ dz, bf16 *__restrict__ dz_t, int ldt,
-                                 float *__restrict__ loss) {
+                                 unsigned long long *__restrict__ loss) {
   const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float sq = 0.f;
   if (b < n) {
@@ -211,44 +243,22 @@ __global__ void loss_grad_kernel(const float *__restrict__ pred, const float *__
 #pragma unroll
     for (int c = 1; c < kOut; c++) dz_t[size_t(c) * ldt + b] = 0;
   }
-  // wave reduction of the squared error, one atomic per wave
+  // wave reduction of the squared error, one fixed-point atomic per wave
+  // (clamped at 2^40 units so that no run of waves wraps the sum)
   for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off);
-  if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, sq);
+  if ((threadIdx.x & 63) == 0 && loss)
+    atomicAdd(loss, static_cast<unsigned long long>(fmin(double(sq) * kLossScale + 0.5, 1099511627776.0)));
 }
 
-__global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__ dz_t, int B, int ldt,
-                                                        float *__restrict__ db) {
-  __shared__ float part[256];
-  const int n = blockIdx.x;
-  float s = 0.f;
-  for (int b = threadIdx.x; b < B; b += 256) s += from_bf16(dz_t[size_t(n) * ldt + b]);
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-    __syncthreads();
+__global__ void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                                  const float *__restrict__ hyper, StepState *__restrict__ advance) {
+  if (hyper) {  // learning rate and bias corrections of this step from device memory
+    lr = hyper[0];
+    bc1 = hyper[2];
+    bc2 = hyper[3];
   }
-  if (threadIdx.x == 0) db[n] = part[0];
-}
-
-__global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
-                            float *__restrict__ v, size_t n, float lr, float b1, float b2, float eps, float bc1,
-                            float bc2, bf16 *__restrict__ w16, bf16 *__restrict__ w16t, int rows, int cols) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
-  w[i] = wi;
-  const bf16 h = to_bf16(wi);
-  if (w16) w16[i] = h;
-  if (w16t) w16t[size_t(i % cols) * rows + i / cols] = h;
-}
-
-__global__ void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  // (the step count is read only by the next step's first kernel)
+  if (advance && blockIdx.x == 0 && threadIdx.x == 0) advance->step += 1;
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int t = 0;
   for (; t < b.count; t++) {
@@ -258,7 +268,8 @@ __global__ void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b
   }
   if (t >= b.count) return;
   const AdamTensor &T = b.t[t];
-  const float gi = T.g[i];
+  float gi = T.g[i];
+  for (int z = 1; z < T.splits; z++) gi += T.g[size_t(z) * T.stride + i];
   const float mi = b1 * T.m[i] + (1.f - b1) * gi;
   const float vi = b2 * T.v[i] + (1.f - b2) * gi * gi;
   T.m[i] = mi;
@@ -270,20 +281,22 @@ __global__ void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b
   if (T.w16t) T.w16t[size_t(i % T.cols) * T.rows + i / T.cols] = h;
 }
 
-__global__ __launch_bounds__(256) void bias_grad_multi_kernel(const BiasGradBatch b) {
-  __shared__ float part[256];
-  int layer = 0, n = blockIdx.x;
-  while (layer < b.count - 1 && n >= b.N[layer]) n -= b.N[layer++];
-  const bf16 *row = b.dz_t[layer] + size_t(n) * b.ldt;
-  float s = 0.f;
-  for (int i = threadIdx.x; i < b.B; i += 256) s += from_bf16(row[i]);
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-    __syncthreads();
+__global__ void begin_step_kernel(StepState *__restrict__ st, const uint32_t *__restrict__ perm, uint32_t per_epoch,
+                                  uint32_t B, uint32_t *__restrict__ idx, uint32_t steps, uint32_t warm, float lr0,
+                                  float lr1, float b1, float b2, uint32_t step0) {
+  const uint32_t s = st->step;
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < B) idx[i] = perm[size_t(s % per_epoch) * B + i];
+  if (i == 0) {
+    const float warmf = s < warm ? float(s + 1) / float(warm) : 1.f;
+    const float prog = float(s) / float(steps > 1 ? steps - 1 : 1);
+    const float f = warmf * (0.05f + 0.95f * 0.5f * (1.f + cosf(3.14159265f * prog)));
+    const float t = float(step0 + s + 1);
+    st->hyper[0] = lr0 * f;
+    st->hyper[1] = lr1 * f;
+    st->hyper[2] = 1.f - powf(b1, t);
+    st->hyper[3] = 1.f - powf(b2, t);
   }
-  if (threadIdx.x == 0) b.db[layer][n] = part[0];
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, bf16 *__restrict__ w16,
@@ -441,10 +454,10 @@ hipError_t gemm(const GemmArgs &g0, hipStream_t s) {
   GemmArgs g = g0;
   if (g.ksplit < 1) g.ksplit = 1;
   if (g.M % 32 || g.N % 32 || g.K % 32 || g.lda % 8 || g.ldb % 8 || (g.K / 32) % g.ksplit) return hipErrorInvalidValue;
-  if (g.ksplit > 1 && (g.epi != kEpiF32 || !g.work)) return hipErrorInvalidValue;
+  if ((g.ksplit > 1 || g.defer || g.bias_part) && (g.epi != kEpiF32 || !g.work)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(gemm_nt_kernel, dim3(unsigned(g.N / 32), unsigned(g.M / 32), unsigned(g.ksplit)), dim3(64), 0, s,
                      g);
-  if (g.ksplit > 1) {
+  if (g.ksplit > 1 && !g.defer) {
     if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
     const size_t n = size_t(g.M) * g.ldo;
     hipLaunchKernelGGL(ksplit_reduce_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.work, n, g.ksplit,
@@ -460,6 +473,15 @@ int auto_ksplit(int M, int N, int K) {
   return split;
 }
 
+int max_ksplit(int M, int N) { return std::max(1, 1024 / std::max(1, (M / 32) * (N / 32))); }
+
+hipError_t sum_slices(const float *work, int splits, size_t n, float *out, hipStream_t s) {
+  if (splits < 1) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(ksplit_reduce_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, work, n, splits, out);
+  return hipGetLastError();
+}
+
 hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t, int ldt,
                     hipStream_t s, const float *desc, const uint32_t *lig, int nfeat) {
   if (n == 0) return hipSuccess;
@@ -471,7 +493,7 @@ hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size
 
 hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, TargetNorm norm,
                      const TargetNorm *table, const uint32_t *lig, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt,
-                     float *loss_acc, hipStream_t s) {
+                     unsigned long long *loss_acc, hipStream_t s) {
   if (table && !lig) return hipErrorInvalidValue;
   hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, energy, idx, n, norm, table,
                      lig, tail_weight, dz, dz_t, ldt, loss_acc);
@@ -484,30 +506,21 @@ hipError_t fold_bias(const float *b, const bf16 *w16, const float *desc, int nli
   return hipGetLastError();
 }
 
-hipError_t bias_grad(const bf16 *dz_t, int N, int B, int ldt, float *db, hipStream_t s) {
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(unsigned(N)), dim3(256), 0, s, dz_t, B, ldt, db);
-  return hipGetLastError();
-}
-
-hipError_t adam(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
-                float bc1, float bc2, bf16 *w16, bf16 *w16t, int rows, int cols, hipStream_t s) {
-  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, eps, bc1, bc2,
-                     w16, w16t, rows, cols);
-  return hipGetLastError();
-}
-
-hipError_t adam_multi(const AdamBatch &b, float lr, float b1, float b2, float eps, float bc1, float bc2,
+hipError_t begin_step(StepState *st, const uint32_t *perm, uint32_t per_epoch, uint32_t B, uint32_t *idx,
+                      uint32_t steps, uint32_t warm, float lr0, float lr1, float b1, float b2, uint32_t step0,
                       hipStream_t s) {
+  if (B == 0 || per_epoch == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(begin_step_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, s, st, perm, per_epoch, B, idx, steps,
+                     warm, lr0, lr1, b1, b2, step0);
+  return hipGetLastError();
+}
+
+hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, float b2, float eps, hipStream_t s,
+                          StepState *advance) {
   size_t total = 0;
   for (int t = 0; t < b.count; t++) total += size_t(b.t[t].rows) * b.t[t].cols;
-  hipLaunchKernelGGL(adam_multi_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, b, lr, b1, b2, eps, bc1, bc2);
-  return hipGetLastError();
-}
-
-hipError_t bias_grad_multi(const BiasGradBatch &b, hipStream_t s) {
-  int rows = 0;
-  for (int l = 0; l < b.count; l++) rows += b.N[l];
-  hipLaunchKernelGGL(bias_grad_multi_kernel, dim3(unsigned(rows)), dim3(256), 0, s, b);
+  hipLaunchKernelGGL(adam_multi_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, b, 0.f, b1, b2, eps, 1.f, 1.f,
+                     hyper, advance);
   return hipGetLastError();
 }
 

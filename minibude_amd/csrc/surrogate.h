@@ -27,15 +27,21 @@ struct Net {
 // backward pass reads).
 struct Acts {
   size_t rows = 0;
-  DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss, work;
+  DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss;
+  // weight and bias gradients of layer l as splits[l] K slices (summed in
+  // order by the Adam step, or by sum_grads)
+  DeviceBuffer gpart[4], bpart[4];
+  int splits[4] = {1, 1, 1, 1};
 };
 
 int alloc_acts(Acts &A, size_t rows, bool training);
 // Forward over `rows` rows already featurised in A.feat.
 int forward(const Net &net, Acts &A, size_t rows, bool training, hipStream_t s);
-// Backward from dz3 (written by the loss kernel) into net.gw / gb.
+// Backward from dz3 (written by the loss kernel): the weight and bias
+// gradients as K slices in A.gpart / bpart.
 int backward(Net &net, Acts &A, size_t rows, hipStream_t s);
-int adam_step(Net &net, float lr, hipStream_t s);
+// The slices summed in order into net.gw / gb.
+int sum_grads(Net &net, const Acts &A, hipStream_t s);
 // Uploads FP32 weights W_l [kK[l]][kN[l]] (zero padded) and biases, clears
 // the Adam state and refreshes the BF16 copies.
 int set_weights(Net &net, const std::vector<std::vector<float>> &w, const std::vector<std::vector<float>> &b,
@@ -73,7 +79,11 @@ struct TrainReport {
 // Adam over mini-batches of the label indices `train` (reshuffled every
 // epoch with a seeded mt19937_64); learning rate: linear warm-up, then
 // cosine decay to 5 %.  loss_curve[e] (if set, e < cap) = epoch e's mean
-// squared error in standardised units.
+// squared error in standardised units.  A step runs as one captured
+// hipGraph replayed step after step (every per-step value is computed on
+// the device from the step count), or, with BUDE_MI355X_TRAIN_GRAPH=0 in the
+// environment, as the same kernels launched step by step: the same network
+// bit for bit.
 int train(Net &net, const TrainData &d, std::vector<uint32_t> train, const TrainParams &p, hipStream_t s,
           float *loss_curve, size_t cap, TrainReport *rep);
 

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <random>
 
 #include "surrogate.h"
@@ -44,11 +45,11 @@ int alloc_acts(Acts &A, size_t rows, bool training) {
     if (!st) st = A.dz3.alloc(rows * O * 2);
     if (!st) st = A.dz3_t.alloc(rows * O * 2);
     if (!st) st = A.loss.alloc(256);
-    // split-K partial sums of the largest weight gradient
-    size_t wmax = 0;
-    for (int l = 0; l < 4; l++)
-      wmax = std::max(wmax, size_t(mlp::auto_ksplit(kK[l], kN[l], int(rows))) * kK[l] * kN[l]);
-    if (!st) st = A.work.alloc(wmax * 4);
+    for (int l = 0; l < 4 && !st; l++) {
+      const size_t cap = size_t(mlp::max_ksplit(kK[l], kN[l]));
+      st = A.gpart[l].alloc(cap * kK[l] * kN[l] * 4);
+      if (!st) st = A.bpart[l].alloc(cap * kN[l] * 4);
+    }
   }
   return st;
 }
@@ -77,27 +78,23 @@ int forward(const Net &net, Acts &A, size_t rows, bool training, hipStream_t s) 
   return hip_check(mlp::gemm(g, s), "mlp gemm output");
 }
 
-// Backward from dz3 (already written by the loss kernel) into net.gw / gb.
+// Backward from dz3 (already written by the loss kernel): weight and bias
+// gradients as K slices.
 int backward(Net &net, Acts &A, size_t rows, hipStream_t s) {
   const int R = int(rows);
-  mlp::BiasGradBatch bg{};
-  bg.count = 4;
-  bg.B = R;
-  bg.ldt = R;
   const void *dz_t_next = A.dz3_t.ptr;  // [N_l][R]
   const void *dz_next = A.dz3.ptr;      // [R][N_l]
   for (int l = 3; l >= 0; l--) {
-    // dW_l = A_{l-1}^T dZ_l  -> [K_l][N_l]
+    // dW_l = A_{l-1}^T dZ_l  -> [K_l][N_l], and db_l = column sums of dZ_l, per K slice
     const void *prev_t = l == 0 ? A.feat_t.ptr : A.a_t[l - 1].ptr;
     auto gw = gemm_args(prev_t, dz_t_next, kK[l], kN[l], R, R, R, mlp::kEpiF32);
-    gw.out_f32 = static_cast<float *>(net.gw[l].ptr);
     gw.ldo = kN[l];
     gw.ksplit = mlp::auto_ksplit(kK[l], kN[l], R);
-    gw.work = static_cast<float *>(A.work.ptr);
+    gw.work = gw.out_f32 = static_cast<float *>(A.gpart[l].ptr);
+    gw.bias_part = static_cast<float *>(A.bpart[l].ptr);
+    gw.defer = 1;
+    A.splits[l] = gw.ksplit;
     if (int st = hip_check(mlp::gemm(gw, s), "mlp gemm dW")) return st;
-    bg.dz_t[l] = static_cast<const bf16 *>(dz_t_next);
-    bg.db[l] = static_cast<float *>(net.gb[l].ptr);
-    bg.N[l] = kN[l];
     if (l == 0) break;
     // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1})  -> [R][K_l]
     auto ga = gemm_args(dz_next, net.w16[l].ptr, R, kK[l], kN[l], kN[l], kN[l], mlp::kEpiBackGelu);
@@ -110,24 +107,21 @@ int backward(Net &net, Acts &A, size_t rows, hipStream_t s) {
     dz_next = A.dz[l - 1].ptr;
     dz_t_next = A.dz_t[l - 1].ptr;
   }
-  return hip_check(mlp::bias_grad_multi(bg, s), "mlp bias grads");
+  return BUDE_OK;
 }
 
-int adam_step(Net &net, float lr, hipStream_t s) {
-  net.step++;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float bc1 = 1.f - std::pow(b1, float(net.step)), bc2 = 1.f - std::pow(b2, float(net.step));
-  mlp::AdamBatch ab{};
+int sum_grads(Net &net, const Acts &A, hipStream_t s) {
   for (int l = 0; l < 4; l++) {
-    ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(net.gw[l].ptr),
-                   static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
-                   static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), kK[l], kN[l]};
-    ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(net.gb[l].ptr),
-                       static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), nullptr, nullptr, 1,
-                       kN[l]};
+    if (int st = hip_check(mlp::sum_slices(static_cast<const float *>(A.gpart[l].ptr), A.splits[l],
+                                           size_t(kK[l]) * kN[l], static_cast<float *>(net.gw[l].ptr), s),
+                           "mlp dW"))
+      return st;
+    if (int st = hip_check(mlp::sum_slices(static_cast<const float *>(A.bpart[l].ptr), A.splits[l], size_t(kN[l]),
+                                           static_cast<float *>(net.gb[l].ptr), s),
+                           "mlp db"))
+      return st;
   }
-  ab.count = 8;
-  return hip_check(mlp::adam_multi(ab, lr, b1, b2, eps, bc1, bc2, s), "adam");
+  return BUDE_OK;
 }
 
 // Upload FP32 weights W_l [kK[l]][kN[l]] (zero padded) and biases; refresh BF16 copies.
@@ -185,48 +179,98 @@ int train(Net &net, const TrainData &d, std::vector<uint32_t> perm, const TrainP
   Acts A;
   if (int st = alloc_acts(A, B, true)) return st;
   std::mt19937_64 rng(p.seed);
-  DeviceBuffer dperm;
+  DeviceBuffer dperm, dstate, didx;
   if (int st = dperm.alloc(ntrain * 4)) return st;
+  if (int st = dstate.alloc(sizeof(mlp::StepState))) return st;
+  if (int st = didx.alloc(B * 4)) return st;
   const uint32_t epochs = std::max<uint32_t>(1, p.epochs);
-  const size_t steps_per_epoch = ntrain / B;
-  const size_t total = steps_per_epoch * epochs;
-  const size_t warm = std::min<size_t>(200, total / 10 + 1);
-  size_t step = 0;
+  const uint32_t per_epoch = uint32_t(ntrain / B);
+  const uint32_t total = per_epoch * epochs;
+  const uint32_t warm = std::min<uint32_t>(200, total / 10 + 1);
+  const uint32_t step0 = uint32_t(net.step);
+  auto *state = static_cast<mlp::StepState *>(dstate.ptr);
+  const uint32_t *idx = static_cast<const uint32_t *>(didx.ptr);
+  // one step with every per-step value (batch indices, learning rate, Adam
+  // bias corrections) computed on the device from the step count, captured
+  // once as a hipGraph and replayed step after step
+  auto one_step = [&]() -> int {
+    if (int st = hip_check(mlp::begin_step(state, static_cast<const uint32_t *>(dperm.ptr), per_epoch, uint32_t(B),
+                                           static_cast<uint32_t *>(didx.ptr), total, warm, p.lr, 0.f, 0.9f, 0.999f,
+                                           step0, s),
+                           "begin step"))
+      return st;
+    if (int st = hip_check(mlp::features(d.poses, d.stride, idx, B, static_cast<bf16 *>(A.feat.ptr),
+                                         static_cast<bf16 *>(A.feat_t.ptr), int(B), s, d.desc, d.lig, d.nfeat),
+                           "features"))
+      return st;
+    if (int st = forward(net, A, B, true, s)) return st;
+    if (int st = hip_check(mlp::loss_grad(static_cast<const float *>(A.pred.ptr), d.energy, idx, B, d.norm, d.table,
+                                          d.lig, p.tail_weight, static_cast<bf16 *>(A.dz3.ptr),
+                                          static_cast<bf16 *>(A.dz3_t.ptr), int(B),
+                                          static_cast<unsigned long long *>(A.loss.ptr),
+                                          s),
+                           "loss"))
+      return st;
+    if (int st = backward(net, A, B, s)) return st;
+    // Adam over the gradient slices; the same launch ends the step
+    mlp::AdamBatch ab{};
+    for (int l = 0; l < 4; l++) {
+      ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(A.gpart[l].ptr),
+                     static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
+                     static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), kK[l], kN[l],
+                     A.splits[l], size_t(kK[l]) * kN[l]};
+      ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(A.bpart[l].ptr),
+                         static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), nullptr, nullptr, 1,
+                         kN[l], A.splits[l], size_t(kN[l])};
+    }
+    ab.count = 8;
+    return hip_check(mlp::adam_multi_dev(ab, mlp::step_hyper(state), 0.9f, 0.999f, 1e-8f, s, state), "adam");
+  };
+  if (int st = hip_check(hipMemsetAsync(dstate.ptr, 0, sizeof(mlp::StepState), s), "hipMemsetAsync")) return st;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  struct GraphGuard {
+    hipGraph_t &g;
+    hipGraphExec_t &e;
+    ~GraphGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } guard{graph, exec};
+  const char *env = std::getenv("BUDE_MI355X_TRAIN_GRAPH");
+  const bool use_graph = !(env && env[0] == '0');
+  if (use_graph) {
+    if (int st = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+      return st;
+    const int cst = one_step();
+    const hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (cst) return cst;
+    if (int st = hip_check(ce, "hipStreamEndCapture")) return st;
+    if (int st = hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate")) return st;
+  }
   TrainReport r{};
   for (uint32_t ep = 0; ep < epochs; ep++) {
+    // the epoch's order: uploaded (and the loss sum cleared) outside the
+    // graph, between replays
     std::shuffle(perm.begin(), perm.end(), rng);
     if (int st = hip_check(hipMemcpyAsync(dperm.ptr, perm.data(), ntrain * 4, hipMemcpyHostToDevice, s),
                            "hipMemcpyAsync"))
       return st;
-    if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 4, s), "hipMemsetAsync")) return st;
-    for (size_t it = 0; it < steps_per_epoch; it++, step++) {
-      const uint32_t *idx = static_cast<const uint32_t *>(dperm.ptr) + it * B;
-      if (int st = hip_check(mlp::features(d.poses, d.stride, idx, B, static_cast<bf16 *>(A.feat.ptr),
-                                           static_cast<bf16 *>(A.feat_t.ptr), int(B), s, d.desc, d.lig, d.nfeat),
-                             "features"))
-        return st;
-      if (int st = forward(net, A, B, true, s)) return st;
-      if (int st = hip_check(mlp::loss_grad(static_cast<const float *>(A.pred.ptr), d.energy, idx, B, d.norm, d.table,
-                                            d.lig, p.tail_weight, static_cast<bf16 *>(A.dz3.ptr),
-                                            static_cast<bf16 *>(A.dz3_t.ptr), int(B),
-                                            static_cast<float *>(A.loss.ptr), s),
-                             "loss"))
-        return st;
-      if (int st = backward(net, A, B, s)) return st;
-      const float warmf = step < warm ? float(step + 1) / float(warm) : 1.f;
-      const float prog = float(step) / float(std::max<size_t>(1, total - 1));
-      const float lr = p.lr * warmf * (0.05f + 0.95f * 0.5f * (1.f + std::cos(3.14159265f * prog)));
-      if (int st = adam_step(net, lr, s)) return st;
+    if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 8, s), "hipMemsetAsync")) return st;
+    for (uint32_t it = 0; it < per_epoch; it++) {
+      const int st = use_graph ? hip_check(hipGraphLaunch(exec, s), "hipGraphLaunch") : one_step();
+      if (st) return st;
     }
-    float lsum = 0.f;
-    if (int st = hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync"))
+    unsigned long long lsum = 0;
+    if (int st = hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync"))
       return st;
     if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
-    const float mse = lsum / float(steps_per_epoch * B);
+    const float mse = float(double(lsum) / mlp::kLossScale / double(size_t(per_epoch) * B));
     if (loss_curve && ep < cap) loss_curve[ep] = mse;
     r.final_loss = mse;
   }
-  r.steps = uint32_t(step);
+  net.step += int(total);
+  r.steps = total;
   if (rep) *rep = r;
   return BUDE_OK;
 }

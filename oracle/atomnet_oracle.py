@@ -20,9 +20,11 @@ pose energy.  act is ReLU.  Loss = mean((y - target)^2).
 
 With ``bf16=True`` values are rounded to BF16 where the GPU stores BF16: the
 interpolated features, the weights, the hidden activations and the
-back-propagated dZ; the grid itself is read in FP32 in training (its BF16
-copy is what inference reads: ``grid_bf16=True``).  With ``bf16=False`` it
-is the plain FP64 network, whose gradients tests pin against central finite
+back-propagated dZ; the grid itself is read in FP32 in training.  Inference
+(``predict_atoms``) reads the grid as FP16 and interpolates with FP16
+weights (per z-pair of corners, wx wy (1 - fz) and wx wy fz, their products
+formed in FP32), as the fused scorer does.  With ``bf16=False`` it is the
+plain FP64 network, whose gradients tests pin against central finite
 differences of its own loss.
 """
 from __future__ import annotations
@@ -70,12 +72,32 @@ def corners(i, f, n):
     return idx, w
 
 
-def features(points, grid, lo, h, n, grid_bf16=False):
+def to_f16(x: np.ndarray) -> np.ndarray:
+    """Round to FP16 (nearest even) and return as float64."""
+    return np.asarray(x, np.float32).astype(np.float16).astype(np.float64)
+
+
+def features(points, grid, lo, h, n):
     """phi (B, F): trilinear interpolation of grid (N, F) at points."""
-    g = to_bf16(grid) if grid_bf16 else np.asarray(grid, np.float64)
     i, f = cell(points, lo, h, n)
     idx, w = corners(i, f, n)
-    return np.einsum("bc,bcf->bf", w, g[idx])
+    return np.einsum("bc,bcf->bf", w, np.asarray(grid, np.float64)[idx])
+
+
+def features_infer(points, grid, lo, h, n):
+    """phi as the fused scorer forms it: FP16 grid values and FP16 weights."""
+    i, f = cell(points, lo, h, n)
+    idx, _ = corners(i, f, n)
+    f32 = np.asarray(f, np.float32)
+    one = np.float32(1.0)
+    wx = [one - f32[:, 0], f32[:, 0]]
+    wy = [one - f32[:, 1], f32[:, 1]]
+    wz = [one - f32[:, 2], f32[:, 2]]
+    w = np.zeros((len(points), 8))
+    for c in range(8):
+        dx, dy, dz = (c >> 2) & 1, (c >> 1) & 1, c & 1
+        w[:, c] = to_f16((wx[dx] * wy[dy]).astype(np.float32) * wz[dz])
+    return np.einsum("bc,bcf->bf", w, to_f16(grid)[idx])
 
 
 def forward_backward(points, slots, target, grid, lo, h, n, W, b, k0: int, bf16: bool = True):
@@ -114,11 +136,12 @@ def forward_backward(points, slots, target, grid, lo, h, n, W, b, k0: int, bf16:
     return y, loss, gG, gW, gb
 
 
-def predict_atoms(points, slots, grid, lo, h, n, W, b, k0: int, grid_bf16=True):
-    """Inference: y (B,) per atom with BF16 grid, features, weights and
-    activations (the one-hot row folded into a per-slot first-layer bias)."""
+def predict_atoms(points, slots, grid, lo, h, n, W, b, k0: int):
+    """Inference: y (B,) per atom with FP16 interpolation, BF16 features,
+    weights and activations (the one-hot row folded into a per-slot
+    first-layer bias)."""
     F = grid.shape[1]
-    phi = to_bf16(features(points, grid, lo, h, n, grid_bf16=grid_bf16))
+    phi = to_bf16(features_infer(points, grid, lo, h, n))
     Wr = [to_bf16(x) for x in W]
     z1 = phi @ Wr[0][:F] + b[0] + Wr[0][F + np.asarray(slots)]
     a1 = to_bf16(act(z1))

@@ -149,3 +149,26 @@ def test_untrained_field_falls_back_everywhere(gpu_device):
     assert out["untrained"][0] == 6 and np.all(out["untrained"][2] == 1.0)
     assert out["trained"][0] <= 1
     assert np.all(out["trained"][1] > 4 * np.maximum(out["untrained"][1], 0.1))
+
+
+@pytest.mark.parametrize("steps", [1, 2, 5, 40])
+def test_graph_training_equals_step_by_step(gpu_device, steps, monkeypatch):
+    # training replays one captured step as a hipGraph; launched step by step
+    # instead (BUDE_MI355X_TRAIN_GRAPH=0) the same kernels must give the same
+    # network, so the same scores, bit for bit, after every number of steps
+    w = c4_like(nproteins=1, nligands=3, nposes=4096, seed=23)
+    cfg = dict(train_points=65536, batch=4096, steps=steps, labels_per_pair=256, gate_z=0.0)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("BUDE_MI355X_TRAIN_GRAPH", mode)
+        with C.Combined(w.forcefield, w.ligands, w.poses, C.default_config("mlp", **cfg), gpu_device) as c:
+            r = c.screen(w.proteins[0])
+            out[mode] = ([c.scores(l) for l in range(3)], r.surrogate_rmse)
+            # a second protein on the same object re-initialises and retrains
+            c.screen(w.proteins[0])
+            again = [c.scores(l) for l in range(3)]
+        for a, b in zip(out[mode][0], again):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for a, b in zip(out["1"][0], out["0"][0]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), steps
+    assert out["1"][1] == out["0"][1]

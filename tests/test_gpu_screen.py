@@ -106,3 +106,43 @@ def test_fused_inference_matches_oracle_and_training_forward(gpu_device):
     assert rel_err(fused, ref) < 2e-3
     train_fwd = S.mlp_forward_backward(feat[:992], np.zeros(992), W, b, gpu_device)[0]
     assert rel_err(fused[:992], train_fwd) < 2e-3
+
+
+@pytest.mark.parametrize("features", [12, 21])
+def test_graph_training_equals_step_by_step(gpu_device, small_workload, features, monkeypatch):
+    # the training loop replays one captured step as a hipGraph (the epoch's
+    # order uploaded between replays); launched step by step the same kernels
+    # give the same network, so the same predictions, bit for bit; with 12
+    # inputs the rotation-matrix rows of the first layer stay zero
+    w = small_workload
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("BUDE_MI355X_TRAIN_GRAPH", mode)
+        cfg = S.default_config(batch=1024, epochs=3, features=features, seed=2)
+        with S.Screen(w.protein, w.ligand, w.forcefield, cfg, gpu_device) as s:
+            st, curve = s.train(w.train_poses[:, :32768])
+            s.load(w.screen_poses[:, :4096])
+            out[mode] = (s.predict(), curve)
+    assert np.array_equal(out["1"][0].view(np.uint32), out["0"][0].view(np.uint32))
+    assert np.array_equal(out["1"][1], out["0"][1])
+    assert out["1"][1][-1] < out["1"][1][0]
+
+
+def test_configured_12_inputs_at_2pct(gpu_device):
+    # the learned prescreen as configured (12 pose inputs, 3 x 256 GELU) on a
+    # scaled copy of its workload: recall of the exact top 1 % at the
+    # configured 2 % rescored.  The pose network does not reach 0.99 there
+    # (the learned per-atom field does, tests/test_gpu_atomnet.py)
+    w = S.screen_workload(ntrain=1 << 20, nscreen=1 << 20, natpro=2048, seed=9, protein_seed=9)
+    cfg = S.default_config(features=12, epochs=20)
+    with S.Screen(w.protein, w.ligand, w.forcefield, cfg, gpu_device) as s:
+        s.train(w.train_poses)
+        s.load(w.screen_poses)
+        exact, _ = s.exhaustive()
+        s.run()
+        idx, _ = s.selected()
+    rec = S.recall(idx, exact, 0.01)
+    print(f"12-input pose MLP: recall@1% with 2% rescored = {rec:.4f}", flush=True)
+    assert rec > 0.5  # far above the 0.02 of a random ranking
+    if rec < 0.99:
+        pytest.xfail(f"the configured pose MLP keeps {rec:.3f} of the exact top 1 % at 2 % rescored, below 0.99")
