@@ -341,7 +341,8 @@ typedef void (*bude_combined_done_fn)(void *user, size_t index);
 int bude_mi355x_combined_screen_batch(bude_combined *c, size_t nproteins, const BudeAtom *const *proteins,
                                       const size_t *natpro, BudeCombinedResult *res, bude_combined_done_fn done,
                                       void *user, double *wall_ms);
-/* Changes the per-pair fraction later screens rescore exactly. */
+/* Changes the per-pair fraction later screens rescore exactly (not below
+ * the configured gate_top_fraction). */
 int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fraction);
 int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePairResult *r);
 /* The last screen's selection of one pair: pose indices (ascending, within

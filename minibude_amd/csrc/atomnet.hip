@@ -188,33 +188,29 @@ __global__ void features_kernel(const float4 *__restrict__ points, const uint32_
 // [B][32]) spread to its 8 corners with the trilinear weights, accumulated
 // as 2^40 fixed point in 64-bit integer atomics (order-independent, so the
 // training is deterministic).
-// One work-item per (point, corner): 8 work-items per point spread the
-// atomics over 8x the waves.
-__global__ void grid_grad_kernel(const float4 *__restrict__ points, const uint32_t *__restrict__ idx, size_t B,
-                                 const field::Grid g, const float *__restrict__ dx0,
-                                 unsigned long long *__restrict__ acc) {
-  const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t b = t >> 3;
-  const int q = int(t & 7);
+// One wave per point, lane = (corner, feature): each atomic instruction
+// then covers the point's 8 corner nodes, 4 cache lines (the two z-corners
+// of a pair share one), where one work-item per (point, corner) spread
+// every instruction over 8 points' 32 lines.
+__global__ __launch_bounds__(256) void grid_grad_kernel(const float4 *__restrict__ points,
+                                                        const uint32_t *__restrict__ idx, size_t B,
+                                                        const field::Grid g, const float *__restrict__ dx0,
+                                                        unsigned long long *__restrict__ acc) {
+  const size_t b = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   if (b >= B) return;
+  const int lane = threadIdx.x & 63, q = lane >> 3, k = lane & 7;
+  const int dx = q >> 2, dy = (q >> 1) & 1, dz = q & 1;
   const float4 p = points[idx ? idx[b] : b];
-  const Corners c = corners(p.x, p.y, p.z, g, 1.0f / g.h);
-  int node = c.node[0];
-  float w = c.w[0];
-#pragma unroll
-  for (int k = 1; k < 8; k++)
-    if (q == k) {
-      node = c.node[k];
-      w = c.w[k];
-    }
-  const float4 d0 = *reinterpret_cast<const float4 *>(dx0 + b * 32);
-  const float4 d1 = *reinterpret_cast<const float4 *>(dx0 + b * 32 + 4);
-  const float d[kF] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-#pragma unroll
-  for (int k = 0; k < kF; k++) {
-    const long long v = llrint(double(w * d[k]) * kGradScale);
-    atomicAdd(acc + size_t(node) * kF + k, static_cast<unsigned long long>(v));
-  }
+  const float inv_h = 1.0f / g.h;
+  int ix, iy, iz;
+  float fx, fy, fz;
+  cell(p.x, g.lo[0], inv_h, g.n[0], ix, fx);
+  cell(p.y, g.lo[1], inv_h, g.n[1], iy, fy);
+  cell(p.z, g.lo[2], inv_h, g.n[2], iz, fz);
+  const int node = ((ix + dx) * g.n[1] + iy + dy) * g.n[2] + iz + dz;
+  const float w = (dx ? fx : 1.f - fx) * (dy ? fy : 1.f - fy) * (dz ? fz : 1.f - fz);
+  const long long v = llrint(double(w * dx0[b * 32 + k]) * kGradScale);
+  atomicAdd(acc + size_t(node) * kF + k, static_cast<unsigned long long>(v));
 }
 
 // Adam on the grid features from the fixed-point accumulators (which it
@@ -368,17 +364,22 @@ __device__ __forceinline__ void fetch_atom(const ScoreArgs &a, const Xform &x, c
   cell(px, a.grid.lo[0], inv_h, a.grid.n[0], ix, f.fx);
   cell(py, a.grid.lo[1], inv_h, a.grid.n[1], iy, f.fy);
   cell(pz, a.grid.lo[2], inv_h, a.grid.n[2], iz, f.fz);
-  const size_t n1 = size_t(a.grid.n[1]), n2m = size_t(a.grid.n[2] - 1);
-  // the four z-pairs (dx, dy), 8 FP16 pairs = 32 bytes each
+  // the four z-pairs (dx, dy), 8 FP16 pairs = 32 bytes each (32-bit
+  // offsets: the host keeps the table under 2^31 words)
+  const uint32_t n1 = uint32_t(a.grid.n[1]), n2m = uint32_t(a.grid.n[2] - 1);
+  const uint32_t base = ((uint32_t(ix) * n1 + uint32_t(iy)) * n2m + uint32_t(iz)) * kF;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const int dx = q >> 1, dy = q & 1;
-    const uint4 *e =
-        reinterpret_cast<const uint4 *>(a.gzp + ((size_t(ix + dx) * n1 + size_t(iy + dy)) * n2m + size_t(iz)) * kF);
+    const uint32_t dx = q >> 1, dy = q & 1;
+    const uint4 *e = reinterpret_cast<const uint4 *>(a.gzp + base + (dx * n1 + dy) * n2m * kF);
     f.v[q][0] = e[0];
     f.v[q][1] = e[1];
   }
 }
+
+// max(z, 0) of a finite float as one integer max on its bits (negative
+// floats are negative integers); fmaxf would add a canonicalisation.
+__device__ __forceinline__ float relu(float z) { return __int_as_float(max(__float_as_int(z), 0)); }
 
 __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_t tend) {
   __shared__ bf16 act[4][64 * kRow];
@@ -481,7 +482,7 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
       for (int i = 0; i < 4; i++) {
         const floatx4 z = acc[jn][i];
         *reinterpret_cast<uint2 *>(buf + (16 * i + r) * kRow + 16 * jn + 4 * grp) =
-            make_uint2(pack_bf16(fmaxf(z[0], 0.f), fmaxf(z[1], 0.f)), pack_bf16(fmaxf(z[2], 0.f), fmaxf(z[3], 0.f)));
+            make_uint2(pack_bf16(relu(z[0]), relu(z[1])), pack_bf16(relu(z[2]), relu(z[3])));
       }
     wave_sync();
     // layer 2 onto b1, ReLU, dot with W2 over this lane's neurons
@@ -495,10 +496,10 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
         const float4 ww = *reinterpret_cast<const float4 *>(w2s + 16 * jn + 4 * grp);
         floatx4 z = floatx4{bb.x, bb.y, bb.z, bb.w};
         z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw1[jn], fb, z, 0, 0, 0);
-        sum = __builtin_fmaf(ww.x, fmaxf(z[0], 0.f), sum);
-        sum = __builtin_fmaf(ww.y, fmaxf(z[1], 0.f), sum);
-        sum = __builtin_fmaf(ww.z, fmaxf(z[2], 0.f), sum);
-        sum = __builtin_fmaf(ww.w, fmaxf(z[3], 0.f), sum);
+        sum = __builtin_fmaf(ww.x, relu(z[0]), sum);
+        sum = __builtin_fmaf(ww.y, relu(z[1]), sum);
+        sum = __builtin_fmaf(ww.z, relu(z[2]), sum);
+        sum = __builtin_fmaf(ww.w, relu(z[3]), sum);
       }
       part[i] = sum;
     }
@@ -652,7 +653,7 @@ int step_grads(Net &net, Acts &A, const float4 *points, const uint32_t *idx, con
                          "atomnet loss"))
     return st;
   if (int st = backward(net, A, B, s)) return st;
-  hipLaunchKernelGGL(grid_grad_kernel, dim3(blocks_for(B * 8, 256)), dim3(256), 0, s, points, idx, B, net.grid,
+  hipLaunchKernelGGL(grid_grad_kernel, dim3(blocks_for(B * 64, 256)), dim3(256), 0, s, points, idx, B, net.grid,
                      static_cast<const float *>(A.dx0.ptr), static_cast<unsigned long long *>(net.gacc.ptr));
   return hip_check(hipGetLastError(), "atomnet grid grad");
 }
@@ -971,7 +972,8 @@ ScoreArgs score_args(const Net &net) {
 hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (a.nligands <= 0 || a.tile0 < 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 ||
-      a.nslots < 1 || a.nslots > kMaxSlots || (ntiles + 3) / 4 > 0x7fffffff)
+      a.nslots < 1 || a.nslots > kMaxSlots || (ntiles + 3) / 4 > 0x7fffffff ||
+      uint64_t(a.grid.n[0]) * uint64_t(a.grid.n[1]) * uint64_t(a.grid.n[2] - 1) * kF >= (uint64_t(1) << 31))
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a, a.tile0 + ntiles);
   return hipGetLastError();

@@ -1393,6 +1393,9 @@ extern "C" int bude_mi355x_combined_screen_batch(bude_combined *c, size_t nprote
 extern "C" int bude_mi355x_combined_set_keep(bude_combined *c, float keep_fraction) {
   if (!c || !(keep_fraction > 0.f) || keep_fraction > 1.f)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_set_keep: keep_fraction must be in (0, 1]");
+  if (keep_fraction < c->cfg.gate_top_fraction)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "bude_mi355x_combined_set_keep: keep_fraction below the configured gate_top_fraction");
   if (c->slots[0].busy || c->slots[1].busy)
     return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "bude_mi355x_combined_set_keep: not during a batch");
   if (int st = bude::hip_check(hipSetDevice(c->device), "hipSetDevice")) return st;

@@ -268,8 +268,17 @@ __global__ void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b
   }
   if (t >= b.count) return;
   const AdamTensor &T = b.t[t];
+  // the slices in order; eight loads in flight at a time
   float gi = T.g[i];
-  for (int z = 1; z < T.splits; z++) gi += T.g[size_t(z) * T.stride + i];
+  int z = 1;
+  for (; z + 8 <= T.splits; z += 8) {
+    float s[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s[u] = T.g[size_t(z + u) * T.stride + i];
+#pragma unroll
+    for (int u = 0; u < 8; u++) gi += s[u];
+  }
+  for (; z < T.splits; z++) gi += T.g[size_t(z) * T.stride + i];
   const float mi = b1 * T.m[i] + (1.f - b1) * gi;
   const float vi = b2 * T.v[i] + (1.f - b2) * gi * gi;
   T.m[i] = mi;

@@ -408,7 +408,7 @@ def test_tied_scores_are_all_rescored(gpu_device, chunks):
     w = small_workload(nposes=1200, nligands=4, nproteins=1, seed=17)
     poses = [np.ascontiguousarray(np.repeat(q[:, :400], 3, axis=1)) for q in w.poses]
     cfg = C.default_config("field", max_val_rmse=1e9, gate_z=0.0, labels_per_pair=48, keep_fraction=0.0085,
-                           pipeline_chunks=chunks)
+                           gate_top_fraction=0.005, pipeline_chunks=chunks)
     with C.Combined(w.forcefield, w.ligands, poses, cfg, gpu_device) as c:
         r = c.screen(w.proteins[0])
         c.exhaustive()
@@ -458,7 +458,7 @@ def test_batch_equals_one_by_one(gpu_device, kind):
     # object, and the callback sees each protein as the one screened last
     w = small_workload(nposes=3000, nligands=5, nproteins=4, seed=19)
     L = len(w.ligands)
-    kw = dict(labels_per_pair=48)
+    kw = dict(labels_per_pair=48, gate_z=0.0)  # (48 samples: a gate would fall back at random)
     if kind == "pose_mlp":
         cfg = lambda: C.default_config("pose_mlp", labels_per_pair=256, batch=64, epochs=1, max_val_rmse=1e9,  # noqa: E731
                                        gate_z=0.0)
