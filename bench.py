@@ -347,6 +347,7 @@ def screen(epochs: int = 0) -> dict:
             "recall_top0.1pct": round(S.recall(idx, exact, 0.001), 4),
             "fallback": bool(pair.fallback),
             "gate_z": round(pair.gate_z, 2),
+            "pair_keep_fraction": round(pair.keep_fraction, 5),
             "one_atom_rmse_energy_units": round(r.surrogate_rmse, 3) if kind == "mlp" else None,
             "phases_ms": {k: round(getattr(r, k), 3) for k in ("label_ms", "train_ms", "score_ms", "select_ms",
                                                                "rescore_ms")},
@@ -428,8 +429,8 @@ def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_p
     for kind in kinds:
         cfg = C.default_config(kind)
         parts = {"label_ms": 0.0, "train_ms": 0.0, "score_ms": 0.0, "select_ms": 0.0, "rescore_ms": 0.0}
-        fb, labels, sel, rmse = 0, 0, 0, []
-        rec, trusted, gate_z, one_ms = [], [], [], 0.0
+        fb, wide, labels, sel, rmse = 0, 0, 0, 0, []
+        rec, trusted, gate_z, keeps, one_ms = [], [], [], [], 0.0
         with C.Combined(w.forcefield, w.ligands, w.poses, cfg, 0) as c:
             # one untimed screen of the first protein sizes the object's
             # buffers, then every protein in one batch call (each one's
@@ -438,6 +439,7 @@ def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_p
             results, screen_ms = c.screen_batch(w.proteins)
             for r in results:
                 fb += r.fallback_pairs
+                wide += r.widened_pairs
                 labels += r.labels
                 sel += r.nselected
                 rmse.append(r.surrogate_rmse)
@@ -456,6 +458,7 @@ def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_p
                     r_l = S.recall(c.selected(l)[0], exact[p][l], 0.01)
                     rec.append(r_l)
                     gate_z.append(pr.gate_z)
+                    keeps.append(pr.keep_fraction)
                     if not pr.fallback:
                         trusted.append(r_l)
         rec = np.array(rec)
@@ -476,6 +479,9 @@ def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_p
             "recall_pairs_measured": int(len(rec)),
             "fallback_pairs": fb,
             "fallback_fraction": round(fb / w.pairs, 4),
+            "widened_pairs": wide,
+            "widened_fraction": round(wide / w.pairs, 4),
+            "keep_fraction_mean_measured": round(float(np.mean(keeps)), 5),
             "gate_z_min_measured": round(float(np.min(gate_z)), 2),
             "labels": labels,
             "rescored": sel,

@@ -248,7 +248,12 @@ int bude_mi355x_screen_destroy(bude_screen *s);
  *      energies spread evenly over the gap the selection then loses a
  *      fraction G(z) / z, G(z) = phi(z) - z Phi(-z), of the exact top
  *      gate_top_fraction (0.0043 at z = 2) -- and the calibrated validation
- *      RMSE (below) must stay within max_val_rmse.
+ *      RMSE (below) must stay within max_val_rmse.  A pair short of gate_z
+ *      at keep_fraction is widened instead of scored exhaustively when a
+ *      larger fraction, at most widen_max, reaches it: the fraction of its
+ *      samples whose exact energy is within gate_z deviations above the
+ *      gate_top_fraction quantile (so its gap does), which its selection
+ *      then keeps.
  *      Surrogates: BUDE_SURROGATE_MLP, the learned prescreen's network with
  *      11 ligand descriptor inputs, one per protein, trained on 15/16 of the
  *      samples; BUDE_SURROGATE_FIELD, single-ligand-atom energy maps of the
@@ -290,6 +295,8 @@ typedef struct BudeCombinedConfig {
   float grid_lr;             /* ATOM_MLP: peak Adam learning rate of the grid features */
   float gate_z;              /* validation: required (quantile gap) / (tail error deviation), see above */
   float gate_top_fraction;   /* validation: the exact top fraction the selection must keep (e.g. 0.01) */
+  float widen_max;           /* validation: largest fraction a pair short of gate_z is widened to instead of
+                                falling back (0: never widen; else keep_fraction .. 1) */
 } BudeCombinedConfig;
 
 typedef struct BudeCombinedResult {
@@ -306,6 +313,7 @@ typedef struct BudeCombinedResult {
   uint64_t labels;           /* sampled poses scored exactly */
   uint32_t fallback_pairs;
   float surrogate_rmse;      /* ATOM_MLP: one-atom energy RMSE on held-out points (energy units) */
+  uint32_t widened_pairs;    /* pairs rescored at a fraction above keep_fraction (see widen_max) */
 } BudeCombinedResult;
 
 typedef struct BudePairResult {
@@ -316,6 +324,8 @@ typedef struct BudePairResult {
                                 targets (the part of the error that changes the ranking); a fallback criterion */
   float gate_z;              /* validation gap / tail error deviation (energy units); fallback below cfg gate_z */
   float sample_recall;       /* of the samples' exact lowest 5 %, the fraction among the surrogate's lowest 10 % */
+  float keep_fraction;       /* the fraction of the pair's surrogate scores rescored (keep_fraction, or more
+                                for a widened pair; 1 for a fallback pair) */
 } BudePairResult;
 
 /* ligands[i] has natlig[i] atoms and poses[i] 6 rows of nposes[i] floats (as

@@ -37,7 +37,7 @@ class CombinedConfig(_c.Structure):
                 ("tail_weight", _c.c_float), ("batch", _c.c_uint32), ("epochs", _c.c_uint32), ("lr", _c.c_float),
                 ("grid_spacing", _c.c_float), ("pipeline_chunks", _c.c_uint32), ("seed", _c.c_uint64),
                 ("train_points", _c.c_uint32), ("steps", _c.c_uint32), ("grid_lr", _c.c_float), ("gate_z", _c.c_float),
-                ("gate_top_fraction", _c.c_float)]
+                ("gate_top_fraction", _c.c_float), ("widen_max", _c.c_float)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -47,7 +47,8 @@ class CombinedResult(_c.Structure):
     _fields_ = [("total_ms", _c.c_double), ("prepare_ms", _c.c_float), ("label_ms", _c.c_float),
                 ("train_ms", _c.c_float), ("score_ms", _c.c_float), ("select_ms", _c.c_float),
                 ("rescore_ms", _c.c_float), ("nposes", _c.c_uint64), ("nselected", _c.c_uint64),
-                ("labels", _c.c_uint64), ("fallback_pairs", _c.c_uint32), ("surrogate_rmse", _c.c_float)]
+                ("labels", _c.c_uint64), ("fallback_pairs", _c.c_uint32), ("surrogate_rmse", _c.c_float),
+                ("widened_pairs", _c.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -56,7 +57,7 @@ class CombinedResult(_c.Structure):
 class PairResult(_c.Structure):
     _fields_ = [("nposes", _c.c_uint32), ("nselected", _c.c_uint32), ("fallback", _c.c_int32),
                 ("val_rmse", _c.c_float), ("val_rmse_calibrated", _c.c_float), ("gate_z", _c.c_float),
-                ("sample_recall", _c.c_float)]
+                ("sample_recall", _c.c_float), ("keep_fraction", _c.c_float)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -67,16 +68,18 @@ def default_config(surrogate: str = "field", **kw) -> CombinedConfig:
     3,000 Adam steps of 32,768 on 2^22 exact one-atom energies with a 0.75 A
     feature grid and validates on 4,096 sampled poses per pair; the field's
     maps are 0.5 A and validate on 1,024; the pose MLP trains on 16,384
-    sampled poses per pair (1/16 held out).  Every kind falls back where its
-    validation gap is under 2 standard deviations of its tail error (top 1 %,
-    keep 2 %: an expected loss of 0.4 % of the top 1 %); the pose MLP also
-    where its calibrated validation RMSE exceeds 0.8."""
+    sampled poses per pair (1/16 held out).  Every kind trusts a pair where
+    its validation gap reaches 2 standard deviations of its tail error (top
+    1 %, keep 2 %: an expected loss of 0.4 % of the top 1 %), widens a pair
+    short of that to the fraction that reaches it, up to 10 %, and scores a
+    pair beyond that exhaustively; the pose MLP also falls back where its
+    calibrated validation RMSE exceeds 0.8."""
     mlp = surrogate == "mlp"
     c = dict(surrogate=SURROGATES[surrogate], labels_per_pair={"pose_mlp": 16384, "mlp": 4096}.get(surrogate, 1024),
              keep_fraction=0.02, max_val_rmse=0.8 if surrogate == "pose_mlp" else float("inf"), clip_quantile=0.1,
              tail_quantile=0.05, tail_weight=8.0, batch=32768 if mlp else 8192, epochs=20, lr=3e-3 if mlp else 4e-3,
              grid_spacing=0.75 if mlp else 0.5, pipeline_chunks=0, seed=0, train_points=(1 << 22) if mlp else 1 << 21,
-             steps=3000, grid_lr=1.5e-2, gate_z=2.0, gate_top_fraction=0.01)
+             steps=3000, grid_lr=1.5e-2, gate_z=2.0, gate_top_fraction=0.01, widen_max=0.1)
     c.update(kw)
     return CombinedConfig(**c)
 

@@ -324,12 +324,13 @@ __global__ void slot_bias_kernel(const float *__restrict__ b0, const bf16 *__res
 // against the pair's two trilinear weights, FP32 accumulation -- then the
 // decoder on MFMA, as the transpose D[neuron][pose] = W^T . x^T so the
 // weights are the (register-held) A operand and the wave's 64 poses four
-// 16-pose B operands read from the wave's own LDS rows: layer 1 over K = 32
-// (the kF features, then the slot's bias as a BF16 (hi, lo) pair against two
-// inputs of value 1, then zeros), ReLU, layer 2 onto b1, ReLU, and the output
-// neuron as a dot product with W2 over the lane's neurons, reduced over the
-// four lane groups that share a pose.
-constexpr int kRow = kH + 8;  // bf16 per LDS row (padding staggers banks)
+// 16-pose B operands: layer 1 over K = 32 (the kF features from the wave's
+// own LDS rows, then the slot's bias as a BF16 (hi, lo) pair against two
+// inputs of value 1, then zeros), ReLU, layer 2 onto b1 with layer 1's
+// accumulators as its B operand in registers, ReLU, and the output neuron
+// as a dot product with W2 over the lane's neurons, summed over atoms and
+// then over the four lane groups that share a pose.
+constexpr int kRow = kF;  // bf16 per LDS row: the point's features
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -407,14 +408,20 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
   const int j = int(t - a.tile_begin[lo]) * 64 + lane;
   const Xform x = make_xform(a.poses, a.stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
   const int r = lane & 15, grp = lane >> 4;
-  // register-held A fragments of W1^T (rows 16 jn + r, k 8 grp ..) and of
-  // the feature rows of W0^T (k 0 .. kF - 1: lane group 0 only)
+  // register-held A fragments of the feature rows of W0^T (k 0 .. kF - 1:
+  // lane group 0 only) and of W1^T with its K (input neuron) order permuted
+  // so that layer 1's accumulators are layer 2's B operand as they stand:
+  // lane (r, grp) holds hidden neurons 16 jn + 4 grp + c of point r (jn =
+  // 0, 1; c = 0..3) after layer 1, so layer 2's K slot 8 grp + e is neuron
+  // 16 (e / 4) + 4 grp + e % 4
   const bf16x8 zero8 = bf16x8{};
   bf16x8 fw0[kH / 16], fw1[kH / 16];
 #pragma unroll
   for (int jn = 0; jn < kH / 16; jn++) {
     fw0[jn] = grp == 0 ? *reinterpret_cast<const bf16x8 *>(a.w0t + size_t(16 * jn + r) * a.k0) : zero8;
-    fw1[jn] = *reinterpret_cast<const bf16x8 *>(a.w1t + size_t(16 * jn + r) * kH + 8 * grp);
+    const bf16 *row = a.w1t + size_t(16 * jn + r) * kH + 4 * grp;
+    const uint2 lo = *reinterpret_cast<const uint2 *>(row), hi = *reinterpret_cast<const uint2 *>(row + 16);
+    fw1[jn] = __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
   }
   // B operand of lane group 1 in layer 1: the two unit inputs the slot's
   // bias pair multiplies (k = kF, kF + 1)
@@ -474,21 +481,16 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
       for (int jn = 0; jn < kH / 16; jn++)
         acc[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[jn], fb, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     }
-    wave_sync();
-    // ReLU -> BF16 rows [pose][neuron]
-#pragma unroll
-    for (int jn = 0; jn < kH / 16; jn++)
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const floatx4 z = acc[jn][i];
-        *reinterpret_cast<uint2 *>(buf + (16 * i + r) * kRow + 16 * jn + 4 * grp) =
-            make_uint2(pack_bf16(relu(z[0]), relu(z[1])), pack_bf16(relu(z[2]), relu(z[3])));
-      }
-    wave_sync();
-    // layer 2 onto b1, ReLU, dot with W2 over this lane's neurons
+    wave_sync();  // (the feature rows are rewritten by the next atom)
+    // layer 2 onto b1 with layer 1's ReLU outputs, in BF16, as its B
+    // operand straight from the accumulators (K permuted, above); ReLU, and
+    // the dot with W2 over this lane's neurons
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const bf16x8 fb = lds8(buf + (16 * i + r) * kRow + 8 * grp);
+      const floatx4 z0 = acc[0][i], z1 = acc[1][i];
+      const bf16x8 fb = __builtin_bit_cast(
+          bf16x8, make_uint4(pack_bf16(relu(z0[0]), relu(z0[1])), pack_bf16(relu(z0[2]), relu(z0[3])),
+                             pack_bf16(relu(z1[0]), relu(z1[1])), pack_bf16(relu(z1[2]), relu(z1[3]))));
       float sum = part[i];
 #pragma unroll
       for (int jn = 0; jn < kH / 16; jn++) {
@@ -503,7 +505,6 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
       }
       part[i] = sum;
     }
-    wave_sync();  // the rows are rewritten by the next atom
   };
   // two fetch buffers in turn: atom i + 1's gathers are in flight while
   // atom i is evaluated.  The fetches are unconditional (past the last atom

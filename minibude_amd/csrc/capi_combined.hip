@@ -160,7 +160,10 @@ struct ProteinSlot {
   hipEvent_t ev[12] = {};
   std::vector<uint32_t> nsel;
   std::vector<int32_t> fallback;
-  std::vector<float> val_rmse, val_rmse_cal, gate_z, sample_recall;
+  std::vector<float> val_rmse, val_rmse_cal, gate_z, sample_recall, keep;
+  // the selection of a protein with widened pairs: the object's segments
+  // with this protein's per-pair k
+  bude::sel::Plan wplan;
   float atom_val_rmse = 0.f;  // learned per-atom field: one-atom RMSE on held-out points (energy units)
   // the sampled poses: pose index within the pair, exact energy, per-pair offsets
   std::vector<uint32_t> lab_j;
@@ -273,7 +276,7 @@ size_t stage_bound(const bude_combined *c, size_t natpro) {
   const size_t L = c->nlig, lpp = std::min<size_t>(c->cfg.labels_per_pair, c->total);
   return natpro * 16 + (natpro + 2) * 4 + c->slot_type.size() * c->ff.size() * sizeof(bude::PairCoef) +
          3 * L * sizeof(bude::LigandMeta) + 2 * L * lpp * 4 + L * (lpp / bude::kSplitTile + 1) * 8 + (L + 1) * 8 +
-         16 * 256;
+         L * sizeof(bude::sel::Segment) + 16 * 256;
 }
 
 // Room for `bytes` of this protein's uploads in the slot's staging buffer
@@ -477,15 +480,19 @@ int launch_surrogate(bude_combined *c, const float *poses, size_t stride, float 
 // threshold's by Q(keep) - e; with Gaussian errors of deviation sd and the
 // exact energies spread evenly over the gap, the exact top fraction loses
 // G(z) / z of its poses, G(z) = phi(z) - z Phi(-z): 0.0093 at z = 1.75,
-// 0.0043 at z = 2.  recall: of the samples' exact lowest 5 %, the fraction
-// among the surrogate's lowest 10 %.
+// 0.0043 at z = 2.  need: the fraction of the samples whose exact energy is
+// at most Q(top) + z_req sd, the smallest rescored fraction whose gap
+// reaches z_req (infinite without a usable slope).  recall: of the
+// samples' exact lowest 5 %, the fraction among the surrogate's lowest 10 %.
 struct Gate {
   float z, recall;
+  double need;
 };
 
-Gate gate_stats(const std::vector<double> &e, const std::vector<double> &p, double keep, double top, double tail_q) {
+Gate gate_stats(const std::vector<double> &e, const std::vector<double> &p, double keep, double top, double tail_q,
+                double z_req) {
   const size_t n = e.size();
-  if (n < 2) return Gate{0.f, 0.f};
+  if (n < 2) return Gate{0.f, 0.f, INFINITY};
   std::vector<size_t> oe(n), op(n);
   std::iota(oe.begin(), oe.end(), size_t(0));
   std::iota(op.begin(), op.end(), size_t(0));
@@ -516,6 +523,13 @@ Gate gate_stats(const std::vector<double> &e, const std::vector<double> &p, doub
   const double sd = a > 0 ? std::sqrt(std::max(0.0, vp - a * cv)) / a : INFINITY;
   Gate g{};
   g.z = sd > 0 ? float(gap / sd) : (gap > 0 ? INFINITY : 0.f);
+  g.need = INFINITY;
+  if (std::isfinite(sd)) {
+    const double thr = q(top) + z_req * sd;
+    size_t below = 0;
+    while (below < n && e[oe[below]] <= thr) below++;
+    g.need = double(below) / double(n);
+  }
   const size_t kt = std::max<size_t>(1, size_t(std::lround(0.05 * double(n))));
   const size_t ks = std::min(n, std::max(kt, size_t(std::lround(0.10 * double(n)))));
   std::vector<char> sel(n, 0);
@@ -630,6 +644,8 @@ int plan_selection(bude_combined *c, float keep) {
     cost += double(c->nposes[i]) * double(c->lig[i].size());
   }
   if (int st = c->plan.prepare(segs)) return st;
+  for (ProteinSlot &ps : c->slots)
+    if (int st = ps.wplan.prepare(segs)) return st;
   size_t K = 1;
   if (c->cfg.surrogate != BUDE_SURROGATE_MLP)
     K = c->cfg.pipeline_chunks ? size_t(c->cfg.pipeline_chunks)
@@ -697,8 +713,10 @@ int check_config(const BudeCombinedConfig *cfg) {
       cfg->surrogate != BUDE_SURROGATE_ATOM_MLP)
     return bude::fail(BUDE_ERR_UNSUPPORTED, "combined: surrogate must be BUDE_SURROGATE_MLP, BUDE_SURROGATE_FIELD or "
                                             "BUDE_SURROGATE_ATOM_MLP");
-  if (!(cfg->gate_z >= 0.f) || !(cfg->gate_top_fraction > 0.f) || cfg->gate_top_fraction > cfg->keep_fraction)
-    return bude::fail(BUDE_ERR_INVALID_ARGUMENT, "combined: gate_z >= 0 and gate_top_fraction in (0, keep_fraction]");
+  if (!(cfg->gate_z >= 0.f) || !(cfg->gate_top_fraction > 0.f) || cfg->gate_top_fraction > cfg->keep_fraction ||
+      !(cfg->widen_max >= 0.f) || cfg->widen_max > 1.f)
+    return bude::fail(BUDE_ERR_INVALID_ARGUMENT,
+                      "combined: gate_z >= 0, gate_top_fraction in (0, keep_fraction], widen_max in [0, 1]");
   if (!(cfg->keep_fraction > 0.f) || cfg->keep_fraction > 1.f || !(cfg->clip_quantile > 0.f) ||
       cfg->clip_quantile > 1.f || cfg->tail_quantile < 0.f || cfg->tail_quantile > 1.f || !(cfg->tail_weight > 0.f) ||
       cfg->max_val_rmse < 0.f)
@@ -1185,6 +1203,8 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
   c->s().val_rmse_cal.assign(L, 0.f);
   c->s().gate_z.assign(L, 0.f);
   c->s().sample_recall.assign(L, 0.f);
+  c->s().keep.assign(L, cfg.keep_fraction);
+  std::vector<double> need(L, 0.0);
   const double tail_q = mlp ? std::min(0.25, double(cfg.clip_quantile)) : 0.25;
   for (size_t l = 0; l < L; l++) {
     const size_t b = lab_off[l] + lab_train[l], e = lab_off[l + 1];
@@ -1195,9 +1215,11 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
         ev.push_back(double(le[i]));
         pv.push_back(mlp ? double(lpred[i]) * norms[l].sigma + norms[l].mu : double(lpred[i]));
       }
-      const Gate g = gate_stats(ev, pv, double(cfg.keep_fraction), double(cfg.gate_top_fraction), tail_q);
+      const Gate g = gate_stats(ev, pv, double(cfg.keep_fraction), double(cfg.gate_top_fraction), tail_q,
+                                double(cfg.gate_z));
       c->s().gate_z[l] = g.z;
       c->s().sample_recall[l] = g.recall;
+      need[l] = g.need;
     }
     double se = 0, sy = 0, sp = 0, syy = 0, spp = 0, syp = 0;
     for (size_t i = b; i < e; i++) {
@@ -1219,8 +1241,21 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     // residual of the best increasing affine map of the scores onto the
     // targets: the error that matters for a ranking
     c->s().val_rmse_cal[l] = float(std::sqrt(vp > 0 && cv > 0 ? std::max(0.0, vy - cv * cv / vp) : vy));
-    c->s().fallback[l] = !(c->s().gate_z[l] >= cfg.gate_z) || !(c->s().val_rmse_cal[l] <= cfg.max_val_rmse);
-    res->fallback_pairs += uint32_t(c->s().fallback[l]);
+    // short of the gate at keep_fraction: widened to the fraction that
+    // reaches it (with a 5 % margin) if that is within widen_max
+    bool fb = !(c->s().val_rmse_cal[l] <= cfg.max_val_rmse);
+    if (!fb && !(c->s().gate_z[l] >= cfg.gate_z)) {
+      const double f = need[l] * 1.05;
+      if (cfg.widen_max > 0.f && f <= double(cfg.widen_max)) {
+        c->s().keep[l] = float(std::max(double(cfg.keep_fraction), f));
+        res->widened_pairs++;
+      } else {
+        fb = true;
+      }
+    }
+    c->s().fallback[l] = fb;
+    if (fb) c->s().keep[l] = 1.f;
+    res->fallback_pairs += uint32_t(fb);
   }
 
   // 7. Without fallback pairs the field screen is already selecting chunk
@@ -1231,7 +1266,8 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
   std::vector<Span> fb;
   for (size_t l = 0; l < L; l++)
     if (c->s().fallback[l]) fb.push_back(all[l]);
-  const bool pipelined = !mlp && fb.empty();
+  const bool widened = res->widened_pairs > 0;
+  const bool pipelined = !mlp && fb.empty() && !widened;
   int64_t expect = 0;
   int nrs = 0;
   if (!pipelined) {
@@ -1240,6 +1276,18 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     nrs = int(o.size());
     if (o.empty()) o.push_back(0);  // every pair fell back: nothing to rescore
     if (int st = c->s().rs_sel.assign(o.data(), o.size() * 4)) return st;
+  }
+  if (widened) {  // this protein's per-pair k, into the slot's own segment table
+    std::vector<bude::sel::Segment> segs(L);
+    expect = 0;
+    for (size_t l = 0; l < L; l++) {  // (a fallback pair selects its lowest keep_fraction of exact energies)
+      const double f = c->s().fallback[l] ? double(cfg.keep_fraction) : double(c->s().keep[l]);
+      const double kf = std::ceil(f * double(c->nposes[l]));
+      const uint32_t k = uint32_t(std::min<double>(double(c->nposes[l]), std::max(1.0, kf)));
+      segs[l] = bude::sel::Segment{int64_t(c->pose_off[l]), uint32_t(c->nposes[l]), k};
+      if (!c->s().fallback[l]) expect += int64_t((k + 63) / 64);
+    }
+    if (int st = stage_upload(c->s(), c->s().wplan.segs, segs.data(), L * sizeof(segs[0]), c->stream)) return st;
   }
   if (mlp) {
     if (int st = record(c, 4)) return st;
@@ -1268,8 +1316,9 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
       if (int st = launch_rescore(c, rs_tiles + c->chunk_tile[k], rs_ntiles + k, c->chunk_expect[k])) return st;
     }
   } else {
-    if (int st = select_and_tables(c, c->plan, 0, static_cast<const int32_t *>(c->s().rs_sel.ptr), nrs, rs_tiles,
-                                   rs_ntiles + K, c->stream))
+    if (int st = select_and_tables(c, widened ? c->s().wplan : c->plan, 0,
+                                   static_cast<const int32_t *>(c->s().rs_sel.ptr), nrs, rs_tiles, rs_ntiles + K,
+                                   c->stream))
       return st;
     if (int st = record(c, 7)) return st;
     if (nrs)
@@ -1419,6 +1468,7 @@ extern "C" int bude_mi355x_combined_pair(bude_combined *c, size_t ligand, BudePa
   r->val_rmse_calibrated = c->s().val_rmse_cal[ligand];
   r->gate_z = c->s().gate_z[ligand];
   r->sample_recall = c->s().sample_recall[ligand];
+  r->keep_fraction = c->s().keep[ligand];
   return bude::ok();
 }
 

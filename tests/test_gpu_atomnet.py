@@ -172,3 +172,41 @@ def test_graph_training_equals_step_by_step(gpu_device, steps, monkeypatch):
     for a, b in zip(out["1"][0], out["0"][0]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), steps
     assert out["1"][1] == out["0"][1]
+
+
+def test_pairs_short_of_the_gate_are_widened_not_scored_exhaustively(gpu_device):
+    # a stricter gate (z >= 3) than the trained field reaches on every pair:
+    # a pair short of it rescores the larger fraction whose gap reaches it
+    # (within widen_max) instead of every pose; its selection is then the
+    # lowest ceil(keep n) of its scores (ties kept) with exact energies, and
+    # keeps at least what the 2 % selection keeps of the exact top 1 %
+    w = c4_like(nproteins=1, nligands=8, nposes=32768, seed=24)
+    L = len(w.ligands)
+    out = {}
+    for name, kw in (("base", dict(gate_z=0.0)), ("strict", dict(gate_z=3.0, widen_max=0.2))):
+        with C.Combined(w.forcefield, w.ligands, w.poses, C.default_config("mlp", **kw), gpu_device) as c:
+            r = c.screen(w.proteins[0])
+            c.exhaustive()
+            pairs = [c.pair(l) for l in range(L)]
+            sel = [c.selected(l) for l in range(L)]
+            out[name] = (r, pairs, c.recall(0.01), [c.scores(l) for l in range(L)], [c.exact(l) for l in range(L)],
+                         sel)
+    r, pairs, rec, scores, exact, sel = out["strict"]
+    print(f"strict gate: widened {r.widened_pairs}, fallbacks {r.fallback_pairs}, keep "
+          f"{[round(p.keep_fraction, 4) for p in pairs]}, z {[round(p.gate_z, 2) for p in pairs]}", flush=True)
+    assert r.widened_pairs >= 1
+    assert r.widened_pairs + r.fallback_pairs == sum(p.gate_z < 3.0 for p in pairs)
+    for l, p in enumerate(pairs):
+        idx, e = sel[l]
+        if p.fallback:
+            assert p.keep_fraction == 1.0
+            continue
+        k = int(np.ceil(p.keep_fraction * p.nposes))
+        if p.gate_z < 3.0:
+            assert 0.02 < p.keep_fraction <= 0.2
+        else:
+            assert p.keep_fraction == pytest.approx(0.02)
+        thr = np.sort(scores[l])[k - 1]
+        assert set(idx.tolist()) == set(np.nonzero(scores[l] <= thr)[0].tolist())
+        assert np.array_equal(e.view(np.uint32), exact[l][idx].view(np.uint32))
+        assert rec[l] >= out["base"][2][l]
