@@ -69,8 +69,9 @@ def default_config(surrogate: str = "field", **kw) -> CombinedConfig:
     feature grid and validates on 4,096 sampled poses per pair; the field's
     maps are 0.5 A and validate on 1,024; the pose MLP trains on 16,384
     sampled poses per pair (1/16 held out).  Every kind trusts a pair where
-    its validation gap reaches 2 standard deviations of its tail error (top
-    1 %, keep 2 %: an expected loss of 0.4 % of the top 1 %), widens a pair
+    its validation gap reaches 2.5 standard deviations of its tail error
+    (top 1 %, keep 2 %: a modelled loss of 0.08 % of the top 1 %; the full-
+    size screen at 2 left one pair of 512 just under 0.99), widens a pair
     short of that to the fraction that reaches it, up to 10 %, and scores a
     pair beyond that exhaustively; the pose MLP also falls back where its
     calibrated validation RMSE exceeds 0.8."""
@@ -79,7 +80,7 @@ def default_config(surrogate: str = "field", **kw) -> CombinedConfig:
              keep_fraction=0.02, max_val_rmse=0.8 if surrogate == "pose_mlp" else float("inf"), clip_quantile=0.1,
              tail_quantile=0.05, tail_weight=8.0, batch=32768 if mlp else 8192, epochs=20, lr=3e-3 if mlp else 4e-3,
              grid_spacing=0.75 if mlp else 0.5, pipeline_chunks=0, seed=0, train_points=(1 << 22) if mlp else 1 << 21,
-             steps=3000, grid_lr=1.5e-2, gate_z=2.0, gate_top_fraction=0.01, widen_max=0.1)
+             steps=3000, grid_lr=1.5e-2, gate_z=2.5, gate_top_fraction=0.01, widen_max=0.1)
     c.update(kw)
     return CombinedConfig(**c)
 

@@ -107,16 +107,20 @@ struct TrainReport {
   uint32_t steps = 0;
 };
 
-// Training activations of one batch (reused from call to call: an
-// allocation can wait for the kernels in flight).
+// Training buffers of one batch (reused from call to call: an allocation
+// can wait for the kernels in flight).  A step is one fused kernel
+// (forward, loss, backward, grid gradient) between the step's batch
+// indices and its Adam updates; its activations never leave LDS.
 struct Workspace {
   size_t rows = 0;
   int k0 = 0;
-  DeviceBuffer x0, x0t, z[2], a[2], at[2], pred, dz3, dz3t, dz[2], dzt[2], dx0, loss, part, vpred;
-  // weight and bias gradients of layer l as splits[l] K slices (summed in
-  // order by the Adam step)
+  DeviceBuffer pred, loss, part, vpred;
+  // weight and bias gradients of layer l as splits[l] slices, one per
+  // work-group of the step (summed in order by the Adam step); room for
+  // `slices` of them
   DeviceBuffer gpart[3], bpart[3];
   int splits[3] = {1, 1, 1};
+  unsigned slices = 0;
   // per-step state in device memory (step count, learning rates, Adam bias
   // corrections) and the step's batch indices, so that one captured step
   // replays as every step
@@ -136,10 +140,12 @@ struct Workspace {
 };
 int reserve(const Net &net, Workspace &ws, size_t batch, size_t nval);
 
-// Training runs as one captured hipGraph of a whole step replayed `steps`
-// times (every per-step value is computed on the device from the step
-// count), or, with BUDE_MI355X_TRAIN_GRAPH=0 in the environment, as the same
-// kernels launched step by step; both give the same network bit for bit.
+// Training runs as one captured hipGraph of a whole step (batch indices,
+// the fused step kernel, Adam over the weights, Adam over the grid)
+// replayed `steps` times (every per-step value is computed on the device
+// from the step count), or, with BUDE_MI355X_TRAIN_GRAPH=0 in the
+// environment, as the same kernels launched step by step; both give the
+// same network bit for bit.
 // Trains on points[0, ntrain) with labels (energy units), validates on
 // points[ntrain, ntrain + nval); sets mu / sigma / clip from the training
 // labels (target y = (min(E, clip) - mu) / sigma, clip = mu + 8 sigma).

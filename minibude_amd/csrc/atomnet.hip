@@ -48,32 +48,6 @@ __device__ __forceinline__ void cell(float w, float lo, float inv_h, int n, int 
   f = __builtin_amdgcn_fmed3f(u - float(i0), 0.0f, 1.0f);
 }
 
-// Trilinear corners of position (x, y, z): node index of corner c = (dx, dy,
-// dz) bits (x high, z low) and its weight.
-struct Corners {
-  int node[8];
-  float w[8];
-};
-
-__device__ __forceinline__ Corners corners(float x, float y, float z, const field::Grid &g, float inv_h) {
-  int ix, iy, iz;
-  float fx, fy, fz;
-  cell(x, g.lo[0], inv_h, g.n[0], ix, fx);
-  cell(y, g.lo[1], inv_h, g.n[1], iy, fy);
-  cell(z, g.lo[2], inv_h, g.n[2], iz, fz);
-  const int s1 = g.n[2], s0 = g.n[1] * g.n[2];
-  const int base = ix * s0 + iy * s1 + iz;
-  Corners c;
-  const float gx[2] = {1.f - fx, fx}, gy[2] = {1.f - fy, fy}, gz[2] = {1.f - fz, fz};
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const int dx = (k >> 2) & 1, dy = (k >> 1) & 1, dz = k & 1;
-    c.node[k] = base + dx * s0 + dy * s1 + dz;
-    c.w[k] = gx[dx] * gy[dy] * gz[dz];
-  }
-  return c;
-}
-
 // ---- training points and labels ---------------------------------------------
 
 __global__ void sample_kernel(const float4 *__restrict__ ligands, const LigandMeta *__restrict__ meta, int nligands,
@@ -142,76 +116,6 @@ __global__ __launch_bounds__(256) void stats_kernel(const float *__restrict__ la
 }
 
 // ---- training kernels -------------------------------------------------------
-
-// Input rows of points idx[b]: [bf16(phi), onehot(slot), 0 ...] (k0 columns)
-// and the transposed copy x0t [k0][B], from the FP32 grid.
-__global__ void features_kernel(const float4 *__restrict__ points, const uint32_t *__restrict__ idx, size_t B,
-                                const float *__restrict__ grid, const field::Grid g, int k0, bf16 *__restrict__ x0,
-                                bf16 *__restrict__ x0t) {
-  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const float4 p = points[idx ? idx[b] : b];
-  const int slot = __float_as_int(p.w);
-  const Corners c = corners(p.x, p.y, p.z, g, 1.0f / g.h);
-  float phi[kF];
-#pragma unroll
-  for (int k = 0; k < kF; k++) phi[k] = 0.f;
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const float4 *v = reinterpret_cast<const float4 *>(grid + size_t(c.node[q]) * kF);
-#pragma unroll
-    for (int h = 0; h < kF / 4; h++) {
-      const float4 u = v[h];
-      phi[4 * h + 0] = __builtin_fmaf(c.w[q], u.x, phi[4 * h + 0]);
-      phi[4 * h + 1] = __builtin_fmaf(c.w[q], u.y, phi[4 * h + 1]);
-      phi[4 * h + 2] = __builtin_fmaf(c.w[q], u.z, phi[4 * h + 2]);
-      phi[4 * h + 3] = __builtin_fmaf(c.w[q], u.w, phi[4 * h + 3]);
-    }
-  }
-  uint32_t *row = reinterpret_cast<uint32_t *>(x0 + b * size_t(k0));
-#pragma unroll
-  for (int k = 0; k < kF; k += 2) {
-    const uint32_t u = pack_bf16(phi[k], phi[k + 1]);
-    row[k / 2] = u;
-    x0t[size_t(k) * B + b] = bf16(u & 0xffffu);
-    x0t[size_t(k + 1) * B + b] = bf16(u >> 16);
-  }
-  for (int k = kF; k < k0; k += 2) {  // one-hot slot columns (BF16 1.0 = 0x3f80)
-    const bf16 h0 = k == kF + slot ? bf16(0x3f80) : bf16(0), h1 = k + 1 == kF + slot ? bf16(0x3f80) : bf16(0);
-    row[k / 2] = uint32_t(h0) | (uint32_t(h1) << 16);
-    x0t[size_t(k) * B + b] = h0;
-    x0t[size_t(k + 1) * B + b] = h1;
-  }
-}
-
-// Grid gradient: every point's dL/dphi (the first kF columns of dx0
-// [B][32]) spread to its 8 corners with the trilinear weights, accumulated
-// as 2^40 fixed point in 64-bit integer atomics (order-independent, so the
-// training is deterministic).
-// One wave per point, lane = (corner, feature): each atomic instruction
-// then covers the point's 8 corner nodes, 4 cache lines (the two z-corners
-// of a pair share one), where one work-item per (point, corner) spread
-// every instruction over 8 points' 32 lines.
-__global__ __launch_bounds__(256) void grid_grad_kernel(const float4 *__restrict__ points,
-                                                        const uint32_t *__restrict__ idx, size_t B,
-                                                        const field::Grid g, const float *__restrict__ dx0,
-                                                        unsigned long long *__restrict__ acc) {
-  const size_t b = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  if (b >= B) return;
-  const int lane = threadIdx.x & 63, q = lane >> 3, k = lane & 7;
-  const int dx = q >> 2, dy = (q >> 1) & 1, dz = q & 1;
-  const float4 p = points[idx ? idx[b] : b];
-  const float inv_h = 1.0f / g.h;
-  int ix, iy, iz;
-  float fx, fy, fz;
-  cell(p.x, g.lo[0], inv_h, g.n[0], ix, fx);
-  cell(p.y, g.lo[1], inv_h, g.n[1], iy, fy);
-  cell(p.z, g.lo[2], inv_h, g.n[2], iz, fz);
-  const int node = ((ix + dx) * g.n[1] + iy + dy) * g.n[2] + iz + dz;
-  const float w = (dx ? fx : 1.f - fx) * (dy ? fy : 1.f - fy) * (dz ? fz : 1.f - fz);
-  const long long v = llrint(double(w * dx0[b * 32 + k]) * kGradScale);
-  atomicAdd(acc + size_t(node) * kF + k, static_cast<unsigned long long>(v));
-}
 
 // Adam on the grid features from the fixed-point accumulators (which it
 // clears), refreshing the BF16 copy.
@@ -535,84 +439,508 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
     a.out[size_t(m.pose_off) + size_t(j)] = a.sigma * (sy + float(m.natlig) * a.b2) + float(m.natlig) * a.mu;
 }
 
+// ---- fused training step ------------------------------------------------------
+// One launch does a training step's forward, loss and backward: one wave per
+// 64 points of the batch (lane = point while the point's own data are
+// formed, then the MFMA lane layout of the scorer), four waves per
+// work-group.
+//   forward   phi from the FP32 grid (trilinear), layer 0 on MFMA over
+//             K = k0 ([phi, one-hot slot]), ReLU, layer 1 on MFMA with layer
+//             0's accumulators as its B operand (W1^T's K order permuted as
+//             in the scorer), ReLU, y = a1 . W2[:, 0] + b2 (VALU dot over the
+//             lane's neurons, summed over the four lane groups), loss
+//             gradient dz3 = bf16(2 (y - t) / batch).
+//   backward  dz1 = bf16(dz3 W2 * relu'(z1)) elementwise; dz0 =
+//             bf16((dz1 W1^T) * relu'(z0)) on MFMA with dz1 from registers
+//             (W1's K order permuted likewise); dL/dphi = dz0 W0[0..kF)^T on
+//             MFMA; weight and bias gradients X^T dZ as MFMAs over the
+//             wave's 64 points from LDS tiles written transposed (bias
+//             gradients against an all-ones A row); the grid gradient
+//             scattered to the 8 corners in 2^-40 fixed point by 64-bit
+//             integer atomics, one point per instruction (lane = (corner,
+//             feature)).
+// The work-group then sums its four waves' gradients in wave order into its
+// own slice of gw / gb, which the Adam step sums slice by slice in order:
+// the step is deterministic.  BF16 rounding sits where the layer-by-layer
+// kernels put it (inputs, weights, activations, dz), so the oracle of those
+// kernels is the oracle of this one.
+constexpr int kTP = 64 + 8;  // bf16 pitch of a transposed [row][64 points] tile (padding staggers banks)
+
+struct alignas(16) StepRow {
+  uint4 phi;  // BF16 features of the point
+  int node;   // grid node of the cell's lower corner
+  float fx, fy, fz;
+  int slot;
+  float yt;   // standardised target
+  float dz3;  // loss gradient (BF16 value)
+  int pad;
+};
+
+// K slot 8 grp + e of a permuted-K operand holds hidden neuron
+// 16 (e / 4) + 4 grp + e % 4: a row's two 4-element halves.
+__device__ __forceinline__ bf16x8 perm_row(const bf16 *row, int grp) {
+  const uint2 lo = *reinterpret_cast<const uint2 *>(row + 4 * grp), hi = *reinterpret_cast<const uint2 *>(row + 16 + 4 * grp);
+  return __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+}
+
+// The B operand of point block i from per-neuron values held as the
+// accumulators of the two 16-neuron blocks (the permuted K order).
+__device__ __forceinline__ bf16x8 acc_operand(const floatx4 &v0, const floatx4 &v1) {
+  return __builtin_bit_cast(bf16x8, make_uint4(pack_bf16(v0[0], v0[1]), pack_bf16(v0[2], v0[3]),
+                                               pack_bf16(v1[0], v1[1]), pack_bf16(v1[2], v1[3])));
+}
+
+__device__ __forceinline__ float bf16_round(float x) { return lo_bf16(pack_bf16(x, 0.f)); }
+
+struct StepArgs {
+  const float4 *points;
+  const uint32_t *idx;    // batch rows -> points (null: rows are points)
+  const float *labels;    // exact one-atom energy of every point
+  size_t n;               // rows in this launch (a multiple of 64)
+  const float *grid;      // FP32 grid features [nodes][kF]
+  field::Grid g;
+  const bf16 *w0t, *w0;   // W0^T [kH][k0], W0 [k0][kH]
+  const bf16 *w1t, *w1;   // W1^T, W1 [kH][kH]
+  const bf16 *w2;         // W2 [kH][kOut] (column 0)
+  const float *b0, *b1, *b2;
+  mlp::TargetNorm norm;
+  float grad_scale;       // 2 / batch
+  float *pred;            // optional: y of every row
+  unsigned long long *loss;  // optional: fixed-point sum of squared errors (mlp::kLossScale)
+  int backward;
+  unsigned long long *gacc;  // grid gradient accumulators [nodes][kF]
+  float *gw[3], *gb[3];   // this launch's work-group slices: gw[l] [nwg][K_l][N_l], gb[l] [nwg][N_l]
+};
+
+template <int K0S>
+__global__ __launch_bounds__(256, 2) void step_kernel(const StepArgs a) {
+  constexpr int K0 = 32 * K0S;
+  constexpr int NP = K0 * kH + kH * kH + 3 * kH + 1;  // a wave's partial gradients (dW2, db2: column 0)
+  __shared__ StepRow rows[4][64];
+  __shared__ __align__(16) float dphi[4][64 * kF];
+  __shared__ __align__(16) bf16 tt[4][64 * kTP + kH * kTP];  // transposed tiles; the wave partials at the end
+  static_assert(sizeof(tt) >= 4 * NP * sizeof(float), "reduction buffer");
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, grp = lane >> 4;
+  const size_t tile = size_t(blockIdx.x) * 4 + wave;
+  const bool active = tile * 64 < a.n;
+  StepRow *R = rows[wave];
+  bf16 *U = tt[wave];
+  const bf16x8 zero8 = bf16x8{};
+  float sq = 0.f;
+  // this wave's partial gradients, in MFMA output layouts
+  floatx4 gW0[K0 / 16][2], gW1[2][2], gW2[2], gB0[2], gB1[2];
+  float gB2 = 0.f;
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+#pragma unroll
+    for (int kb = 0; kb < K0 / 16; kb++) gW0[kb][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    gW1[0][b] = gW1[1][b] = gW2[b] = gB0[b] = gB1[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+  }
+  if (active) {
+    const float inv_h = 1.0f / a.g.h;
+    const int s1 = a.g.n[2], s0 = a.g.n[1] * a.g.n[2];
+    // ---- the point of this lane: features, target
+    {
+      const size_t row = tile * 64 + lane;
+      const uint32_t q = a.idx ? a.idx[row] : uint32_t(row);
+      const float4 p = a.points[q];
+      const float e = a.labels[q];
+      int ix, iy, iz;
+      float fx, fy, fz;
+      cell(p.x, a.g.lo[0], inv_h, a.g.n[0], ix, fx);
+      cell(p.y, a.g.lo[1], inv_h, a.g.n[1], iy, fy);
+      cell(p.z, a.g.lo[2], inv_h, a.g.n[2], iz, fz);
+      const int node = ix * s0 + iy * s1 + iz;
+      const float gx[2] = {1.f - fx, fx}, gy[2] = {1.f - fy, fy}, gz[2] = {1.f - fz, fz};
+      float phi[kF];
+#pragma unroll
+      for (int k = 0; k < kF; k++) phi[k] = 0.f;
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
+        const float w = gx[dx] * gy[dy] * gz[dz];
+        const float4 *v = reinterpret_cast<const float4 *>(a.grid + size_t(node + dx * s0 + dy * s1 + dz) * kF);
+        const float4 u0 = v[0], u1 = v[1];
+        phi[0] = __builtin_fmaf(w, u0.x, phi[0]);
+        phi[1] = __builtin_fmaf(w, u0.y, phi[1]);
+        phi[2] = __builtin_fmaf(w, u0.z, phi[2]);
+        phi[3] = __builtin_fmaf(w, u0.w, phi[3]);
+        phi[4] = __builtin_fmaf(w, u1.x, phi[4]);
+        phi[5] = __builtin_fmaf(w, u1.y, phi[5]);
+        phi[6] = __builtin_fmaf(w, u1.z, phi[6]);
+        phi[7] = __builtin_fmaf(w, u1.w, phi[7]);
+      }
+      StepRow row_{};
+      row_.phi = make_uint4(pack_bf16(phi[0], phi[1]), pack_bf16(phi[2], phi[3]), pack_bf16(phi[4], phi[5]),
+                            pack_bf16(phi[6], phi[7]));
+      row_.node = node;
+      row_.fx = fx;
+      row_.fy = fy;
+      row_.fz = fz;
+      row_.slot = __float_as_int(p.w);
+      row_.yt = (fminf(e, a.norm.clip) - a.norm.mu) * a.norm.inv_sigma;
+      R[lane] = row_;
+    }
+    wave_sync();
+    // ---- layer 0: [phi, one-hot] (B, from the rows) x W0^T (A, from L2)
+    floatx4 z0[2][4];
+    {
+      bf16x8 fw0[2][K0S];
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+        for (int s = 0; s < K0S; s++)
+          fw0[jn][s] = *reinterpret_cast<const bf16x8 *>(a.w0t + size_t(16 * jn + r) * K0 + 32 * s + 8 * grp);
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++) {
+        const float4 b = *reinterpret_cast<const float4 *>(a.b0 + 16 * jn + 4 * grp);
+#pragma unroll
+        for (int i = 0; i < 4; i++) z0[jn][i] = floatx4{b.x, b.y, b.z, b.w};
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const StepRow &pr = R[16 * i + r];
+        const uint4 ph = pr.phi;
+        const int slot = pr.slot;
+#pragma unroll
+        for (int s = 0; s < K0S; s++) {
+          // the one-hot column kF + slot, if it falls in this lane's 8 columns
+          const int e = kF + slot - (32 * s + 8 * grp);
+          const uint32_t one = (e >= 0 && e < 8) ? (0x3f80u << (16 * (e & 1))) : 0u;
+          const int wsel = e >> 1;
+          uint4 v = make_uint4(wsel == 0 ? one : 0u, wsel == 1 ? one : 0u, wsel == 2 ? one : 0u, wsel == 3 ? one : 0u);
+          if (s == 0 && grp == 0) v = ph;  // (columns 0..7: the features; the slot column is >= kF)
+          const bf16x8 fb = __builtin_bit_cast(bf16x8, v);
+#pragma unroll
+          for (int jn = 0; jn < 2; jn++) z0[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw0[jn][s], fb, z0[jn][i], 0, 0, 0);
+        }
+      }
+    }
+    // ReLU masks (bit (jn * 4 + i) * 4 + c) and the BF16 activations
+    uint32_t m0 = 0, m1 = 0;
+    floatx4 a0[2][4];
+#pragma unroll
+    for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const float z = z0[jn][i][c];
+          m0 |= uint32_t(z > 0.f) << ((jn * 4 + i) * 4 + c);
+          a0[jn][i][c] = bf16_round(z > 0.f ? z : 0.f);
+        }
+    // ---- layer 1
+    floatx4 a1[2][4];
+    {
+      bf16x8 fw1[2];
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++) fw1[jn] = perm_row(a.w1t + size_t(16 * jn + r) * kH, grp);
+      floatx4 z1[2][4];
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++) {
+        const float4 b = *reinterpret_cast<const float4 *>(a.b1 + 16 * jn + 4 * grp);
+#pragma unroll
+        for (int i = 0; i < 4; i++) z1[jn][i] = floatx4{b.x, b.y, b.z, b.w};
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const bf16x8 fb = acc_operand(a0[0][i], a0[1][i]);
+#pragma unroll
+        for (int jn = 0; jn < 2; jn++) z1[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw1[jn], fb, z1[jn][i], 0, 0, 0);
+      }
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const float z = z1[jn][i][c];
+            m1 |= uint32_t(z > 0.f) << ((jn * 4 + i) * 4 + c);
+            a1[jn][i][c] = bf16_round(z > 0.f ? z : 0.f);
+          }
+    }
+    // ---- output, loss gradient
+    float w2[2][4];
+#pragma unroll
+    for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) w2[jn][c] = lo_bf16(uint32_t(a.w2[size_t(16 * jn + 4 * grp + c) * kOut]));
+    const float b2 = a.b2[0];
+    float dz3[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      float part = 0.f;
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) part = __builtin_fmaf(a1[jn][i][c], w2[jn][c], part);
+      part += __shfl_xor(part, 16);
+      part += __shfl_xor(part, 32);
+      const float y = part + b2;
+      const float d = y - R[16 * i + r].yt;
+      if (grp == i) {
+        sq += d * d;
+        if (a.pred) a.pred[tile * 64 + 16 * i + r] = y;
+      }
+      dz3[i] = bf16_round(a.grad_scale * d);
+    }
+    if (a.backward) {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        if (grp == i) R[16 * i + r].dz3 = dz3[i];
+      // transposed activations: A0T [n][p] at U, A1T at U + kH kTP
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const int n = 16 * jn + 4 * grp + c, p = 16 * i + r;
+            U[n * kTP + p] = bf16(pack_bf16(a0[jn][i][c], 0.f));
+            U[(kH + n) * kTP + p] = bf16(pack_bf16(a1[jn][i][c], 0.f));
+          }
+      // dz1 = dz3 W2 * relu'(z1), exact product then BF16
+      floatx4 dz1[2][4];
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            dz1[jn][i][c] = ((m1 >> ((jn * 4 + i) * 4 + c)) & 1u) ? bf16_round(dz3[i] * w2[jn][c]) : 0.f;
+      wave_sync();
+      // dW2 = A1T . dz3 (column 0), db2
+      const bf16x8 ones = __builtin_bit_cast(bf16x8, make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u));
+      const bf16x8 onesA = r == 0 ? ones : zero8;
+#pragma unroll
+      for (int s = 0; s < 2; s++) {
+        uint32_t dv[4] = {0u, 0u, 0u, 0u};
+        if (r == 0)
+#pragma unroll
+          for (int h = 0; h < 4; h++)
+            dv[h] = pack_bf16(R[32 * s + 8 * grp + 2 * h].dz3, R[32 * s + 8 * grp + 2 * h + 1].dz3);
+        const bf16x8 fb = __builtin_bit_cast(bf16x8, make_uint4(dv[0], dv[1], dv[2], dv[3]));
+#pragma unroll
+        for (int jn = 0; jn < 2; jn++) {
+          const bf16x8 fa = lds8(U + (kH + 16 * jn + r) * kTP + 32 * s + 8 * grp);
+          gW2[jn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, gW2[jn], 0, 0, 0);
+        }
+      }
+      {
+        float v = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) v += grp == i ? dz3[i] : 0.f;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
+        gB2 = v;
+      }
+      wave_sync();
+      // DZ1T [n][p] over A1T
+#pragma unroll
+      for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            U[(kH + 16 * jn + 4 * grp + c) * kTP + 16 * i + r] = bf16(pack_bf16(dz1[jn][i][c], 0.f));
+      wave_sync();
+      // dW1 = A0T . DZ1T^T, db1 = ones . DZ1T^T
+#pragma unroll
+      for (int s = 0; s < 2; s++) {
+        bf16x8 fa[2], fb[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          fa[j] = lds8(U + (16 * j + r) * kTP + 32 * s + 8 * grp);
+          fb[j] = lds8(U + (kH + 16 * j + r) * kTP + 32 * s + 8 * grp);
+        }
+#pragma unroll
+        for (int jo = 0; jo < 2; jo++) {
+#pragma unroll
+          for (int ji = 0; ji < 2; ji++) gW1[ji][jo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[ji], fb[jo], gW1[ji][jo], 0, 0, 0);
+          gB1[jo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(onesA, fb[jo], gB1[jo], 0, 0, 0);
+        }
+      }
+      // dz0 = (dz1 W1^T) * relu'(z0): A = W1 rows (K permuted), B = dz1 from registers
+      floatx4 dz0[2][4];
+      {
+        bf16x8 fw[2];
+#pragma unroll
+        for (int ji = 0; ji < 2; ji++) fw[ji] = perm_row(a.w1 + size_t(16 * ji + r) * kH, grp);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const bf16x8 fb = acc_operand(dz1[0][i], dz1[1][i]);
+#pragma unroll
+          for (int ji = 0; ji < 2; ji++) {
+            const floatx4 v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw[ji], fb, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < 4; c++) dz0[ji][i][c] = ((m0 >> ((ji * 4 + i) * 4 + c)) & 1u) ? bf16_round(v[c]) : 0.f;
+          }
+        }
+      }
+      // dL/dphi = dz0 W0[0..kF)^T: A = W0 rows 0..kF-1 (K permuted), rows kF..15 zero
+      {
+        const bf16x8 fw = r < kF ? perm_row(a.w0 + size_t(r) * kH, grp) : zero8;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const floatx4 v = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw, acc_operand(dz0[0][i], dz0[1][i]),
+                                                                    floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+          if (grp < 2) *reinterpret_cast<float4 *>(&dphi[wave][(16 * i + r) * kF + 4 * grp]) = make_float4(v[0], v[1], v[2], v[3]);
+        }
+      }
+      wave_sync();
+      // X0T [k][p] (k < k0) at U, DZ0T [n][p] at U + 64 kTP
+      {
+        uint4 *z = reinterpret_cast<uint4 *>(U);
+        for (int t = lane; t < K0 * kTP / 8; t += 64) z[t] = make_uint4(0u, 0u, 0u, 0u);
+        const uint4 ph = R[lane].phi;
+        const uint32_t pw[4] = {ph.x, ph.y, ph.z, ph.w};
+#pragma unroll
+        for (int k = 0; k < kF; k++) U[k * kTP + lane] = bf16((pw[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+        U[(kF + R[lane].slot) * kTP + lane] = bf16(0x3f80);
+#pragma unroll
+        for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+              U[(64 + 16 * jn + 4 * grp + c) * kTP + 16 * i + r] = bf16(pack_bf16(dz0[jn][i][c], 0.f));
+      }
+      wave_sync();
+      // dW0 = X0T . DZ0T^T, db0 = ones . DZ0T^T
+#pragma unroll
+      for (int s = 0; s < 2; s++) {
+        bf16x8 fb[2];
+#pragma unroll
+        for (int jn = 0; jn < 2; jn++) fb[jn] = lds8(U + (64 + 16 * jn + r) * kTP + 32 * s + 8 * grp);
+#pragma unroll
+        for (int kb = 0; kb < K0 / 16; kb++) {
+          const bf16x8 fa = lds8(U + (16 * kb + r) * kTP + 32 * s + 8 * grp);
+#pragma unroll
+          for (int jn = 0; jn < 2; jn++) gW0[kb][jn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[jn], gW0[kb][jn], 0, 0, 0);
+        }
+#pragma unroll
+        for (int jn = 0; jn < 2; jn++) gB0[jn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(onesA, fb[jn], gB0[jn], 0, 0, 0);
+      }
+      // grid gradient: one point per instruction, lane = (corner, feature)
+      {
+        const int cq = lane >> 3, k = lane & 7;
+        const int dx = cq >> 2, dy = (cq >> 1) & 1, dz = cq & 1;
+        const int off = dx * s0 + dy * s1 + dz;
+        for (int pt = 0; pt < 64; pt++) {
+          const StepRow &pr = R[pt];
+          const float w = (dx ? pr.fx : 1.f - pr.fx) * (dy ? pr.fy : 1.f - pr.fy) * (dz ? pr.fz : 1.f - pr.fz);
+          const long long v = llrint(double(w * dphi[wave][pt * kF + k]) * kGradScale);
+          atomicAdd(a.gacc + size_t(pr.node + off) * kF + k, static_cast<unsigned long long>(v));
+        }
+      }
+    }
+  }
+  // squared error of the wave's points (fixed point, as the layer kernels' loss)
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off);
+  if (lane == 0 && active && a.loss)
+    atomicAdd(a.loss, static_cast<unsigned long long>(fmin(double(sq) * mlp::kLossScale + 0.5, 1099511627776.0)));
+  if (!a.backward) return;  // (uniform over the launch: no barrier below is reached)
+  // ---- the work-group's gradients: each wave's partials to LDS, summed in wave order
+  __syncthreads();
+  float *red = reinterpret_cast<float *>(&tt[0][0]);
+  float *P = red + wave * NP;
+  // layout: dW0 [K0][kH], dW1 [kH][kH], dW2 [kH], db0 [kH], db1 [kH], db2
+#pragma unroll
+  for (int kb = 0; kb < K0 / 16; kb++)
+#pragma unroll
+    for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) P[(16 * kb + 4 * grp + c) * kH + 16 * jn + r] = gW0[kb][jn][c];
+#pragma unroll
+  for (int ji = 0; ji < 2; ji++)
+#pragma unroll
+    for (int jo = 0; jo < 2; jo++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) P[K0 * kH + (16 * ji + 4 * grp + c) * kH + 16 * jo + r] = gW1[ji][jo][c];
+#pragma unroll
+  for (int jn = 0; jn < 2; jn++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (r == 0) P[K0 * kH + kH * kH + 16 * jn + 4 * grp + c] = gW2[jn][c];
+      if (grp == 0 && c == 0) {
+        P[K0 * kH + kH * kH + kH + 16 * jn + r] = gB0[jn][0];
+        P[K0 * kH + kH * kH + 2 * kH + 16 * jn + r] = gB1[jn][0];
+      }
+    }
+  if (lane == 0) P[NP - 1] = gB2;
+  __syncthreads();
+  const size_t wg = blockIdx.x;
+  for (int t = tid; t < K0 * kH; t += 256) a.gw[0][wg * K0 * kH + t] = ((red[t] + red[NP + t]) + red[2 * NP + t]) + red[3 * NP + t];
+  for (int t = tid; t < kH * kH; t += 256) {
+    const int o = K0 * kH + t;
+    a.gw[1][wg * kH * kH + t] = ((red[o] + red[NP + o]) + red[2 * NP + o]) + red[3 * NP + o];
+  }
+  for (int t = tid; t < kH * kOut; t += 256) {  // W2 [kH][kOut]: column 0
+    const int o = K0 * kH + kH * kH + t / kOut;
+    a.gw[2][wg * kH * kOut + t] = t % kOut ? 0.f : ((red[o] + red[NP + o]) + red[2 * NP + o]) + red[3 * NP + o];
+  }
+  if (tid < kH) {
+    const int o0 = K0 * kH + kH * kH + kH + tid, o1 = o0 + kH;
+    a.gb[0][wg * kH + tid] = ((red[o0] + red[NP + o0]) + red[2 * NP + o0]) + red[3 * NP + o0];
+    a.gb[1][wg * kH + tid] = ((red[o1] + red[NP + o1]) + red[2 * NP + o1]) + red[3 * NP + o1];
+  }
+  if (tid < kOut) {
+    const int o = NP - 1;
+    a.gb[2][wg * kOut + tid] = tid ? 0.f : ((red[o] + red[NP + o]) + red[2 * NP + o]) + red[3 * NP + o];
+  }
+}
+
 unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
 
 uint64_t mix_seed(uint64_t seed, uint64_t k) { return (seed ^ (k * 0xd6e8feb86659fd93ull)) * 0x9e3779b97f4a7c15ull; }
 
-mlp::GemmArgs gemm_args(const void *a, const void *b, int M, int N, int K, int lda, int ldb, int epi) {
-  mlp::GemmArgs g{};
-  g.a = static_cast<const bf16 *>(a);
-  g.b = static_cast<const bf16 *>(b);
-  g.M = M;
-  g.N = N;
-  g.K = K;
-  g.lda = lda;
-  g.ldb = ldb;
-  g.epi = epi;
-  return g;
-}
-
 using Acts = Workspace;
 
-// Forward of B rows whose inputs are in A.x0; training keeps z, a, a^T.
-int forward(const Net &net, Acts &A, size_t B, bool training, hipStream_t s) {
-  const int R = int(B);
-  const void *in = A.x0.ptr;
-  int in_ld = net.k0;
-  for (int l = 0; l < 2; l++) {
-    auto g = gemm_args(in, net.w16t[l].ptr, R, kH, layer_k(net, l), in_ld, layer_k(net, l), mlp::kEpiHiddenRelu);
-    g.bias = static_cast<const float *>(net.b[l].ptr);
-    g.out_f32 = training ? static_cast<float *>(A.z[l].ptr) : nullptr;
-    g.out_bf16 = static_cast<bf16 *>(A.a[l].ptr);
-    g.out_t = training ? static_cast<bf16 *>(A.at[l].ptr) : nullptr;
-    g.ldo = kH;
-    g.ldt = R;
-    if (int st = hip_check(mlp::gemm(g, s), "atomnet gemm forward")) return st;
-    in = g.out_bf16;
-    in_ld = kH;
-  }
-  auto g = gemm_args(in, net.w16t[2].ptr, R, kOut, kH, kH, kH, mlp::kEpiOut);
-  g.bias = static_cast<const float *>(net.b[2].ptr);
-  g.out_f32 = static_cast<float *>(A.pred.ptr);
-  return hip_check(mlp::gemm(g, s), "atomnet gemm output");
-}
-
-// Backward from dz3 (written by the loss kernel): weight and bias gradients
-// as K slices into A.gpart / bpart, dL/dx0 (first 32 columns) into A.dx0.
-int backward(Net &net, Acts &A, size_t B, hipStream_t s) {
-  const int R = int(B);
-  const void *dzt_next = A.dz3t.ptr, *dz_next = A.dz3.ptr;
-  for (int l = 2; l >= 0; l--) {
-    const int K = layer_k(net, l), N = layer_n(l);
-    const void *prev_t = l == 0 ? A.x0t.ptr : A.at[l - 1].ptr;
-    // dW_l = A_{l-1}^T dZ_l and db_l = column sums of dZ_l, per K slice
-    auto gw = gemm_args(prev_t, dzt_next, K, N, R, R, R, mlp::kEpiF32);
-    gw.ldo = N;
-    gw.ksplit = mlp::auto_ksplit(K, N, R);
-    gw.work = gw.out_f32 = static_cast<float *>(A.gpart[l].ptr);
-    gw.bias_part = static_cast<float *>(A.bpart[l].ptr);
-    gw.defer = 1;
-    A.splits[l] = gw.ksplit;
-    if (int st = hip_check(mlp::gemm(gw, s), "atomnet gemm dW")) return st;
-    if (l == 0) {
-      // dL/dx0 for the grid: dz1 W0^T over the first 32 rows of W0
-      auto gx = gemm_args(dz_next, net.w16[0].ptr, R, 32, kH, kH, kH, mlp::kEpiF32);
-      gx.out_f32 = static_cast<float *>(A.dx0.ptr);
-      gx.ldo = 32;
-      if (int st = hip_check(mlp::gemm(gx, s), "atomnet gemm dx0")) return st;
-      break;
+// The fused step over rows [0, n) of `points` (through idx, if set):
+// forward (y into pred, if set; squared error into A.loss), and with
+// `backward` the gradients -- grid into net.gacc, weights and biases as
+// one slice per work-group into A.gpart / bpart (A.splits slices).
+int launch_step(Net &net, Acts &A, const float4 *points, const uint32_t *idx, const float *labels, size_t n,
+                size_t batch, mlp::TargetNorm norm, float *pred, bool backward, hipStream_t s) {
+  if (n == 0 || n % 64) return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: rows must be a multiple of 64");
+  StepArgs a{};
+  a.points = points;
+  a.idx = idx;
+  a.labels = labels;
+  a.n = n;
+  a.grid = static_cast<const float *>(net.g.ptr);
+  a.g = net.grid;
+  a.w0t = static_cast<const bf16 *>(net.w16t[0].ptr);
+  a.w0 = static_cast<const bf16 *>(net.w16[0].ptr);
+  a.w1t = static_cast<const bf16 *>(net.w16t[1].ptr);
+  a.w1 = static_cast<const bf16 *>(net.w16[1].ptr);
+  a.w2 = static_cast<const bf16 *>(net.w16[2].ptr);
+  a.b0 = static_cast<const float *>(net.b[0].ptr);
+  a.b1 = static_cast<const float *>(net.b[1].ptr);
+  a.b2 = static_cast<const float *>(net.b[2].ptr);
+  a.norm = norm;
+  a.grad_scale = 2.0f / float(batch);
+  a.pred = pred;
+  a.loss = backward ? static_cast<unsigned long long *>(A.loss.ptr) : nullptr;
+  a.backward = backward ? 1 : 0;
+  a.gacc = static_cast<unsigned long long *>(net.gacc.ptr);
+  const unsigned blocks = blocks_for(n, 256);
+  if (backward) {
+    if (A.slices < blocks) return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: workspace sized for fewer rows");
+    for (int l = 0; l < 3; l++) {
+      a.gw[l] = static_cast<float *>(A.gpart[l].ptr);
+      a.gb[l] = static_cast<float *>(A.bpart[l].ptr);
+      A.splits[l] = int(blocks);
     }
-    auto ga = gemm_args(dz_next, net.w16[l].ptr, R, kH, N, N, N, mlp::kEpiBackRelu);
-    ga.zprev = static_cast<const float *>(A.z[l - 1].ptr);
-    ga.out_bf16 = static_cast<bf16 *>(A.dz[l - 1].ptr);
-    ga.out_t = static_cast<bf16 *>(A.dzt[l - 1].ptr);
-    ga.ldo = kH;
-    ga.ldt = R;
-    if (int st = hip_check(mlp::gemm(ga, s), "atomnet gemm dA")) return st;
-    dz_next = A.dz[l - 1].ptr;
-    dzt_next = A.dzt[l - 1].ptr;
   }
-  return BUDE_OK;
+  if (net.k0 == 32) {
+    hipLaunchKernelGGL(step_kernel<1>, dim3(blocks), dim3(256), 0, s, a);
+  } else if (net.k0 == 64) {
+    hipLaunchKernelGGL(step_kernel<2>, dim3(blocks), dim3(256), 0, s, a);
+  } else {
+    return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: input width must be 32 or 64");
+  }
+  return hip_check(hipGetLastError(), "atomnet step");
 }
 
 // Adam on every parameter with this step's values from the device state,
@@ -640,25 +968,6 @@ int adam_step_dev(Net &net, const Acts &A, mlp::StepState *state, hipStream_t s)
   return hip_check(hipGetLastError(), "atomnet grid adam");
 }
 
-// Forward + loss + backward of one batch (grid gradients accumulated in net.gacc).
-int step_grads(Net &net, Acts &A, const float4 *points, const uint32_t *idx, const float *labels, size_t B,
-               mlp::TargetNorm norm, hipStream_t s) {
-  hipLaunchKernelGGL(features_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, s, points, idx, B,
-                     static_cast<const float *>(net.g.ptr), net.grid, net.k0, static_cast<bf16 *>(A.x0.ptr),
-                     static_cast<bf16 *>(A.x0t.ptr));
-  if (int st = hip_check(hipGetLastError(), "atomnet features")) return st;
-  if (int st = forward(net, A, B, true, s)) return st;
-  if (int st = hip_check(mlp::loss_grad(static_cast<const float *>(A.pred.ptr), labels, idx, B, norm, nullptr, nullptr,
-                                        1.f, static_cast<bf16 *>(A.dz3.ptr), static_cast<bf16 *>(A.dz3t.ptr), int(B),
-                                        static_cast<unsigned long long *>(A.loss.ptr), s),
-                         "atomnet loss"))
-    return st;
-  if (int st = backward(net, A, B, s)) return st;
-  hipLaunchKernelGGL(grid_grad_kernel, dim3(blocks_for(B * 64, 256)), dim3(256), 0, s, points, idx, B, net.grid,
-                     static_cast<const float *>(A.dx0.ptr), static_cast<unsigned long long *>(net.gacc.ptr));
-  return hip_check(hipGetLastError(), "atomnet grid grad");
-}
-
 int alloc_zero(DeviceBuffer &d, size_t bytes) {
   if (int st = d.alloc(bytes)) return st;
   return hip_check(hipMemset(d.ptr, 0, bytes), "hipMemset");
@@ -667,22 +976,11 @@ int alloc_zero(DeviceBuffer &d, size_t bytes) {
 }  // namespace
 
 int reserve(const Net &net, Workspace &A, size_t B, size_t nval) {
+  const size_t slices = (B + 255) / 256;
   if (A.rows >= B && A.k0 == net.k0 && A.vpred.bytes >= std::max<size_t>(nval, 1) * 4) return BUDE_OK;
   A.rows = B;
   A.k0 = net.k0;
-  int st = A.x0.reserve(B * net.k0 * 2);
-  if (!st) st = A.x0t.reserve(B * net.k0 * 2);
-  for (int l = 0; l < 2 && !st; l++) {
-    st = A.z[l].reserve(B * kH * 4);
-    if (!st) st = A.a[l].reserve(B * kH * 2);
-    if (!st) st = A.at[l].reserve(B * kH * 2);
-    if (!st) st = A.dz[l].reserve(B * kH * 2);
-    if (!st) st = A.dzt[l].reserve(B * kH * 2);
-  }
-  if (!st) st = A.pred.reserve(B * 4);
-  if (!st) st = A.dz3.reserve(B * kOut * 2);
-  if (!st) st = A.dz3t.reserve(B * kOut * 2);
-  if (!st) st = A.dx0.reserve(B * 32 * 4);
+  int st = A.pred.reserve(B * 4);
   if (!st) st = A.loss.reserve(256);
   if (!st) st = A.part.reserve(1024 * 2 * 8);
   if (!st) st = A.vpred.reserve(std::max<size_t>(nval, 1) * 4);
@@ -690,10 +988,10 @@ int reserve(const Net &net, Workspace &A, size_t B, size_t nval) {
   if (!st) st = A.idx.reserve(B * 4);
   for (int l = 0; l < 3 && !st; l++) {
     const int K = layer_k(net, l), N = layer_n(l);
-    const size_t cap = mlp::max_ksplit(K, N);
-    st = A.gpart[l].reserve(cap * K * N * 4);
-    if (!st) st = A.bpart[l].reserve(cap * N * 4);
+    st = A.gpart[l].reserve(slices * K * N * 4);
+    if (!st) st = A.bpart[l].reserve(slices * N * 4);
   }
+  if (!st) A.slices = unsigned(slices);
   return st;
 }
 
@@ -845,7 +1143,7 @@ int train(Net &net, Workspace &A, const float4 *points, const float *labels, con
                                            p.grid_lr, 0.9f, 0.999f, step0, s),
                            "atomnet begin step"))
       return st;
-    if (int st = step_grads(net, A, points, idx, labels, B, norm, s)) return st;
+    if (int st = launch_step(net, A, points, idx, labels, B, B, norm, nullptr, true, s)) return st;
     return adam_step_dev(net, A, state, s);
   };
   if (int st = hip_check(hipMemsetAsync(A.state.ptr, 0, sizeof(mlp::StepState), s), "hipMemsetAsync")) return st;
@@ -872,22 +1170,13 @@ int train(Net &net, Workspace &A, const float4 *points, const float *labels, con
   r.steps = steps;
   unsigned long long lsum = 0;
   if (int st = hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return st;
-  // held-out RMSE in energy units (forward of the training kernels)
+  // held-out RMSE in energy units (the training step's forward)
   double se = 0;
   if (nval) {
     DeviceBuffer &vpred = A.vpred;
-    for (size_t v0 = 0; v0 < nval; v0 += B) {
-      const size_t nb = std::min(B, nval - v0);
-      hipLaunchKernelGGL(features_kernel, dim3(blocks_for(nb, 256)), dim3(256), 0, s, points + ntrain + v0, nullptr, nb,
-                         static_cast<const float *>(net.g.ptr), net.grid, net.k0, static_cast<bf16 *>(A.x0.ptr),
-                         static_cast<bf16 *>(A.x0t.ptr));
-      if (int st = hip_check(hipGetLastError(), "atomnet features")) return st;
-      if (int st = forward(net, A, nb, false, s)) return st;
-      if (int st = hip_check(hipMemcpyAsync(static_cast<float *>(vpred.ptr) + v0, A.pred.ptr, nb * 4,
-                                            hipMemcpyDeviceToDevice, s),
-                             "hipMemcpyAsync"))
-        return st;
-    }
+    if (int st = launch_step(net, A, points + ntrain, nullptr, labels + ntrain, nval, B, norm,
+                             static_cast<float *>(vpred.ptr), false, s))
+      return st;
     hipLaunchKernelGGL(sq_err_kernel, dim3(256), dim3(256), 0, s, static_cast<const float *>(vpred.ptr),
                        labels + ntrain, nval, net.sigma, net.mu, net.clip, static_cast<double *>(A.part.ptr));
     if (int st = hip_check(hipGetLastError(), "atomnet val")) return st;
@@ -913,7 +1202,9 @@ int forward_backward(Net &net, const float4 *points, const float *target, size_t
   if (int st = reserve(net, A, batch, 0)) return st;
   if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 8, s), "hipMemsetAsync")) return st;
   const mlp::TargetNorm norm{INFINITY, 0.f, 1.f, -INFINITY};
-  if (int st = step_grads(net, A, points, nullptr, target, batch, norm, s)) return st;
+  if (int st = launch_step(net, A, points, nullptr, target, batch, batch, norm, static_cast<float *>(A.pred.ptr), true,
+                           s))
+    return st;
   // the K slices of the weight and bias gradients, summed in order into net.gw / gb
   for (int l = 0; l < 3; l++) {
     const size_t K = size_t(layer_k(net, l)), N = size_t(layer_n(l));

@@ -32,8 +32,6 @@ enum Epilogue : int {
   kEpiOut = 1,     // pred[m] = acc[m][0] + b[0]
   kEpiBackGelu = 2,  // dz = acc * gelu'(zprev[m][n]); store dz (bf16) and dz^T (bf16)
   kEpiF32 = 3,     // store acc (f32), e.g. a weight gradient
-  kEpiHiddenRelu = 4,  // as kEpiHidden with a = max(z, 0)
-  kEpiBackRelu = 5,    // as kEpiBackGelu with the ReLU derivative (zprev > 0)
 };
 
 struct GemmArgs {

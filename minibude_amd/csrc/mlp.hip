@@ -157,13 +157,13 @@ __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
       for (int q = 0; q < 4; q++) {
         const size_t o = size_t(mb + q) * g.ldo + n;
         const float c = acc[i][j][q];
-        if (g.epi == kEpiHidden || g.epi == kEpiHiddenRelu) {
+        if (g.epi == kEpiHidden) {
           const float z = c + g.bias[n];
           if (g.out_f32) g.out_f32[o] = z;
-          t[q] = to_bf16(g.epi == kEpiHidden ? gelu(z) : fmaxf(z, 0.f));
+          t[q] = to_bf16(gelu(z));
           g.out_bf16[o] = t[q];
-        } else if (g.epi == kEpiBackGelu || g.epi == kEpiBackRelu) {
-          t[q] = to_bf16(g.epi == kEpiBackGelu ? c * gelu_grad(g.zprev[o]) : (g.zprev[o] > 0.f ? c : 0.f));
+        } else if (g.epi == kEpiBackGelu) {
+          t[q] = to_bf16(c * gelu_grad(g.zprev[o]));
           g.out_bf16[o] = t[q];
         } else if (g.ksplit > 1 || g.defer) {
           g.work[size_t(blockIdx.z) * size_t(g.M) * g.ldo + o] = c;
