@@ -85,65 +85,75 @@ __device__ __forceinline__ void pose_features(const float *__restrict__ poses, s
   f[20] = cx * cy;
 }
 
+// One wave per (16 BM) x (16 BN) tile of C: BM x BN MFMA blocks, the next K
+// step's BM + BN fragments loaded while the current ones multiply.  32 x 32
+// (BM = BN = 2) for the weight gradients and small products; 64 x 64 for
+// the large activation products, which then read their A rows half as
+// often and issue 16 MFMAs per K step.
+template <int BM, int BN>
 __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
   const int lane = threadIdx.x;
-  const int n0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int n0 = blockIdx.x * 16 * BN, m0 = blockIdx.y * 16 * BM;
   const int kc = g.K / g.ksplit, kbeg = blockIdx.z * kc, kend = kbeg + kc;
   const int r = lane & 15, grp = lane >> 4;
-  const bf16 *a0 = g.a + size_t(m0 + r) * g.lda + 8 * grp;
-  const bf16 *a1 = a0 + size_t(16) * g.lda;
-  const bf16 *b0 = g.b + size_t(n0 + r) * g.ldb + 8 * grp;
-  const bf16 *b1 = b0 + size_t(16) * g.ldb;
-  floatx4 acc[2][2];
+  const bf16 *ap = g.a + size_t(m0 + r) * g.lda + 8 * grp;
+  const bf16 *bp = g.b + size_t(n0 + r) * g.ldb + 8 * grp;
+  floatx4 acc[BM][BN];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < BM; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < BN; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   // row sums of B over this K slice (the bias gradient when B is dZ^T),
   // by the waves of the first row of tiles: lane (r, grp) sums its own
   // fragment columns, the four lane groups are added at the end
   const bool bsum = g.bias_part && blockIdx.y == 0;
-  float bs0 = 0.f, bs1 = 0.f;
+  float bs[BN];
+#pragma unroll
+  for (int j = 0; j < BN; j++) bs[j] = 0.f;
 
-  bf16x8 fa0 = load8(a0 + kbeg), fa1 = load8(a1 + kbeg);
-  bf16x8 fb0 = load8(b0 + kbeg), fb1 = load8(b1 + kbeg);
+  bf16x8 fa[BM], fb[BN];
+#pragma unroll
+  for (int i = 0; i < BM; i++) fa[i] = load8(ap + size_t(16 * i) * g.lda + kbeg);
+#pragma unroll
+  for (int j = 0; j < BN; j++) fb[j] = load8(bp + size_t(16 * j) * g.ldb + kbeg);
   for (int k = kbeg; k < kend; k += 32) {
     const int kn = k + 32 < kend ? k + 32 : k;  // (the last step reloads its own fragments)
-    const bf16x8 na0 = load8(a0 + kn), na1 = load8(a1 + kn);
-    const bf16x8 nb0 = load8(b0 + kn), nb1 = load8(b1 + kn);
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa0, fb1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa1, fb1, acc[1][1], 0, 0, 0);
-    if (bsum) {
-      const uint4 u0 = __builtin_bit_cast(uint4, fb0), u1 = __builtin_bit_cast(uint4, fb1);
-      const uint32_t w0[4] = {u0.x, u0.y, u0.z, u0.w}, w1[4] = {u1.x, u1.y, u1.z, u1.w};
+    bf16x8 na[BM], nb[BN];
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        bs0 += __uint_as_float(w0[e] << 16) + __uint_as_float(w0[e] & 0xffff0000u);
-        bs1 += __uint_as_float(w1[e] << 16) + __uint_as_float(w1[e] & 0xffff0000u);
+    for (int i = 0; i < BM; i++) na[i] = load8(ap + size_t(16 * i) * g.lda + kn);
+#pragma unroll
+    for (int j = 0; j < BN; j++) nb[j] = load8(bp + size_t(16 * j) * g.ldb + kn);
+#pragma unroll
+    for (int i = 0; i < BM; i++)
+#pragma unroll
+      for (int j = 0; j < BN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    if (bsum) {
+#pragma unroll
+      for (int j = 0; j < BN; j++) {
+        const uint4 u = __builtin_bit_cast(uint4, fb[j]);
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) bs[j] += __uint_as_float(w[e] << 16) + __uint_as_float(w[e] & 0xffff0000u);
       }
     }
-    fa0 = na0;
-    fa1 = na1;
-    fb0 = nb0;
-    fb1 = nb1;
+#pragma unroll
+    for (int i = 0; i < BM; i++) fa[i] = na[i];
+#pragma unroll
+    for (int j = 0; j < BN; j++) fb[j] = nb[j];
   }
   if (bsum) {
-    bs0 += __shfl_xor(bs0, 16);
-    bs0 += __shfl_xor(bs0, 32);
-    bs1 += __shfl_xor(bs1, 16);
-    bs1 += __shfl_xor(bs1, 32);
-    if (grp == 0) {
-      g.bias_part[size_t(blockIdx.z) * g.N + n0 + r] = bs0;
-      g.bias_part[size_t(blockIdx.z) * g.N + n0 + 16 + r] = bs1;
+#pragma unroll
+    for (int j = 0; j < BN; j++) {
+      bs[j] += __shfl_xor(bs[j], 16);
+      bs[j] += __shfl_xor(bs[j], 32);
+      if (grp == 0) g.bias_part[size_t(blockIdx.z) * g.N + n0 + 16 * j + r] = bs[j];
     }
   }
 
 #pragma unroll
-  for (int i = 0; i < 2; i++) {
+  for (int i = 0; i < BM; i++) {
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < BN; j++) {
       const int n = n0 + 16 * j + r;
       const int mb = m0 + 16 * i + 4 * grp;
       if (g.epi == kEpiOut) {
@@ -173,6 +183,74 @@ __global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
       }
       if (g.out_t && g.epi != kEpiF32) {
         bf16 *dst = g.out_t + size_t(n) * g.ldt + mb;  // 4 consecutive rows -> one 8-byte store
+        *reinterpret_cast<uint2 *>(dst) =
+            make_uint2(uint32_t(t[0]) | (uint32_t(t[1]) << 16), uint32_t(t[2]) | (uint32_t(t[3]) << 16));
+      }
+    }
+  }
+}
+
+// C = A B^T for K = 32 KS known at compile time (no split): every K step's
+// fragments are requested at once, so the wave waits for one memory
+// latency instead of KS in a row, then the epilogue of gemm_nt_kernel.
+template <int KS>
+__global__ __launch_bounds__(64) void gemm_nt_preload_kernel(const GemmArgs g) {
+  const int lane = threadIdx.x;
+  const int n0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+  const int r = lane & 15, grp = lane >> 4;
+  const bf16 *ap = g.a + size_t(m0 + r) * g.lda + 8 * grp;
+  const bf16 *bp = g.b + size_t(n0 + r) * g.ldb + 8 * grp;
+  bf16x8 fa[KS][2], fb[KS][2];
+#pragma unroll
+  for (int k = 0; k < KS; k++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      fa[k][i] = load8(ap + size_t(16 * i) * g.lda + 32 * k);
+      fb[k][i] = load8(bp + size_t(16 * i) * g.ldb + 32 * k);
+    }
+  __builtin_amdgcn_sched_barrier(0);  // (all loads issued before the first multiply)
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < KS; k++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k][i], fb[k][j], acc[i][j], 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int n = n0 + 16 * j + r;
+      const int mb = m0 + 16 * i + 4 * grp;
+      if (g.epi == kEpiOut) {
+        if (n == 0)
+#pragma unroll
+          for (int q = 0; q < 4; q++) g.out_f32[mb + q] = acc[i][j][q] + g.bias[0];
+        continue;
+      }
+      bf16 t[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const size_t o = size_t(mb + q) * g.ldo + n;
+        const float c = acc[i][j][q];
+        if (g.epi == kEpiHidden) {
+          const float z = c + g.bias[n];
+          if (g.out_f32) g.out_f32[o] = z;
+          t[q] = to_bf16(gelu(z));
+          g.out_bf16[o] = t[q];
+        } else if (g.epi == kEpiBackGelu) {
+          t[q] = to_bf16(c * gelu_grad(g.zprev[o]));
+          g.out_bf16[o] = t[q];
+        } else {
+          g.out_f32[o] = c;
+        }
+      }
+      if (g.out_t && g.epi != kEpiF32) {
+        bf16 *dst = g.out_t + size_t(n) * g.ldt + mb;
         *reinterpret_cast<uint2 *>(dst) =
             make_uint2(uint32_t(t[0]) | (uint32_t(t[1]) << 16), uint32_t(t[2]) | (uint32_t(t[3]) << 16));
       }
@@ -464,8 +542,14 @@ hipError_t gemm(const GemmArgs &g0, hipStream_t s) {
   if (g.ksplit < 1) g.ksplit = 1;
   if (g.M % 32 || g.N % 32 || g.K % 32 || g.lda % 8 || g.ldb % 8 || (g.K / 32) % g.ksplit) return hipErrorInvalidValue;
   if ((g.ksplit > 1 || g.defer || g.bias_part) && (g.epi != kEpiF32 || !g.work)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(gemm_nt_kernel, dim3(unsigned(g.N / 32), unsigned(g.M / 32), unsigned(g.ksplit)), dim3(64), 0, s,
-                     g);
+  // K = 256 without a split (the hidden layers' products): every fragment requested at once
+  if (g.ksplit == 1 && g.K == 256 && !g.defer && !g.bias_part) {
+    hipLaunchKernelGGL((gemm_nt_preload_kernel<8>), dim3(unsigned(g.N / 32), unsigned(g.M / 32), 1u), dim3(64), 0, s,
+                       g);
+  } else {
+    hipLaunchKernelGGL((gemm_nt_kernel<2, 2>), dim3(unsigned(g.N / 32), unsigned(g.M / 32), unsigned(g.ksplit)),
+                       dim3(64), 0, s, g);
+  }
   if (g.ksplit > 1 && !g.defer) {
     if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
     const size_t n = size_t(g.M) * g.ldo;
