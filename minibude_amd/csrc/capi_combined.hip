@@ -171,7 +171,11 @@ struct ProteinSlot {
   std::vector<size_t> lab_offs;
   uint64_t protein_sum = 0;
   // begun and not yet ended; how its selection and rescoring run
-  bool busy = false, valid = false, pipelined = false, mlp = false;
+  // overlap: the next protein of a batch may begin before this one ends (its
+  // selection ran chunk by chunk, or as one pass over the scores that the
+  // next protein's scoring waits for: ev[10]; not with fallback pairs, whose
+  // exhaustive pass writes shared tables, or the pose MLP)
+  bool busy = false, valid = false, pipelined = false, mlp = false, overlap = false;
   uint32_t *nsel_host = nullptr;  // pinned: the counts' copy lands here without blocking the host
   // pinned staging of the protein's uploads (a synchronous copy would wait
   // for the previous protein's kernels on the other streams)
@@ -1104,6 +1108,11 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     // which the host waits for, and, in a batch, beside the previous
     // protein's rescoring, which it issues ahead of too)
     const bool overlap = c->slots[c->cur ^ 1].busy;
+    {  // a previous protein still in flight may select from the scores this one overwrites
+      const ProteinSlot &other = c->slots[c->cur ^ 1];
+      if (other.busy && !other.pipelined)
+        if (int st = bude::hip_check(hipStreamWaitEvent(c->stream2, other.ev[10], 0), "hipStreamWaitEvent")) return st;
+    }
     for (size_t k = 0; k < K; k++) {
       const size_t a = c->chunk_lig[k], b = c->chunk_lig[k + 1];
       if (int st = launch_surrogate(c, poses, c->total, pred, c->meta_all, c->tb_all, tb_all[a], tb_all[b] - tb_all[a],
@@ -1320,6 +1329,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
                                    static_cast<const int32_t *>(c->s().rs_sel.ptr), nrs, rs_tiles, rs_ntiles + K,
                                    c->stream))
       return st;
+    if (int st = record(c, 10)) return st;  // the scores are read
     if (int st = record(c, 7)) return st;
     if (nrs)
       if (int st = launch_rescore(c, rs_tiles, rs_ntiles + K, expect)) return st;
@@ -1331,6 +1341,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     return st;
   if (int st = record(c, 8)) return st;
   c->s().pipelined = pipelined;
+  c->s().overlap = pipelined || (!mlp && fb.empty());
   c->s().mlp = mlp;
   c->s().K = K;
   res->nposes = c->total;
@@ -1399,9 +1410,8 @@ extern "C" int bude_mi355x_combined_screen_batch(bude_combined *c, size_t nprote
   const auto t0 = std::chrono::steady_clock::now();
   // protein i begins in one slot while protein i - 1, in the other, is
   // rescored; i - 1 then ends (and its callback runs with it as the protein
-  // screened last).  A protein whose selection ran outside the pipeline (the
-  // MLP, or fallback pairs) shares buffers with the next one's scoring and
-  // ends first.
+  // screened last).  A protein with fallback pairs, or screened by the pose
+  // MLP, shares buffers with the next one's scoring and ends first.
   int prev = -1, st = BUDE_OK;
   auto finish = [&](int slot, size_t i) -> int {
     c->cur = slot;
@@ -1411,7 +1421,7 @@ extern "C" int bude_mi355x_combined_screen_batch(bude_combined *c, size_t nprote
     return BUDE_OK;
   };
   for (size_t i = 0; i < nproteins && !st; i++) {
-    if (prev >= 0 && !c->slots[prev].pipelined) {
+    if (prev >= 0 && !c->slots[prev].overlap) {
       st = finish(prev, i - 1);
       if (st) break;
       prev = -1;

@@ -449,7 +449,7 @@ def test_automatic_chunking_follows_set_keep(gpu_device):
             assert r.nselected == total
 
 
-@pytest.mark.parametrize("kind", ["field1", "field3", "fallback", "mlp", "pose_mlp"])
+@pytest.mark.parametrize("kind", ["field1", "field3", "fallback", "mlp", "mlp_widened", "pose_mlp"])
 def test_batch_equals_one_by_one(gpu_device, kind):
     # screen_batch overlaps each protein's scoring with the previous one's
     # rescoring (or, with fallback pairs or the MLP, runs them one after
@@ -465,6 +465,9 @@ def test_batch_equals_one_by_one(gpu_device, kind):
     elif kind == "mlp":
         cfg = lambda: C.default_config("mlp", labels_per_pair=256, train_points=65536, batch=4096,  # noqa: E731
                                        steps=50, gate_z=0.0)
+    elif kind == "mlp_widened":  # pairs widened, none falling back: the proteins still overlap
+        cfg = lambda: C.default_config("mlp", labels_per_pair=256, train_points=65536, batch=4096,  # noqa: E731
+                                       steps=300, gate_z=2.0, widen_max=1.0)
     else:
         kw.update(max_val_rmse=0.0 if kind == "fallback" else 1e9, pipeline_chunks=3 if kind == "field3" else 1)
         cfg = lambda: C.default_config("field", **kw)  # noqa: E731
@@ -488,7 +491,7 @@ def test_batch_equals_one_by_one(gpu_device, kind):
                 c.exhaustive()
                 exact_ok.append(all(np.array_equal(c.exact(l)[idx].view(np.uint32), e.view(np.uint32))
                                     for l, (idx, e) in enumerate(got[i][0])))
-            if i < 3 and kind in ("field1", "field3"):
+            if i < 3 and kind in ("field1", "field3", "mlp_widened"):
                 try:
                     c.scores(0)
                 except capi.BudeError:
@@ -497,6 +500,12 @@ def test_batch_equals_one_by_one(gpu_device, kind):
         res, wall = c.screen_batch(w.proteins, done)
         assert sorted(got) == [0, 1, 2, 3] and wall > 0
         assert exact_ok == [True]
+        if kind == "mlp_widened":
+            wide, fb = [r.widened_pairs for r in res], [r.fallback_pairs for r in res]
+            print(f"widened {wide}, fallbacks {fb}, scores of the next protein begun at {scores_err}", flush=True)
+            assert sum(wide) > 0
+            # a protein without fallback pairs lets the next one begin before it ends
+            assert scores_err == [i for i in range(3) if fb[i] == 0]
         if kind in ("field1", "field3"):
             assert scores_err == [0, 1, 2]  # the next protein's scoring had begun
         for i in range(4):
