@@ -210,3 +210,23 @@ def test_pairs_short_of_the_gate_are_widened_not_scored_exhaustively(gpu_device)
         assert set(idx.tolist()) == set(np.nonzero(scores[l] <= thr)[0].tolist())
         assert np.array_equal(e.view(np.uint32), exact[l][idx].view(np.uint32))
         assert rec[l] >= out["base"][2][l]
+
+
+def test_learned_field_meets_the_prescreen_target_at_2pct(gpu_device):
+    # the learned-prescreen workload (the single-GPU deck's 4,096-atom
+    # protein, a 26-atom ligand) with 2^21 of its screening poses through the
+    # learned per-atom field: at the configured 2 % rescored the exact top
+    # 1 % is kept to >= 99 % (bench.py measures the same at 16.8M poses)
+    from minibude_amd import screen as S
+    w = S.screen_workload(ntrain=64, nscreen=1 << 21)
+    with C.Combined(w.forcefield, [w.ligand], [w.screen_poses], C.default_config("mlp"), gpu_device) as c:
+        r = c.screen(w.protein)
+        idx, _ = c.selected(0)
+        pair = c.pair(0)
+        c.exhaustive()
+        exact = c.exact(0)
+    rec = S.recall(idx, exact, 0.01)
+    print(f"learned field on the prescreen workload: recall@1% {rec:.4f} at keep {pair.keep_fraction:.4f}, "
+          f"gate z {pair.gate_z:.2f}, fallback {pair.fallback}, train {r.train_ms:.0f} ms", flush=True)
+    assert not pair.fallback and pair.keep_fraction <= 0.025
+    assert rec >= 0.99
