@@ -51,8 +51,10 @@ struct Net {
   size_t nodes = 0;
   // grid features: FP32 master [nodes][kF], Adam moments, int64 fixed-point
   // gradient accumulators (cleared by the Adam step); for inference the
-  // z-pair table gzp [n0][n1][n2 - 1][kF] of FP16 pairs (feature k of node
-  // (i, j, l) and of node (i, j, l + 1)), one packed dot product per pair
+  // cell table gzp [n0 - 1][n1 - 1][n2 - 1][4][kF] of FP16 pairs: per cell
+  // (i, j, l) and corner column q = (dx, dy), feature k of node (i + dx,
+  // j + dy, l) and of node (i + dx, j + dy, l + 1) -- a cell's 8 corners
+  // in one 128-byte record, one packed dot product per pair
   DeviceBuffer g, gm, gv, gacc, gzp;
   // layers l = 0..2: W_l [K_l][N_l] (FP32 master, Adam moments, gradient,
   // BF16 row-major and transposed copies), b_l [N_l]
@@ -165,7 +167,7 @@ int forward_backward(Net &net, const float4 *points, const float *target, size_t
 int finalize(Net &net, hipStream_t s);
 
 struct ScoreArgs {
-  const uint32_t *gzp;        // z-pair table (Net::gzp)
+  const uint32_t *gzp;        // cell table (Net::gzp)
   field::Grid grid;
   const bf16 *w0t;            // W0^T [kH][k0]: columns 0..kF-1 read
   int k0;

@@ -157,16 +157,18 @@ __device__ __forceinline__ uint32_t pack_half2(float lo, float hi) {
   return __builtin_bit_cast(uint32_t, half2_t{_Float16(lo), _Float16(hi)});
 }
 
-// FP32 grid -> z-pair table: out[((i n1 + j)(n2 - 1) + l) kF + k] = FP16
-// (g[i][j][l][k], g[i][j][l + 1][k]).
-__global__ void zpair_kernel(const float *__restrict__ g, const field::Grid grid, uint32_t *__restrict__ out) {
-  const size_t n2m = size_t(grid.n[2] - 1);
-  const size_t n = size_t(grid.n[0]) * size_t(grid.n[1]) * n2m * kF;
+// FP32 grid -> cell table: out[(cell 4 + q) kF + k] = FP16 (g[node][k],
+// g[node + 1][k]), node = the cell's corner (i + dx, j + dy, l) of column
+// q = 2 dx + dy: the 8 corners a trilinear interpolation reads, in one
+// 128-byte record per cell.
+__global__ void cell_table_kernel(const float *__restrict__ g, const field::Grid grid, uint32_t *__restrict__ out) {
+  const size_t c1 = size_t(grid.n[1] - 1), c2 = size_t(grid.n[2] - 1);
+  const size_t n = size_t(grid.n[0] - 1) * c1 * c2 * 4 * kF;
   const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int k = int(t % kF);
-  const size_t e = t / kF, l = e % n2m, ij = e / n2m;
-  const size_t node = ij * size_t(grid.n[2]) + l;
+  const int k = int(t % kF), q = int((t / kF) % 4);
+  const size_t cell = t / (4 * kF), l = cell % c2, j = (cell / c2) % c1, i = cell / (c2 * c1);
+  const size_t node = ((i + size_t(q >> 1)) * size_t(grid.n[1]) + j + size_t(q & 1)) * size_t(grid.n[2]) + l;
   out[t] = pack_half2(g[node * kF + k], g[(node + 1) * kF + k]);
 }
 
@@ -269,16 +271,16 @@ __device__ __forceinline__ void fetch_atom(const ScoreArgs &a, const Xform &x, c
   cell(px, a.grid.lo[0], inv_h, a.grid.n[0], ix, f.fx);
   cell(py, a.grid.lo[1], inv_h, a.grid.n[1], iy, f.fy);
   cell(pz, a.grid.lo[2], inv_h, a.grid.n[2], iz, f.fz);
-  // the four z-pairs (dx, dy), 8 FP16 pairs = 32 bytes each (32-bit
-  // offsets: the host keeps the table under 2^31 words)
-  const uint32_t n1 = uint32_t(a.grid.n[1]), n2m = uint32_t(a.grid.n[2] - 1);
-  const uint32_t base = ((uint32_t(ix) * n1 + uint32_t(iy)) * n2m + uint32_t(iz)) * kF;
+  // the cell's four z-pairs (dx, dy), 8 FP16 pairs = 32 bytes each, one
+  // 128-byte record (32-bit offsets: the host keeps the table under 2^31
+  // words)
+  const uint32_t c1 = uint32_t(a.grid.n[1] - 1), c2 = uint32_t(a.grid.n[2] - 1);
+  const uint4 *e = reinterpret_cast<const uint4 *>(
+      a.gzp + ((uint32_t(ix) * c1 + uint32_t(iy)) * c2 + uint32_t(iz)) * (4 * kF));
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const uint32_t dx = q >> 1, dy = q & 1;
-    const uint4 *e = reinterpret_cast<const uint4 *>(a.gzp + base + (dx * n1 + dy) * n2m * kF);
-    f.v[q][0] = e[0];
-    f.v[q][1] = e[1];
+    f.v[q][0] = e[2 * q];
+    f.v[q][1] = e[2 * q + 1];
   }
 }
 
@@ -1001,7 +1003,7 @@ int set_params(Net &net, const float *grid, const float *const w[3], const float
   if (!st) st = alloc_zero(net.gm, ng * 4);
   if (!st) st = alloc_zero(net.gv, ng * 4);
   if (!st) st = alloc_zero(net.gacc, ng * 8);
-  if (!st) st = net.gzp.alloc(ng * 4);
+  if (!st) st = net.gzp.alloc(net.nodes * 4 * kF * 4);  // (cells < nodes)
   for (int l = 0; l < 3 && !st; l++) {
     const int K = layer_k(net, l), N = layer_n(l);
     const size_t nw = size_t(K) * N;
@@ -1040,7 +1042,7 @@ int init(Net &net, const field::Grid &grid, int nslots, uint64_t seed, hipStream
   if (!st) st = net.gm.reserve(ng * 4);
   if (!st) st = net.gv.reserve(ng * 4);
   if (!st) st = net.gacc.reserve(ng * 8);
-  if (!st) st = net.gzp.reserve(ng * 4);
+  if (!st) st = net.gzp.reserve(net.nodes * 4 * kF * 4);
   for (int l = 0; l < 3 && !st; l++) {
     const size_t nw = size_t(layer_k(net, l)) * layer_n(l), nb = size_t(layer_n(l));
     for (DeviceBuffer *d : {&net.w[l], &net.mw[l], &net.vw[l], &net.gw[l]})
@@ -1231,10 +1233,11 @@ int forward_backward(Net &net, const float4 *points, const float *target, size_t
 }
 
 int finalize(Net &net, hipStream_t s) {
-  const size_t nz = size_t(net.grid.n[0]) * size_t(net.grid.n[1]) * size_t(net.grid.n[2] - 1) * kF;
-  hipLaunchKernelGGL(zpair_kernel, dim3(blocks_for(nz, 256)), dim3(256), 0, s, static_cast<const float *>(net.g.ptr),
-                     net.grid, static_cast<uint32_t *>(net.gzp.ptr));
-  if (int st = hip_check(hipGetLastError(), "atomnet z-pairs")) return st;
+  const size_t nz =
+      size_t(net.grid.n[0] - 1) * size_t(net.grid.n[1] - 1) * size_t(net.grid.n[2] - 1) * 4 * kF;
+  hipLaunchKernelGGL(cell_table_kernel, dim3(blocks_for(nz, 256)), dim3(256), 0, s,
+                     static_cast<const float *>(net.g.ptr), net.grid, static_cast<uint32_t *>(net.gzp.ptr));
+  if (int st = hip_check(hipGetLastError(), "atomnet cell table")) return st;
   hipLaunchKernelGGL(slot_bias_kernel, dim3(blocks_for(size_t(net.nslots) * kH, 256)), dim3(256), 0, s,
                      static_cast<const float *>(net.b[0].ptr), static_cast<const bf16 *>(net.w16[0].ptr), net.nslots,
                      static_cast<uint32_t *>(net.slot_bias.ptr));
@@ -1265,7 +1268,8 @@ hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (a.nligands <= 0 || a.tile0 < 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 ||
       a.nslots < 1 || a.nslots > kMaxSlots || (ntiles + 3) / 4 > 0x7fffffff ||
-      uint64_t(a.grid.n[0]) * uint64_t(a.grid.n[1]) * uint64_t(a.grid.n[2] - 1) * kF >= (uint64_t(1) << 31))
+      uint64_t(a.grid.n[0] - 1) * uint64_t(a.grid.n[1] - 1) * uint64_t(a.grid.n[2] - 1) * 4 * kF >=
+          (uint64_t(1) << 31))
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a, a.tile0 + ntiles);
   return hipGetLastError();
