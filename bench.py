@@ -396,7 +396,8 @@ def counter_summary() -> dict:
     return {}
 
 
-def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_proteins: int = 1) -> dict:
+def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_proteins: int = 1,
+             steps: int = 0) -> dict:
     """The combined screen with each surrogate on the same workload: every
     protein screened in one batch call (timed), then exact energies of every
     pair of the first `recall_proteins` proteins (one exhaustive batched pass
@@ -427,7 +428,7 @@ def combined(reduced: bool, kinds=("field", "mlp"), nproteins: int = 0, recall_p
     ex_ms: dict[int, float] = {}
     nrec = min(recall_proteins, len(w.proteins))
     for kind in kinds:
-        cfg = C.default_config(kind)
+        cfg = C.default_config(kind, **({"steps": steps} if steps and kind == "mlp" else {}))
         parts = {"label_ms": 0.0, "train_ms": 0.0, "score_ms": 0.0, "select_ms": 0.0, "rescore_ms": 0.0}
         fb, wide, labels, sel, rmse = 0, 0, 0, 0, []
         rec, trusted, gate_z, keeps, one_ms = [], [], [], [], 0.0
@@ -523,6 +524,8 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--combined-recall-proteins", type=int, default=1,
                     help="proteins whose pairs get exhaustive energies for recall")
     ap.add_argument("--combined-proteins", type=int, default=0, help="screen only the first N proteins")
+    ap.add_argument("--combined-steps", type=int, default=0, help="training steps of the learned per-atom field "
+                    "(default: its configuration's 3,000; fewer only to keep a counter run short)")
     ap.add_argument("--copy-calibration", type=int, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration) and exit")
     a = ap.parse_args(argv)
@@ -627,7 +630,8 @@ def main(argv: list[str] | None = None) -> int:
         "batched": None if a.no_batched else batched(),
         "screen": None if a.no_screen else screen(a.screen_epochs),
         "combined": None if a.no_combined else combined(a.combined_reduced, tuple(a.combined_surrogates.split(",")),
-                                                         a.combined_proteins, a.combined_recall_proteins),
+                                                         a.combined_proteins, a.combined_recall_proteins,
+                                                         a.combined_steps),
         "device": device_name(0),
     }
     print(json.dumps(out), flush=True)
