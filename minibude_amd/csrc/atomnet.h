@@ -59,9 +59,8 @@ struct Net {
   // layers l = 0..2: W_l [K_l][N_l] (FP32 master, Adam moments, gradient,
   // BF16 row-major and transposed copies), b_l [N_l]
   DeviceBuffer w[3], b[3], mw[3], vw[3], mb[3], vb[3], gw[3], gb[3], w16[3], w16t[3];
-  // [nslots][kH] first-layer bias of each slot, b0 + W0[kF + s], as the BF16
-  // pair (hi, lo) with hi + lo = the FP32 bias to 2^-16: inference feeds it
-  // to the layer's MFMA as two extra inputs of value 1
+  // [nslots][kH] first-layer bias of each slot, b0 + W0[kF + s] (FP32):
+  // inference starts the layer's MFMA accumulators from it
   DeviceBuffer slot_bias;
   float mu = 0.f, sigma = 1.f, clip = 0.f;
   float b2 = 0.f;  // b_2[0], read back by finalize
@@ -171,7 +170,7 @@ struct ScoreArgs {
   field::Grid grid;
   const bf16 *w0t;            // W0^T [kH][k0]: columns 0..kF-1 read
   int k0;
-  const uint32_t *slot_bias;  // [nslots][kH] BF16 (hi, lo) pairs
+  const float *slot_bias;     // [nslots][kH] b0 + W0[kF + s]
   int nslots;
   const bf16 *w1t;            // W1^T [kH][kH]
   const float *b1;            // [kH]

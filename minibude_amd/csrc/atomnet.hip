@@ -211,32 +211,40 @@ __global__ void init_kernel(float *__restrict__ w, float *__restrict__ m, float 
   v[i] = 0.f;
 }
 
-// slot_bias[s][j] = b0[j] + W0[kF + s][j] (W0's BF16 copy, row-major [k0][kH])
-// as the BF16 pair hi = bf16(b), lo = bf16(b - hi) packed (hi low).
+// slot_bias[s][j] = b0[j] + W0[kF + s][j] (W0's BF16 copy, row-major [k0][kH]).
 __global__ void slot_bias_kernel(const float *__restrict__ b0, const bf16 *__restrict__ w16, int nslots,
-                                 uint32_t *__restrict__ out) {
+                                 float *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslots * kH) return;
   const int s = i / kH, j = i % kH;
-  const float b = b0[j] + __uint_as_float(uint32_t(w16[size_t(kF + s) * kH + j]) << 16);
-  const float hi = __uint_as_float(pack_bf16(b, 0.f) << 16);
-  out[i] = (pack_bf16(b, 0.f) & 0xffffu) | (pack_bf16(b - hi, 0.f) << 16);
+  out[i] = b0[j] + __uint_as_float(uint32_t(w16[size_t(kF + s) * kH + j]) << 16);
 }
 
 // ---- fused scorer -------------------------------------------------------------
-// One wave per 64 poses of one ligand (lane = pose).  Per ligand atom: the
-// transformed position, the interpolated features -- four z-pairs of FP16
-// features (32 bytes each), one packed FP16 dot product per pair and feature
-// against the pair's two trilinear weights, FP32 accumulation -- then the
-// decoder on MFMA, as the transpose D[neuron][pose] = W^T . x^T so the
-// weights are the (register-held) A operand and the wave's 64 poses four
-// 16-pose B operands: layer 1 over K = 32 (the kF features from the wave's
-// own LDS rows, then the slot's bias as a BF16 (hi, lo) pair against two
-// inputs of value 1, then zeros), ReLU, layer 2 onto b1 with layer 1's
-// accumulators as its B operand in registers, ReLU, and the output neuron
-// as a dot product with W2 over the lane's neurons, summed over atoms and
-// then over the four lane groups that share a pose.
-constexpr int kRow = kF;  // bf16 per LDS row: the point's features
+// One wave per 64 poses of one ligand (lane = pose).  Per ligand atom:
+//   gather    each pose's transformed atom, its cell and the four FP16
+//             weight pairs (wx wy (1 - fz), wx wy fz) of the cell's z-pairs
+//             go through the wave's LDS to the eight lanes that then load the
+//             cell's 128-byte record together (16 bytes each: one z-pair's
+//             four features at both z): eight loads bring the 64 records,
+//             each touching 8 cache lines, where one record per lane would
+//             touch 64 -- the vector L1's line lookups bound the
+//             one-record-per-lane form;
+//   features  each lane interpolates its four features of its z-pair (one
+//             packed FP16 dot product each, FP32), rounds them to BF16 and
+//             writes them to the pose's LDS row, K slot 8 q + f (pair q,
+//             feature f);
+//   decoder   on MFMA, as the transpose D[neuron][pose] = W^T . x^T so the
+//             weights are the (register-held) A operand and the wave's 64
+//             poses four 16-pose B operands: layer 1 over K = 32 with W0's
+//             feature rows repeated for the four pairs (the sum over pairs
+//             is the MFMA's own), onto the slot's FP32 bias (b0 + W0[kF +
+//             slot]); ReLU; layer 2 onto b1 with layer 1's accumulators as
+//             its B operand in registers; ReLU; the output neuron as a third
+//             MFMA with W2 in every row of A, accumulated over the atoms in
+//             its accumulators.  The ReLUs act on BF16 pairs (one packed
+//             16-bit integer max each), after the rounding the next layer's
+//             operand needs anyway.
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -251,47 +259,47 @@ __device__ __forceinline__ float fdot2(uint32_t a, uint32_t b, float c) {
 }
 
 constexpr int kStage = 128;  // ligand atoms staged in LDS per wave at a time
+constexpr int kRowPitch = 5;  // uint4 per pose row of layer 1's B operand: 4 pair slots + 1 (staggers LDS banks)
 
-// The four z-pairs around one transformed ligand atom and its cell
-// fractions: the gathers of the next atom are issued before the current
-// one is computed, so their latency hides behind its network.
+// One atom's share of a lane: the 16 bytes of each of its eight records
+// (record of pose 8 m + lane / 8, bytes 16 (lane % 8) ..).
 struct AtomFetch {
-  uint4 v[4][2];
-  float fx, fy, fz;
+  uint4 v[8];
   int slot;
 };
-
-__device__ __forceinline__ void fetch_atom(const ScoreArgs &a, const Xform &x, const float4 la, float inv_h,
-                                           AtomFetch &f) {
-  f.slot = __float_as_int(la.w);  // (the same in every lane)
-  const float px = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
-  const float py = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
-  const float pz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
-  int ix, iy, iz;
-  cell(px, a.grid.lo[0], inv_h, a.grid.n[0], ix, f.fx);
-  cell(py, a.grid.lo[1], inv_h, a.grid.n[1], iy, f.fy);
-  cell(pz, a.grid.lo[2], inv_h, a.grid.n[2], iz, f.fz);
-  // the cell's four z-pairs (dx, dy), 8 FP16 pairs = 32 bytes each, one
-  // 128-byte record (32-bit offsets: the host keeps the table under 2^31
-  // words)
-  const uint32_t c1 = uint32_t(a.grid.n[1] - 1), c2 = uint32_t(a.grid.n[2] - 1);
-  const uint4 *e = reinterpret_cast<const uint4 *>(
-      a.gzp + ((uint32_t(ix) * c1 + uint32_t(iy)) * c2 + uint32_t(iz)) * (4 * kF));
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    f.v[q][0] = e[2 * q];
-    f.v[q][1] = e[2 * q + 1];
-  }
-}
 
 // max(z, 0) of a finite float as one integer max on its bits (negative
 // floats are negative integers); fmaxf would add a canonicalisation.
 __device__ __forceinline__ float relu(float z) { return __int_as_float(max(__float_as_int(z), 0)); }
 
+// ReLU of two BF16 values packed in one word, as one packed 16-bit integer
+// max with zero: a BF16 value with its sign bit set (negative, or -0) is a
+// negative int16.  Equal, bit for bit, to packing relu() of the FP32 values.
+typedef short short2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(short2_t, v), short2_t{0, 0}));
+}
+
+// ReLU of the neurons of one 16-point block held as the accumulators of
+// the two 16-neuron blocks, as the next layer's BF16 B operand (K slot
+// 8 grp + e = neuron 16 (e / 4) + 4 grp + e % 4).
+__device__ __forceinline__ bf16x8 relu_operand(const floatx4 &v0, const floatx4 &v1) {
+  return __builtin_bit_cast(bf16x8, make_uint4(relu_bf16x2(pack_bf16(v0[0], v0[1])), relu_bf16x2(pack_bf16(v0[2], v0[3])),
+                                               relu_bf16x2(pack_bf16(v1[0], v1[1])), relu_bf16x2(pack_bf16(v1[2], v1[3]))));
+}
+
+// K slot 8 grp + e of a permuted-K operand holds hidden neuron
+// 16 (e / 4) + 4 grp + e % 4: a row's two 4-element halves.
+__device__ __forceinline__ bf16x8 perm_row(const bf16 *row, int grp) {
+  const uint2 lo = *reinterpret_cast<const uint2 *>(row + 4 * grp), hi = *reinterpret_cast<const uint2 *>(row + 16 + 4 * grp);
+  return __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+}
+
 __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_t tend) {
-  __shared__ bf16 act[4][64 * kRow];
+  __shared__ uint4 xrows[4][64 * kRowPitch];  // per wave: layer 1's B rows, 4 pairs x 8 BF16 features per pose
+  __shared__ uint4 xfer[4][2][64 * 2];        // per wave, per atom parity: per pose 4 (record byte offset, weight pair)
   __shared__ float w2s[kH], b1s[kH];
-  __shared__ uint32_t sbias[kMaxSlots * kH];
+  __shared__ float sbias[kMaxSlots * kH];
   __shared__ float4 latoms[4][kStage];
   const int tid = threadIdx.x;
   if (tid < kH) {
@@ -314,28 +322,36 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
   const int j = int(t - a.tile_begin[lo]) * 64 + lane;
   const Xform x = make_xform(a.poses, a.stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
   const int r = lane & 15, grp = lane >> 4;
-  // register-held A fragments of the feature rows of W0^T (k 0 .. kF - 1:
-  // lane group 0 only) and of W1^T with its K (input neuron) order permuted
-  // so that layer 1's accumulators are layer 2's B operand as they stand:
-  // lane (r, grp) holds hidden neurons 16 jn + 4 grp + c of point r (jn =
-  // 0, 1; c = 0..3) after layer 1, so layer 2's K slot 8 grp + e is neuron
-  // 16 (e / 4) + 4 grp + e % 4
-  const bf16x8 zero8 = bf16x8{};
+  // register-held A fragments: layer 1's, the feature row 16 jn + r of W0^T
+  // in every lane group (K slot 8 q + f: pair q, feature f); layer 2's,
+  // W1^T with its K (input neuron) order permuted so that layer 1's
+  // accumulators are layer 2's B operand as they stand: lane (r, grp) holds
+  // hidden neurons 16 jn + 4 grp + c of point r (jn = 0, 1; c = 0..3) after
+  // layer 1, so layer 2's K slot 8 grp + e is neuron 16 (e / 4) + 4 grp +
+  // e % 4; the output layer's, W2 in that order in every row
   bf16x8 fw0[kH / 16], fw1[kH / 16];
 #pragma unroll
   for (int jn = 0; jn < kH / 16; jn++) {
-    fw0[jn] = grp == 0 ? *reinterpret_cast<const bf16x8 *>(a.w0t + size_t(16 * jn + r) * a.k0) : zero8;
-    const bf16 *row = a.w1t + size_t(16 * jn + r) * kH + 4 * grp;
-    const uint2 lo = *reinterpret_cast<const uint2 *>(row), hi = *reinterpret_cast<const uint2 *>(row + 16);
-    fw1[jn] = __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+    fw0[jn] = *reinterpret_cast<const bf16x8 *>(a.w0t + size_t(16 * jn + r) * a.k0);
+    fw1[jn] = perm_row(a.w1t + size_t(16 * jn + r) * kH, grp);
   }
-  // B operand of lane group 1 in layer 1: the two unit inputs the slot's
-  // bias pair multiplies (k = kF, kF + 1)
-  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-  const bf16x8 ones2 = __builtin_bit_cast(bf16x8, u32x4{0x3f803f80u, 0u, 0u, 0u});
-  bf16 *buf = act[wave];
+  const bf16x8 fw2 = __builtin_bit_cast(
+      bf16x8, make_uint4(pack_bf16(w2s[4 * grp], w2s[4 * grp + 1]), pack_bf16(w2s[4 * grp + 2], w2s[4 * grp + 3]),
+                         pack_bf16(w2s[16 + 4 * grp], w2s[17 + 4 * grp]),
+                         pack_bf16(w2s[18 + 4 * grp], w2s[19 + 4 * grp])));
+  // this lane's place in the cooperative gathers: record of pose 8 m + pl,
+  // 16-byte chunk ch = (pair ch / 2, features 4 (ch % 2) ..)
+  const int pl = lane >> 3, ch = lane & 7;
+  uint4 *rows = xrows[wave];
+  uint2 *wrow = reinterpret_cast<uint2 *>(rows + pl * kRowPitch) + ch;  // + 8 m * kRowPitch uint4 per m
+  const uint4 *brow = rows + r * kRowPitch + grp;                       // + 16 i * kRowPitch per block i
+  const char *table = reinterpret_cast<const char *>(a.gzp);
   const float inv_h = 1.0f / a.grid.h;
-  float part[4] = {0.f, 0.f, 0.f, 0.f};  // this lane's share of pose block i, summed over atoms
+  // the output neuron of point 16 i + r (every accumulator row holds it),
+  // summed over atoms
+  floatx4 out[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) out[i] = floatx4{0.f, 0.f, 0.f, 0.f};
   // the ligand's atoms, staged kStage at a time in the wave's LDS rows (read
   // as broadcasts: no global load, and so no wait, inside the atom loop)
   const int natlig = __builtin_amdgcn_readfirstlane(m.natlig);
@@ -346,70 +362,84 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
     for (int i = lane; i < kStage && base + i < natlig; i += 64) ls[i] = a.ligands[lig_off + base + i];
     wave_sync();
   };
-  // atom i's gathers; the first atom of a new stage reloads the rows
-  auto fetch = [&](int i, AtomFetch &f) {
+  // atom i's gathers through the transfer rows of its parity; the first
+  // atom of a new stage reloads the rows
+  auto fetch = [&](int i, AtomFetch &f, int par) {
+    uint2 *xf = reinterpret_cast<uint2 *>(xfer[wave][par]);
     if (i > 0 && i % kStage == 0) stage(i);
-    fetch_atom(a, x, ls[i % kStage], inv_h, f);
-  };
-  // one atom through the network, its output added to part[]
-  auto eval = [&](const AtomFetch &cur) {
-    const int slot = __builtin_amdgcn_readfirstlane(cur.slot);
-    const float fz = cur.fz;
-    const float wx[2] = {1.f - cur.fx, cur.fx}, wy[2] = {1.f - cur.fy, cur.fy};
-    float phi[kF];
-#pragma unroll
-    for (int k = 0; k < kF; k++) phi[k] = 0.f;
+    const float4 la = ls[i % kStage];
+    f.slot = __float_as_int(la.w);  // (the same in every lane)
+    const float px = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
+    const float py = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
+    const float pz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
+    int ix, iy, iz;
+    float fx, fy, fz;
+    cell(px, a.grid.lo[0], inv_h, a.grid.n[0], ix, fx);
+    cell(py, a.grid.lo[1], inv_h, a.grid.n[1], iy, fy);
+    cell(pz, a.grid.lo[2], inv_h, a.grid.n[2], iz, fz);
+    // the cell's record (4 z-pairs (dx, dy) x 8 features x 2 z, FP16: 128
+    // bytes; 32-bit byte offsets: the host keeps the table under 2^31 bytes)
+    const uint32_t c1 = uint32_t(a.grid.n[1] - 1), c2 = uint32_t(a.grid.n[2] - 1);
+    const uint32_t off = ((uint32_t(ix) * c1 + uint32_t(iy)) * c2 + uint32_t(iz)) * (16 * kF);
+    const float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy};
+    uint32_t w2[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const float wxy = wx[q >> 1] * wy[q & 1];
-      const uint32_t w2 = pack_half2(wxy * (1.f - fz), wxy * fz);
-      const uint32_t u[8] = {cur.v[q][0].x, cur.v[q][0].y, cur.v[q][0].z, cur.v[q][0].w,
-                             cur.v[q][1].x, cur.v[q][1].y, cur.v[q][1].z, cur.v[q][1].w};
-#pragma unroll
-      for (int k = 0; k < kF; k++) phi[k] = fdot2(w2, u[k], phi[k]);
+      w2[q] = pack_half2(wxy * (1.f - fz), wxy * fz);
     }
-    *reinterpret_cast<uint4 *>(buf + lane * kRow) =
-        make_uint4(pack_bf16(phi[0], phi[1]), pack_bf16(phi[2], phi[3]), pack_bf16(phi[4], phi[5]),
-                   pack_bf16(phi[6], phi[7]));
+    wave_sync();  // (the reads of these rows by the atom before last are done)
+    reinterpret_cast<uint4 *>(xf)[2 * lane] = make_uint4(off, w2[0], off, w2[1]);
+    reinterpret_cast<uint4 *>(xf)[2 * lane + 1] = make_uint4(off, w2[2], off, w2[3]);
     wave_sync();
-    // layer 1: features and the slot's bias on MFMA
-    bf16x8 fa[kH / 16];
+#pragma unroll
+    for (int mm = 0; mm < 8; mm++) {
+      const uint32_t e = xf[4 * (8 * mm + pl) + (ch >> 1)].x;
+      f.v[mm] = *reinterpret_cast<const uint4 *>(table + (e + uint32_t(16 * ch)));
+    }
+  };
+  // one atom through the network, its output added to out[]
+  auto eval = [&](const AtomFetch &cur, int par) {
+    const int slot = __builtin_amdgcn_readfirstlane(cur.slot);
+    const uint2 *xf = reinterpret_cast<const uint2 *>(xfer[wave][par]);
+    // this lane's four features of its z-pair for its eight poses
+#pragma unroll
+    for (int mm = 0; mm < 8; mm++) {
+      const uint32_t w = xf[4 * (8 * mm + pl) + (ch >> 1)].y;
+      const uint4 v = cur.v[mm];
+      wrow[mm * 8 * kRowPitch * 2] = make_uint2(pack_bf16(fdot2(w, v.x, 0.f), fdot2(w, v.y, 0.f)),
+                                                pack_bf16(fdot2(w, v.z, 0.f), fdot2(w, v.w, 0.f)));
+    }
+    wave_sync();
+    // layer 1 onto the slot's bias
+    floatx4 sb[kH / 16];
 #pragma unroll
     for (int jn = 0; jn < kH / 16; jn++) {
-      const uint32_t b = sbias[slot * kH + 16 * jn + r];
-      fa[jn] = grp == 0 ? fw0[jn] : (grp == 1 ? __builtin_bit_cast(bf16x8, u32x4{b, 0u, 0u, 0u}) : zero8);
+      const float4 v = *reinterpret_cast<const float4 *>(sbias + slot * kH + 16 * jn + 4 * grp);
+      sb[jn] = floatx4{v.x, v.y, v.z, v.w};
     }
     floatx4 acc[kH / 16][4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const bf16x8 fb = grp == 0 ? lds8(buf + (16 * i + r) * kRow) : (grp == 1 ? ones2 : zero8);
+      const bf16x8 fb = __builtin_bit_cast(bf16x8, brow[16 * i * kRowPitch]);
 #pragma unroll
       for (int jn = 0; jn < kH / 16; jn++)
-        acc[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[jn], fb, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        acc[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw0[jn], fb, sb[jn], 0, 0, 0);
     }
-    wave_sync();  // (the feature rows are rewritten by the next atom)
-    // layer 2 onto b1 with layer 1's ReLU outputs, in BF16, as its B
-    // operand straight from the accumulators (K permuted, above); ReLU, and
-    // the dot with W2 over this lane's neurons
+    wave_sync();  // (the rows are rewritten by the next atom)
+    // layer 2 onto b1 with layer 1's ReLU outputs as its B operand straight
+    // from the accumulators (K permuted, above); the output neuron on its
+    // ReLU outputs likewise
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const floatx4 z0 = acc[0][i], z1 = acc[1][i];
-      const bf16x8 fb = __builtin_bit_cast(
-          bf16x8, make_uint4(pack_bf16(relu(z0[0]), relu(z0[1])), pack_bf16(relu(z0[2]), relu(z0[3])),
-                             pack_bf16(relu(z1[0]), relu(z1[1])), pack_bf16(relu(z1[2]), relu(z1[3]))));
-      float sum = part[i];
+      const bf16x8 fb = relu_operand(acc[0][i], acc[1][i]);
+      floatx4 z[kH / 16];
 #pragma unroll
       for (int jn = 0; jn < kH / 16; jn++) {
         const float4 bb = *reinterpret_cast<const float4 *>(b1s + 16 * jn + 4 * grp);
-        const float4 ww = *reinterpret_cast<const float4 *>(w2s + 16 * jn + 4 * grp);
-        floatx4 z = floatx4{bb.x, bb.y, bb.z, bb.w};
-        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw1[jn], fb, z, 0, 0, 0);
-        sum = __builtin_fmaf(ww.x, relu(z[0]), sum);
-        sum = __builtin_fmaf(ww.y, relu(z[1]), sum);
-        sum = __builtin_fmaf(ww.z, relu(z[2]), sum);
-        sum = __builtin_fmaf(ww.w, relu(z[3]), sum);
+        z[jn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw1[jn], fb, floatx4{bb.x, bb.y, bb.z, bb.w}, 0, 0, 0);
       }
-      part[i] = sum;
+      out[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw2, relu_operand(z[0], z[1]), out[i], 0, 0, 0);
     }
   };
   // two fetch buffers in turn: atom i + 1's gathers are in flight while
@@ -422,21 +452,15 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
   }
   AtomFetch f0{}, f1{};
   stage(0);
-  fetch(0, f0);
+  fetch(0, f0, 0);
   for (int il = 0; il < natlig; il += 2) {
-    fetch(min(il + 1, natlig - 1), f1);
-    eval(f0);
-    fetch(min(il + 2, natlig - 1), f0);
-    if (il + 1 < natlig) eval(f1);
+    fetch(min(il + 1, natlig - 1), f1, 1);
+    eval(f0, 0);
+    fetch(min(il + 2, natlig - 1), f0, 0);
+    if (il + 1 < natlig) eval(f1, 1);
   }
-  // sum over the four lane groups of pose block i, kept by group i:
-  // groups {g, g ^ 2} first (each keeps its half of the blocks), then {g, g ^ 1}
-  const bool upper = grp >= 2;
-  const float s0 = upper ? part[0] : part[2], s1 = upper ? part[1] : part[3];
-  const float m0 = (upper ? part[2] : part[0]) + __shfl_xor(s0, 32);
-  const float m1 = (upper ? part[3] : part[1]) + __shfl_xor(s1, 32);
-  const bool odd = grp & 1;
-  const float sy = (odd ? m1 : m0) + __shfl_xor(odd ? m0 : m1, 16);
+  // pose j = 16 grp + r of the tile is point r of block grp
+  const float sy = grp == 0 ? out[0][0] : (grp == 1 ? out[1][0] : (grp == 2 ? out[2][0] : out[3][0]));
   if (j < m.nposes)
     a.out[size_t(m.pose_off) + size_t(j)] = a.sigma * (sy + float(m.natlig) * a.b2) + float(m.natlig) * a.mu;
 }
@@ -477,13 +501,6 @@ struct alignas(16) StepRow {
   float dz3;  // loss gradient (BF16 value)
   int pad;
 };
-
-// K slot 8 grp + e of a permuted-K operand holds hidden neuron
-// 16 (e / 4) + 4 grp + e % 4: a row's two 4-element halves.
-__device__ __forceinline__ bf16x8 perm_row(const bf16 *row, int grp) {
-  const uint2 lo = *reinterpret_cast<const uint2 *>(row + 4 * grp), hi = *reinterpret_cast<const uint2 *>(row + 16 + 4 * grp);
-  return __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
-}
 
 // The B operand of point block i from per-neuron values held as the
 // accumulators of the two 16-neuron blocks (the permuted K order).
@@ -1020,7 +1037,7 @@ int set_params(Net &net, const float *grid, const float *const w[3], const float
                                   static_cast<bf16 *>(net.w16t[l].ptr), s),
                      "to_bf16");
   }
-  if (!st) st = net.slot_bias.alloc(size_t(net.nslots) * kH * 4);  // BF16 pairs
+  if (!st) st = net.slot_bias.alloc(size_t(net.nslots) * kH * 4);
   net.step = 0;
   if (!st) st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   return st;
@@ -1240,7 +1257,7 @@ int finalize(Net &net, hipStream_t s) {
   if (int st = hip_check(hipGetLastError(), "atomnet cell table")) return st;
   hipLaunchKernelGGL(slot_bias_kernel, dim3(blocks_for(size_t(net.nslots) * kH, 256)), dim3(256), 0, s,
                      static_cast<const float *>(net.b[0].ptr), static_cast<const bf16 *>(net.w16[0].ptr), net.nslots,
-                     static_cast<uint32_t *>(net.slot_bias.ptr));
+                     static_cast<float *>(net.slot_bias.ptr));
   if (int st = hip_check(hipGetLastError(), "atomnet slot bias")) return st;
   if (int st = hip_check(hipMemcpyAsync(&net.b2, net.b[2].ptr, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync"))
     return st;
@@ -1253,7 +1270,7 @@ ScoreArgs score_args(const Net &net) {
   a.grid = net.grid;
   a.w0t = static_cast<const bf16 *>(net.w16t[0].ptr);
   a.k0 = net.k0;
-  a.slot_bias = static_cast<const uint32_t *>(net.slot_bias.ptr);
+  a.slot_bias = static_cast<const float *>(net.slot_bias.ptr);
   a.nslots = net.nslots;
   a.w1t = static_cast<const bf16 *>(net.w16t[1].ptr);
   a.b1 = static_cast<const float *>(net.b[1].ptr);
@@ -1268,7 +1285,7 @@ hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (a.nligands <= 0 || a.tile0 < 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 ||
       a.nslots < 1 || a.nslots > kMaxSlots || (ntiles + 3) / 4 > 0x7fffffff ||
-      uint64_t(a.grid.n[0] - 1) * uint64_t(a.grid.n[1] - 1) * uint64_t(a.grid.n[2] - 1) * 4 * kF >=
+      uint64_t(a.grid.n[0] - 1) * uint64_t(a.grid.n[1] - 1) * uint64_t(a.grid.n[2] - 1) * 16 * kF >=
           (uint64_t(1) << 31))
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a, a.tile0 + ntiles);

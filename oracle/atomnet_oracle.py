@@ -23,7 +23,10 @@ interpolated features, the weights, the hidden activations and the
 back-propagated dZ; the grid itself is read in FP32 in training.  Inference
 (``predict_atoms``) reads the grid as FP16 and interpolates with FP16
 weights (per z-pair of corners, wx wy (1 - fz) and wx wy fz, their products
-formed in FP32), as the fused scorer does.  With ``bf16=False`` it is the
+formed in FP32), as the fused scorer does: each z-pair's interpolated
+features are rounded to BF16 on their own and the first layer sums the four
+pairs' contributions (the scorer's MFMA sums them over its K dimension),
+onto the slot's FP32 bias b0 + W0[F + slot].  With ``bf16=False`` it is the
 plain FP64 network, whose gradients tests pin against central finite
 differences of its own loss.
 """
@@ -84,8 +87,10 @@ def features(points, grid, lo, h, n):
     return np.einsum("bc,bcf->bf", w, np.asarray(grid, np.float64)[idx])
 
 
-def features_infer(points, grid, lo, h, n):
-    """phi as the fused scorer forms it: FP16 grid values and FP16 weights."""
+def features_infer_pairs(points, grid, lo, h, n):
+    """(B, 4, F): the features each z-pair of corners q = 2 dx + dy (corners
+    2q at z and 2q + 1 at z + 1) contributes, as the fused scorer forms
+    them: FP16 grid values and FP16 weights."""
     i, f = cell(points, lo, h, n)
     idx, _ = corners(i, f, n)
     f32 = np.asarray(f, np.float32)
@@ -97,7 +102,13 @@ def features_infer(points, grid, lo, h, n):
     for c in range(8):
         dx, dy, dz = (c >> 2) & 1, (c >> 1) & 1, c & 1
         w[:, c] = to_f16((wx[dx] * wy[dy]).astype(np.float32) * wz[dz])
-    return np.einsum("bc,bcf->bf", w, to_f16(grid)[idx])
+    g = to_f16(grid)[idx]  # (B, 8, F)
+    return (w[:, 0::2, None] * g[:, 0::2] + w[:, 1::2, None] * g[:, 1::2])
+
+
+def features_infer(points, grid, lo, h, n):
+    """phi as the fused scorer forms it (the sum of the four z-pairs)."""
+    return features_infer_pairs(points, grid, lo, h, n).sum(1)
 
 
 def forward_backward(points, slots, target, grid, lo, h, n, W, b, k0: int, bf16: bool = True):
@@ -137,13 +148,13 @@ def forward_backward(points, slots, target, grid, lo, h, n, W, b, k0: int, bf16:
 
 
 def predict_atoms(points, slots, grid, lo, h, n, W, b, k0: int):
-    """Inference: y (B,) per atom with FP16 interpolation, BF16 features,
-    weights and activations (the one-hot row folded into a per-slot
-    first-layer bias)."""
+    """Inference: y (B,) per atom with FP16 interpolation, BF16 features
+    (per z-pair), weights and activations (the one-hot row folded into a
+    per-slot first-layer bias)."""
     F = grid.shape[1]
-    phi = to_bf16(features_infer(points, grid, lo, h, n))
+    phi4 = to_bf16(features_infer_pairs(points, grid, lo, h, n))
     Wr = [to_bf16(x) for x in W]
-    z1 = phi @ Wr[0][:F] + b[0] + Wr[0][F + np.asarray(slots)]
+    z1 = np.einsum("bqf,fh->bh", phi4, Wr[0][:F]) + b[0] + Wr[0][F + np.asarray(slots)]
     a1 = to_bf16(act(z1))
     a2 = to_bf16(act(a1 @ Wr[1] + b[1]))
     return (a2 @ Wr[2] + b[2])[:, 0]

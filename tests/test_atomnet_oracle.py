@@ -108,6 +108,29 @@ def test_inference_folds_the_one_hot_row_into_the_bias():
     assert np.max(np.abs(y_infer - y_train)) < 0.05 * np.max(np.abs(y_train))
 
 
+def test_inference_pairs_group_the_two_z_corners_of_each_xy_corner():
+    n = (4, 4, 4)
+    lo, h = np.zeros(3), 1.0
+    pts = np.random.default_rng(11).uniform(0.05, 2.95, (300, 3))
+    i, f = A.cell(pts, lo, h, n)
+    for d in range(8):  # one corner (dx, dy, dz) of every point's cell carries the only nonzero node value
+        dx, dy, dz = (d >> 2) & 1, (d >> 1) & 1, d & 1
+        node = ((i[:, 0] + dx) * n[1] + i[:, 1] + dy) * n[2] + i[:, 2] + dz
+        for b in range(len(pts)):
+            grid = np.zeros((int(np.prod(n)), F))
+            grid[node[b]] = 1.0 + np.arange(F)
+            pairs = A.features_infer_pairs(pts[[b]], grid, lo, h, n)[0]
+            w = (f[b, 0] if dx else 1 - f[b, 0]) * (f[b, 1] if dy else 1 - f[b, 1]) * (f[b, 2] if dz else 1 - f[b, 2])
+            q = 2 * dx + dy
+            assert np.all(np.delete(pairs, q, 0) == 0.0)
+            # FP16 weight: 2^-11 relative
+            assert np.allclose(pairs[q], w * (1.0 + np.arange(F)), rtol=1e-3, atol=1e-6)
+    # and the pairs sum to the trilinear interpolation (FP16 rounding apart)
+    grid = np.random.default_rng(12).normal(0, 1, (int(np.prod(n)), F))
+    assert np.allclose(A.features_infer_pairs(pts, grid, lo, h, n).sum(1), A.features(pts, grid, lo, h, n),
+                       atol=4e-3)
+
+
 def test_pose_score_is_the_sum_over_atoms():
     n = (6, 6, 6)
     lo, h = np.full(3, -5.0), 2.0
