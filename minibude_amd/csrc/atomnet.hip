@@ -258,6 +258,26 @@ __device__ __forceinline__ float fdot2(uint32_t a, uint32_t b, float c) {
   return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, a), __builtin_bit_cast(half2_t, b), c, false);
 }
 
+// The four features of one z-pair chunk, interpolated and rounded to BF16
+// pairs: four v_dot2_f32_f16 with a zero addend (the fdot2 builtin becomes
+// the accumulating VOP2 form after a move of zero into each accumulator),
+// then the packs, after the 3 wait states a DOT result needs before another
+// VALU instruction reads it (s_nop 1 plus the two DOTs after the second).
+__device__ __forceinline__ uint2 pair_features(uint32_t w, uint4 v) {
+  uint32_t o0, o1;
+  float t0, t1, t2, t3;
+  asm("v_dot2_f32_f16 %2, %6, %7, 0\n\t"
+      "v_dot2_f32_f16 %3, %6, %8, 0\n\t"
+      "v_dot2_f32_f16 %4, %6, %9, 0\n\t"
+      "v_dot2_f32_f16 %5, %6, %10, 0\n\t"
+      "s_nop 2\n\t"
+      "v_cvt_pk_bf16_f32 %0, %2, %3\n\t"
+      "v_cvt_pk_bf16_f32 %1, %4, %5"
+      : "=v"(o0), "=v"(o1), "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
+      : "v"(w), "v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
+  return make_uint2(o0, o1);
+}
+
 constexpr int kStage = 128;  // ligand atoms staged in LDS per wave at a time
 constexpr int kRowPitch = 5;  // uint4 per pose row of layer 1's B operand: 4 pair slots + 1 (staggers LDS banks)
 
@@ -297,7 +317,11 @@ __device__ __forceinline__ bf16x8 perm_row(const bf16 *row, int grp) {
 
 __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_t tend) {
   __shared__ uint4 xrows[4][64 * kRowPitch];  // per wave: layer 1's B rows, 4 pairs x 8 BF16 features per pose
-  __shared__ uint4 xfer[4][2][64 * 2];        // per wave, per atom parity: per pose 4 (record byte offset, weight pair)
+  // per wave and atom parity: each pose's record byte offset, and its weight
+  // pair of z-pair q at [q][pose] (rows padded to 80 words: the readers of
+  // one instruction, 8 poses x 4 pairs, hit 32 distinct banks)
+  __shared__ uint32_t xoff[4][2][64];
+  __shared__ uint32_t xw[4][2][4][80];
   __shared__ float w2s[kH], b1s[kH];
   __shared__ float sbias[kMaxSlots * kH];
   __shared__ float4 latoms[4][kStage];
@@ -365,7 +389,6 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
   // atom i's gathers through the transfer rows of its parity; the first
   // atom of a new stage reloads the rows
   auto fetch = [&](int i, AtomFetch &f, int par) {
-    uint2 *xf = reinterpret_cast<uint2 *>(xfer[wave][par]);
     if (i > 0 && i % kStage == 0) stage(i);
     const float4 la = ls[i % kStage];
     f.slot = __float_as_int(la.w);  // (the same in every lane)
@@ -389,26 +412,23 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
       w2[q] = pack_half2(wxy * (1.f - fz), wxy * fz);
     }
     wave_sync();  // (the reads of these rows by the atom before last are done)
-    reinterpret_cast<uint4 *>(xf)[2 * lane] = make_uint4(off, w2[0], off, w2[1]);
-    reinterpret_cast<uint4 *>(xf)[2 * lane + 1] = make_uint4(off, w2[2], off, w2[3]);
+    xoff[wave][par][lane] = off;
+#pragma unroll
+    for (int q = 0; q < 4; q++) xw[wave][par][q][lane] = w2[q];
     wave_sync();
 #pragma unroll
-    for (int mm = 0; mm < 8; mm++) {
-      const uint32_t e = xf[4 * (8 * mm + pl) + (ch >> 1)].x;
-      f.v[mm] = *reinterpret_cast<const uint4 *>(table + (e + uint32_t(16 * ch)));
-    }
+    for (int mm = 0; mm < 8; mm++)
+      f.v[mm] = *reinterpret_cast<const uint4 *>(table + (xoff[wave][par][8 * mm + pl] + uint32_t(16 * ch)));
   };
   // one atom through the network, its output added to out[]
   auto eval = [&](const AtomFetch &cur, int par) {
     const int slot = __builtin_amdgcn_readfirstlane(cur.slot);
-    const uint2 *xf = reinterpret_cast<const uint2 *>(xfer[wave][par]);
     // this lane's four features of its z-pair for its eight poses
 #pragma unroll
     for (int mm = 0; mm < 8; mm++) {
-      const uint32_t w = xf[4 * (8 * mm + pl) + (ch >> 1)].y;
+      const uint32_t w = xw[wave][par][ch >> 1][8 * mm + pl];
       const uint4 v = cur.v[mm];
-      wrow[mm * 8 * kRowPitch * 2] = make_uint2(pack_bf16(fdot2(w, v.x, 0.f), fdot2(w, v.y, 0.f)),
-                                                pack_bf16(fdot2(w, v.z, 0.f), fdot2(w, v.w, 0.f)));
+      wrow[mm * 8 * kRowPitch * 2] = pair_features(w, v);
     }
     wave_sync();
     // layer 1 onto the slot's bias
