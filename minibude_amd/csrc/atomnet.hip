@@ -279,7 +279,12 @@ __device__ __forceinline__ uint2 pair_features(uint32_t w, uint4 v) {
 }
 
 constexpr int kStage = 128;  // ligand atoms staged in LDS per wave at a time
-constexpr int kRowPitch = 5;  // uint4 per pose row of layer 1's B operand: 4 pair slots + 1 (staggers LDS banks)
+// Layer 1's B rows in LDS: pose p's four pair slots (uint4 each) start at
+// uint4 4 p + 4 (p / 16) + (p / 4) % 4.  The stagger makes both accesses
+// conflict-free: the 8-byte feature writes (32 lanes = 4 poses x 64 bytes
+// fill the 64 banks) and the 16-byte operand reads (16 lanes = 16 poses of
+// one pair slot, 16 distinct 4-bank groups).
+constexpr int kRowsPerWave = 272;
 
 // One atom's share of a lane: the 16 bytes of each of its eight records
 // (record of pose 8 m + lane / 8, bytes 16 (lane % 8) ..).
@@ -316,7 +321,7 @@ __device__ __forceinline__ bf16x8 perm_row(const bf16 *row, int grp) {
 }
 
 __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_t tend) {
-  __shared__ uint4 xrows[4][64 * kRowPitch];  // per wave: layer 1's B rows, 4 pairs x 8 BF16 features per pose
+  __shared__ uint4 xrows[4][kRowsPerWave];  // per wave: layer 1's B rows, 4 pairs x 8 BF16 features per pose
   // per wave and atom parity: each pose's record byte offset, and its weight
   // pair of z-pair q at [q][pose] (rows padded to 80 words: the readers of
   // one instruction, 8 poses x 4 pairs, hit 32 distinct banks)
@@ -367,10 +372,28 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
   // 16-byte chunk ch = (pair ch / 2, features 4 (ch % 2) ..)
   const int pl = lane >> 3, ch = lane & 7;
   uint4 *rows = xrows[wave];
-  uint2 *wrow = reinterpret_cast<uint2 *>(rows + pl * kRowPitch) + ch;  // + 8 m * kRowPitch uint4 per m
-  const uint4 *brow = rows + r * kRowPitch + grp;                       // + 16 i * kRowPitch per block i
+  // pose 8 m + pl: uint4 32 m + 4 (m / 2) + 2 (m % 2) + 4 pl + pl / 4
+  uint2 *wrow = reinterpret_cast<uint2 *>(rows + 4 * pl + (pl >> 2)) + ch;
+  // pose 16 i + r: uint4 68 i + 4 r + (r / 4) % 4
+  const uint4 *brow = rows + 4 * r + (r >> 2) + grp;
   const char *table = reinterpret_cast<const char *>(a.gzp);
+  // the pose transform in grid coordinates, u = (R x + t - lo) / h
   const float inv_h = 1.0f / a.grid.h;
+  float xg[12];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    xg[4 * d] = x.m[4 * d] * inv_h;
+    xg[4 * d + 1] = x.m[4 * d + 1] * inv_h;
+    xg[4 * d + 2] = x.m[4 * d + 2] * inv_h;
+    xg[4 * d + 3] = (x.m[4 * d + 3] - a.grid.lo[d]) * inv_h;
+  }
+  // cell index and fraction along one axis from u, clamped to the grid: for
+  // finite u the values cell() gives
+  auto cell_u = [](float u, int n, int &i0, float &f) {
+    const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, float(n - 1));
+    i0 = min(int(uc), n - 2);
+    f = uc - float(i0);
+  };
   // the output neuron of point 16 i + r (every accumulator row holds it),
   // summed over atoms
   floatx4 out[4];
@@ -392,14 +415,11 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
     if (i > 0 && i % kStage == 0) stage(i);
     const float4 la = ls[i % kStage];
     f.slot = __float_as_int(la.w);  // (the same in every lane)
-    const float px = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
-    const float py = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
-    const float pz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
     int ix, iy, iz;
     float fx, fy, fz;
-    cell(px, a.grid.lo[0], inv_h, a.grid.n[0], ix, fx);
-    cell(py, a.grid.lo[1], inv_h, a.grid.n[1], iy, fy);
-    cell(pz, a.grid.lo[2], inv_h, a.grid.n[2], iz, fz);
+    cell_u(xg[3] + la.x * xg[0] + la.y * xg[1] + la.z * xg[2], a.grid.n[0], ix, fx);
+    cell_u(xg[7] + la.x * xg[4] + la.y * xg[5] + la.z * xg[6], a.grid.n[1], iy, fy);
+    cell_u(xg[11] + la.x * xg[8] + la.y * xg[9] + la.z * xg[10], a.grid.n[2], iz, fz);
     // the cell's record (4 z-pairs (dx, dy) x 8 features x 2 z, FP16: 128
     // bytes; 32-bit byte offsets: the host keeps the table under 2^31 bytes)
     const uint32_t c1 = uint32_t(a.grid.n[1] - 1), c2 = uint32_t(a.grid.n[2] - 1);
@@ -428,7 +448,7 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
     for (int mm = 0; mm < 8; mm++) {
       const uint32_t w = xw[wave][par][ch >> 1][8 * mm + pl];
       const uint4 v = cur.v[mm];
-      wrow[mm * 8 * kRowPitch * 2] = pair_features(w, v);
+      wrow[2 * (32 * mm + 4 * (mm >> 1) + 2 * (mm & 1))] = pair_features(w, v);
     }
     wave_sync();
     // layer 1 onto the slot's bias
@@ -441,7 +461,7 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
     floatx4 acc[kH / 16][4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const bf16x8 fb = __builtin_bit_cast(bf16x8, brow[16 * i * kRowPitch]);
+      const bf16x8 fb = __builtin_bit_cast(bf16x8, brow[68 * i]);
 #pragma unroll
       for (int jn = 0; jn < kH / 16; jn++)
         acc[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw0[jn], fb, sb[jn], 0, 0, 0);
