@@ -117,7 +117,7 @@ struct Workspace {
   int k0 = 0;
   DeviceBuffer pred, loss, part, vpred;
   // weight and bias gradients of layer l as splits[l] slices, one per
-  // work-group of the step (summed in order by the Adam step); room for
+  // work-group of the step (summed in a fixed order by the Adam step); room for
   // `slices` of them
   DeviceBuffer gpart[3], bpart[3];
   int splits[3] = {1, 1, 1};

@@ -31,7 +31,6 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(floatx2_t{lo, hi}, bf16x2_t));
 }
 __device__ __forceinline__ float lo_bf16(uint32_t v) { return __uint_as_float(v << 16); }
-__device__ __forceinline__ float hi_bf16(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += 0x9e3779b97f4a7c15ull;
@@ -254,10 +253,6 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ bf16x8 lds8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 
-__device__ __forceinline__ float fdot2(uint32_t a, uint32_t b, float c) {
-  return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, a), __builtin_bit_cast(half2_t, b), c, false);
-}
-
 // The four features of one z-pair chunk, interpolated and rounded to BF16
 // pairs: four v_dot2_f32_f16 with a zero addend (the fdot2 builtin becomes
 // the accumulating VOP2 form after a move of zero into each accumulator),
@@ -292,10 +287,6 @@ struct AtomFetch {
   uint4 v[8];
   int slot;
 };
-
-// max(z, 0) of a finite float as one integer max on its bits (negative
-// floats are negative integers); fmaxf would add a canonicalisation.
-__device__ __forceinline__ float relu(float z) { return __int_as_float(max(__float_as_int(z), 0)); }
 
 // ReLU of two BF16 values packed in one word, as one packed 16-bit integer
 // max with zero: a BF16 value with its sign bit set (negative, or -0) is a
@@ -508,7 +499,7 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
 // ---- fused training step ------------------------------------------------------
 // One launch does a training step's forward, loss and backward: one wave per
 // 64 points of the batch (lane = point while the point's own data are
-// formed, then the MFMA lane layout of the scorer), four waves per
+// formed, then the MFMA lane layout of the scorer), kStepWaves waves per
 // work-group.
 //   forward   phi from the FP32 grid (trilinear), layer 0 on MFMA over
 //             K = k0 ([phi, one-hot slot]), ReLU, layer 1 on MFMA with layer
@@ -525,12 +516,15 @@ __global__ __launch_bounds__(256, 3) void score_kernel(const ScoreArgs a, int64_
 //             scattered to the 8 corners in 2^-40 fixed point by 64-bit
 //             integer atomics, one point per instruction (lane = (corner,
 //             feature)).
-// The work-group then sums its four waves' gradients in wave order into its
-// own slice of gw / gb, which the Adam step sums slice by slice in order:
-// the step is deterministic.  BF16 rounding sits where the layer-by-layer
+// The work-group then sums its waves' gradients in wave order into its own
+// slice of gw / gb, which the Adam step sums in a fixed order: the step is
+// deterministic.  BF16 rounding sits where the layer-by-layer
 // kernels put it (inputs, weights, activations, dz), so the oracle of those
 // kernels is the oracle of this one.
 constexpr int kTP = 64 + 8;  // bf16 pitch of a transposed [row][64 points] tile (padding staggers banks)
+// Waves (of 64 points) per work-group of the fused step: a batch of 32,768
+// points is then 256 work-groups, one per CU.
+constexpr int kStepWaves = 2;
 
 struct alignas(16) StepRow {
   uint4 phi;  // BF16 features of the point
@@ -553,7 +547,16 @@ __device__ __forceinline__ float bf16_round(float x) { return lo_bf16(pack_bf16(
 
 struct StepArgs {
   const float4 *points;
-  const uint32_t *idx;    // batch rows -> points (null: rows are points)
+  // batch rows -> points: with perm set, row r of step s (the count in
+  // state) is point perm[(s mod per_epoch) n + r], and the launch also
+  // writes the step's learning rates and bias corrections (sched) into
+  // state for the Adam kernels after it; else through idx, if set; else
+  // rows are points
+  const uint32_t *perm;
+  uint32_t per_epoch;
+  mlp::StepState *state;
+  mlp::Schedule sched;
+  const uint32_t *idx;
   const float *labels;    // exact one-atom energy of every point
   size_t n;               // rows in this launch (a multiple of 64)
   const float *grid;      // FP32 grid features [nodes][kF]
@@ -572,16 +575,18 @@ struct StepArgs {
 };
 
 template <int K0S>
-__global__ __launch_bounds__(256, 2) void step_kernel(const StepArgs a) {
+__global__ __launch_bounds__(64 * kStepWaves, 2) void step_kernel(const StepArgs a) {
   constexpr int K0 = 32 * K0S;
   constexpr int NP = K0 * kH + kH * kH + 3 * kH + 1;  // a wave's partial gradients (dW2, db2: column 0)
-  __shared__ StepRow rows[4][64];
-  __shared__ __align__(16) float dphi[4][64 * kF];
-  __shared__ __align__(16) bf16 tt[4][64 * kTP + kH * kTP];  // transposed tiles; the wave partials at the end
-  static_assert(sizeof(tt) >= 4 * NP * sizeof(float), "reduction buffer");
+  constexpr int NT = 64 * kStepWaves;
+  __shared__ StepRow rows[kStepWaves][64];
+  __shared__ __align__(16) float dphi[kStepWaves][64 * kF];
+  __shared__ __align__(16) bf16 tt[kStepWaves][64 * kTP + kH * kTP];  // transposed tiles; the wave partials at the end
+  static_assert(sizeof(tt) >= kStepWaves * NP * sizeof(float), "reduction buffer");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, grp = lane >> 4;
-  const size_t tile = size_t(blockIdx.x) * 4 + wave;
+  const size_t tile = size_t(blockIdx.x) * kStepWaves + wave;
   const bool active = tile * 64 < a.n;
+  if (a.perm && blockIdx.x == 0 && tid == 0) mlp::schedule_hyper(a.sched, a.state->step, a.state->hyper);
   StepRow *R = rows[wave];
   bf16 *U = tt[wave];
   const bf16x8 zero8 = bf16x8{};
@@ -601,7 +606,8 @@ __global__ __launch_bounds__(256, 2) void step_kernel(const StepArgs a) {
     // ---- the point of this lane: features, target
     {
       const size_t row = tile * 64 + lane;
-      const uint32_t q = a.idx ? a.idx[row] : uint32_t(row);
+      const uint32_t q = a.perm ? a.perm[size_t(a.state->step % a.per_epoch) * a.n + row]
+                                : (a.idx ? a.idx[row] : uint32_t(row));
       const float4 p = a.points[q];
       const float e = a.labels[q];
       int ix, iy, iz;
@@ -929,24 +935,22 @@ __global__ __launch_bounds__(256, 2) void step_kernel(const StepArgs a) {
   if (lane == 0) P[NP - 1] = gB2;
   __syncthreads();
   const size_t wg = blockIdx.x;
-  for (int t = tid; t < K0 * kH; t += 256) a.gw[0][wg * K0 * kH + t] = ((red[t] + red[NP + t]) + red[2 * NP + t]) + red[3 * NP + t];
-  for (int t = tid; t < kH * kH; t += 256) {
-    const int o = K0 * kH + t;
-    a.gw[1][wg * kH * kH + t] = ((red[o] + red[NP + o]) + red[2 * NP + o]) + red[3 * NP + o];
-  }
-  for (int t = tid; t < kH * kOut; t += 256) {  // W2 [kH][kOut]: column 0
-    const int o = K0 * kH + kH * kH + t / kOut;
-    a.gw[2][wg * kH * kOut + t] = t % kOut ? 0.f : ((red[o] + red[NP + o]) + red[2 * NP + o]) + red[3 * NP + o];
-  }
+  auto wsum = [&](int o) {  // the waves' partials of entry o, in wave order
+    float v = red[o];
+#pragma unroll
+    for (int w = 1; w < kStepWaves; w++) v += red[w * NP + o];
+    return v;
+  };
+  for (int t = tid; t < K0 * kH; t += NT) a.gw[0][wg * K0 * kH + t] = wsum(t);
+  for (int t = tid; t < kH * kH; t += NT) a.gw[1][wg * kH * kH + t] = wsum(K0 * kH + t);
+  for (int t = tid; t < kH * kOut; t += NT)  // W2 [kH][kOut]: column 0
+    a.gw[2][wg * kH * kOut + t] = t % kOut ? 0.f : wsum(K0 * kH + kH * kH + t / kOut);
   if (tid < kH) {
     const int o0 = K0 * kH + kH * kH + kH + tid, o1 = o0 + kH;
-    a.gb[0][wg * kH + tid] = ((red[o0] + red[NP + o0]) + red[2 * NP + o0]) + red[3 * NP + o0];
-    a.gb[1][wg * kH + tid] = ((red[o1] + red[NP + o1]) + red[2 * NP + o1]) + red[3 * NP + o1];
+    a.gb[0][wg * kH + tid] = wsum(o0);
+    a.gb[1][wg * kH + tid] = wsum(o1);
   }
-  if (tid < kOut) {
-    const int o = NP - 1;
-    a.gb[2][wg * kOut + tid] = tid ? 0.f : ((red[o] + red[NP + o]) + red[2 * NP + o]) + red[3 * NP + o];
-  }
+  if (tid < kOut) a.gb[2][wg * kOut + tid] = tid ? 0.f : wsum(NP - 1);
 }
 
 unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
@@ -955,16 +959,32 @@ uint64_t mix_seed(uint64_t seed, uint64_t k) { return (seed ^ (k * 0xd6e8feb8665
 
 using Acts = Workspace;
 
+// A training run's batch rows (see StepArgs::perm).
+struct StepRows {
+  const uint32_t *perm;
+  uint32_t per_epoch;
+  mlp::StepState *state;
+  mlp::Schedule sched;
+};
+
 // The fused step over rows [0, n) of `points` (through idx, if set):
 // forward (y into pred, if set; squared error into A.loss), and with
 // `backward` the gradients -- grid into net.gacc, weights and biases as
 // one slice per work-group into A.gpart / bpart (A.splits slices).
 int launch_step(Net &net, Acts &A, const float4 *points, const uint32_t *idx, const float *labels, size_t n,
-                size_t batch, mlp::TargetNorm norm, float *pred, bool backward, hipStream_t s) {
+                size_t batch, mlp::TargetNorm norm, float *pred, bool backward, hipStream_t s,
+                const StepRows *rows = nullptr) {
   if (n == 0 || n % 64) return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: rows must be a multiple of 64");
   StepArgs a{};
   a.points = points;
   a.idx = idx;
+  if (rows) {
+    if (rows->per_epoch == 0) return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: empty epoch");
+    a.perm = rows->perm;
+    a.per_epoch = rows->per_epoch;
+    a.state = rows->state;
+    a.sched = rows->sched;
+  }
   a.labels = labels;
   a.n = n;
   a.grid = static_cast<const float *>(net.g.ptr);
@@ -983,7 +1003,7 @@ int launch_step(Net &net, Acts &A, const float4 *points, const uint32_t *idx, co
   a.loss = backward ? static_cast<unsigned long long *>(A.loss.ptr) : nullptr;
   a.backward = backward ? 1 : 0;
   a.gacc = static_cast<unsigned long long *>(net.gacc.ptr);
-  const unsigned blocks = blocks_for(n, 256);
+  const unsigned blocks = blocks_for(n, 64 * kStepWaves);
   if (backward) {
     if (A.slices < blocks) return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: workspace sized for fewer rows");
     for (int l = 0; l < 3; l++) {
@@ -993,9 +1013,9 @@ int launch_step(Net &net, Acts &A, const float4 *points, const uint32_t *idx, co
     }
   }
   if (net.k0 == 32) {
-    hipLaunchKernelGGL(step_kernel<1>, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(step_kernel<1>, dim3(blocks), dim3(64 * kStepWaves), 0, s, a);
   } else if (net.k0 == 64) {
-    hipLaunchKernelGGL(step_kernel<2>, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(step_kernel<2>, dim3(blocks), dim3(64 * kStepWaves), 0, s, a);
   } else {
     return fail(BUDE_ERR_INVALID_ARGUMENT, "atomnet: input width must be 32 or 64");
   }
@@ -1035,7 +1055,7 @@ int alloc_zero(DeviceBuffer &d, size_t bytes) {
 }  // namespace
 
 int reserve(const Net &net, Workspace &A, size_t B, size_t nval) {
-  const size_t slices = (B + 255) / 256;
+  const size_t slices = (B + 64 * kStepWaves - 1) / (64 * kStepWaves);
   if (A.rows >= B && A.k0 == net.k0 && A.vpred.bytes >= std::max<size_t>(nval, 1) * 4) return BUDE_OK;
   A.rows = B;
   A.k0 = net.k0;
@@ -1195,14 +1215,14 @@ int train(Net &net, Workspace &A, const float4 *points, const float *labels, con
   const uint32_t warm = std::min<uint32_t>(100, steps / 10 + 1);
   const uint32_t step0 = uint32_t(net.step);
   auto *state = static_cast<mlp::StepState *>(A.state.ptr);
-  uint32_t *idx = static_cast<uint32_t *>(A.idx.ptr);
-  // one training step, every per-step value taken from the device state
+  // one training step, every per-step value taken from the device state:
+  // the fused step reads its rows through the permutation at the step count
+  // and writes the step's learning rates for the two Adam launches, the
+  // grid's Adam advances the count
+  const StepRows rows{perm, uint32_t(ntrain / B), state,
+                      mlp::Schedule{steps, warm, step0, p.lr, p.grid_lr, 0.9f, 0.999f}};
   auto one_step = [&]() -> int {
-    if (int st = hip_check(mlp::begin_step(state, perm, uint32_t(ntrain / B), uint32_t(B), idx, steps, warm, p.lr,
-                                           p.grid_lr, 0.9f, 0.999f, step0, s),
-                           "atomnet begin step"))
-      return st;
-    if (int st = launch_step(net, A, points, idx, labels, B, B, norm, nullptr, true, s)) return st;
+    if (int st = launch_step(net, A, points, nullptr, labels, B, B, norm, nullptr, true, s, &rows)) return st;
     return adam_step_dev(net, A, state, s);
   };
   if (int st = hip_check(hipMemsetAsync(A.state.ptr, 0, sizeof(mlp::StepState), s), "hipMemsetAsync")) return st;

@@ -109,10 +109,29 @@ inline const float *step_hyper(const StepState *st) {
   return reinterpret_cast<const float *>(reinterpret_cast<const char *>(st) + 8);
 }
 
+// The learning-rate schedule of a training run and Adam's bias corrections:
+// linear warm-up over `warm` steps, then cosine decay to 5 % at the last of
+// `steps`; step0 counts the steps of earlier runs on the same network.
+struct Schedule {
+  uint32_t steps, warm, step0;
+  float lr0, lr1, b1, b2;
+};
+// hyper (lr0, lr1, 1 - b1^t, 1 - b2^t) of step s of the run.
+__device__ inline void schedule_hyper(const Schedule &c, uint32_t s, float *hyper) {
+  const float warmf = s < c.warm ? float(s + 1) / float(c.warm) : 1.f;
+  const float prog = float(s) / float(c.steps > 1 ? c.steps - 1 : 1);
+  const float f = warmf * (0.05f + 0.95f * 0.5f * (1.f + cosf(3.14159265f * prog)));
+  const float t = float(c.step0 + s + 1);
+  hyper[0] = c.lr0 * f;
+  hyper[1] = c.lr1 * f;
+  hyper[2] = 1.f - powf(c.b1, t);
+  hyper[3] = 1.f - powf(c.b2, t);
+}
+
 // One Adam step over up to 8 parameter tensors in a single launch.
 struct AdamTensor {
   float *w;
-  const float *g;    // gradient: `splits` slices `stride` floats apart, summed in order
+  const float *g;    // gradient: `splits` slices `stride` floats apart, summed in a fixed order
   float *m, *v;
   bf16 *w16, *w16t;  // optional BF16 copies (row-major and transposed)
   int rows, cols;    // w is rows x cols row-major

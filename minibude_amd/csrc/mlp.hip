@@ -328,8 +328,19 @@ __global__ void loss_grad_kernel(const float *__restrict__ pred, const float *__
     atomicAdd(loss, static_cast<unsigned long long>(fmin(double(sq) * kLossScale + 0.5, 1099511627776.0)));
 }
 
-__global__ void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b2, float eps, float bc1, float bc2,
-                                  const float *__restrict__ hyper, StepState *__restrict__ advance) {
+// One work-group of W waves per 64 consecutive entries of the flat index
+// over the batch's tensors: wave w sums the lane's entry over slices w,
+// w + W, ... (each load a coalesced row of 64 entries), the work-group adds
+// the waves' partial sums in wave order, and the first wave applies Adam.
+// The slice sums are spread over the waves so that their loads are in
+// flight together (W = 16 for the hundreds of slices of the fused step, 4
+// for the K splits of the GEMMs); the order of every sum is fixed.
+template <int W>
+__global__ __launch_bounds__(64 * W) void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b2, float eps,
+                                                            float bc1, float bc2, const float *__restrict__ hyper,
+                                                            StepState *__restrict__ advance) {
+  constexpr int kAdamWaves = W;
+  __shared__ float part[kAdamWaves][64];
   if (hyper) {  // learning rate and bias corrections of this step from device memory
     lr = hyper[0];
     bc1 = hyper[2];
@@ -337,26 +348,35 @@ __global__ void adam_multi_kernel(const AdamBatch b, float lr, float b1, float b
   }
   // (the step count is read only by the next step's first kernel)
   if (advance && blockIdx.x == 0 && threadIdx.x == 0) advance->step += 1;
-  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  size_t i = size_t(blockIdx.x) * 64 + lane;
   int t = 0;
   for (; t < b.count; t++) {
     const size_t n = size_t(b.t[t].rows) * b.t[t].cols;
     if (i < n) break;
     i -= n;
   }
-  if (t >= b.count) return;
-  const AdamTensor &T = b.t[t];
-  // the slices in order; eight loads in flight at a time
-  float gi = T.g[i];
-  int z = 1;
-  for (; z + 8 <= T.splits; z += 8) {
-    float s[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) s[u] = T.g[size_t(z + u) * T.stride + i];
-#pragma unroll
-    for (int u = 0; u < 8; u++) gi += s[u];
+  const bool valid = t < b.count;
+  float gs = 0.f;
+  if (valid) {
+    const float *g = b.t[t].g + i;
+    const size_t stride = b.t[t].stride;
+    const int splits = b.t[t].splits;
+    int z = wave;
+    for (; z + 3 * kAdamWaves < splits; z += 4 * kAdamWaves) {
+      const float s0 = g[size_t(z) * stride], s1 = g[size_t(z + kAdamWaves) * stride];
+      const float s2 = g[size_t(z + 2 * kAdamWaves) * stride], s3 = g[size_t(z + 3 * kAdamWaves) * stride];
+      gs = ((gs + s0) + s1) + s2 + s3;
+    }
+    for (; z < splits; z += kAdamWaves) gs += g[size_t(z) * stride];
   }
-  for (; z < T.splits; z++) gi += T.g[size_t(z) * T.stride + i];
+  part[wave][lane] = gs;
+  __syncthreads();
+  if (wave != 0 || !valid) return;
+  float gi = part[0][lane];
+#pragma unroll
+  for (int w = 1; w < kAdamWaves; w++) gi += part[w][lane];
+  const AdamTensor &T = b.t[t];
   const float mi = b1 * T.m[i] + (1.f - b1) * gi;
   const float vi = b2 * T.v[i] + (1.f - b2) * gi * gi;
   T.m[i] = mi;
@@ -374,16 +394,7 @@ __global__ void begin_step_kernel(StepState *__restrict__ st, const uint32_t *__
   const uint32_t s = st->step;
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < B) idx[i] = perm[size_t(s % per_epoch) * B + i];
-  if (i == 0) {
-    const float warmf = s < warm ? float(s + 1) / float(warm) : 1.f;
-    const float prog = float(s) / float(steps > 1 ? steps - 1 : 1);
-    const float f = warmf * (0.05f + 0.95f * 0.5f * (1.f + cosf(3.14159265f * prog)));
-    const float t = float(step0 + s + 1);
-    st->hyper[0] = lr0 * f;
-    st->hyper[1] = lr1 * f;
-    st->hyper[2] = 1.f - powf(b1, t);
-    st->hyper[3] = 1.f - powf(b2, t);
-  }
+  if (i == 0) schedule_hyper(Schedule{steps, warm, step0, lr0, lr1, b1, b2}, s, st->hyper);
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, bf16 *__restrict__ w16,
@@ -611,9 +622,17 @@ hipError_t begin_step(StepState *st, const uint32_t *perm, uint32_t per_epoch, u
 hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, float b2, float eps, hipStream_t s,
                           StepState *advance) {
   size_t total = 0;
-  for (int t = 0; t < b.count; t++) total += size_t(b.t[t].rows) * b.t[t].cols;
-  hipLaunchKernelGGL(adam_multi_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, b, 0.f, b1, b2, eps, 1.f, 1.f,
-                     hyper, advance);
+  int splits = 1;
+  for (int t = 0; t < b.count; t++) {
+    total += size_t(b.t[t].rows) * b.t[t].cols;
+    splits = std::max(splits, b.t[t].splits);
+  }
+  if (splits > 32)
+    hipLaunchKernelGGL(adam_multi_kernel<16>, dim3(blocks_for(total, 64)), dim3(64 * 16), 0, s, b, 0.f, b1, b2, eps,
+                       1.f, 1.f, hyper, advance);
+  else
+    hipLaunchKernelGGL(adam_multi_kernel<4>, dim3(blocks_for(total, 64)), dim3(64 * 4), 0, s, b, 0.f, b1, b2, eps, 1.f,
+                       1.f, hyper, advance);
   return hipGetLastError();
 }
 

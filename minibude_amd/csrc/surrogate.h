@@ -28,8 +28,8 @@ struct Net {
 struct Acts {
   size_t rows = 0;
   DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss;
-  // weight and bias gradients of layer l as splits[l] K slices (summed in
-  // order by the Adam step, or by sum_grads)
+  // weight and bias gradients of layer l as splits[l] K slices (summed in a
+  // fixed order by the Adam step, in slice order by sum_grads)
   DeviceBuffer gpart[4], bpart[4];
   int splits[4] = {1, 1, 1, 1};
 };
