@@ -1,7 +1,7 @@
 // atomnet.hip -- the learned per-atom field (see atomnet.h): training
-// points and their exact labels, the training step (BF16 MFMA GEMMs of
-// mlp::gemm, deterministic fixed-point grid gradients, Adam), and the fused
-// scorer that sums the network over a ligand's atoms for every pose.
+// points and their exact labels, the fused training step (forward and
+// backward on BF16 MFMA, deterministic fixed-point grid gradients, Adam), and
+// the fused scorer that sums the network over a ligand's atoms for every pose.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -495,17 +495,15 @@ extern "C" int bude_mi355x_mlp_forward_backward(int device, size_t batch, const 
                                                   s),
                                "to_bf16"))
     return st;
-  if (int st = bude::surrogate::forward(net, A, batch, true, s)) return st;
   if (int st = bude::hip_check(hipMemsetAsync(A.loss.ptr, 0, 8, s), "hipMemsetAsync")) return st;
-  if (int st = bude::hip_check(
-          bude::mlp::loss_grad(static_cast<const float *>(A.pred.ptr), static_cast<const float *>(tgt.ptr), nullptr,
-                               batch, bude::mlp::TargetNorm{3.0e38f, 0.f, 1.f, -3.0e38f}, nullptr, nullptr, 1.f,
-                               static_cast<bf16 *>(A.dz3.ptr),
-                               static_cast<bf16 *>(A.dz3_t.ptr), int(batch),
-                               static_cast<unsigned long long *>(A.loss.ptr), s),
-          "loss"))
-    return st;
-  if (int st = bude::surrogate::backward(net, A, batch, s)) return st;
+  bude::mlp::RowsArgs in{};
+  in.feat = static_cast<const bf16 *>(A.feat.ptr);
+  in.energy = static_cast<const float *>(tgt.ptr);
+  in.norm = bude::mlp::TargetNorm{3.0e38f, 0.f, 1.f, -3.0e38f};
+  in.tail_weight = 1.f;
+  in.pred = static_cast<float *>(A.pred.ptr);
+  if (int st = bude::surrogate::rows_step(net, A, batch, in, s)) return st;
+  if (int st = bude::surrogate::weight_grads(net, A, batch, s)) return st;
   if (int st = bude::surrogate::sum_grads(net, A, s)) return st;
   if (int st = bude::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
   if (int st = bude::hip_check(hipMemcpy(pred, A.pred.ptr, batch * 4, hipMemcpyDeviceToHost), "hipMemcpy")) return st;

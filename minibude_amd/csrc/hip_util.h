@@ -59,4 +59,25 @@ struct DeviceBuffer {
   }
 };
 
+// Owning pinned host allocation (hipHostMalloc / hipHostFree): the source of
+// copies that run asynchronously to the host.
+struct PinnedBuffer {
+  void *ptr = nullptr;
+  PinnedBuffer() = default;
+  PinnedBuffer(const PinnedBuffer &) = delete;
+  PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+  ~PinnedBuffer() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  int alloc(size_t n) {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    if (int st = hip_check(hipHostMalloc(&ptr, n ? n : 4, hipHostMallocDefault), "hipHostMalloc")) {
+      ptr = nullptr;
+      return st;
+    }
+    return BUDE_OK;
+  }
+};
+
 }  // namespace bude

@@ -1,7 +1,7 @@
 // mlp.h -- the pose-energy surrogate: a 4-layer MLP (21 pose features ->
 // 256 -> 256 -> 256 -> 1, GELU) on BF16 MFMA with FP32 accumulation, its
-// training kernels (backward GEMMs, loss, Adam), and the gather that passes
-// the surrogate's best poses to exact rescoring.
+// training kernels (the fused rows kernel, the weight-gradient products,
+// Adam) and its fused inference kernel.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,72 +26,12 @@ struct TargetNorm {
   float clip, mu, inv_sigma, tail;
 };
 
-// GEMM epilogues.
-enum Epilogue : int {
-  kEpiHidden = 0,  // z = acc + b[n]; a = gelu(z); store z (f32, optional), a (bf16), a^T (bf16, optional)
-  kEpiOut = 1,     // pred[m] = acc[m][0] + b[0]
-  kEpiBackGelu = 2,  // dz = acc * gelu'(zprev[m][n]); store dz (bf16) and dz^T (bf16)
-  kEpiF32 = 3,     // store acc (f32), e.g. a weight gradient
-};
-
-struct GemmArgs {
-  const bf16 *a;  // [M][lda], rows k-contiguous
-  const bf16 *b;  // [N][ldb], rows k-contiguous: C = A * B^T
-  int M, N, K, lda, ldb;
-  int epi;
-  const float *bias;   // [N]            (kEpiHidden, kEpiOut)
-  float *out_f32;      // [M][ldo]       (kEpiHidden z, kEpiF32 acc, kEpiOut pred[M])
-  bf16 *out_bf16;      // [M][ldo]       (kEpiHidden a, kEpiBackGelu dz)
-  bf16 *out_t;         // [N][ldt]       (transposed copy, optional)
-  const float *zprev;  // [M][ldo]       (kEpiBackGelu)
-  int ldo, ldt;
-  int ksplit;          // kEpiF32 only: > 1 splits K over ksplit work-groups per tile
-  float *work;         // [ksplit][M][ldo] partial sums when ksplit > 1
-  int defer;           // kEpiF32: leave the K slices in `work` (slice 0 of ksplit = 1 too) for
-                       // the consumer to sum (Adam over AdamTensor slices), no reduction pass
-  float *bias_part;    // kEpiF32, optional: [ksplit][N] row sums of B over each K slice (the
-                       // bias gradient when B is dZ^T)
-};
-
-// C = A * B^T on 32x32 tiles (M, N, K multiples of 32), each wave loading
-// the next K step's fragments while it multiplies the current ones.  With
-// ksplit > 1 (kEpiF32) every tile's K range is split over ksplit
-// work-groups writing partial sums to `work`, then summed in a fixed order
-// (deterministic) by a second pass, or by the consumer (`defer`).
-hipError_t gemm(const GemmArgs &g, hipStream_t s);
-
 // out[i] = sum over z < splits of work[z * n + i], z ascending.
 hipError_t sum_slices(const float *work, int splits, size_t n, float *out, hipStream_t s);
 
-// Split count that gives a kEpiF32 GEMM about 1,024 waves (K split in
-// multiples of 256).
-int auto_ksplit(int M, int N, int K);
-// Largest split auto_ksplit returns for an M x N product (any K).
-int max_ksplit(int M, int N);
-
-// Features of poses idx[i] (or i when idx is null) from 6 pose rows of
-// `stride` floats: row i of out [n][kIn] (bf16), and out_t [kIn][ldt] if set.
-// With desc [nlig][kDesc] and lig (ligand of each pose row), columns kFeat..
-// hold desc[lig[q]]; otherwise they are zero.  Feature columns from nfeat
-// up are zero (nfeat = 12: the six pose parameters and the sines and
-// cosines of the angles only; a network whose first-layer rows for the
-// other inputs start at zero then keeps them at zero).
-hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t,
-                    int ldt, hipStream_t s, const float *desc = nullptr, const uint32_t *lig = nullptr,
-                    int nfeat = kFeat);
-
-// Weighted MSE on the transformed target of pose q = idx[b] (or b):
-// y = (min(E, clip) - mu) * inv_sigma with the TargetNorm table[lig[q]]
-// (device memory) when table is set, else `norm`; weight tail_weight when
-// E < tail else 1.
-// dz[b][0] = 2 w (pred - y) / n, other columns 0 (also transposed);
-// loss_acc += sum (pred - y)^2 in fixed point (kLossScale units per 1,
-// integer atomics: the same sum whatever order the waves land in; for
-// monitoring).
+// Squared errors are summed in fixed point, kLossScale units per 1 (integer
+// atomics: the same sum whatever order the waves land in; for monitoring).
 constexpr double kLossScale = 16777216.0;
-hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, TargetNorm norm,
-                     const TargetNorm *table, const uint32_t *lig, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt,
-                     unsigned long long *loss_acc, hipStream_t s);
 
 // Per-ligand first-layer bias with the descriptor inputs folded in, for the
 // fused inference kernel (whose descriptor columns are zero):
@@ -128,6 +68,74 @@ __device__ inline void schedule_hyper(const Schedule &c, uint32_t s, float *hype
   hyper[3] = 1.f - powf(c.b2, t);
 }
 
+// The fused rows kernel of a training step (forward, loss, backward to dZ0
+// for kRowBlock rows per work-group; see mlp.hip).  Input rows: features
+// computed from poses (pose_features, the first nfeat of them, desc[lig[q]]
+// in the last kDesc columns if desc is set; X^T written to feat_t), or the
+// BF16 feature rows feat [n][kIn].  Row r of the launch is sample q =
+// perm[(state->step mod per_epoch) n + r] with perm set (the launch then
+// also writes the step's schedule into state->hyper), else idx[r], else r.
+// Target of sample q: y = (min(E, clip) - mu) * inv_sigma with table[lig[q]]
+// (device memory) when table is set, else norm; dz3 = bf16(2 w (pred - y) /
+// n), w = tail_weight when E < tail, else 1.  Outputs: pred (optional), the
+// loss sum (kLossScale fixed point),
+// and the transposed BF16 rows A_l^T, dZ_l^T [kHidden][n] and dz3^T (row 0
+// of [kOut][n]) for the weight-gradient products.
+constexpr int kRowBlock = 32;
+struct RowsArgs {
+  const float *poses;
+  size_t stride;
+  const float *desc;
+  const uint32_t *lig;
+  int nfeat;
+  const bf16 *feat;
+  const uint32_t *perm;
+  uint32_t per_epoch;
+  StepState *state;
+  Schedule sched;
+  const uint32_t *idx;
+  size_t n;  // rows, a multiple of kRowBlock
+  const float *energy;
+  TargetNorm norm;
+  const TargetNorm *table;
+  float tail_weight;
+  const bf16 *wt[3];   // W_l^T [N_l][K_l], l < 3
+  const bf16 *w[4];    // W_l [K_l][N_l] (w[0] unused; w[3]: column 0 read)
+  const float *bias[4];
+  float *pred;
+  unsigned long long *loss;
+  bf16 *feat_t;        // [kIn][n]
+  bf16 *a_t[3], *dz_t[3];
+  bf16 *dz3_t;
+};
+hipError_t rows(const RowsArgs &a, hipStream_t s);
+
+// The weight gradients of a training step in one launch: for each product
+// d, D[M][N] = P[M][ld] Q[N][ld]^T (P = A_{l-1}^T or X^T, Q = dZ_l^T, the
+// transposed rows of the batch) as `splits` K slices in work [splits][M][N],
+// and the row sums of Q per slice in bias_part [splits][N] (the bias
+// gradient).  M, N and ld multiples of 32; `splits` = dw_splits(ld).
+constexpr int kDwSplits = 16;
+// The K slices of a batch of ld rows: the largest power of two up to
+// kDwSplits that divides ld into multiples of 32 rows.
+inline int dw_splits(size_t ld) {
+  int s = kDwSplits;
+  while (s > 1 && ld % (32 * size_t(s))) s /= 2;
+  return s;
+}
+struct DwProduct {
+  const bf16 *p, *q;
+  int M, N;
+  float *work, *bias_part;
+};
+struct DwArgs {
+  DwProduct d[4];
+  int count;
+  size_t ld;
+  int splits;
+};
+hipError_t weight_grads(const DwArgs &a, hipStream_t s);
+
 // One Adam step over up to 8 parameter tensors in a single launch.
 struct AdamTensor {
   float *w;
@@ -148,14 +156,6 @@ struct AdamBatch {
 // `advance` set the launch also ends the step: advance->step += 1.
 hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, float b2, float eps, hipStream_t s,
                           StepState *advance = nullptr);
-
-// Start of step s = st->step: idx[i] = perm[(s % per_epoch) * B + i] for
-// i < B (the batch), and the step's rates: linear warm-up over `warm` steps,
-// then cosine decay to 5 % of lr0 / lr1 at the last of `steps`; t = step0 +
-// s + 1 for the bias corrections.
-hipError_t begin_step(StepState *st, const uint32_t *perm, uint32_t per_epoch, uint32_t B, uint32_t *idx,
-                      uint32_t steps, uint32_t warm, float lr0, float lr1, float b1, float b2, uint32_t step0,
-                      hipStream_t s);
 
 // FP32 -> BF16 of a rows x cols matrix and its transpose.
 hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s);

@@ -1,10 +1,8 @@
 // mlp.hip -- kernels of the pose-energy surrogate (see mlp.h).
 //
-// GEMMs run on gfx950 BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
-// accumulate): one wavefront owns a 32x32 tile of C = A * B^T as 2x2 MFMA
-// blocks, both operands k-contiguous, so every fragment is one 16-byte load
-// per lane, and the next K step's four fragments are in flight while the
-// current four multiply.  MFMA 16x16x32 lane layout (lane l, r = l % 16, g = l / 16):
+// Products run on gfx950 BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
+// accumulate), operands k-contiguous, so every fragment is one 16-byte load
+// per lane.  MFMA 16x16x32 lane layout (lane l, r = l % 16, g = l / 16):
 //   A fragment  A[r][8g .. 8g+7]       B fragment  B^T[8g .. 8g+7][r]
 //   accumulator C[4g .. 4g+3][r]
 #include <hip/hip_runtime.h>
@@ -46,11 +44,12 @@ __device__ __forceinline__ float gelu_sigma(float z) {
 
 __device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
 
-// d/dz [z s(z)], s = sigmoid(2u): s + z s (1 - s) 2u'(z),
-// 2u' = 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2).
-__device__ __forceinline__ float gelu_grad(float z) {
+// GELU(z) and its derivative d/dz [z s(z)] = s + z s (1 - s) 2u'(z),
+// s = sigmoid(2u), 2u' = 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2): one s for both.
+__device__ __forceinline__ void gelu_both(float z, float &a, float &g) {
   const float s = gelu_sigma(z);
-  return s + z * s * (1.0f - s) * __builtin_fmaf(z * z, 0.21406444f, 1.5957691f);
+  a = z * s;
+  g = s + z * s * (1.0f - s) * __builtin_fmaf(z * z, 0.21406444f, 1.5957691f);
 }
 
 __device__ __forceinline__ bf16x8 load8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
@@ -85,247 +84,378 @@ __device__ __forceinline__ void pose_features(const float *__restrict__ poses, s
   f[20] = cx * cy;
 }
 
-// One wave per (16 BM) x (16 BN) tile of C: BM x BN MFMA blocks, the next K
-// step's BM + BN fragments loaded while the current ones multiply.  32 x 32
-// (BM = BN = 2) for the weight gradients and small products; 64 x 64 for
-// the large activation products, which then read their A rows half as
-// often and issue 16 MFMAs per K step.
-template <int BM, int BN>
-__global__ __launch_bounds__(64) void gemm_nt_kernel(const GemmArgs g) {
-  const int lane = threadIdx.x;
-  const int n0 = blockIdx.x * 16 * BN, m0 = blockIdx.y * 16 * BM;
-  const int kc = g.K / g.ksplit, kbeg = blockIdx.z * kc, kend = kbeg + kc;
-  const int r = lane & 15, grp = lane >> 4;
-  const bf16 *ap = g.a + size_t(m0 + r) * g.lda + 8 * grp;
-  const bf16 *bp = g.b + size_t(n0 + r) * g.ldb + 8 * grp;
-  floatx4 acc[BM][BN];
-#pragma unroll
-  for (int i = 0; i < BM; i++)
-#pragma unroll
-    for (int j = 0; j < BN; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  // row sums of B over this K slice (the bias gradient when B is dZ^T),
-  // by the waves of the first row of tiles: lane (r, grp) sums its own
-  // fragment columns, the four lane groups are added at the end
-  const bool bsum = g.bias_part && blockIdx.y == 0;
-  float bs[BN];
-#pragma unroll
-  for (int j = 0; j < BN; j++) bs[j] = 0.f;
+// ---- the rows of a training step, fused ---------------------------------------
+// One work-group of kRBW = 8 waves per kRB = 32 rows of the batch does what a
+// training step computes row by row: the input features, the forward pass of
+// the three hidden layers and of the output neuron, the loss and its
+// gradient, and the backward pass down to dZ0.  It writes the transposed
+// BF16 rows the weight-gradient products read (X^T in pose mode, A_l^T,
+// dZ_l^T, dz3^T) and the loss; the weight gradients are the products after
+// it.  The layers run transposed, D[neuron][row] = W^T X^T: wave w owns
+// neurons 32 w .. 32 w + 31 of every hidden layer (kNB = 2 MFMA blocks) and
+// all 32 rows (2 blocks), two waves per SIMD, so that one's MFMAs run beside
+// the other's epilogue; lane (r, grp) of row block i holds row 2 r + i, so a
+// lane's two rows are adjacent in the transposed outputs (one 4-byte store
+// per neuron).  The weights are the A operands, read from global memory (all
+// work-groups read the same 0.3 MB: L2); the layer below is the B operand,
+// from LDS, row 2 r + i kept at LDS row r + 16 i (the 16 rows one operand
+// read touches then fall on distinct bank groups).  GELU'(Z_l), rounded to
+// BF16, waits in LDS for the backward pass, where the lane that wrote it
+// reads it back: wave w holds the same neurons of dA_l as of Z_l.  Each
+// product's weight fragments are all loaded two products ahead (two register
+// buffers; the first two while the input rows are formed), so their latency
+// passes behind the work in between.
+constexpr int kRB = kRowBlock;
+constexpr int kRBW = 8;                    // waves per work-group
+constexpr int kNB = kHidden / (16 * kRBW);  // 16-neuron MFMA blocks per wave
+constexpr int kRBPitch = kHidden + 8;  // bf16 per LDS row (528 bytes: 4-bank stagger between rows)
+constexpr int kRBIn = kIn + 8;
 
-  bf16x8 fa[BM], fb[BN];
+// A fragments of the wave's 16 kNB rows of A (lda bf16 apart in global memory)
+// for all K / 32 k steps at once: issued a layer ahead, so their latency
+// passes behind the current layer's epilogue and barrier.
+template <int K>
+__device__ __forceinline__ void rows_load(const bf16 *__restrict__ A, int lda, int wave, int r, int grp,
+                                          bf16x8 (&fa)[8][kNB]) {
+  const bf16 *ap = A + size_t(16 * kNB * wave + r) * lda + 8 * grp;
 #pragma unroll
-  for (int i = 0; i < BM; i++) fa[i] = load8(ap + size_t(16 * i) * g.lda + kbeg);
+  for (int k = 0; k < K / 32; k++)
 #pragma unroll
-  for (int j = 0; j < BN; j++) fb[j] = load8(bp + size_t(16 * j) * g.ldb + kbeg);
-  for (int k = kbeg; k < kend; k += 32) {
-    const int kn = k + 32 < kend ? k + 32 : k;  // (the last step reloads its own fragments)
-    bf16x8 na[BM], nb[BN];
+    for (int jn = 0; jn < kNB; jn++) fa[k][jn] = load8(ap + size_t(16 * jn) * lda + 32 * k);
+}
+
+// acc[jn][i] = sum_k A[16 kNB w + 16 jn + r][k] B[k][row 2 r + i] from the
+// loaded fragments, B rows in LDS (row 2 r + i at r + 16 i).
+template <int K>
+__device__ __forceinline__ void rows_mma(const bf16x8 (&fa)[8][kNB], const bf16 *B, int pitch, int r, int grp,
+                                         floatx4 (&acc)[kNB][2]) {
 #pragma unroll
-    for (int i = 0; i < BM; i++) na[i] = load8(ap + size_t(16 * i) * g.lda + kn);
+  for (int jn = 0; jn < kNB; jn++) acc[jn][0] = acc[jn][1] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < BN; j++) nb[j] = load8(bp + size_t(16 * j) * g.ldb + kn);
+  for (int k = 0; k < K / 32; k++) {
+    bf16x8 fb[2];
 #pragma unroll
-    for (int i = 0; i < BM; i++)
+    for (int i = 0; i < 2; i++) fb[i] = *reinterpret_cast<const bf16x8 *>(B + (r + 16 * i) * pitch + 32 * k + 8 * grp);
 #pragma unroll
-      for (int j = 0; j < BN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
-    if (bsum) {
+    for (int jn = 0; jn < kNB; jn++)
 #pragma unroll
-      for (int j = 0; j < BN; j++) {
-        const uint4 u = __builtin_bit_cast(uint4, fb[j]);
-        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+      for (int i = 0; i < 2; i++) acc[jn][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k][jn], fb[i], acc[jn][i], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
+
+// v[jn][i][c] (neuron 16 kNB w + 16 jn + 4 grp + c, row 2 r + i) as BF16: to the
+// LDS rows (if set) and to the transposed rows out_t [neuron][ld], rows
+// row0 + 2 r, + 1 in one 4-byte store per neuron.
+__device__ __forceinline__ void rows_store(const uint32_t (&v)[kNB][2][2], bf16 *lds, bf16 *__restrict__ out_t,
+                                           size_t ld, size_t row0, int wave, int r, int grp) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) bs[j] += __uint_as_float(w[e] << 16) + __uint_as_float(w[e] & 0xffff0000u);
-      }
+  for (int jn = 0; jn < kNB; jn++) {
+    const int n0 = 16 * kNB * wave + 16 * jn + 4 * grp;
+    if (lds) {
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+        *reinterpret_cast<uint2 *>(lds + (r + 16 * i) * kRBPitch + n0) = make_uint2(v[jn][i][0], v[jn][i][1]);
     }
 #pragma unroll
-    for (int i = 0; i < BM; i++) fa[i] = na[i];
-#pragma unroll
-    for (int j = 0; j < BN; j++) fb[j] = nb[j];
-  }
-  if (bsum) {
-#pragma unroll
-    for (int j = 0; j < BN; j++) {
-      bs[j] += __shfl_xor(bs[j], 16);
-      bs[j] += __shfl_xor(bs[j], 32);
-      if (grp == 0) g.bias_part[size_t(blockIdx.z) * g.N + n0 + 16 * j + r] = bs[j];
-    }
-  }
-
-#pragma unroll
-  for (int i = 0; i < BM; i++) {
-#pragma unroll
-    for (int j = 0; j < BN; j++) {
-      const int n = n0 + 16 * j + r;
-      const int mb = m0 + 16 * i + 4 * grp;
-      if (g.epi == kEpiOut) {
-        if (n == 0)
-#pragma unroll
-          for (int q = 0; q < 4; q++) g.out_f32[mb + q] = acc[i][j][q] + g.bias[0];
-        continue;
-      }
-      bf16 t[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const size_t o = size_t(mb + q) * g.ldo + n;
-        const float c = acc[i][j][q];
-        if (g.epi == kEpiHidden) {
-          const float z = c + g.bias[n];
-          if (g.out_f32) g.out_f32[o] = z;
-          t[q] = to_bf16(gelu(z));
-          g.out_bf16[o] = t[q];
-        } else if (g.epi == kEpiBackGelu) {
-          t[q] = to_bf16(c * gelu_grad(g.zprev[o]));
-          g.out_bf16[o] = t[q];
-        } else if (g.ksplit > 1 || g.defer) {
-          g.work[size_t(blockIdx.z) * size_t(g.M) * g.ldo + o] = c;
-        } else {
-          g.out_f32[o] = c;
-        }
-      }
-      if (g.out_t && g.epi != kEpiF32) {
-        bf16 *dst = g.out_t + size_t(n) * g.ldt + mb;  // 4 consecutive rows -> one 8-byte store
-        *reinterpret_cast<uint2 *>(dst) =
-            make_uint2(uint32_t(t[0]) | (uint32_t(t[1]) << 16), uint32_t(t[2]) | (uint32_t(t[3]) << 16));
-      }
+    for (int c = 0; c < 4; c++) {
+      const uint32_t e0 = c & 1 ? v[jn][0][c >> 1] >> 16 : v[jn][0][c >> 1] & 0xffffu;
+      const uint32_t e1 = c & 1 ? v[jn][1][c >> 1] & 0xffff0000u : v[jn][1][c >> 1] << 16;
+      *reinterpret_cast<uint32_t *>(out_t + size_t(n0 + c) * ld + row0 + 2 * r) = e0 | e1;
     }
   }
 }
 
-// C = A B^T for K = 32 KS known at compile time (no split): every K step's
-// fragments are requested at once, so the wave waits for one memory
-// latency instead of KS in a row, then the epilogue of gemm_nt_kernel.
-template <int KS>
-__global__ __launch_bounds__(64) void gemm_nt_preload_kernel(const GemmArgs g) {
-  const int lane = threadIdx.x;
-  const int n0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
-  const int r = lane & 15, grp = lane >> 4;
-  const bf16 *ap = g.a + size_t(m0 + r) * g.lda + 8 * grp;
-  const bf16 *bp = g.b + size_t(n0 + r) * g.ldb + 8 * grp;
-  bf16x8 fa[KS][2], fb[KS][2];
+__global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
+  __shared__ __align__(16) bf16 xs[kRB][kRBIn];
+  __shared__ __align__(16) bf16 hs[2][kRB][kRBPitch];
+  __shared__ __align__(16) bf16 gs[3][kRB][kRBPitch];  // GELU'(Z_l), read back by the lane that wrote it
+  __shared__ float ypart[kRBW][kRB];
+  __shared__ float dz3s[kRB];
+  __shared__ uint32_t qs[kRB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, grp = lane >> 4;
+  const size_t row0 = size_t(blockIdx.x) * kRB;
+  if (a.perm && blockIdx.x == 0 && tid == 0) schedule_hyper(a.sched, a.state->step, a.state->hyper);
+  // the weight fragments of the products, two products ahead in two buffers:
+  // layers 0 and 1 in flight while the input rows are formed
+  bf16x8 fx[8][kNB], fy[8][kNB];
+  rows_load<kIn>(a.wt[0], kIn, wave, r, grp, fx);
+  rows_load<kHidden>(a.wt[1], kHidden, wave, r, grp, fy);
+  // ---- the input row of thread p < kRB (LDS row p / 2 + 16 (p % 2))
+  if (tid < kRB) {
+    const size_t row = row0 + tid;
+    const uint32_t q = a.perm ? a.perm[size_t(a.state->step % a.per_epoch) * a.n + row]
+                              : (a.idx ? a.idx[row] : uint32_t(row));
+    qs[tid] = q;
+    uint4 h[kIn / 8];
+    if (a.feat) {
 #pragma unroll
-  for (int k = 0; k < KS; k++)
+      for (int v = 0; v < kIn / 8; v++) h[v] = reinterpret_cast<const uint4 *>(a.feat + row * kIn)[v];
+    } else {
+      float f[kIn];
+      pose_features(a.poses, a.stride, q, f);
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-      fa[k][i] = load8(ap + size_t(16 * i) * g.lda + 32 * k);
-      fb[k][i] = load8(bp + size_t(16 * i) * g.ldb + 32 * k);
+      for (int c = 0; c < kFeat; c++)
+        if (c >= a.nfeat) f[c] = 0.f;
+      if (a.desc) {
+        const float *d = a.desc + size_t(a.lig[q]) * kDesc;
+#pragma unroll
+        for (int c = 0; c < kDesc; c++) f[kFeat + c] = d[c];
+      }
+#pragma unroll
+      for (int v = 0; v < kIn / 8; v++)
+        h[v] = make_uint4(pack_bf16(f[8 * v], f[8 * v + 1]), pack_bf16(f[8 * v + 2], f[8 * v + 3]),
+                          pack_bf16(f[8 * v + 4], f[8 * v + 5]), pack_bf16(f[8 * v + 6], f[8 * v + 7]));
+      const uint32_t *hw = reinterpret_cast<const uint32_t *>(h);
+#pragma unroll
+      for (int c = 0; c < kIn; c++) a.feat_t[size_t(c) * a.n + row] = bf16(hw[c >> 1] >> (16 * (c & 1)));
     }
-  __builtin_amdgcn_sched_barrier(0);  // (all loads issued before the first multiply)
-  floatx4 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+    for (int v = 0; v < kIn / 8; v++) reinterpret_cast<uint4 *>(&xs[(tid >> 1) + 16 * (tid & 1)][0])[v] = h[v];
+  }
+  __syncthreads();
+  // ---- forward: three hidden layers, GELU'(Z) kept in LDS
+  float w3[kNB][4];  // W3[n][0] of the wave's neurons n = 16 kNB w + 16 jn + 4 grp + c
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int jn = 0; jn < kNB; jn++)
 #pragma unroll
-  for (int k = 0; k < KS; k++)
+    for (int c = 0; c < 4; c++) w3[jn][c] = from_bf16(a.w[3][size_t(16 * kNB * wave + 16 * jn + 4 * grp + c) * kOut]);
+  float yd[2] = {0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+  for (int l = 0; l < 3; l++) {
+    floatx4 acc[kNB][2];
+    // products 0 .. 4 (layers 0, 1, 2, backward W2, W1) alternate between
+    // the buffers; product p + 2's fragments load as soon as p's are used
+    bf16x8 (&fa)[8][kNB] = l & 1 ? fy : fx;
+    if (l == 0)
+      rows_mma<kIn>(fa, &xs[0][0], kRBIn, r, grp, acc);
+    else
+      rows_mma<kHidden>(fa, &hs[(l - 1) & 1][0][0], kRBPitch, r, grp, acc);
+    if (l == 0) rows_load<kHidden>(a.wt[2], kHidden, wave, r, grp, fx);
+    if (l == 1) rows_load<kHidden>(a.w[2], kHidden, wave, r, grp, fy);
+    if (l == 2) rows_load<kHidden>(a.w[1], kHidden, wave, r, grp, fx);
+    uint32_t v[kNB][2][2];
 #pragma unroll
-      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[k][i], fb[k][j], acc[i][j], 0, 0, 0);
+    for (int jn = 0; jn < kNB; jn++) {
+      const float4 bb = *reinterpret_cast<const float4 *>(a.bias[l] + 16 * kNB * wave + 16 * jn + 4 * grp);
+      const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        float av[4], gv[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          gelu_both(acc[jn][i][c] + bv[c], av[c], gv[c]);
+        }
+        v[jn][i][0] = pack_bf16(av[0], av[1]);
+        v[jn][i][1] = pack_bf16(av[2], av[3]);
+        *reinterpret_cast<uint2 *>(&gs[l][r + 16 * i][16 * kNB * wave + 16 * jn + 4 * grp]) =
+            make_uint2(pack_bf16(gv[0], gv[1]), pack_bf16(gv[2], gv[3]));
+        if (l == 2)
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const uint32_t u = v[jn][i][c >> 1];
+            yd[i] = __builtin_fmaf(c & 1 ? bf16_hi(u) : bf16_lo(u), w3[jn][c], yd[i]);
+          }
+      }
+    }
+    rows_store(v, &hs[l & 1][0][0], a.a_t[l], a.n, row0, wave, r, grp);
+    __syncthreads();
+  }
+  // ---- output neuron, loss, dz3 (the four lane groups, then the four waves, in order)
 #pragma unroll
   for (int i = 0; i < 2; i++) {
+    yd[i] += __shfl_xor(yd[i], 16);
+    yd[i] += __shfl_xor(yd[i], 32);
+  }
+  if (grp == 0) {
+    ypart[wave][2 * r] = yd[0];
+    ypart[wave][2 * r + 1] = yd[1];
+  }
+  __syncthreads();
+  if (tid < kRB) {
+    float y = ypart[0][tid];
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int n = n0 + 16 * j + r;
-      const int mb = m0 + 16 * i + 4 * grp;
-      if (g.epi == kEpiOut) {
-        if (n == 0)
+    for (int w = 1; w < kRBW; w++) y += ypart[w][tid];
+    y += a.bias[3][0];
+    const uint32_t q = qs[tid];
+    const float e = a.energy[q];
+    const TargetNorm t = a.table ? a.table[a.lig[q]] : a.norm;
+    const float d = y - (fminf(e, t.clip) - t.mu) * t.inv_sigma;
+    const float wgt = e < t.tail ? a.tail_weight : 1.0f;
+    const uint32_t gq = pack_bf16(2.0f * wgt * d / float(a.n), 0.f) & 0xffffu;
+    dz3s[tid] = bf16_lo(gq);
+    a.dz3_t[row0 + tid] = bf16(gq);
+    if (a.pred) a.pred[row0 + tid] = y;
+    // the squared errors of the 32 rows: one fixed-point atomic per work-group
+    float sq = d * d;
+    for (int off = 16; off > 0; off >>= 1) sq += __shfl_down(sq, off, 32);
+    if (tid == 0 && a.loss)
+      atomicAdd(a.loss, static_cast<unsigned long long>(fmin(double(sq) * kLossScale + 0.5, 1099511627776.0)));
+  }
+  __syncthreads();
+  // ---- backward: dZ2 = dz3 W3^T * GELU'(Z2) (rank one), then two products
+  {
+    const float dq[2] = {dz3s[2 * r], dz3s[2 * r + 1]};
+    uint32_t v[kNB][2][2];
 #pragma unroll
-          for (int q = 0; q < 4; q++) g.out_f32[mb + q] = acc[i][j][q] + g.bias[0];
-        continue;
-      }
-      bf16 t[4];
+    for (int jn = 0; jn < kNB; jn++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const size_t o = size_t(mb + q) * g.ldo + n;
-        const float c = acc[i][j][q];
-        if (g.epi == kEpiHidden) {
-          const float z = c + g.bias[n];
-          if (g.out_f32) g.out_f32[o] = z;
-          t[q] = to_bf16(gelu(z));
-          g.out_bf16[o] = t[q];
-        } else if (g.epi == kEpiBackGelu) {
-          t[q] = to_bf16(c * gelu_grad(g.zprev[o]));
-          g.out_bf16[o] = t[q];
-        } else {
-          g.out_f32[o] = c;
+      for (int i = 0; i < 2; i++) {
+        const uint2 gp = *reinterpret_cast<const uint2 *>(&gs[2][r + 16 * i][16 * kNB * wave + 16 * jn + 4 * grp]);
+        float dv[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const uint32_t gu = c < 2 ? gp.x : gp.y;
+          dv[c] = (w3[jn][c] * dq[i]) * (c & 1 ? bf16_hi(gu) : bf16_lo(gu));
         }
+        v[jn][i][0] = pack_bf16(dv[0], dv[1]);
+        v[jn][i][1] = pack_bf16(dv[2], dv[3]);
       }
-      if (g.out_t && g.epi != kEpiF32) {
-        bf16 *dst = g.out_t + size_t(n) * g.ldt + mb;
-        *reinterpret_cast<uint2 *>(dst) =
-            make_uint2(uint32_t(t[0]) | (uint32_t(t[1]) << 16), uint32_t(t[2]) | (uint32_t(t[3]) << 16));
+    rows_store(v, &hs[1][0][0], a.dz_t[2], a.n, row0, wave, r, grp);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int l = 2; l >= 1; l--) {
+    // dZ_{l-1} = (dZ_l W_l^T) * GELU'(Z_{l-1}): A = W_l rows (input neurons), B = dZ_l from LDS
+    floatx4 acc[kNB][2];
+    rows_mma<kHidden>(l == 2 ? fy : fx, &hs[(l + 1) & 1][0][0], kRBPitch, r, grp, acc);  // dZ2: hs[1], dZ1: hs[0]
+    uint32_t v[kNB][2][2];
+#pragma unroll
+    for (int jn = 0; jn < kNB; jn++)
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        const uint2 gp = *reinterpret_cast<const uint2 *>(&gs[l - 1][r + 16 * i][16 * kNB * wave + 16 * jn + 4 * grp]);
+        float dv[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const uint32_t gu = c < 2 ? gp.x : gp.y;
+          dv[c] = acc[jn][i][c] * (c & 1 ? bf16_hi(gu) : bf16_lo(gu));
+        }
+        v[jn][i][0] = pack_bf16(dv[0], dv[1]);
+        v[jn][i][1] = pack_bf16(dv[2], dv[3]);
       }
-    }
+    rows_store(v, l == 2 ? &hs[0][0][0] : nullptr, a.dz_t[l - 1], a.n, row0, wave, r, grp);
+    if (l == 2) __syncthreads();
   }
 }
 
-// out[i] = sum over z of work[z][i], z ascending (fixed order).
+// ---- the weight gradients of a step, one launch ------------------------------
+// One wave per (product, 64 x 64 tile, K slice): 4 x 4 MFMA blocks (fewer at
+// the edge of a 32-row or 32-column product) over a slice of ld / splits
+// batch rows, two k steps of fragments in flight while the two before them
+// multiply (the rows were just written by the rows kernel: their latency is
+// long).  The waves of the first tile row also sum Q's fragments: the bias
+// gradient.
+struct DwItem {
+  int d, m0, n0, split;
+};
+
+__device__ __forceinline__ DwItem dw_item(const DwArgs &a, int item) {
+  DwItem it{0, 0, 0, 0};
+  for (int d = 0; d < a.count; d++) {
+    const int tn = (a.d[d].N + 63) / 64, tm = (a.d[d].M + 63) / 64, n = tn * tm * a.splits;
+    if (item < n) {
+      it.d = d;
+      it.split = item % a.splits;
+      const int tile = item / a.splits;
+      it.n0 = 64 * (tile % tn);
+      it.m0 = 64 * (tile / tn);
+      return it;
+    }
+    item -= n;
+  }
+  it.d = -1;
+  return it;
+}
+
+__global__ __launch_bounds__(64) void dw_kernel(const DwArgs a) {
+  const DwItem it = dw_item(a, blockIdx.x);
+  if (it.d < 0) return;
+  const DwProduct &d = a.d[it.d];
+  const int lane = threadIdx.x, r = lane & 15, grp = lane >> 4;
+  const int bm = min(4, (d.M - it.m0) / 16), bn = min(4, (d.N - it.n0) / 16);
+  const size_t kc = a.ld / a.splits, k0 = size_t(it.split) * kc;
+  const int nsteps = int(kc / 32);
+  const bf16 *pp = d.p + size_t(it.m0 + r) * a.ld + k0 + 8 * grp;
+  const bf16 *qp = d.q + size_t(it.n0 + r) * a.ld + k0 + 8 * grp;
+  floatx4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float bs[4] = {0.f, 0.f, 0.f, 0.f};
+  const bool bsum = d.bias_part && it.m0 == 0;
+  // group g = k steps 2 g, 2 g + 1 (the last may have one): fragments [step][i or j]
+  bf16x8 fp[2][2][4], fq[2][2][4];
+  auto load = [&](int g, bf16x8 (&ap)[2][4], bf16x8 (&aq)[2][4]) {
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      if (2 * g + t >= nsteps) break;
+      const size_t k = size_t(64 * g + 32 * t);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        if (i < bm) ap[t][i] = load8(pp + size_t(16 * i) * a.ld + k);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (j < bn) aq[t][j] = load8(qp + size_t(16 * j) * a.ld + k);
+    }
+  };
+  auto mul = [&](int g, const bf16x8 (&ap)[2][4], const bf16x8 (&aq)[2][4]) {
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      if (2 * g + t >= nsteps) break;
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (i < bm && j < bn) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[t][i], aq[t][j], acc[i][j], 0, 0, 0);
+      if (bsum) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (j < bn) {
+            const uint4 u = __builtin_bit_cast(uint4, aq[t][j]);
+            const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int e = 0; e < 4; e++) bs[j] += __uint_as_float(w[e] << 16) + __uint_as_float(w[e] & 0xffff0000u);
+          }
+      }
+    }
+  };
+  const int groups = (nsteps + 1) / 2;
+  load(0, fp[0], fq[0]);
+  for (int g = 0; g < groups; g += 2) {
+    if (g + 1 < groups) load(g + 1, fp[1], fq[1]);
+    mul(g, fp[0], fq[0]);
+    if (g + 2 < groups) load(g + 2, fp[0], fq[0]);
+    if (g + 1 < groups) mul(g + 1, fp[1], fq[1]);
+  }
+  float *out = d.work + size_t(it.split) * size_t(d.M) * d.N;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (i < bm && j < bn)
+#pragma unroll
+        for (int c = 0; c < 4; c++) out[size_t(it.m0 + 16 * i + 4 * grp + c) * d.N + it.n0 + 16 * j + r] = acc[i][j][c];
+  if (bsum) {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (j < bn) {
+        float v = bs[j] + __shfl_xor(bs[j], 16);
+        v += __shfl_xor(v, 32);
+        if (grp == 0) d.bias_part[size_t(it.split) * d.N + it.n0 + 16 * j + r] = v;
+      }
+  }
+}
+
 __global__ void ksplit_reduce_kernel(const float *__restrict__ work, size_t n, int splits, float *__restrict__ out) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float acc = 0.f;
   for (int z = 0; z < splits; z++) acc += work[size_t(z) * n + i];
   out[i] = acc;
-}
-
-__global__ void features_kernel(const float *__restrict__ poses, size_t stride, const uint32_t *__restrict__ idx,
-                                size_t n, bf16 *__restrict__ out, bf16 *__restrict__ out_t, int ldt,
-                                const float *__restrict__ desc, const uint32_t *__restrict__ lig, int nfeat) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const size_t q = idx ? idx[i] : i;
-  float f[kIn];
-  pose_features(poses, stride, q, f);
-#pragma unroll
-  for (int c = 0; c < kFeat; c++)
-    if (c >= nfeat) f[c] = 0.f;
-  if (desc) {
-    const float *d = desc + size_t(lig[q]) * kDesc;
-#pragma unroll
-    for (int c = 0; c < kDesc; c++) f[kFeat + c] = d[c];
-  }
-  bf16 h[kIn];
-#pragma unroll
-  for (int c = 0; c < kIn; c++) h[c] = to_bf16(f[c]);
-  uint4 *dst = reinterpret_cast<uint4 *>(out + i * kIn);
-#pragma unroll
-  for (int v = 0; v < kIn / 8; v++)
-    dst[v] = make_uint4(uint32_t(h[8 * v + 0]) | (uint32_t(h[8 * v + 1]) << 16),
-                        uint32_t(h[8 * v + 2]) | (uint32_t(h[8 * v + 3]) << 16),
-                        uint32_t(h[8 * v + 4]) | (uint32_t(h[8 * v + 5]) << 16),
-                        uint32_t(h[8 * v + 6]) | (uint32_t(h[8 * v + 7]) << 16));
-  if (out_t)
-#pragma unroll
-    for (int c = 0; c < kIn; c++) out_t[size_t(c) * ldt + i] = h[c];
-}
-
-__global__ void loss_grad_kernel(const float *__restrict__ pred, const float *__restrict__ energy,
-                                 const uint32_t *__restrict__ idx, size_t n, TargetNorm norm,
-                                 const TargetNorm *__restrict__ table, const uint32_t *__restrict__ lig,
-                                 float tail_weight, bf16 *__restrict__ dz, bf16 *__restrict__ dz_t, int ldt,
-                                 unsigned long long *__restrict__ loss) {
-  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  float sq = 0.f;
-  if (b < n) {
-    const size_t q = idx ? idx[b] : b;
-    const float e = energy[q];
-    const TargetNorm t = table ? table[lig[q]] : norm;
-    const float y = (fminf(e, t.clip) - t.mu) * t.inv_sigma;
-  const float d = pred[b] - y;
-    sq = d * d;
-    const float wgt = e < t.tail ? tail_weight : 1.0f;
-    const bf16 g = to_bf16(2.0f * wgt * d / float(n));
-    dz[b * kOut] = g;
-#pragma unroll
-    for (int c = 1; c < kOut; c++) dz[b * kOut + c] = 0;
-    dz_t[b] = g;
-#pragma unroll
-    for (int c = 1; c < kOut; c++) dz_t[size_t(c) * ldt + b] = 0;
-  }
-  // wave reduction of the squared error, one fixed-point atomic per wave
-  // (clamped at 2^40 units so that no run of waves wraps the sum)
-  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off);
-  if ((threadIdx.x & 63) == 0 && loss)
-    atomicAdd(loss, static_cast<unsigned long long>(fmin(double(sq) * kLossScale + 0.5, 1099511627776.0)));
 }
 
 // One work-group of W waves per 64 consecutive entries of the flat index
@@ -386,15 +516,6 @@ __global__ __launch_bounds__(64 * W) void adam_multi_kernel(const AdamBatch b, f
   const bf16 h = to_bf16(wi);
   if (T.w16) T.w16[i] = h;
   if (T.w16t) T.w16t[size_t(i % T.cols) * T.rows + i / T.cols] = h;
-}
-
-__global__ void begin_step_kernel(StepState *__restrict__ st, const uint32_t *__restrict__ perm, uint32_t per_epoch,
-                                  uint32_t B, uint32_t *__restrict__ idx, uint32_t steps, uint32_t warm, float lr0,
-                                  float lr1, float b1, float b2, uint32_t step0) {
-  const uint32_t s = st->step;
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < B) idx[i] = perm[size_t(s % per_epoch) * B + i];
-  if (i == 0) schedule_hyper(Schedule{steps, warm, step0, lr0, lr1, b1, b2}, s, st->hyper);
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, bf16 *__restrict__ w16,
@@ -548,36 +669,25 @@ unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
 
 }  // namespace
 
-hipError_t gemm(const GemmArgs &g0, hipStream_t s) {
-  GemmArgs g = g0;
-  if (g.ksplit < 1) g.ksplit = 1;
-  if (g.M % 32 || g.N % 32 || g.K % 32 || g.lda % 8 || g.ldb % 8 || (g.K / 32) % g.ksplit) return hipErrorInvalidValue;
-  if ((g.ksplit > 1 || g.defer || g.bias_part) && (g.epi != kEpiF32 || !g.work)) return hipErrorInvalidValue;
-  // K = 256 without a split (the hidden layers' products): every fragment requested at once
-  if (g.ksplit == 1 && g.K == 256 && !g.defer && !g.bias_part) {
-    hipLaunchKernelGGL((gemm_nt_preload_kernel<8>), dim3(unsigned(g.N / 32), unsigned(g.M / 32), 1u), dim3(64), 0, s,
-                       g);
-  } else {
-    hipLaunchKernelGGL((gemm_nt_kernel<2, 2>), dim3(unsigned(g.N / 32), unsigned(g.M / 32), unsigned(g.ksplit)),
-                       dim3(64), 0, s, g);
-  }
-  if (g.ksplit > 1 && !g.defer) {
-    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
-    const size_t n = size_t(g.M) * g.ldo;
-    hipLaunchKernelGGL(ksplit_reduce_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, g.work, n, g.ksplit,
-                       g.out_f32);
-  }
+hipError_t rows(const RowsArgs &a, hipStream_t s) {
+  if (a.n == 0 || a.n % kRB || (!a.feat && (!a.poses || !a.feat_t)) || (a.perm && (!a.state || a.per_epoch == 0)) ||
+      !a.energy || !a.dz3_t)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rows_kernel, dim3(unsigned(a.n / kRB)), dim3(64 * kRBW), 0, s, a);
   return hipGetLastError();
 }
 
-int auto_ksplit(int M, int N, int K) {
-  const int tiles = (M / 32) * (N / 32);
-  int split = 1;
-  while (split * 2 * tiles <= 1024 && (K / 32) % (split * 2) == 0 && K / (split * 2) >= 256) split *= 2;
-  return split;
+hipError_t weight_grads(const DwArgs &a, hipStream_t s) {
+  if (a.count < 1 || a.count > 4 || a.splits < 1 || a.ld == 0 || a.ld % (32 * size_t(a.splits)))
+    return hipErrorInvalidValue;
+  int items = 0;
+  for (int d = 0; d < a.count; d++) {
+    if (a.d[d].M % 32 || a.d[d].N % 32 || !a.d[d].p || !a.d[d].q || !a.d[d].work) return hipErrorInvalidValue;
+    items += ((a.d[d].M + 63) / 64) * ((a.d[d].N + 63) / 64) * a.splits;
+  }
+  hipLaunchKernelGGL(dw_kernel, dim3(unsigned(items)), dim3(64), 0, s, a);
+  return hipGetLastError();
 }
-
-int max_ksplit(int M, int N) { return std::max(1, 1024 / std::max(1, (M / 32) * (N / 32))); }
 
 hipError_t sum_slices(const float *work, int splits, size_t n, float *out, hipStream_t s) {
   if (splits < 1) return hipErrorInvalidValue;
@@ -586,36 +696,9 @@ hipError_t sum_slices(const float *work, int splits, size_t n, float *out, hipSt
   return hipGetLastError();
 }
 
-hipError_t features(const float *poses, size_t stride, const uint32_t *idx, size_t n, bf16 *out, bf16 *out_t, int ldt,
-                    hipStream_t s, const float *desc, const uint32_t *lig, int nfeat) {
-  if (n == 0) return hipSuccess;
-  if ((desc && !lig) || nfeat < 0 || nfeat > kFeat) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(features_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, poses, stride, idx, n, out, out_t,
-                     ldt, desc, lig, nfeat);
-  return hipGetLastError();
-}
-
-hipError_t loss_grad(const float *pred, const float *energy, const uint32_t *idx, size_t n, TargetNorm norm,
-                     const TargetNorm *table, const uint32_t *lig, float tail_weight, bf16 *dz, bf16 *dz_t, int ldt,
-                     unsigned long long *loss_acc, hipStream_t s) {
-  if (table && !lig) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, pred, energy, idx, n, norm, table,
-                     lig, tail_weight, dz, dz_t, ldt, loss_acc);
-  return hipGetLastError();
-}
-
 hipError_t fold_bias(const float *b, const bf16 *w16, const float *desc, int nlig, float *out, hipStream_t s) {
   if (nlig <= 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fold_bias_kernel, dim3(unsigned(nlig)), dim3(kHidden), 0, s, b, w16, desc, out);
-  return hipGetLastError();
-}
-
-hipError_t begin_step(StepState *st, const uint32_t *perm, uint32_t per_epoch, uint32_t B, uint32_t *idx,
-                      uint32_t steps, uint32_t warm, float lr0, float lr1, float b1, float b2, uint32_t step0,
-                      hipStream_t s) {
-  if (B == 0 || per_epoch == 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(begin_step_kernel, dim3(blocks_for(B, 256)), dim3(256), 0, s, st, perm, per_epoch, B, idx, steps,
-                     warm, lr0, lr1, b1, b2, step0);
   return hipGetLastError();
 }
 

@@ -23,11 +23,12 @@ struct Net {
   int step = 0;
 };
 
-// Activations of one batch of `rows` poses (training keeps everything the
-// backward pass reads).
+// The rows of one batch of `rows` poses: BF16 features, and in training the
+// transposed rows the weight-gradient products read (X^T, A_l^T, dZ_l^T,
+// dz3^T), predictions and the loss sum.
 struct Acts {
   size_t rows = 0;
-  DeviceBuffer feat, feat_t, z[3], a[3], a_t[3], pred, dz3, dz3_t, dz[3], dz_t[3], loss;
+  DeviceBuffer feat, feat_t, a_t[3], pred, dz3_t, dz_t[3], loss;
   // weight and bias gradients of layer l as splits[l] K slices (summed in a
   // fixed order by the Adam step, in slice order by sum_grads)
   DeviceBuffer gpart[4], bpart[4];
@@ -35,11 +36,12 @@ struct Acts {
 };
 
 int alloc_acts(Acts &A, size_t rows, bool training);
-// Forward over `rows` rows already featurised in A.feat.
-int forward(const Net &net, Acts &A, size_t rows, bool training, hipStream_t s);
-// Backward from dz3 (written by the loss kernel): the weight and bias
-// gradients as K slices in A.gpart / bpart.
-int backward(Net &net, Acts &A, size_t rows, hipStream_t s);
+// The fused rows kernel over `rows` rows (mlp::rows) with the network's
+// weights and A's buffers filled in from in's inputs and targets.
+int rows_step(const Net &net, Acts &A, size_t rows, const mlp::RowsArgs &in, hipStream_t s);
+// From the transposed rows: the weight and bias gradients as K slices in
+// A.gpart / bpart.
+int weight_grads(Net &net, Acts &A, size_t rows, hipStream_t s);
 // The slices summed in order into net.gw / gb.
 int sum_grads(Net &net, const Acts &A, hipStream_t s);
 // Uploads FP32 weights W_l [kK[l]][kN[l]] (zero padded) and biases, clears

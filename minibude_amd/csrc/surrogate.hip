@@ -5,48 +5,31 @@
 #include <cmath>
 #include <cstdlib>
 #include <random>
+#include <vector>
 
 #include "surrogate.h"
 
 namespace bude {
 namespace surrogate {
 
-namespace {
-
-mlp::GemmArgs gemm_args(const void *a, const void *b, int M, int N, int K, int lda, int ldb, int epi) {
-  mlp::GemmArgs g{};
-  g.a = static_cast<const bf16 *>(a);
-  g.b = static_cast<const bf16 *>(b);
-  g.M = M;
-  g.N = N;
-  g.K = K;
-  g.lda = lda;
-  g.ldb = ldb;
-  g.epi = epi;
-  return g;
-}
-
-}  // namespace
-
 int alloc_acts(Acts &A, size_t rows, bool training) {
   A.rows = rows;
   const size_t H = mlp::kHidden, O = mlp::kOut, I = mlp::kIn;
   int st = A.feat.alloc(rows * I * 2);
   if (!st) st = A.pred.alloc(rows * 4);
-  for (int l = 0; l < 3 && !st; l++) st = A.a[l].alloc(rows * H * 2);
   if (training) {
     if (!st) st = A.feat_t.alloc(rows * I * 2);
     for (int l = 0; l < 3 && !st; l++) {
-      st = A.z[l].alloc(rows * H * 4);
-      if (!st) st = A.a_t[l].alloc(rows * H * 2);
-      if (!st) st = A.dz[l].alloc(rows * H * 2);
+      st = A.a_t[l].alloc(rows * H * 2);
       if (!st) st = A.dz_t[l].alloc(rows * H * 2);
     }
-    if (!st) st = A.dz3.alloc(rows * O * 2);
+    // dz3^T: row 0 is written every step, rows 1.. stay zero (the padded
+    // output columns of the last weight gradient)
     if (!st) st = A.dz3_t.alloc(rows * O * 2);
+    if (!st) st = hip_check(hipMemset(A.dz3_t.ptr, 0, rows * O * 2), "hipMemset");
     if (!st) st = A.loss.alloc(256);
     for (int l = 0; l < 4 && !st; l++) {
-      const size_t cap = size_t(mlp::max_ksplit(kK[l], kN[l]));
+      const size_t cap = size_t(mlp::kDwSplits);
       st = A.gpart[l].alloc(cap * kK[l] * kN[l] * 4);
       if (!st) st = A.bpart[l].alloc(cap * kN[l] * 4);
     }
@@ -54,60 +37,43 @@ int alloc_acts(Acts &A, size_t rows, bool training) {
   return st;
 }
 
-// Forward over `rows` rows already featurised in A.feat; training stores
-// z, a, a^T for the backward pass.
-int forward(const Net &net, Acts &A, size_t rows, bool training, hipStream_t s) {
-  const int R = int(rows);
-  const void *in = A.feat.ptr;
-  int in_ld = mlp::kIn;
+int rows_step(const Net &net, Acts &A, size_t rows, const mlp::RowsArgs &in, hipStream_t s) {
+  mlp::RowsArgs a = in;
+  a.n = rows;
   for (int l = 0; l < 3; l++) {
-    auto g = gemm_args(in, net.w16t[l].ptr, R, kN[l], kK[l], in_ld, kK[l], mlp::kEpiHidden);
-    g.bias = static_cast<const float *>(net.b[l].ptr);
-    g.out_f32 = training ? static_cast<float *>(A.z[l].ptr) : nullptr;
-    g.out_bf16 = static_cast<bf16 *>(A.a[training ? l : (l & 1)].ptr);
-    g.out_t = training ? static_cast<bf16 *>(A.a_t[l].ptr) : nullptr;
-    g.ldo = kN[l];
-    g.ldt = R;
-    if (int st = hip_check(mlp::gemm(g, s), "mlp gemm forward")) return st;
-    in = g.out_bf16;
-    in_ld = kN[l];
+    a.wt[l] = static_cast<const bf16 *>(net.w16t[l].ptr);
+    a.a_t[l] = static_cast<bf16 *>(A.a_t[l].ptr);
+    a.dz_t[l] = static_cast<bf16 *>(A.dz_t[l].ptr);
   }
-  auto g = gemm_args(in, net.w16t[3].ptr, R, kN[3], kK[3], in_ld, kK[3], mlp::kEpiOut);
-  g.bias = static_cast<const float *>(net.b[3].ptr);
-  g.out_f32 = static_cast<float *>(A.pred.ptr);
-  return hip_check(mlp::gemm(g, s), "mlp gemm output");
+  for (int l = 0; l < 4; l++) {
+    a.w[l] = static_cast<const bf16 *>(net.w16[l].ptr);
+    a.bias[l] = static_cast<const float *>(net.b[l].ptr);
+  }
+  a.loss = static_cast<unsigned long long *>(A.loss.ptr);
+  a.feat_t = static_cast<bf16 *>(A.feat_t.ptr);
+  a.dz3_t = static_cast<bf16 *>(A.dz3_t.ptr);
+  return hip_check(mlp::rows(a, s), "mlp rows");
 }
 
-// Backward from dz3 (already written by the loss kernel): weight and bias
-// gradients as K slices.
-int backward(Net &net, Acts &A, size_t rows, hipStream_t s) {
-  const int R = int(rows);
-  const void *dz_t_next = A.dz3_t.ptr;  // [N_l][R]
-  const void *dz_next = A.dz3.ptr;      // [R][N_l]
-  for (int l = 3; l >= 0; l--) {
-    // dW_l = A_{l-1}^T dZ_l  -> [K_l][N_l], and db_l = column sums of dZ_l, per K slice
-    const void *prev_t = l == 0 ? A.feat_t.ptr : A.a_t[l - 1].ptr;
-    auto gw = gemm_args(prev_t, dz_t_next, kK[l], kN[l], R, R, R, mlp::kEpiF32);
-    gw.ldo = kN[l];
-    gw.ksplit = mlp::auto_ksplit(kK[l], kN[l], R);
-    gw.work = gw.out_f32 = static_cast<float *>(A.gpart[l].ptr);
-    gw.bias_part = static_cast<float *>(A.bpart[l].ptr);
-    gw.defer = 1;
-    A.splits[l] = gw.ksplit;
-    if (int st = hip_check(mlp::gemm(gw, s), "mlp gemm dW")) return st;
-    if (l == 0) break;
-    // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1})  -> [R][K_l]
-    auto ga = gemm_args(dz_next, net.w16[l].ptr, R, kK[l], kN[l], kN[l], kN[l], mlp::kEpiBackGelu);
-    ga.zprev = static_cast<const float *>(A.z[l - 1].ptr);
-    ga.out_bf16 = static_cast<bf16 *>(A.dz[l - 1].ptr);
-    ga.out_t = static_cast<bf16 *>(A.dz_t[l - 1].ptr);
-    ga.ldo = kK[l];
-    ga.ldt = R;
-    if (int st = hip_check(mlp::gemm(ga, s), "mlp gemm dA")) return st;
-    dz_next = A.dz[l - 1].ptr;
-    dz_t_next = A.dz_t[l - 1].ptr;
+// dW_l = A_{l-1}^T dZ_l -> [K_l][N_l] and db_l = the column sums of dZ_l,
+// as mlp::dw_splits(rows) K slices (split over the batch rows) from the transposed
+// rows, all four layers in one launch.
+int weight_grads(Net &net, Acts &A, size_t rows, hipStream_t s) {
+  (void)net;
+  mlp::DwArgs a{};
+  a.count = 4;
+  a.ld = rows;
+  a.splits = mlp::dw_splits(rows);
+  for (int l = 0; l < 4; l++) {
+    a.d[l].p = static_cast<const bf16 *>(l == 0 ? A.feat_t.ptr : A.a_t[l - 1].ptr);
+    a.d[l].q = static_cast<const bf16 *>(l == 3 ? A.dz3_t.ptr : A.dz_t[l].ptr);
+    a.d[l].M = kK[l];
+    a.d[l].N = kN[l];
+    a.d[l].work = static_cast<float *>(A.gpart[l].ptr);
+    a.d[l].bias_part = static_cast<float *>(A.bpart[l].ptr);
+    A.splits[l] = a.splits;
   }
-  return BUDE_OK;
+  return hip_check(mlp::weight_grads(a, s), "mlp weight gradients");
 }
 
 int sum_grads(Net &net, const Acts &A, hipStream_t s) {
@@ -179,39 +145,35 @@ int train(Net &net, const TrainData &d, std::vector<uint32_t> perm, const TrainP
   Acts A;
   if (int st = alloc_acts(A, B, true)) return st;
   std::mt19937_64 rng(p.seed);
-  DeviceBuffer dperm, dstate, didx;
+  DeviceBuffer dperm, dstate;
   if (int st = dperm.alloc(ntrain * 4)) return st;
   if (int st = dstate.alloc(sizeof(mlp::StepState))) return st;
-  if (int st = didx.alloc(B * 4)) return st;
   const uint32_t epochs = std::max<uint32_t>(1, p.epochs);
   const uint32_t per_epoch = uint32_t(ntrain / B);
   const uint32_t total = per_epoch * epochs;
   const uint32_t warm = std::min<uint32_t>(200, total / 10 + 1);
   const uint32_t step0 = uint32_t(net.step);
   auto *state = static_cast<mlp::StepState *>(dstate.ptr);
-  const uint32_t *idx = static_cast<const uint32_t *>(didx.ptr);
   // one step with every per-step value (batch indices, learning rate, Adam
   // bias corrections) computed on the device from the step count, captured
   // once as a hipGraph and replayed step after step
+  mlp::RowsArgs in{};
+  in.poses = d.poses;
+  in.stride = d.stride;
+  in.desc = d.desc;
+  in.lig = d.lig;
+  in.nfeat = d.nfeat;
+  in.perm = static_cast<const uint32_t *>(dperm.ptr);
+  in.per_epoch = per_epoch;
+  in.state = state;
+  in.sched = mlp::Schedule{total, warm, step0, p.lr, 0.f, 0.9f, 0.999f};
+  in.energy = d.energy;
+  in.norm = d.norm;
+  in.table = d.table;
+  in.tail_weight = p.tail_weight;
   auto one_step = [&]() -> int {
-    if (int st = hip_check(mlp::begin_step(state, static_cast<const uint32_t *>(dperm.ptr), per_epoch, uint32_t(B),
-                                           static_cast<uint32_t *>(didx.ptr), total, warm, p.lr, 0.f, 0.9f, 0.999f,
-                                           step0, s),
-                           "begin step"))
-      return st;
-    if (int st = hip_check(mlp::features(d.poses, d.stride, idx, B, static_cast<bf16 *>(A.feat.ptr),
-                                         static_cast<bf16 *>(A.feat_t.ptr), int(B), s, d.desc, d.lig, d.nfeat),
-                           "features"))
-      return st;
-    if (int st = forward(net, A, B, true, s)) return st;
-    if (int st = hip_check(mlp::loss_grad(static_cast<const float *>(A.pred.ptr), d.energy, idx, B, d.norm, d.table,
-                                          d.lig, p.tail_weight, static_cast<bf16 *>(A.dz3.ptr),
-                                          static_cast<bf16 *>(A.dz3_t.ptr), int(B),
-                                          static_cast<unsigned long long *>(A.loss.ptr),
-                                          s),
-                           "loss"))
-      return st;
-    if (int st = backward(net, A, B, s)) return st;
+    if (int st = rows_step(net, A, B, in, s)) return st;
+    if (int st = weight_grads(net, A, B, s)) return st;
     // Adam over the gradient slices; the same launch ends the step
     mlp::AdamBatch ab{};
     for (int l = 0; l < 4; l++) {
@@ -248,24 +210,53 @@ int train(Net &net, const TrainData &d, std::vector<uint32_t> perm, const TrainP
     if (int st = hip_check(ce, "hipStreamEndCapture")) return st;
     if (int st = hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate")) return st;
   }
-  TrainReport r{};
+  // epoch e's order is shuffled on the host while epoch e - 1 runs on the
+  // device: two pinned staging buffers (one is refilled only after its last
+  // copy has ended), the epoch's loss sum set aside on the device, and one
+  // synchronization at the end
+  PinnedBuffer host_perm[2];
+  DeviceBuffer dlosses;
+  hipEvent_t copied[2] = {nullptr, nullptr};
+  struct EventGuard {
+    hipEvent_t (&e)[2];
+    ~EventGuard() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } event_guard{copied};
+  for (int k = 0; k < 2; k++) {
+    if (int st = host_perm[k].alloc(ntrain * 4)) return st;
+    if (int st = hip_check(hipEventCreateWithFlags(&copied[k], hipEventDisableTiming), "hipEventCreate")) return st;
+  }
+  if (int st = dlosses.alloc(size_t(epochs) * 8)) return st;
   for (uint32_t ep = 0; ep < epochs; ep++) {
-    // the epoch's order: uploaded (and the loss sum cleared) outside the
-    // graph, between replays
     std::shuffle(perm.begin(), perm.end(), rng);
-    if (int st = hip_check(hipMemcpyAsync(dperm.ptr, perm.data(), ntrain * 4, hipMemcpyHostToDevice, s),
+    const int k = int(ep & 1);
+    if (ep >= 2)
+      if (int st = hip_check(hipEventSynchronize(copied[k]), "hipEventSynchronize")) return st;
+    std::copy(perm.begin(), perm.end(), static_cast<uint32_t *>(host_perm[k].ptr));
+    if (int st = hip_check(hipMemcpyAsync(dperm.ptr, host_perm[k].ptr, ntrain * 4, hipMemcpyHostToDevice, s),
                            "hipMemcpyAsync"))
       return st;
+    if (int st = hip_check(hipEventRecord(copied[k], s), "hipEventRecord")) return st;
     if (int st = hip_check(hipMemsetAsync(A.loss.ptr, 0, 8, s), "hipMemsetAsync")) return st;
     for (uint32_t it = 0; it < per_epoch; it++) {
       const int st = use_graph ? hip_check(hipGraphLaunch(exec, s), "hipGraphLaunch") : one_step();
       if (st) return st;
     }
-    unsigned long long lsum = 0;
-    if (int st = hip_check(hipMemcpyAsync(&lsum, A.loss.ptr, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync"))
+    if (int st = hip_check(hipMemcpyAsync(static_cast<unsigned long long *>(dlosses.ptr) + ep, A.loss.ptr, 8,
+                                          hipMemcpyDeviceToDevice, s),
+                           "hipMemcpyAsync"))
       return st;
-    if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
-    const float mse = float(double(lsum) / mlp::kLossScale / double(size_t(per_epoch) * B));
+  }
+  std::vector<unsigned long long> lsum(epochs);
+  if (int st = hip_check(hipMemcpyAsync(lsum.data(), dlosses.ptr, size_t(epochs) * 8, hipMemcpyDeviceToHost, s),
+                         "hipMemcpyAsync"))
+    return st;
+  if (int st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize")) return st;
+  TrainReport r{};
+  for (uint32_t ep = 0; ep < epochs; ep++) {
+    const float mse = float(double(lsum[ep]) / mlp::kLossScale / double(size_t(per_epoch) * B));
     if (loss_curve && ep < cap) loss_curve[ep] = mse;
     r.final_loss = mse;
   }

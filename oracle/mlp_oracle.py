@@ -5,8 +5,8 @@ kernels; the product never imports it.
 Network: 21 features -> 256 -> 256 -> 256 -> 1, GELU (tanh form,
 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))) on the three hidden layers, loss = mean((pred - target)^2).  With ``bf16=True`` values are
 rounded to BF16 (round to nearest even) exactly where the GPU stores BF16:
-inputs, weights, hidden activations and the back-propagated dZ; everything
-else is FP64.  With ``bf16=False`` it is the plain FP64 network, whose
+inputs, weights, hidden activations, GELU'(Z) of the hidden layers (kept for
+the backward pass) and the back-propagated dZ; everything else is FP64.  With ``bf16=False`` it is the plain FP64 network, whose
 gradients tests/test_mlp_oracle.py pins against central finite differences.
 """
 from __future__ import annotations
@@ -59,7 +59,7 @@ def forward_backward(feat, target, W, b, bf16: bool = True):
         gb[l] = dz.sum(axis=0)
         if l == 0:
             break
-        dz = r((dz @ Wr[l].T) * gelu_grad(zs[l - 1]))
+        dz = r((dz @ Wr[l].T) * r(gelu_grad(zs[l - 1])))
     return pred, loss, gW, gb
 
 
