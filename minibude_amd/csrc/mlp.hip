@@ -183,12 +183,13 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
   bf16x8 fx[8][kNB], fy[8][kNB];
   rows_load<kIn>(a.wt[0], kIn, wave, r, grp, fx);
   rows_load<kHidden>(a.wt[1], kHidden, wave, r, grp, fy);
-  // ---- the input row of thread p < kRB (LDS row p / 2 + 16 (p % 2))
+  // ---- the input row p = tid < kRB (LDS row p / 2 + 16 (p % 2))
   if (tid < kRB) {
-    const size_t row = row0 + tid;
+    const int p = tid;
+    const size_t row = row0 + p;
     const uint32_t q = a.perm ? a.perm[size_t(a.state->step % a.per_epoch) * a.n + row]
                               : (a.idx ? a.idx[row] : uint32_t(row));
-    qs[tid] = q;
+    qs[p] = q;
     uint4 h[kIn / 8];
     if (a.feat) {
 #pragma unroll
@@ -213,8 +214,9 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
       for (int c = 0; c < kIn; c++) a.feat_t[size_t(c) * a.n + row] = bf16(hw[c >> 1] >> (16 * (c & 1)));
     }
 #pragma unroll
-    for (int v = 0; v < kIn / 8; v++) reinterpret_cast<uint4 *>(&xs[(tid >> 1) + 16 * (tid & 1)][0])[v] = h[v];
+    for (int v = 0; v < kIn / 8; v++) reinterpret_cast<uint4 *>(&xs[(p >> 1) + 16 * (p & 1)][0])[v] = h[v];
   }
+
   __syncthreads();
   // ---- forward: three hidden layers, GELU'(Z) kept in LDS
   float w3[kNB][4];  // W3[n][0] of the wave's neurons n = 16 kNB w + 16 jn + 4 grp + c
@@ -387,19 +389,19 @@ __global__ __launch_bounds__(64) void dw_kernel(const DwArgs a) {
     for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   float bs[4] = {0.f, 0.f, 0.f, 0.f};
   const bool bsum = d.bias_part && it.m0 == 0;
-  // group g = k steps 2 g, 2 g + 1 (the last may have one): fragments [step][i or j]
+  // group g = k steps 2 g, 2 g + 1: fragments [step][i or j].  Every load
+  // is unconditional (blocks past the tile's edge and steps past the slice
+  // reload valid ones, unused), so the loads' waits count the fragments in
+  // flight instead of draining them.
   bf16x8 fp[2][2][4], fq[2][2][4];
   auto load = [&](int g, bf16x8 (&ap)[2][4], bf16x8 (&aq)[2][4]) {
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-      if (2 * g + t >= nsteps) break;
-      const size_t k = size_t(64 * g + 32 * t);
+      const size_t k = size_t(32 * min(2 * g + t, nsteps - 1));
 #pragma unroll
-      for (int i = 0; i < 4; i++)
-        if (i < bm) ap[t][i] = load8(pp + size_t(16 * i) * a.ld + k);
+      for (int i = 0; i < 4; i++) ap[t][i] = load8(pp + size_t(16 * min(i, bm - 1)) * a.ld + k);
 #pragma unroll
-      for (int j = 0; j < 4; j++)
-        if (j < bn) aq[t][j] = load8(qp + size_t(16 * j) * a.ld + k);
+      for (int j = 0; j < 4; j++) aq[t][j] = load8(qp + size_t(16 * min(j, bn - 1)) * a.ld + k);
     }
   };
   auto mul = [&](int g, const bf16x8 (&ap)[2][4], const bf16x8 (&aq)[2][4]) {
@@ -426,9 +428,9 @@ __global__ __launch_bounds__(64) void dw_kernel(const DwArgs a) {
   const int groups = (nsteps + 1) / 2;
   load(0, fp[0], fq[0]);
   for (int g = 0; g < groups; g += 2) {
-    if (g + 1 < groups) load(g + 1, fp[1], fq[1]);
+    load(min(g + 1, groups - 1), fp[1], fq[1]);
     mul(g, fp[0], fq[0]);
-    if (g + 2 < groups) load(g + 2, fp[0], fq[0]);
+    load(min(g + 2, groups - 1), fp[0], fq[0]);
     if (g + 1 < groups) mul(g + 1, fp[1], fq[1]);
   }
   float *out = d.work + size_t(it.split) * size_t(d.M) * d.N;
