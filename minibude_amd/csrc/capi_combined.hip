@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -211,6 +212,11 @@ struct bude_combined {
   std::vector<int32_t> slot_of_type;  // forcefield type -> map slot (-1: unused)
   std::vector<int32_t> slot_type;     // map slot -> forcefield type
   DeviceBuffer ligands;               // float4: x, y, z, slot bits
+  // the field scorer's atoms, grouped by map slot for the XCDs
+  // (field::split_atoms), the groups' bounds, and partial-sum scratch of the
+  // chunk scores (second stream) and the sampled poses' scores (third)
+  int field_groups = 1;
+  DeviceBuffer fgrouped, fgroup_begin, fpart, fpart_lab;
   DeviceBuffer poses;                 // 6 rows of `total`
   DeviceBuffer pred, exact_all;
   DeviceBuffer desc;                  // [nlig][kDesc]
@@ -433,23 +439,58 @@ int prepare_field(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer
 
 // Interpolated field scores (out[off + j]) of tiles [tile0, tile0 + ntiles)
 // of the spans prepared by prepare_field, from the current cells.
+// The field scorer's atom groups (field::split_atoms): one per XCD, or one
+// in all when BUDE_MI355X_FIELD_GROUPS=1 (the form without the split, kept
+// to measure the split against).
+int group_field_atoms(bude_combined *c, const std::vector<float> &lig4) {
+  const char *env = std::getenv("BUDE_MI355X_FIELD_GROUPS");
+  c->field_groups = env && std::strcmp(env, "1") == 0 ? 1 : bude::field::kXcdGroups;
+  std::vector<float4> atoms(lig4.size() / 4), grouped;
+  std::memcpy(atoms.data(), lig4.data(), atoms.size() * sizeof(float4));
+  std::vector<int32_t> off(c->lig_off), begin;
+  off.push_back(int32_t(atoms.size()));
+  bude::field::split_atoms(c->field_groups, atoms, off, c->nposes, int(c->slot_type.size()), grouped, begin);
+  if (int st = c->fgrouped.upload(grouped.data(), grouped.size() * sizeof(float4))) return st;
+  return c->fgroup_begin.upload(begin.data(), begin.size() * sizeof(int32_t));
+}
+
+// Tiles of one field launch when the atoms are split over the XCDs: 16.8M
+// poses, 512 MiB of partial sums at 8 groups.
+constexpr int64_t kFieldLaunchTiles = int64_t(1) << 18;
+
+// Bytes of partial-sum scratch a field launch over `ntiles` tiles needs.
+size_t field_partial_bytes(const bude_combined *c, int64_t ntiles) {
+  return c->field_groups > 1 ? size_t(std::min(ntiles, kFieldLaunchTiles)) * size_t(c->field_groups) * 64 * 4 : 0;
+}
+
+// partial: scratch of field_partial_bytes(ntiles), used by nothing else on
+// another stream meanwhile.
 int launch_field(bude_combined *c, const float *poses, size_t stride, float *out, const DeviceBuffer &meta,
                  const DeviceBuffer &tile_begin, int64_t tile0, int64_t ntiles, hipStream_t stream,
-                 bool raise_priority = false) {
+                 DeviceBuffer &partial, bool raise_priority = false) {
   if (ntiles == 0) return BUDE_OK;
+  if (partial.bytes < field_partial_bytes(c, ntiles))
+    return bude::fail(BUDE_ERR_RUNTIME, "field score: partial-sum scratch too small");
   bude::field::ScoreArgs a{};
   a.cells = static_cast<const uint4 *>(c->cells.ptr);
   a.grid = c->grid;
-  a.ligands = static_cast<const float4 *>(c->ligands.ptr);
+  a.grouped = static_cast<const float4 *>(c->fgrouped.ptr);
+  a.group_begin = static_cast<const int32_t *>(c->fgroup_begin.ptr);
+  a.ngroups = c->field_groups;
+  a.partial = static_cast<float *>(partial.ptr);
   a.meta = static_cast<const bude::LigandMeta *>(meta.ptr);
   a.tile_begin = static_cast<const int64_t *>(tile_begin.ptr);
   a.nligands = int(c->nlig);
   a.poses = poses;
   a.stride = stride;
   a.out = out;
-  a.tile0 = tile0;
   a.raise_priority = raise_priority;
-  return bude::hip_check(bude::field::score(a, ntiles, stream), "field score");
+  const int64_t step = c->field_groups > 1 ? kFieldLaunchTiles : ntiles;
+  for (int64_t t = 0; t < ntiles; t += step) {
+    a.tile0 = tile0 + t;
+    if (int st = bude::hip_check(bude::field::score(a, std::min(step, ntiles - t), stream), "field score")) return st;
+  }
+  return BUDE_OK;
 }
 
 // Surrogate scores of the per-atom kinds (the field's maps or the learned
@@ -457,9 +498,9 @@ int launch_field(bude_combined *c, const float *poses, size_t stride, float *out
 // prepare_field.
 int launch_surrogate(bude_combined *c, const float *poses, size_t stride, float *out, const DeviceBuffer &meta,
                      const DeviceBuffer &tile_begin, int64_t tile0, int64_t ntiles, hipStream_t stream,
-                     bool raise_priority = false) {
+                     DeviceBuffer &partial, bool raise_priority = false) {
   if (c->cfg.surrogate != BUDE_SURROGATE_ATOM_MLP)
-    return launch_field(c, poses, stride, out, meta, tile_begin, tile0, ntiles, stream, raise_priority);
+    return launch_field(c, poses, stride, out, meta, tile_begin, tile0, ntiles, stream, partial, raise_priority);
   if (ntiles == 0) return BUDE_OK;
   bude::atomnet::ScoreArgs a = bude::atomnet::score_args(c->anet);
   a.ligands = static_cast<const float4 *>(c->ligands.ptr);
@@ -883,6 +924,7 @@ extern "C" int bude_mi355x_combined_create(int device, const BudeFFParams *force
 
   int st = plan_selection(c.get(), cfg->keep_fraction);
   if (!st) st = c->ligands.upload(lig4.data(), lig4.size() * 4);
+  if (!st && cfg->surrogate == BUDE_SURROGATE_FIELD) st = group_field_atoms(c.get(), lig4);
   if (!st) st = c->desc.upload(desc.data(), desc.size() * 4);
   if (!st) st = c->poses.alloc(6 * total * sizeof(float));
   for (size_t i = 0; i < nligands && !st; i++)
@@ -1064,6 +1106,13 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     }
     ft_all = c->tb_all_host[L];
     if (int st = prepare_field(c, lab_spans, c->meta_lf, c->tb_label, &ft_label, nullptr, c->stream3)) return st;
+    if (!atom && c->field_groups > 1) {  // partial sums of the largest chunk, and of the sampled poses
+      int64_t most = 0;
+      for (size_t k = 0; k + 1 < c->chunk_lig.size(); k++)
+        most = std::max(most, c->tb_all_host[c->chunk_lig[k + 1]] - c->tb_all_host[c->chunk_lig[k]]);
+      if (int st = c->fpart.reserve(field_partial_bytes(c, most))) return st;
+      if (int st = c->fpart_lab.reserve(field_partial_bytes(c, ft_label))) return st;
+    }
   }
   const size_t K = c->chunk_lig.size() - 1;
   int2 *rs_tiles = static_cast<int2 *>(c->s().tiles_rs.ptr);
@@ -1116,7 +1165,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     for (size_t k = 0; k < K; k++) {
       const size_t a = c->chunk_lig[k], b = c->chunk_lig[k + 1];
       if (int st = launch_surrogate(c, poses, c->total, pred, c->meta_all, c->tb_all, tb_all[a], tb_all[b] - tb_all[a],
-                                    c->stream2, k > 0 || overlap))
+                                    c->stream2, c->fpart, k > 0 || overlap))
         return st;
       if (int st = bude::hip_check(hipEventRecord(c->s().chunk_ev[2 * k], c->stream2), "hipEventRecord")) return st;
       if (int st = select_and_tables(c, *c->chunk_plan[k], a, static_cast<const int32_t *>(c->rs_order.ptr) + a,
@@ -1132,7 +1181,8 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     return st;
   if (!mlp) {  // field scores of the sampled poses, once the cells exist
     if (int st = bude::hip_check(hipStreamWaitEvent(c->stream3, c->s().ev[3], 0), "hipStreamWaitEvent")) return st;
-    if (int st = launch_surrogate(c, lpos, nlab, lp, c->meta_lf, c->tb_label, 0, ft_label, c->stream3)) return st;
+    if (int st = launch_surrogate(c, lpos, nlab, lp, c->meta_lf, c->tb_label, 0, ft_label, c->stream3, c->fpart_lab))
+      return st;
     if (int st = bude::hip_check(hipMemcpyAsync(lpred.data(), lp, nlab * 4, hipMemcpyDeviceToHost, c->stream3),
                                  "hipMemcpyAsync"))
       return st;

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "bude_internal.h"
 #include "fasten_batch.h"
@@ -44,23 +45,50 @@ hipError_t build_maps(const float4 *protein, const int32_t *group_begin, const P
 // cells has nslots * g.cells() entries.
 hipError_t pack_cells(const float *maps, int nslots, const Grid &g, uint4 *cells, hipStream_t s);
 
+// The device's work-groups go round-robin to its 8 XCDs (work-group i to
+// XCD i mod 8), each with its own 4 MB L2.  The cells of every map slot
+// (143 MB for 34 slots at 0.5 A) do not fit one L2, and a wave's 64 poses
+// put its atoms anywhere in the box, so every cell read misses L2.  The
+// scorer therefore splits each ligand's atoms into kXcdGroups groups by map
+// slot (split_atoms) and gives group g to work-groups i with i mod 8 == g:
+// each L2 then caches only its group's slots.  The groups' partial sums of
+// a pose are added in group order by a second kernel.
+constexpr int kXcdGroups = 8;
+
+// Assignment of the ligands' atoms to `ngroups` groups (1 or kXcdGroups):
+// items (ligand, slot) weighted by atoms x poses, slots in decreasing order
+// of their total weight (ties by slot), each slot's ligands in order, filled
+// into groups 0, 1, ... up to an equal share of the total each, so a group
+// holds a few whole slots and at most two partial ones.  grouped holds every
+// ligand's atoms, group by group, each group's in their ligand order;
+// group_begin[l * (ngroups + 1) + g] is the index in grouped of ligand l's
+// first atom of group g (group_begin[l * (ngroups + 1) + ngroups] its end).
+// ligands: x, y, z, map slot (int bits); ligand l's atoms are
+// [lig_off[l], lig_off[l + 1]); nposes[l] its pose count.
+void split_atoms(int ngroups, const std::vector<float4> &ligands, const std::vector<int32_t> &lig_off,
+                 const std::vector<size_t> &nposes, int nslots, std::vector<float4> &grouped,
+                 std::vector<int32_t> &group_begin);
+
 struct ScoreArgs {
-  const uint4 *cells;         // nslots * grid.cells() packed cells (pack_cells)
+  const uint4 *cells;          // nslots * grid.cells() packed cells (pack_cells)
   Grid grid;
-  const float4 *ligands;      // x, y, z, map slot (int bits)
-  const LigandMeta *meta;     // lig_off, natlig, nposes, pose_off of every ligand
-  const int64_t *tile_begin;  // nligands + 1: first 64-pose tile of each ligand
+  const float4 *grouped;       // x, y, z, map slot (int bits), grouped by split_atoms
+  const int32_t *group_begin;  // nligands * (ngroups + 1) (split_atoms)
+  int ngroups;                 // 1 or kXcdGroups
+  const LigandMeta *meta;      // nposes, pose_off of every ligand
+  const int64_t *tile_begin;   // nligands + 1: first 64-pose tile of each ligand
   int nligands;
-  const float *poses;         // 6 rows of `stride` floats
+  const float *poses;          // 6 rows of `stride` floats
   size_t stride;
-  float *out;                 // out[meta.pose_off + j]
-  int64_t tile0;              // first tile scored
-  bool raise_priority;        // waves issue ahead of a concurrent kernel's
+  float *out;                  // out[meta.pose_off + j]
+  float *partial;              // ngroups > 1: ngroups * ntiles * 64 floats of scratch
+  int64_t tile0;               // first tile scored
+  bool raise_priority;         // waves issue ahead of a concurrent kernel's
 };
 
 // Interpolated energy of every pose of tiles [tile0, tile0 + ntiles): one
-// wave per tile of 64 poses of one ligand; every tile is tile0 = 0, ntiles =
-// tile_begin[nligands].
+// wave per tile of 64 poses of one ligand and group of its atoms; every tile
+// is tile0 = 0, ntiles = tile_begin[nligands].
 hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s);
 
 }  // namespace field

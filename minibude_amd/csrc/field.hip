@@ -1,6 +1,9 @@
 // field.hip -- single-atom energy maps and the interpolating pose scorer.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+
 #include "fasten_core.h"
 #include "field.h"
 
@@ -167,36 +170,48 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ map
   cells[size_t(blockIdx.y) * nc + c] = v.u;
 }
 
-__global__ __launch_bounds__(256) void score_kernel(const uint4 *__restrict__ cells, const Grid g,
-                                                    const float4 *__restrict__ ligands,
-                                                    const LigandMeta *__restrict__ meta,
-                                                    const int64_t *__restrict__ tile_begin, int nligands,
-                                                    const float *__restrict__ poses, size_t stride,
-                                                    float *__restrict__ out, int64_t tile0, int64_t tend,
-                                                    int raise_priority) {
-  // Optional issue priority over a concurrent exact pass (the pipelined
-  // combined screen rescoring the previous chunk): these waves wait on cell
-  // reads most of the time and get the VALU the moment their data arrives.
-  if (raise_priority) __builtin_amdgcn_s_setprio(2);
-  const int lane = threadIdx.x & 63;
-  const int64_t t = tile0 + int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (t >= tend) return;
-  // ligand of this tile: last l with tile_begin[l] <= t (wave-uniform search)
+// The ligand of tile t: the last l with tile_begin[l] <= t (wave-uniform).
+__device__ __forceinline__ int tile_ligand(const int64_t *__restrict__ tile_begin, int nligands, int64_t t) {
   int lo = 0, hi = nligands - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (tile_begin[mid] <= t) lo = mid;
     else hi = mid - 1;
   }
-  const LigandMeta m = meta[lo];
-  const int j = int(t - tile_begin[lo]) * 64 + lane;
+  return lo;
+}
+
+// Work-group b scores atom group b mod ngroups of the 4 tiles
+// tile0 + 4 (b / ngroups) + wave: with ngroups = 8 every work-group of one
+// XCD reads the cells of one group's map slots.
+__global__ __launch_bounds__(256) void score_kernel(const uint4 *__restrict__ cells, const Grid g,
+                                                    const float4 *__restrict__ grouped,
+                                                    const int32_t *__restrict__ group_begin, int ngroups,
+                                                    const LigandMeta *__restrict__ meta,
+                                                    const int64_t *__restrict__ tile_begin, int nligands,
+                                                    const float *__restrict__ poses, size_t stride,
+                                                    float *__restrict__ out, float *__restrict__ partial,
+                                                    int64_t tile0, int64_t tend, int raise_priority) {
+  // Optional issue priority over a concurrent exact pass (the pipelined
+  // combined screen rescoring the previous chunk): these waves wait on cell
+  // reads most of the time and get the VALU the moment their data arrives.
+  if (raise_priority) __builtin_amdgcn_s_setprio(2);
+  const int lane = threadIdx.x & 63;
+  const int grp = int(blockIdx.x % unsigned(ngroups));
+  const int64_t t = tile0 + int64_t(blockIdx.x / unsigned(ngroups)) * 4 + (threadIdx.x >> 6);
+  if (t >= tend) return;
+  const int l = tile_ligand(tile_begin, nligands, t);
+  const LigandMeta m = meta[l];
+  const int j = int(t - tile_begin[l]) * 64 + lane;
   const Xform x = make_xform(poses, stride, size_t(m.pose_off) + size_t(j < m.nposes ? j : m.nposes - 1));
   const float inv_h = 1.0f / g.h;
   const size_t nc = g.cells();
   const int c2 = g.n[2] - 1, c12 = (g.n[1] - 1) * c2;
+  const int a0 = group_begin[size_t(l) * size_t(ngroups + 1) + size_t(grp)];
+  const int a1 = group_begin[size_t(l) * size_t(ngroups + 1) + size_t(grp) + 1];
   float e = 0.f;
-  for (int il = 0; il < m.natlig; ++il) {
-    const float4 la = ligands[m.lig_off + il];
+  for (int il = a0; il < a1; ++il) {
+    const float4 la = grouped[il];
     const float wx = x.m[3] + la.x * x.m[0] + la.y * x.m[1] + la.z * x.m[2];
     const float wy = x.m[7] + la.x * x.m[4] + la.y * x.m[5] + la.z * x.m[6];
     const float wz = x.m[11] + la.x * x.m[8] + la.y * x.m[9] + la.z * x.m[10];
@@ -215,7 +230,31 @@ __global__ __launch_bounds__(256) void score_kernel(const uint4 *__restrict__ ce
     const float c0 = __builtin_fmaf(fy, c01 - c00, c00), c1 = __builtin_fmaf(fy, c11 - c10, c10);
     e += __builtin_fmaf(fx, c1 - c0, c0);
   }
-  if (j < m.nposes) out[size_t(m.pose_off) + size_t(j)] = e;
+  if (ngroups == 1) {
+    if (j < m.nposes) out[size_t(m.pose_off) + size_t(j)] = e;
+  } else {
+    // every lane of the tile, so the sum kernel reads whole rows
+    partial[size_t(grp) * size_t(tend - tile0) * 64 + size_t(t - tile0) * 64 + size_t(lane)] = e;
+  }
+}
+
+// out[pose] = the groups' partial sums of the pose added in group order
+// (one wave per tile of tiles [tile0, tend)).
+__global__ __launch_bounds__(256) void sum_kernel(const float *__restrict__ partial, int ngroups,
+                                                  const LigandMeta *__restrict__ meta,
+                                                  const int64_t *__restrict__ tile_begin, int nligands,
+                                                  float *__restrict__ out, int64_t tile0, int64_t tend) {
+  const int lane = threadIdx.x & 63;
+  const int64_t t = tile0 + int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (t >= tend) return;
+  const int l = tile_ligand(tile_begin, nligands, t);
+  const LigandMeta m = meta[l];
+  const int j = int(t - tile_begin[l]) * 64 + lane;
+  if (j >= m.nposes) return;
+  const size_t n = size_t(tend - tile0) * 64, i = size_t(t - tile0) * 64 + size_t(lane);
+  float e = partial[i];
+  for (int grp = 1; grp < ngroups; grp++) e += partial[size_t(grp) * n + i];
+  out[size_t(m.pose_off) + size_t(j)] = e;
 }
 
 }  // namespace
@@ -246,12 +285,68 @@ hipError_t pack_cells(const float *maps, int nslots, const Grid &g, uint4 *cells
 hipError_t score(const ScoreArgs &a, int64_t ntiles, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (a.nligands <= 0 || a.tile0 < 0 || a.grid.n[0] < 2 || a.grid.n[1] < 2 || a.grid.n[2] < 2 ||
-      (ntiles + 3) / 4 > 0x7fffffff)
+      !(a.ngroups == 1 || a.ngroups == kXcdGroups) || (a.ngroups > 1 && !a.partial) ||
+      (ntiles + 3) / 4 * a.ngroups > 0x7fffffff)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(score_kernel, dim3(unsigned((ntiles + 3) / 4)), dim3(256), 0, s, a.cells, a.grid, a.ligands,
-                     a.meta, a.tile_begin, a.nligands, a.poses, a.stride, a.out, a.tile0, a.tile0 + ntiles,
-                     int(a.raise_priority));
+  const unsigned blocks = unsigned((ntiles + 3) / 4);
+  hipLaunchKernelGGL(score_kernel, dim3(blocks * unsigned(a.ngroups)), dim3(256), 0, s, a.cells, a.grid, a.grouped,
+                     a.group_begin, a.ngroups, a.meta, a.tile_begin, a.nligands, a.poses, a.stride, a.out, a.partial,
+                     a.tile0, a.tile0 + ntiles, int(a.raise_priority));
+  if (a.ngroups > 1)
+    hipLaunchKernelGGL(sum_kernel, dim3(blocks), dim3(256), 0, s, a.partial, a.ngroups, a.meta, a.tile_begin,
+                       a.nligands, a.out, a.tile0, a.tile0 + ntiles);
   return hipGetLastError();
+}
+
+namespace {
+int slot_bits(float w) {
+  int s;
+  std::memcpy(&s, &w, sizeof s);
+  return s;
+}
+}  // namespace
+
+void split_atoms(int ngroups, const std::vector<float4> &ligands, const std::vector<int32_t> &lig_off,
+                 const std::vector<size_t> &nposes, int nslots, std::vector<float4> &grouped,
+                 std::vector<int32_t> &group_begin) {
+  const size_t L = nposes.size();
+  // weight of every (ligand, slot) item and of every slot
+  std::vector<double> item(L * size_t(nslots), 0.0), total(size_t(nslots), 0.0);
+  for (size_t l = 0; l < L; l++)
+    for (int32_t a = lig_off[l]; a < lig_off[l + 1]; a++) {
+      const int s = slot_bits(ligands[size_t(a)].w);
+      item[l * size_t(nslots) + size_t(s)] += double(nposes[l]);
+      total[size_t(s)] += double(nposes[l]);
+    }
+  std::vector<int> order(static_cast<size_t>(nslots), 0);
+  for (int s = 0; s < nslots; s++) order[size_t(s)] = s;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return total[size_t(x)] > total[size_t(y)]; });
+  double sum = 0.0;
+  for (double w : total) sum += w;
+  // group of each item: walk the items slot by slot, ligand by ligand, and
+  // move to the next group once the running weight passes its share
+  std::vector<int> group_of(L * size_t(nslots), 0);
+  double run = 0.0;
+  int grp = 0;
+  for (int s : order)
+    for (size_t l = 0; l < L; l++) {
+      const double w = item[l * size_t(nslots) + size_t(s)];
+      if (w == 0.0) continue;
+      group_of[l * size_t(nslots) + size_t(s)] = grp;
+      run += w;
+      while (grp + 1 < ngroups && run >= sum * double(grp + 1) / double(ngroups)) grp++;
+    }
+  grouped.clear();
+  group_begin.assign(L * size_t(ngroups + 1), 0);
+  for (size_t l = 0; l < L; l++) {
+    for (int g = 0; g < ngroups; g++) {
+      group_begin[l * size_t(ngroups + 1) + size_t(g)] = int32_t(grouped.size());
+      for (int32_t a = lig_off[l]; a < lig_off[l + 1]; a++)
+        if (group_of[l * size_t(nslots) + size_t(slot_bits(ligands[size_t(a)].w))] == g)
+          grouped.push_back(ligands[size_t(a)]);
+    }
+    group_begin[l * size_t(ngroups + 1) + size_t(ngroups)] = int32_t(grouped.size());
+  }
 }
 
 }  // namespace field
