@@ -492,7 +492,7 @@ extern "C" int bude_mi355x_mlp_forward_backward(int device, size_t batch, const 
   if (int st = tgt.upload(target, batch * 4)) return st;
   if (int st = bude::hip_check(bude::mlp::to_bf16(static_cast<const float *>(f32.ptr), int(batch), bude::mlp::kIn,
                                                   static_cast<bf16 *>(A.feat.ptr), static_cast<bf16 *>(A.feat_t.ptr),
-                                                  s),
+                                                  s, A.ld_t, true),
                                "to_bf16"))
     return st;
   if (int st = bude::hip_check(hipMemsetAsync(A.loss.ptr, 0, 8, s), "hipMemsetAsync")) return st;

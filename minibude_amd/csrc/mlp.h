@@ -79,9 +79,26 @@ __device__ inline void schedule_hyper(const Schedule &c, uint32_t s, float *hype
 // (device memory) when table is set, else norm; dz3 = bf16(2 w (pred - y) /
 // n), w = tail_weight when E < tail, else 1.  Outputs: pred (optional), the
 // loss sum (kLossScale fixed point),
-// and the transposed BF16 rows A_l^T, dZ_l^T [kHidden][n] and dz3^T (row 0
-// of [kOut][n]) for the weight-gradient products.
+// and the transposed BF16 rows A_l^T, dZ_l^T [kHidden][ld_t] and dz3^T (row
+// 0 of [kOut][ld_t]), fragment-tiled (tiled_index), for the weight-gradient
+// products.
 constexpr int kRowBlock = 32;
+
+// The transposed rows of a batch (X^T, A_l^T, dZ_l^T, dz3^T: M x K, K the
+// batch) are stored as MFMA fragments: the 16 x 32 block of rows 16 mb ..
+// 16 mb + 15 and columns 32 kb .. 32 kb + 31 is 1 KB contiguous at block
+// mb * (K / 32) + kb, in the order a 16x16x32 operand takes it (lane
+// r + 16 g: row r, columns 8 g .. 8 g + 7).  A fragment is then one
+// contiguous 1 KB load (8 whole lines) instead of 64 bytes of 16 rows (16
+// lines, half used each), which the weight-gradient product's loads wait
+// on far less.
+__host__ __device__ inline size_t tiled_index(size_t m, size_t k, size_t K) {
+  return ((m >> 4) * (K >> 5) + (k >> 5)) * 512 + ((m & 15) + 16 * ((k & 31) >> 3)) * 8 + (k & 7);
+}
+
+// Columns of the tiled transposed rows of a batch of n: n rounded up to the
+// 32 of a block.
+inline size_t t_ld(size_t n) { return (n + 31) / 32 * 32; }
 struct RowsArgs {
   const float *poses;
   size_t stride;
@@ -94,7 +111,8 @@ struct RowsArgs {
   StepState *state;
   Schedule sched;
   const uint32_t *idx;
-  size_t n;  // rows, a multiple of kRowBlock
+  size_t n;     // rows, a multiple of kRowBlock
+  size_t ld_t;  // columns of the tiled transposed rows (t_ld(n): a multiple of 32, >= n)
   const float *energy;
   TargetNorm norm;
   const TargetNorm *table;
@@ -104,23 +122,24 @@ struct RowsArgs {
   const float *bias[4];
   float *pred;
   unsigned long long *loss;
-  bf16 *feat_t;        // [kIn][n]
+  bf16 *feat_t;        // [kIn][ld_t], tiled
   bf16 *a_t[3], *dz_t[3];
   bf16 *dz3_t;
 };
 hipError_t rows(const RowsArgs &a, hipStream_t s);
 
 // The weight gradients of a training step in one launch: for each product
-// d, D[M][N] = P[M][ld] Q[N][ld]^T (P = A_{l-1}^T or X^T, Q = dZ_l^T, the
-// transposed rows of the batch) as `splits` K slices in work [splits][M][N],
-// and the row sums of Q per slice in bias_part [splits][N] (the bias
-// gradient).  M, N and ld multiples of 32; `splits` = dw_splits(ld).
+// d, D[M][N] = P[M][0:k] Q[N][0:k]^T (P = A_{l-1}^T or X^T, Q = dZ_l^T, the
+// transposed rows of the batch, tiled with ld columns: tiled_index) as
+// `splits` K slices in work [splits][M][N], and the row sums of Q per slice
+// in bias_part [splits][N] (the bias gradient).  M, N and k multiples of 32,
+// ld a multiple of 32 and >= k; `splits` = dw_splits(k).
 constexpr int kDwSplits = 16;
-// The K slices of a batch of ld rows: the largest power of two up to
-// kDwSplits that divides ld into multiples of 32 rows.
-inline int dw_splits(size_t ld) {
+// The K slices of a batch of k rows: the largest power of two up to
+// kDwSplits that divides k into multiples of 32 rows.
+inline int dw_splits(size_t k) {
   int s = kDwSplits;
-  while (s > 1 && ld % (32 * size_t(s))) s /= 2;
+  while (s > 1 && k % (32 * size_t(s))) s /= 2;
   return s;
 }
 struct DwProduct {
@@ -131,7 +150,8 @@ struct DwProduct {
 struct DwArgs {
   DwProduct d[4];
   int count;
-  size_t ld;
+  size_t k;   // batch rows summed over
+  size_t ld;  // columns of the tiled P and Q
   int splits;
 };
 hipError_t weight_grads(const DwArgs &a, hipStream_t s);
@@ -157,8 +177,11 @@ struct AdamBatch {
 hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, float b2, float eps, hipStream_t s,
                           StepState *advance = nullptr);
 
-// FP32 -> BF16 of a rows x cols matrix and its transpose.
-hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s);
+// FP32 -> BF16 of a rows x cols matrix and its transpose (cols x ldt, ldt 0:
+// rows; tiled: fragment-tiled, tiled_index, cols a multiple of 16 and ldt of
+// 32).
+hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s, size_t ldt = 0,
+                   bool tiled = false);
 
 // Fused inference: predictions (standardised units) of n poses through the
 // whole network, activations kept in LDS; wt[l] = W_l^T [N_l][K_l] (BF16),

@@ -147,8 +147,8 @@ __device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v 
 __device__ __forceinline__ float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
 
 // v[jn][i][c] (neuron 16 kNB w + 16 jn + 4 grp + c, row 2 r + i) as BF16: to the
-// LDS rows (if set) and to the transposed rows out_t [neuron][ld], rows
-// row0 + 2 r, + 1 in one 4-byte store per neuron.
+// LDS rows (if set) and to the tiled transposed rows out_t [neuron][ld], rows
+// row0 + 2 r, + 1 (adjacent in a tile) in one 4-byte store per neuron.
 __device__ __forceinline__ void rows_store(const uint32_t (&v)[kNB][2][2], bf16 *lds, bf16 *__restrict__ out_t,
                                            size_t ld, size_t row0, int wave, int r, int grp) {
 #pragma unroll
@@ -163,7 +163,7 @@ __device__ __forceinline__ void rows_store(const uint32_t (&v)[kNB][2][2], bf16 
     for (int c = 0; c < 4; c++) {
       const uint32_t e0 = c & 1 ? v[jn][0][c >> 1] >> 16 : v[jn][0][c >> 1] & 0xffffu;
       const uint32_t e1 = c & 1 ? v[jn][1][c >> 1] & 0xffff0000u : v[jn][1][c >> 1] << 16;
-      *reinterpret_cast<uint32_t *>(out_t + size_t(n0 + c) * ld + row0 + 2 * r) = e0 | e1;
+      *reinterpret_cast<uint32_t *>(out_t + tiled_index(size_t(n0 + c), row0 + 2 * r, ld)) = e0 | e1;
     }
   }
 }
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
                           pack_bf16(f[8 * v + 4], f[8 * v + 5]), pack_bf16(f[8 * v + 6], f[8 * v + 7]));
       const uint32_t *hw = reinterpret_cast<const uint32_t *>(h);
 #pragma unroll
-      for (int c = 0; c < kIn; c++) a.feat_t[size_t(c) * a.n + row] = bf16(hw[c >> 1] >> (16 * (c & 1)));
+      for (int c = 0; c < kIn; c++) a.feat_t[tiled_index(c, row, a.ld_t)] = bf16(hw[c >> 1] >> (16 * (c & 1)));
     }
 #pragma unroll
     for (int v = 0; v < kIn / 8; v++) reinterpret_cast<uint4 *>(&xs[(p >> 1) + 16 * (p & 1)][0])[v] = h[v];
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
           }
       }
     }
-    rows_store(v, &hs[l & 1][0][0], a.a_t[l], a.n, row0, wave, r, grp);
+    rows_store(v, &hs[l & 1][0][0], a.a_t[l], a.ld_t, row0, wave, r, grp);
     __syncthreads();
   }
   // ---- output neuron, loss, dz3 (the four lane groups, then the four waves, in order)
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
     const float wgt = e < t.tail ? a.tail_weight : 1.0f;
     const uint32_t gq = pack_bf16(2.0f * wgt * d / float(a.n), 0.f) & 0xffffu;
     dz3s[tid] = bf16_lo(gq);
-    a.dz3_t[row0 + tid] = bf16(gq);
+    a.dz3_t[tiled_index(0, row0 + tid, a.ld_t)] = bf16(gq);
     if (a.pred) a.pred[row0 + tid] = y;
     // the squared errors of the 32 rows: one fixed-point atomic per work-group
     float sq = d * d;
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
         v[jn][i][0] = pack_bf16(dv[0], dv[1]);
         v[jn][i][1] = pack_bf16(dv[2], dv[3]);
       }
-    rows_store(v, &hs[1][0][0], a.dz_t[2], a.n, row0, wave, r, grp);
+    rows_store(v, &hs[1][0][0], a.dz_t[2], a.ld_t, row0, wave, r, grp);
   }
   __syncthreads();
 #pragma unroll
@@ -338,18 +338,24 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
         v[jn][i][0] = pack_bf16(dv[0], dv[1]);
         v[jn][i][1] = pack_bf16(dv[2], dv[3]);
       }
-    rows_store(v, l == 2 ? &hs[0][0][0] : nullptr, a.dz_t[l - 1], a.n, row0, wave, r, grp);
+    rows_store(v, l == 2 ? &hs[0][0][0] : nullptr, a.dz_t[l - 1], a.ld_t, row0, wave, r, grp);
     if (l == 2) __syncthreads();
   }
 }
 
 // ---- the weight gradients of a step, one launch ------------------------------
-// One wave per (product, 64 x 64 tile, K slice): 4 x 4 MFMA blocks (fewer at
-// the edge of a 32-row or 32-column product) over a slice of ld / splits
-// batch rows, two k steps of fragments in flight while the two before them
-// multiply (the rows were just written by the rows kernel: their latency is
-// long).  The waves of the first tile row also sum Q's fragments: the bias
-// gradient.
+// One wave per (product, 64 x 64 tile, K slice): BM x BN 16 x 16 MFMA blocks
+// (2 at the edge of a 32-row or 32-column product) over a slice of k / splits
+// batch rows.  The launch has fewer waves than the device has SIMDs, so a
+// wave's own instruction stream is the kernel's time: the tile shape is a
+// template parameter (no branch around an MFMA, no accumulator copies), the
+// fragments come by buffer loads at per-lane row offsets computed once plus
+// a scalar step offset (no 64-bit address arithmetic per load), four k steps
+// of fragments load while the four before them multiply (the rows were just
+// written by the rows kernel: their latency is long; the re-reads of a slice
+// by the tiles beside it hit L2, since a slice's waves all run on one XCD:
+// item mod 8 = split mod 8), and the waves of the first tile row also sum
+// Q's rows, the bias gradient, as an MFMA against rows of ones.
 struct DwItem {
   int d, m0, n0, split;
 };
@@ -372,83 +378,123 @@ __device__ __forceinline__ DwItem dw_item(const DwArgs &a, int item) {
   return it;
 }
 
+typedef int intx4 __attribute__((ext_vector_type(4)));
+constexpr int kRawBufferWord3 = 0x00020000;  // raw (unformatted) buffer access, 32-bit elements
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_buffer(const void *base, size_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, int(bytes), kRawBufferWord3);
+}
+
+template <int BM, int BN, bool BSUM>
+__device__ __forceinline__ void dw_tile(const DwArgs &a, const DwProduct &d, const DwItem &it) {
+  constexpr int kG = 4;  // k steps per group of fragments
+  const int lane = threadIdx.x, r = lane & 15, grp = lane >> 4;
+  const int kc = int(a.k / size_t(a.splits)), nsteps = kc / 32;
+  const uint32_t k0 = uint32_t(it.split) * uint32_t(kc);
+  const __amdgpu_buffer_rsrc_t rp = raw_buffer(d.p, size_t(d.M) * a.ld * 2);
+  const __amdgpu_buffer_rsrc_t rq = raw_buffer(d.q, size_t(d.N) * a.ld * 2);
+  // byte offsets of the lane's 16 bytes of the fragments at the slice's first
+  // k step (tiled operands: a fragment is 1 KB, the next k step the next KB)
+  const uint32_t kb = uint32_t(a.ld / 32), kb0 = k0 / 32u;
+  uint32_t vp[BM], vq[BN];
+#pragma unroll
+  for (int i = 0; i < BM; i++) vp[i] = 1024u * (uint32_t(it.m0 / 16 + i) * kb + kb0) + 16u * uint32_t(lane);
+#pragma unroll
+  for (int j = 0; j < BN; j++) vq[j] = 1024u * (uint32_t(it.n0 / 16 + j) * kb + kb0) + 16u * uint32_t(lane);
+  floatx4 acc[BM][BN], accb[BN];
+#pragma unroll
+  for (int i = 0; i < BM; i++)
+#pragma unroll
+    for (int j = 0; j < BN; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < BN; j++) accb[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  bf16x8 ones;
+#pragma unroll
+  for (int e = 0; e < 8; e++) ones[e] = __builtin_bit_cast(__bf16, uint16_t(0x3f80));
+  // step s of the slice: byte offset 1024 s (uniform)
+  auto load_step = [&](int s, bf16x8 (&ap)[BM], bf16x8 (&aq)[BN]) {
+    const int so = 1024 * s;
+#pragma unroll
+    for (int i = 0; i < BM; i++) ap[i] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rp, vp[i], so, 0));
+#pragma unroll
+    for (int j = 0; j < BN; j++) aq[j] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rq, vq[j], so, 0));
+  };
+  auto mul_step = [&](const bf16x8 (&ap)[BM], const bf16x8 (&aq)[BN]) {
+#pragma unroll
+    for (int i = 0; i < BM; i++)
+#pragma unroll
+      for (int j = 0; j < BN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[i], aq[j], acc[i][j], 0, 0, 0);
+    if constexpr (BSUM) {
+#pragma unroll
+      for (int j = 0; j < BN; j++) accb[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, aq[j], accb[j], 0, 0, 0);
+    }
+  };
+  bf16x8 fp[2][kG][BM], fq[2][kG][BN];
+  const int groups = nsteps / kG;  // whole groups; the remaining steps one at a time after them
+  if (groups > 0) {
+#pragma unroll
+    for (int t = 0; t < kG; t++) load_step(t, fp[0][t], fq[0][t]);
+    // (the scheduling barriers keep each group's loads ahead of the
+    // multiplies before them: left alone, the compiler moves every load next
+    // to its first use)
+    for (int g = 0; g < groups; g += 2) {
+      const int g1 = min(g + 1, groups - 1), g2 = min(g + 2, groups - 1);  // (reloads of the last group: unused)
+#pragma unroll
+      for (int t = 0; t < kG; t++) load_step(kG * g1 + t, fp[1][t], fq[1][t]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < kG; t++) mul_step(fp[0][t], fq[0][t]);
+      __builtin_amdgcn_sched_barrier(0);
+      if (g + 1 < groups) {
+#pragma unroll
+        for (int t = 0; t < kG; t++) load_step(kG * g2 + t, fp[0][t], fq[0][t]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < kG; t++) mul_step(fp[1][t], fq[1][t]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  for (int s = kG * groups; s < nsteps; s++) {
+    load_step(s, fp[0][0], fq[0][0]);
+    mul_step(fp[0][0], fq[0][0]);
+  }
+  // D to the slice: element (row m0 + 16 i + 4 grp + c, column n0 + 16 j + r)
+  float *out = d.work + size_t(it.split) * size_t(d.M) * size_t(d.N) + size_t(it.m0 + 4 * grp) * size_t(d.N) +
+               size_t(it.n0 + r);
+#pragma unroll
+  for (int i = 0; i < BM; i++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      float *o = out + size_t(16 * i + c) * size_t(d.N);
+#pragma unroll
+      for (int j = 0; j < BN; j++) o[16 * j] = acc[i][j][c];
+    }
+  if constexpr (BSUM) {
+    if (grp == 0)
+#pragma unroll
+      for (int j = 0; j < BN; j++) d.bias_part[size_t(it.split) * d.N + it.n0 + 16 * j + r] = accb[j][0];
+  }
+}
+
 __global__ __launch_bounds__(64) void dw_kernel(const DwArgs a) {
   const DwItem it = dw_item(a, blockIdx.x);
   if (it.d < 0) return;
   const DwProduct &d = a.d[it.d];
-  const int lane = threadIdx.x, r = lane & 15, grp = lane >> 4;
-  const int bm = min(4, (d.M - it.m0) / 16), bn = min(4, (d.N - it.n0) / 16);
-  const size_t kc = a.ld / a.splits, k0 = size_t(it.split) * kc;
-  const int nsteps = int(kc / 32);
-  const bf16 *pp = d.p + size_t(it.m0 + r) * a.ld + k0 + 8 * grp;
-  const bf16 *qp = d.q + size_t(it.n0 + r) * a.ld + k0 + 8 * grp;
-  floatx4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  float bs[4] = {0.f, 0.f, 0.f, 0.f};
+  const bool wide_m = d.M - it.m0 >= 64, wide_n = d.N - it.n0 >= 64;
   const bool bsum = d.bias_part && it.m0 == 0;
-  // group g = k steps 2 g, 2 g + 1: fragments [step][i or j].  Every load
-  // is unconditional (blocks past the tile's edge and steps past the slice
-  // reload valid ones, unused), so the loads' waits count the fragments in
-  // flight instead of draining them.
-  bf16x8 fp[2][2][4], fq[2][2][4];
-  auto load = [&](int g, bf16x8 (&ap)[2][4], bf16x8 (&aq)[2][4]) {
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const size_t k = size_t(32 * min(2 * g + t, nsteps - 1));
-#pragma unroll
-      for (int i = 0; i < 4; i++) ap[t][i] = load8(pp + size_t(16 * min(i, bm - 1)) * a.ld + k);
-#pragma unroll
-      for (int j = 0; j < 4; j++) aq[t][j] = load8(qp + size_t(16 * min(j, bn - 1)) * a.ld + k);
-    }
-  };
-  auto mul = [&](int g, const bf16x8 (&ap)[2][4], const bf16x8 (&aq)[2][4]) {
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      if (2 * g + t >= nsteps) break;
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-          if (i < bm && j < bn) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[t][i], aq[t][j], acc[i][j], 0, 0, 0);
-      if (bsum) {
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-          if (j < bn) {
-            const uint4 u = __builtin_bit_cast(uint4, aq[t][j]);
-            const uint32_t w[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-            for (int e = 0; e < 4; e++) bs[j] += __uint_as_float(w[e] << 16) + __uint_as_float(w[e] & 0xffff0000u);
-          }
-      }
-    }
-  };
-  const int groups = (nsteps + 1) / 2;
-  load(0, fp[0], fq[0]);
-  for (int g = 0; g < groups; g += 2) {
-    load(min(g + 1, groups - 1), fp[1], fq[1]);
-    mul(g, fp[0], fq[0]);
-    load(min(g + 2, groups - 1), fp[0], fq[0]);
-    if (g + 1 < groups) mul(g + 1, fp[1], fq[1]);
-  }
-  float *out = d.work + size_t(it.split) * size_t(d.M) * d.N;
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-      if (i < bm && j < bn)
-#pragma unroll
-        for (int c = 0; c < 4; c++) out[size_t(it.m0 + 16 * i + 4 * grp + c) * d.N + it.n0 + 16 * j + r] = acc[i][j][c];
-  if (bsum) {
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-      if (j < bn) {
-        float v = bs[j] + __shfl_xor(bs[j], 16);
-        v += __shfl_xor(v, 32);
-        if (grp == 0) d.bias_part[size_t(it.split) * d.N + it.n0 + 16 * j + r] = v;
-      }
+  if (wide_m && wide_n) {
+    if (bsum) dw_tile<4, 4, true>(a, d, it);
+    else dw_tile<4, 4, false>(a, d, it);
+  } else if (wide_m) {
+    if (bsum) dw_tile<4, 2, true>(a, d, it);
+    else dw_tile<4, 2, false>(a, d, it);
+  } else if (wide_n) {
+    if (bsum) dw_tile<2, 4, true>(a, d, it);
+    else dw_tile<2, 4, false>(a, d, it);
+  } else {
+    if (bsum) dw_tile<2, 2, true>(a, d, it);
+    else dw_tile<2, 2, false>(a, d, it);
   }
 }
 
@@ -521,12 +567,12 @@ __global__ __launch_bounds__(64 * W) void adam_multi_kernel(const AdamBatch b, f
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, bf16 *__restrict__ w16,
-                               bf16 *__restrict__ w16t) {
+                               bf16 *__restrict__ w16t, size_t ldt, int tiled) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= size_t(rows) * cols) return;
   const bf16 h = to_bf16(w[i]);
   if (w16) w16[i] = h;
-  if (w16t) w16t[size_t(i % cols) * rows + i / cols] = h;
+  if (w16t) w16t[tiled ? tiled_index(i % cols, i / cols, ldt) : size_t(i % cols) * ldt + i / cols] = h;
 }
 
 // ---- fused inference ---------------------------------------------------------
@@ -672,19 +718,24 @@ unsigned blocks_for(size_t n, unsigned t) { return unsigned((n + t - 1) / t); }
 }  // namespace
 
 hipError_t rows(const RowsArgs &a, hipStream_t s) {
-  if (a.n == 0 || a.n % kRB || (!a.feat && (!a.poses || !a.feat_t)) || (a.perm && (!a.state || a.per_epoch == 0)) ||
-      !a.energy || !a.dz3_t)
+  if (a.n == 0 || a.n % kRB || a.ld_t < a.n || a.ld_t % 32 || (!a.feat && (!a.poses || !a.feat_t)) ||
+      (a.perm && (!a.state || a.per_epoch == 0)) || !a.energy || !a.dz3_t)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(rows_kernel, dim3(unsigned(a.n / kRB)), dim3(64 * kRBW), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t weight_grads(const DwArgs &a, hipStream_t s) {
-  if (a.count < 1 || a.count > 4 || a.splits < 1 || a.ld == 0 || a.ld % (32 * size_t(a.splits)))
+  if (a.count < 1 || a.count > 4 || a.splits < 1 || a.k == 0 || a.k % (32 * size_t(a.splits)) || a.ld < a.k ||
+      a.ld % 32)
     return hipErrorInvalidValue;
   int items = 0;
   for (int d = 0; d < a.count; d++) {
-    if (a.d[d].M % 32 || a.d[d].N % 32 || !a.d[d].p || !a.d[d].q || !a.d[d].work) return hipErrorInvalidValue;
+    // (buffer offsets and sizes are 32-bit: operands and slices under 2 GiB)
+    if (a.d[d].M % 32 || a.d[d].N % 32 || !a.d[d].p || !a.d[d].q || !a.d[d].work ||
+        size_t(std::max(a.d[d].M, a.d[d].N)) * a.ld * 2 >= (size_t(1) << 31) ||
+        size_t(a.d[d].M) * size_t(a.d[d].N) * 4 >= (size_t(1) << 31))
+      return hipErrorInvalidValue;
     items += ((a.d[d].M + 63) / 64) * ((a.d[d].N + 63) / 64) * a.splits;
   }
   hipLaunchKernelGGL(dw_kernel, dim3(unsigned(items)), dim3(64), 0, s, a);
@@ -721,9 +772,12 @@ hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, floa
   return hipGetLastError();
 }
 
-hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s) {
+hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hipStream_t s, size_t ldt,
+                   bool tiled) {
+  if (ldt == 0) ldt = size_t(rows);
+  if (ldt < size_t(rows) || (tiled && (cols % 16 || ldt % 32))) return hipErrorInvalidValue;
   hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(size_t(rows) * cols, 256)), dim3(256), 0, s, w, rows, cols, w16,
-                     w16t);
+                     w16t, ldt, int(tiled));
   return hipGetLastError();
 }
 

@@ -25,9 +25,10 @@ struct Net {
 
 // The rows of one batch of `rows` poses: BF16 features, and in training the
 // transposed rows the weight-gradient products read (X^T, A_l^T, dZ_l^T,
-// dz3^T), predictions and the loss sum.
+// dz3^T: fragment-tiled with ld_t = mlp::t_ld(rows) columns), predictions and
+// the loss sum.
 struct Acts {
-  size_t rows = 0;
+  size_t rows = 0, ld_t = 0;
   DeviceBuffer feat, feat_t, a_t[3], pred, dz3_t, dz_t[3], loss;
   // weight and bias gradients of layer l as splits[l] K slices (summed in a
   // fixed order by the Adam step, in slice order by sum_grads)

@@ -14,19 +14,20 @@ namespace surrogate {
 
 int alloc_acts(Acts &A, size_t rows, bool training) {
   A.rows = rows;
-  const size_t H = mlp::kHidden, O = mlp::kOut, I = mlp::kIn;
+  A.ld_t = mlp::t_ld(rows);
+  const size_t H = mlp::kHidden, O = mlp::kOut, I = mlp::kIn, ld = A.ld_t;
   int st = A.feat.alloc(rows * I * 2);
   if (!st) st = A.pred.alloc(rows * 4);
   if (training) {
-    if (!st) st = A.feat_t.alloc(rows * I * 2);
+    if (!st) st = A.feat_t.alloc(ld * I * 2);
     for (int l = 0; l < 3 && !st; l++) {
-      st = A.a_t[l].alloc(rows * H * 2);
-      if (!st) st = A.dz_t[l].alloc(rows * H * 2);
+      st = A.a_t[l].alloc(ld * H * 2);
+      if (!st) st = A.dz_t[l].alloc(ld * H * 2);
     }
     // dz3^T: row 0 is written every step, rows 1.. stay zero (the padded
     // output columns of the last weight gradient)
-    if (!st) st = A.dz3_t.alloc(rows * O * 2);
-    if (!st) st = hip_check(hipMemset(A.dz3_t.ptr, 0, rows * O * 2), "hipMemset");
+    if (!st) st = A.dz3_t.alloc(ld * O * 2);
+    if (!st) st = hip_check(hipMemset(A.dz3_t.ptr, 0, ld * O * 2), "hipMemset");
     if (!st) st = A.loss.alloc(256);
     for (int l = 0; l < 4 && !st; l++) {
       const size_t cap = size_t(mlp::kDwSplits);
@@ -40,6 +41,7 @@ int alloc_acts(Acts &A, size_t rows, bool training) {
 int rows_step(const Net &net, Acts &A, size_t rows, const mlp::RowsArgs &in, hipStream_t s) {
   mlp::RowsArgs a = in;
   a.n = rows;
+  a.ld_t = A.ld_t;
   for (int l = 0; l < 3; l++) {
     a.wt[l] = static_cast<const bf16 *>(net.w16t[l].ptr);
     a.a_t[l] = static_cast<bf16 *>(A.a_t[l].ptr);
@@ -62,7 +64,8 @@ int weight_grads(Net &net, Acts &A, size_t rows, hipStream_t s) {
   (void)net;
   mlp::DwArgs a{};
   a.count = 4;
-  a.ld = rows;
+  a.k = rows;
+  a.ld = A.ld_t;
   a.splits = mlp::dw_splits(rows);
   for (int l = 0; l < 4; l++) {
     a.d[l].p = static_cast<const bf16 *>(l == 0 ? A.feat_t.ptr : A.a_t[l - 1].ptr);
