@@ -117,8 +117,9 @@ struct RowsArgs {
   TargetNorm norm;
   const TargetNorm *table;
   float tail_weight;
-  const bf16 *wt[3];   // W_l^T [N_l][K_l], l < 3
-  const bf16 *w[4];    // W_l [K_l][N_l] (w[0] unused; w[3]: column 0 read)
+  const bf16 *wt[3];   // W_l^T [N_l][K_l], l < 3, fragment-tiled (tiled_index(n, k, K_l))
+  const bf16 *w[4];    // W_l [K_l][N_l]: w[1], w[2] fragment-tiled (tiled_index(k, n, N_l)); w[3] row-major,
+                       // column 0 read; w[0] unused
   const float *bias[4];
   float *pred;
   unsigned long long *loss;
@@ -165,6 +166,7 @@ struct AdamTensor {
   int rows, cols;    // w is rows x cols row-major
   int splits;        // 0 or 1: g is the gradient itself
   size_t stride;
+  bf16 *w16x = nullptr, *w16tx = nullptr;  // optional fragment-tiled BF16 copies of w and w^T (tiled_index)
 };
 struct AdamBatch {
   AdamTensor t[8];
@@ -176,6 +178,11 @@ struct AdamBatch {
 // `advance` set the launch also ends the step: advance->step += 1.
 hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, float b2, float eps, hipStream_t s,
                           StepState *advance = nullptr);
+
+// FP32 -> BF16 fragment-tiled copies (tiled_index) of a rows x cols matrix
+// (wx, rows a multiple of 16, cols of 32) and of its transpose (wtx, cols a
+// multiple of 16, rows of 32); either may be null.
+hipError_t tile_bf16(const float *w, int rows, int cols, bf16 *wx, bf16 *wtx, hipStream_t s);
 
 // FP32 -> BF16 of a rows x cols matrix and its transpose (cols x ldt, ldt 0:
 // rows; tiled: fragment-tiled, tiled_index, cols a multiple of 16 and ldt of

@@ -111,17 +111,18 @@ constexpr int kNB = kHidden / (16 * kRBW);  // 16-neuron MFMA blocks per wave
 constexpr int kRBPitch = kHidden + 8;  // bf16 per LDS row (528 bytes: 4-bank stagger between rows)
 constexpr int kRBIn = kIn + 8;
 
-// A fragments of the wave's 16 kNB rows of A (lda bf16 apart in global memory)
-// for all K / 32 k steps at once: issued a layer ahead, so their latency
-// passes behind the current layer's epilogue and barrier.
+// A fragments of the wave's 16 kNB rows of the fragment-tiled [256][K] A
+// for all K / 32 k steps at once (each one contiguous 1 KB load): issued a
+// layer ahead, so their latency passes behind the current layer's epilogue
+// and barrier.
 template <int K>
-__device__ __forceinline__ void rows_load(const bf16 *__restrict__ A, int lda, int wave, int r, int grp,
+__device__ __forceinline__ void rows_load(const bf16 *__restrict__ A, int wave, int r, int grp,
                                           bf16x8 (&fa)[8][kNB]) {
-  const bf16 *ap = A + size_t(16 * kNB * wave + r) * lda + 8 * grp;
+  const bf16 *ap = A + 8 * (r + 16 * grp);
 #pragma unroll
   for (int k = 0; k < K / 32; k++)
 #pragma unroll
-    for (int jn = 0; jn < kNB; jn++) fa[k][jn] = load8(ap + size_t(16 * jn) * lda + 32 * k);
+    for (int jn = 0; jn < kNB; jn++) fa[k][jn] = load8(ap + size_t((kNB * wave + jn) * (K / 32) + k) * 512);
 }
 
 // acc[jn][i] = sum_k A[16 kNB w + 16 jn + r][k] B[k][row 2 r + i] from the
@@ -146,9 +147,44 @@ __device__ __forceinline__ void rows_mma(const bf16x8 (&fa)[8][kNB], const bf16 
 __device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
 
+// The 4 x 4 transpose of w[c] across the lanes q = lane mod 4 of each quad:
+// afterwards lane q holds w[c] of lane c, word q (two exchanges of two words
+// each, by DPP quad permutations).
+__device__ __forceinline__ void quad_transpose(uint32_t (&w)[4], int q) {
+  {  // lanes q, q ^ 1 swap the words whose bit 0 differs from q's
+    const bool odd = q & 1;
+    const uint32_t s0 = odd ? w[0] : w[1], s1 = odd ? w[2] : w[3];
+    const uint32_t r0 = uint32_t(__builtin_amdgcn_mov_dpp(int(s0), 0xb1, 0xf, 0xf, false));  // quad_perm 1 0 3 2
+    const uint32_t r1 = uint32_t(__builtin_amdgcn_mov_dpp(int(s1), 0xb1, 0xf, 0xf, false));
+    if (odd) {
+      w[0] = r0;
+      w[2] = r1;
+    } else {
+      w[1] = r0;
+      w[3] = r1;
+    }
+  }
+  {  // lanes q, q ^ 2 swap the words whose bit 1 differs from q's
+    const bool hi = q & 2;
+    const uint32_t s0 = hi ? w[0] : w[2], s1 = hi ? w[1] : w[3];
+    const uint32_t r0 = uint32_t(__builtin_amdgcn_mov_dpp(int(s0), 0x4e, 0xf, 0xf, false));  // quad_perm 2 3 0 1
+    const uint32_t r1 = uint32_t(__builtin_amdgcn_mov_dpp(int(s1), 0x4e, 0xf, 0xf, false));
+    if (hi) {
+      w[0] = r0;
+      w[1] = r1;
+    } else {
+      w[2] = r0;
+      w[3] = r1;
+    }
+  }
+}
+
 // v[jn][i][c] (neuron 16 kNB w + 16 jn + 4 grp + c, row 2 r + i) as BF16: to the
-// LDS rows (if set) and to the tiled transposed rows out_t [neuron][ld], rows
-// row0 + 2 r, + 1 (adjacent in a tile) in one 4-byte store per neuron.
+// LDS rows (if set) and to the tiled transposed rows out_t [neuron][ld].  A
+// lane's word c holds rows 2 r, 2 r + 1 of neuron 4 grp + c; after the quad
+// transpose lane (r, grp) holds rows 8 (r / 4) .. + 7 of neuron 4 grp + r % 4,
+// 16 contiguous bytes of the tile, so one store per lane writes the whole
+// 16-neuron x 32-row tile (1 KB).
 __device__ __forceinline__ void rows_store(const uint32_t (&v)[kNB][2][2], bf16 *lds, bf16 *__restrict__ out_t,
                                            size_t ld, size_t row0, int wave, int r, int grp) {
 #pragma unroll
@@ -159,12 +195,17 @@ __device__ __forceinline__ void rows_store(const uint32_t (&v)[kNB][2][2], bf16 
       for (int i = 0; i < 2; i++)
         *reinterpret_cast<uint2 *>(lds + (r + 16 * i) * kRBPitch + n0) = make_uint2(v[jn][i][0], v[jn][i][1]);
     }
+    uint32_t w[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       const uint32_t e0 = c & 1 ? v[jn][0][c >> 1] >> 16 : v[jn][0][c >> 1] & 0xffffu;
       const uint32_t e1 = c & 1 ? v[jn][1][c >> 1] & 0xffff0000u : v[jn][1][c >> 1] << 16;
-      *reinterpret_cast<uint32_t *>(out_t + tiled_index(size_t(n0 + c), row0 + 2 * r, ld)) = e0 | e1;
+      w[c] = e0 | e1;
     }
+    quad_transpose(w, r & 3);
+    const size_t tile = tiled_index(size_t(16 * kNB * wave + 16 * jn), row0, ld);
+    const int slot = 4 * grp + (r & 3) + 16 * (r >> 2);
+    *reinterpret_cast<uint4 *>(out_t + tile + size_t(8 * slot)) = make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
@@ -181,8 +222,8 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
   // the weight fragments of the products, two products ahead in two buffers:
   // layers 0 and 1 in flight while the input rows are formed
   bf16x8 fx[8][kNB], fy[8][kNB];
-  rows_load<kIn>(a.wt[0], kIn, wave, r, grp, fx);
-  rows_load<kHidden>(a.wt[1], kHidden, wave, r, grp, fy);
+  rows_load<kIn>(a.wt[0], wave, r, grp, fx);
+  rows_load<kHidden>(a.wt[1], wave, r, grp, fy);
   // ---- the input row p = tid < kRB (LDS row p / 2 + 16 (p % 2))
   if (tid < kRB) {
     const int p = tid;
@@ -235,9 +276,9 @@ __global__ __launch_bounds__(64 * kRBW) void rows_kernel(const RowsArgs a) {
       rows_mma<kIn>(fa, &xs[0][0], kRBIn, r, grp, acc);
     else
       rows_mma<kHidden>(fa, &hs[(l - 1) & 1][0][0], kRBPitch, r, grp, acc);
-    if (l == 0) rows_load<kHidden>(a.wt[2], kHidden, wave, r, grp, fx);
-    if (l == 1) rows_load<kHidden>(a.w[2], kHidden, wave, r, grp, fy);
-    if (l == 2) rows_load<kHidden>(a.w[1], kHidden, wave, r, grp, fx);
+    if (l == 0) rows_load<kHidden>(a.wt[2], wave, r, grp, fx);
+    if (l == 1) rows_load<kHidden>(a.w[2], wave, r, grp, fy);
+    if (l == 2) rows_load<kHidden>(a.w[1], wave, r, grp, fx);
     uint32_t v[kNB][2][2];
 #pragma unroll
     for (int jn = 0; jn < kNB; jn++) {
@@ -564,6 +605,17 @@ __global__ __launch_bounds__(64 * W) void adam_multi_kernel(const AdamBatch b, f
   const bf16 h = to_bf16(wi);
   if (T.w16) T.w16[i] = h;
   if (T.w16t) T.w16t[size_t(i % T.cols) * T.rows + i / T.cols] = h;
+  if (T.w16x) T.w16x[tiled_index(i / T.cols, i % T.cols, size_t(T.cols))] = h;
+  if (T.w16tx) T.w16tx[tiled_index(i % T.cols, i / T.cols, size_t(T.rows))] = h;
+}
+
+__global__ void tile_bf16_kernel(const float *__restrict__ w, int rows, int cols, bf16 *__restrict__ wx,
+                                 bf16 *__restrict__ wtx) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(rows) * cols) return;
+  const bf16 h = to_bf16(w[i]);
+  if (wx) wx[tiled_index(i / cols, i % cols, size_t(cols))] = h;
+  if (wtx) wtx[tiled_index(i % cols, i / cols, size_t(rows))] = h;
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, bf16 *__restrict__ w16,
@@ -769,6 +821,14 @@ hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, floa
   else
     hipLaunchKernelGGL(adam_multi_kernel<4>, dim3(blocks_for(total, 64)), dim3(64 * 4), 0, s, b, 0.f, b1, b2, eps, 1.f,
                        1.f, hyper, advance);
+  return hipGetLastError();
+}
+
+hipError_t tile_bf16(const float *w, int rows, int cols, bf16 *wx, bf16 *wtx, hipStream_t s) {
+  if (rows <= 0 || cols <= 0 || (wx && (rows % 16 || cols % 32)) || (wtx && (cols % 16 || rows % 32)))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tile_bf16_kernel, dim3(blocks_for(size_t(rows) * cols, 256)), dim3(256), 0, s, w, rows, cols, wx,
+                     wtx);
   return hipGetLastError();
 }
 

@@ -20,6 +20,9 @@ constexpr int kN[4] = {mlp::kHidden, mlp::kHidden, mlp::kHidden, mlp::kOut};
 
 struct Net {
   DeviceBuffer w[4], b[4], mw[4], vw[4], mb[4], vb[4], gw[4], gb[4], w16[4], w16t[4];
+  // fragment-tiled BF16 copies the training rows kernel reads: W_l^T (l < 3)
+  // and W_l (l = 1, 2)
+  DeviceBuffer w16x[4], w16tx[4];
   int step = 0;
 };
 

@@ -43,12 +43,12 @@ int rows_step(const Net &net, Acts &A, size_t rows, const mlp::RowsArgs &in, hip
   a.n = rows;
   a.ld_t = A.ld_t;
   for (int l = 0; l < 3; l++) {
-    a.wt[l] = static_cast<const bf16 *>(net.w16t[l].ptr);
+    a.wt[l] = static_cast<const bf16 *>(net.w16tx[l].ptr);
     a.a_t[l] = static_cast<bf16 *>(A.a_t[l].ptr);
     a.dz_t[l] = static_cast<bf16 *>(A.dz_t[l].ptr);
   }
   for (int l = 0; l < 4; l++) {
-    a.w[l] = static_cast<const bf16 *>(net.w16[l].ptr);
+    a.w[l] = static_cast<const bf16 *>(l == 1 || l == 2 ? net.w16x[l].ptr : net.w16[l].ptr);
     a.bias[l] = static_cast<const float *>(net.b[l].ptr);
   }
   a.loss = static_cast<unsigned long long *>(A.loss.ptr);
@@ -117,6 +117,12 @@ int set_weights(Net &net, const std::vector<std::vector<float>> &w, const std::v
                                               static_cast<bf16 *>(net.w16[l].ptr),
                                               static_cast<bf16 *>(net.w16t[l].ptr), s),
                            "to_bf16");
+    if (!st && l < 3) st = net.w16tx[l].alloc(nw * 2);
+    if (!st && (l == 1 || l == 2)) st = net.w16x[l].alloc(nw * 2);
+    if (!st && l < 3)
+      st = hip_check(mlp::tile_bf16(static_cast<const float *>(net.w[l].ptr), kK[l], kN[l],
+                                    static_cast<bf16 *>(net.w16x[l].ptr), static_cast<bf16 *>(net.w16tx[l].ptr), s),
+                     "tile_bf16");
     if (st) return st;
   }
   net.step = 0;
@@ -183,7 +189,8 @@ int train(Net &net, const TrainData &d, std::vector<uint32_t> perm, const TrainP
       ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(A.gpart[l].ptr),
                      static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
                      static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), kK[l], kN[l],
-                     A.splits[l], size_t(kK[l]) * kN[l]};
+                     A.splits[l], size_t(kK[l]) * kN[l], static_cast<bf16 *>(net.w16x[l].ptr),
+                     static_cast<bf16 *>(net.w16tx[l].ptr)};
       ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(A.bpart[l].ptr),
                          static_cast<float *>(net.mb[l].ptr), static_cast<float *>(net.vb[l].ptr), nullptr, nullptr, 1,
                          kN[l], A.splits[l], size_t(kN[l])};
