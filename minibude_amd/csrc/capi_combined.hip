@@ -1233,7 +1233,7 @@ int screen_begin(bude_combined *c, const BudeAtom *protein, size_t natpro) {
     if (int st = bude::surrogate::train(c->net, td, std::move(train), tp, c->stream, nullptr, 0, nullptr)) return st;
     if (int st = c->bias0.reserve(L * bude::mlp::kHidden * 4)) return st;
     if (int st = bude::hip_check(bude::mlp::fold_bias(static_cast<const float *>(c->net.b[0].ptr),
-                                                      static_cast<const bude::mlp::bf16 *>(c->net.w16[0].ptr),
+                                                      static_cast<const bude::mlp::bf16 *>(c->net.w16tx[0].ptr),
                                                       static_cast<const float *>(c->desc.ptr), int(L),
                                                       static_cast<float *>(c->bias0.ptr), c->stream),
                                  "fold_bias"))

@@ -554,8 +554,8 @@ extern "C" int bude_mi355x_mlp_predict(int device, const float *const poses[6], 
   if (int st = out.alloc(n * 4)) return st;
   hipDeviceProp_t props{};
   if (int st = bude::hip_check(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties")) return st;
-  const bf16 *wt[4] = {static_cast<const bf16 *>(net.w16t[0].ptr), static_cast<const bf16 *>(net.w16t[1].ptr),
-                       static_cast<const bf16 *>(net.w16t[2].ptr), static_cast<const bf16 *>(net.w16t[3].ptr)};
+  const bf16 *wt[4] = {static_cast<const bf16 *>(net.w16tx[0].ptr), static_cast<const bf16 *>(net.w16tx[1].ptr),
+                       static_cast<const bf16 *>(net.w16tx[2].ptr), nullptr};
   const float *bias[4] = {static_cast<const float *>(net.b[0].ptr), static_cast<const float *>(net.b[1].ptr),
                           static_cast<const float *>(net.b[2].ptr), static_cast<const float *>(net.b[3].ptr)};
   if (int st = bude::hip_check(bude::mlp::infer(static_cast<const float *>(dp.ptr), n, n, wt,

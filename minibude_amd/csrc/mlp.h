@@ -35,8 +35,9 @@ constexpr double kLossScale = 16777216.0;
 
 // Per-ligand first-layer bias with the descriptor inputs folded in, for the
 // fused inference kernel (whose descriptor columns are zero):
-// out[l][j] = b[j] + sum_c bf16(desc[l][c]) * w16[kFeat + c][j], w16 = W_0 [kIn][kHidden].
-hipError_t fold_bias(const float *b, const bf16 *w16, const float *desc, int nlig, float *out, hipStream_t s);
+// out[l][j] = b[j] + sum_c bf16(desc[l][c]) * W_0[kFeat + c][j], read from
+// w0tx = W_0^T [kHidden][kIn] fragment-tiled (tiled_index).
+hipError_t fold_bias(const float *b, const bf16 *w0tx, const float *desc, int nlig, float *out, hipStream_t s);
 
 // Device-side state of a training loop, so that one captured hipGraph of a
 // step replays as every step: the step count, and this step's two learning
@@ -191,8 +192,9 @@ hipError_t to_bf16(const float *w, int rows, int cols, bf16 *w16, bf16 *w16t, hi
                    bool tiled = false);
 
 // Fused inference: predictions (standardised units) of n poses through the
-// whole network, activations kept in LDS; wt[l] = W_l^T [N_l][K_l] (BF16),
-// w3 = W_3 [256][kOut] (column 0 used); `blocks` persistent work-groups.
+// whole network, activations kept in LDS; wt[l] = W_l^T [N_l][K_l] (BF16,
+// fragment-tiled: tiled_index(n, k, K_l); l < 3), w3 = W_3 [256][kOut]
+// row-major (column 0 used); `blocks` persistent work-groups.
 hipError_t infer(const float *poses, size_t stride, size_t n, const bf16 *const wt[4], const bf16 *w3,
                  const float *const bias[4], float *out, int blocks, hipStream_t s);
 

@@ -659,16 +659,18 @@ __device__ __forceinline__ void fused_layer(bf16 *__restrict__ buf, const bf16 *
 #pragma unroll
     for (int i = 0; i < kTile / 16; i++) acc[j][i] = floatx4{b4.x, b4.y, b4.z, b4.w};
   }
-  const bf16 *wrow = wt + size_t(32 * wave + r) * K + 8 * grp;
+  // W^T fragment-tiled: neuron block 2 w + j, k step k / 32 is 1 KB at
+  // ((2 w + j) K / 32 + k / 32) KB, this lane's 16 bytes at 16 lane
+  const bf16 *wl = wt + 8 * lane;
   bf16x8 fw[2];
 #pragma unroll
-  for (int j = 0; j < 2; j++) fw[j] = load8(wrow + size_t(16 * j) * K);
+  for (int j = 0; j < 2; j++) fw[j] = load8(wl + size_t((2 * wave + j) * (K / 32)) * 512);
 #pragma unroll 1
   for (int k = 0; k < K; k += 32) {
     bf16x8 fn[2];
     const int kn = k + 32 < K ? k + 32 : k;  // (the last step reloads its own fragments)
 #pragma unroll
-    for (int j = 0; j < 2; j++) fn[j] = load8(wrow + size_t(16 * j) * K + kn);
+    for (int j = 0; j < 2; j++) fn[j] = load8(wl + size_t((2 * wave + j) * (K / 32) + kn / 32) * 512);
 #pragma unroll
     for (int i = 0; i < kTile / 16; i++) {
       // (two halves: the compiler would otherwise hold all eight B fragments)
@@ -754,14 +756,14 @@ __global__ __launch_bounds__(kInferThreads, 4) void mlp_infer_kernel(
   }
 }
 
-__global__ __launch_bounds__(256) void fold_bias_kernel(const float *__restrict__ b, const bf16 *__restrict__ w16,
+__global__ __launch_bounds__(256) void fold_bias_kernel(const float *__restrict__ b, const bf16 *__restrict__ w0tx,
                                                         const float *__restrict__ desc, float *__restrict__ out) {
   const int l = blockIdx.x, j = threadIdx.x;
   float acc = b[j];
 #pragma unroll
   for (int c = 0; c < kDesc; c++)
-    acc = __builtin_fmaf(from_bf16(to_bf16(desc[size_t(l) * kDesc + c])), from_bf16(w16[size_t(kFeat + c) * kHidden + j]),
-                         acc);
+    acc = __builtin_fmaf(from_bf16(to_bf16(desc[size_t(l) * kDesc + c])),
+                         from_bf16(w0tx[tiled_index(size_t(j), size_t(kFeat + c), kIn)]), acc);
   out[size_t(l) * kHidden + j] = acc;
 }
 
@@ -801,9 +803,9 @@ hipError_t sum_slices(const float *work, int splits, size_t n, float *out, hipSt
   return hipGetLastError();
 }
 
-hipError_t fold_bias(const float *b, const bf16 *w16, const float *desc, int nlig, float *out, hipStream_t s) {
+hipError_t fold_bias(const float *b, const bf16 *w0tx, const float *desc, int nlig, float *out, hipStream_t s) {
   if (nlig <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fold_bias_kernel, dim3(unsigned(nlig)), dim3(kHidden), 0, s, b, w16, desc, out);
+  hipLaunchKernelGGL(fold_bias_kernel, dim3(unsigned(nlig)), dim3(kHidden), 0, s, b, w0tx, desc, out);
   return hipGetLastError();
 }
 

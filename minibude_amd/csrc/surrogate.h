@@ -19,10 +19,12 @@ constexpr int kK[4] = {mlp::kIn, mlp::kHidden, mlp::kHidden, mlp::kHidden};
 constexpr int kN[4] = {mlp::kHidden, mlp::kHidden, mlp::kHidden, mlp::kOut};
 
 struct Net {
-  DeviceBuffer w[4], b[4], mw[4], vw[4], mb[4], vb[4], gw[4], gb[4], w16[4], w16t[4];
-  // fragment-tiled BF16 copies the training rows kernel reads: W_l^T (l < 3)
-  // and W_l (l = 1, 2)
-  DeviceBuffer w16x[4], w16tx[4];
+  DeviceBuffer w[4], b[4], mw[4], vw[4], mb[4], vb[4], gw[4], gb[4];
+  // BF16 copies the kernels read, refreshed by every Adam step:
+  // fragment-tiled W_l^T (l < 3: the forward products of training and
+  // inference, the descriptor fold) and W_l (l = 1, 2: the backward
+  // products), and W_3 row-major in w16[3] (the output neuron's column)
+  DeviceBuffer w16[4], w16x[4], w16tx[4];
   int step = 0;
 };
 

@@ -48,7 +48,7 @@ int rows_step(const Net &net, Acts &A, size_t rows, const mlp::RowsArgs &in, hip
     a.dz_t[l] = static_cast<bf16 *>(A.dz_t[l].ptr);
   }
   for (int l = 0; l < 4; l++) {
-    a.w[l] = static_cast<const bf16 *>(l == 1 || l == 2 ? net.w16x[l].ptr : net.w16[l].ptr);
+    a.w[l] = static_cast<const bf16 *>(l == 1 || l == 2 ? net.w16x[l].ptr : net.w16[l].ptr);  // (w[0] unused)
     a.bias[l] = static_cast<const float *>(net.b[l].ptr);
   }
   a.loss = static_cast<unsigned long long *>(A.loss.ptr);
@@ -110,13 +110,11 @@ int set_weights(Net &net, const std::vector<std::vector<float>> &w, const std::v
         st = d->alloc(size_t(kN[l]) * 4);
         if (!st) st = hip_check(hipMemset(d->ptr, 0, size_t(kN[l]) * 4), "hipMemset");
       }
-    if (!st) st = net.w16[l].alloc(nw * 2);
-    if (!st) st = net.w16t[l].alloc(nw * 2);
-    if (!st)
+    if (!st && l == 3) st = net.w16[l].alloc(nw * 2);
+    if (!st && l == 3)
       st = hip_check(mlp::to_bf16(static_cast<const float *>(net.w[l].ptr), kK[l], kN[l],
-                                              static_cast<bf16 *>(net.w16[l].ptr),
-                                              static_cast<bf16 *>(net.w16t[l].ptr), s),
-                           "to_bf16");
+                                  static_cast<bf16 *>(net.w16[l].ptr), nullptr, s),
+                     "to_bf16");
     if (!st && l < 3) st = net.w16tx[l].alloc(nw * 2);
     if (!st && (l == 1 || l == 2)) st = net.w16x[l].alloc(nw * 2);
     if (!st && l < 3)
@@ -188,7 +186,7 @@ int train(Net &net, const TrainData &d, std::vector<uint32_t> perm, const TrainP
     for (int l = 0; l < 4; l++) {
       ab.t[2 * l] = {static_cast<float *>(net.w[l].ptr), static_cast<const float *>(A.gpart[l].ptr),
                      static_cast<float *>(net.mw[l].ptr), static_cast<float *>(net.vw[l].ptr),
-                     static_cast<bf16 *>(net.w16[l].ptr), static_cast<bf16 *>(net.w16t[l].ptr), kK[l], kN[l],
+                     l == 3 ? static_cast<bf16 *>(net.w16[l].ptr) : nullptr, nullptr, kK[l], kN[l],
                      A.splits[l], size_t(kK[l]) * kN[l], static_cast<bf16 *>(net.w16x[l].ptr),
                      static_cast<bf16 *>(net.w16tx[l].ptr)};
       ab.t[2 * l + 1] = {static_cast<float *>(net.b[l].ptr), static_cast<const float *>(A.bpart[l].ptr),
@@ -278,8 +276,8 @@ int train(Net &net, const TrainData &d, std::vector<uint32_t> perm, const TrainP
 
 int predict(const Net &net, const float *poses, size_t stride, size_t n, float *out, int blocks, hipStream_t s,
             const float *bias0) {
-  const bf16 *wt[4] = {static_cast<const bf16 *>(net.w16t[0].ptr), static_cast<const bf16 *>(net.w16t[1].ptr),
-                       static_cast<const bf16 *>(net.w16t[2].ptr), static_cast<const bf16 *>(net.w16t[3].ptr)};
+  const bf16 *wt[4] = {static_cast<const bf16 *>(net.w16tx[0].ptr), static_cast<const bf16 *>(net.w16tx[1].ptr),
+                       static_cast<const bf16 *>(net.w16tx[2].ptr), nullptr};
   const float *bias[4] = {bias0 ? bias0 : static_cast<const float *>(net.b[0].ptr),
                           static_cast<const float *>(net.b[1].ptr), static_cast<const float *>(net.b[2].ptr),
                           static_cast<const float *>(net.b[3].ptr)};
