@@ -1,8 +1,9 @@
 // mlp.hip -- kernels of the pose-energy surrogate (see mlp.h).
 //
 // Products run on gfx950 BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
-// accumulate), operands k-contiguous, so every fragment is one 16-byte load
-// per lane.  MFMA 16x16x32 lane layout (lane l, r = l % 16, g = l / 16):
+// accumulate); operands read from global memory are fragment-tiled
+// (tiled_index), so every fragment is one contiguous 1 KB load (16 bytes per
+// lane).  MFMA 16x16x32 lane layout (lane l, r = l % 16, g = l / 16):
 //   A fragment  A[r][8g .. 8g+7]       B fragment  B^T[8g .. 8g+7][r]
 //   accumulator C[4g .. 4g+3][r]
 #include <hip/hip_runtime.h>
@@ -632,7 +633,7 @@ __global__ void to_bf16_kernel(const float *__restrict__ w, int rows, int cols, 
 // network with the activations in one LDS buffer (pose-major rows):
 // features -> 3 x (MFMA GEMM + bias + GELU) -> output layer, also on MFMA.
 // A hidden layer computes its transpose, D[neuron][pose] = W . act^T, so the
-// weights are the A operand (streamed from L2 as k-contiguous rows of W^T,
+// weights are the A operand (streamed from L2 as fragment tiles of W^T,
 // the next k-step's fragments loaded while the current one multiplies) and
 // the activations the B operand (16-byte LDS reads along k); a lane's four
 // results are then four consecutive neurons of one pose (accumulated onto
