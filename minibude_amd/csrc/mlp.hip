@@ -582,11 +582,21 @@ __global__ __launch_bounds__(64 * W) void adam_multi_kernel(const AdamBatch b, f
     const float *g = b.t[t].g + i;
     const size_t stride = b.t[t].stride;
     const int splits = b.t[t].splits;
+    // slices wave, wave + W, ... added in that order, eight (or four) loads in flight
     int z = wave;
+    for (; z + 7 * kAdamWaves < splits; z += 8 * kAdamWaves) {
+      float sl[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) sl[u] = g[size_t(z + u * kAdamWaves) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; u++) gs += sl[u];
+    }
     for (; z + 3 * kAdamWaves < splits; z += 4 * kAdamWaves) {
-      const float s0 = g[size_t(z) * stride], s1 = g[size_t(z + kAdamWaves) * stride];
-      const float s2 = g[size_t(z + 2 * kAdamWaves) * stride], s3 = g[size_t(z + 3 * kAdamWaves) * stride];
-      gs = ((gs + s0) + s1) + s2 + s3;
+      float sl[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) sl[u] = g[size_t(z + u * kAdamWaves) * stride];
+#pragma unroll
+      for (int u = 0; u < 4; u++) gs += sl[u];
     }
     for (; z < splits; z += kAdamWaves) gs += g[size_t(z) * stride];
   }
