@@ -22,12 +22,16 @@ def rel_err(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_forward_backward_matches_oracle(gpu_device):
+@pytest.mark.parametrize("n", [256, 2560, 8192])
+def test_forward_backward_matches_oracle(gpu_device, n):
+    # n = 256: every weight-gradient slice is one k step (the loop's tail
+    # alone); 2,560: one group of four k steps and a tail step (an odd group
+    # count); 8,192: four groups, the pipelined loop of training
     rng = np.random.default_rng(3)
     W, b = M.init(seed=11)
-    poses = D.synth_poses(12, 256)
+    poses = D.synth_poses(12, n)
     feat = S.features(poses)
-    target = rng.normal(size=256)
+    target = rng.normal(size=n)
     pred, loss, gW, gb = S.mlp_forward_backward(feat, target, W, b, gpu_device)
     rp, rl, rgW, rgb = M.forward_backward(feat, target, W, b, bf16=True)
     # BF16 storage points are emulated by the oracle; the rest is FP32 (GPU)
