@@ -141,9 +141,9 @@ struct Workspace {
 };
 int reserve(const Net &net, Workspace &ws, size_t batch, size_t nval);
 
-// Training runs as one captured hipGraph of a whole step (batch indices,
-// the fused step kernel, Adam over the weights, Adam over the grid)
-// replayed `steps` times (every per-step value is computed on the device
+// Training runs as one captured hipGraph of a whole step (the fused step
+// kernel, which reads its batch through the epoch's permutation, then one
+// Adam launch over the weights and the grid) replayed `steps` times (every per-step value is computed on the device
 // from the step count), or, with BUDE_MI355X_TRAIN_GRAPH=0 in the
 // environment, as the same kernels launched step by step; both give the
 // same network bit for bit.

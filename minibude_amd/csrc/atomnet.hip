@@ -116,32 +116,6 @@ __global__ __launch_bounds__(256) void stats_kernel(const float *__restrict__ la
 
 // ---- training kernels -------------------------------------------------------
 
-// Adam on the grid features from the fixed-point accumulators (which it
-// clears), refreshing the BF16 copy.
-// With `state` set, this step's rates come from it, and the last kernel of
-// the step advances its step count.
-__global__ void grid_adam_kernel(float *__restrict__ w, unsigned long long *__restrict__ acc, float *__restrict__ m,
-                                 float *__restrict__ v, bf16 *__restrict__ w16, size_t n, float lr, float b1, float b2,
-                                 float eps, float bc1, float bc2, mlp::StepState *__restrict__ state) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (state) {
-    lr = state->hyper[1];
-    bc1 = state->hyper[2];
-    bc2 = state->hyper[3];
-    if (i == 0) state->step += 1;  // (no kernel of this step reads it after begin_step)
-  }
-  if (i >= n) return;
-  const float gi = float(double(static_cast<long long>(acc[i])) / kGradScale);
-  acc[i] = 0ull;
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
-  w[i] = wi;
-  if (w16) w16[i] = bf16(pack_bf16(wi, 0.f) & 0xffffu);
-}
-
 // Fixed-point accumulators -> FP32 gradient (forward_backward for tests).
 __global__ void grid_grad_out_kernel(unsigned long long *__restrict__ acc, float *__restrict__ out, size_t n) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1038,13 +1012,10 @@ int adam_step_dev(Net &net, const Acts &A, mlp::StepState *state, hipStream_t s)
                        N, A.splits[l], size_t(N)};
   }
   ab.count = 6;
-  if (int st = hip_check(mlp::adam_multi_dev(ab, mlp::step_hyper(state), b1, b2, eps, s), "atomnet adam")) return st;
-  const size_t n = net.nodes * kF;
-  hipLaunchKernelGGL(grid_adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, static_cast<float *>(net.g.ptr),
-                     static_cast<unsigned long long *>(net.gacc.ptr), static_cast<float *>(net.gm.ptr),
-                     static_cast<float *>(net.gv.ptr), static_cast<bf16 *>(nullptr), n, 0.f, b1, b2, eps, 1.f, 1.f,
-                     state);
-  return hip_check(hipGetLastError(), "atomnet grid adam");
+  // the grid, from its fixed-point accumulators (cleared), in the same launch
+  ab.fixed = {static_cast<float *>(net.g.ptr), static_cast<unsigned long long *>(net.gacc.ptr),
+              static_cast<float *>(net.gm.ptr), static_cast<float *>(net.gv.ptr), net.nodes * kF, kGradScale};
+  return hip_check(mlp::adam_multi_dev(ab, mlp::step_hyper(state), b1, b2, eps, s, state), "atomnet adam");
 }
 
 int alloc_zero(DeviceBuffer &d, size_t bytes) {

@@ -169,9 +169,21 @@ struct AdamTensor {
   size_t stride;
   bf16 *w16x = nullptr, *w16tx = nullptr;  // optional fragment-tiled BF16 copies of w and w^T (tiled_index)
 };
+// A tensor whose gradient arrives as a fixed-point sum of integer atomics:
+// g = acc / scale, and the Adam step clears acc.  Its learning rate is
+// hyper[1] (adam_multi_dev with hyper set).
+struct AdamFixed {
+  float *w;
+  unsigned long long *acc;
+  float *m, *v;
+  size_t n;  // 0: none
+  double scale;
+};
 struct AdamBatch {
   AdamTensor t[8];
   int count;
+  AdamFixed fixed{};  // optional: its work-groups follow the tensors' in the same launch
+  int tensor_blocks = 0;  // (set by adam_multi_dev)
 };
 // The learning rate and the bias corrections are read from device memory
 // when the kernel runs: hyper[0] = lr, hyper[2] = bc1, hyper[3] = bc2 (so a

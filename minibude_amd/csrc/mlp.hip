@@ -568,6 +568,19 @@ __global__ __launch_bounds__(64 * W) void adam_multi_kernel(const AdamBatch b, f
   }
   // (the step count is read only by the next step's first kernel)
   if (advance && blockIdx.x == 0 && threadIdx.x == 0) advance->step += 1;
+  if (int(blockIdx.x) >= b.tensor_blocks) {  // the fixed-point tensor: 64 W entries per work-group
+    const AdamFixed &F = b.fixed;
+    const size_t k = size_t(blockIdx.x - b.tensor_blocks) * (64 * W) + threadIdx.x;
+    if (k >= F.n) return;
+    const float gk = float(double(static_cast<long long>(F.acc[k])) / F.scale);
+    F.acc[k] = 0ull;
+    const float mk = b1 * F.m[k] + (1.f - b1) * gk;
+    const float vk = b2 * F.v[k] + (1.f - b2) * gk * gk;
+    F.m[k] = mk;
+    F.v[k] = vk;
+    F.w[k] = F.w[k] - hyper[1] * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+    return;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   size_t i = size_t(blockIdx.x) * 64 + lane;
   int t = 0;
@@ -828,12 +841,18 @@ hipError_t adam_multi_dev(const AdamBatch &b, const float *hyper, float b1, floa
     total += size_t(b.t[t].rows) * b.t[t].cols;
     splits = std::max(splits, b.t[t].splits);
   }
-  if (splits > 32)
-    hipLaunchKernelGGL(adam_multi_kernel<16>, dim3(blocks_for(total, 64)), dim3(64 * 16), 0, s, b, 0.f, b1, b2, eps,
-                       1.f, 1.f, hyper, advance);
+  if (total == 0 || !hyper || (b.fixed.n && (!b.fixed.w || !b.fixed.acc || !b.fixed.m || !b.fixed.v)))
+    return hipErrorInvalidValue;
+  const int W = splits > 32 ? 16 : 4;
+  AdamBatch a = b;
+  a.tensor_blocks = int(blocks_for(total, 64));
+  const unsigned blocks = unsigned(a.tensor_blocks) + blocks_for(b.fixed.n, 64u * W);
+  if (W == 16)
+    hipLaunchKernelGGL(adam_multi_kernel<16>, dim3(blocks), dim3(64 * 16), 0, s, a, 0.f, b1, b2, eps, 1.f, 1.f, hyper,
+                       advance);
   else
-    hipLaunchKernelGGL(adam_multi_kernel<4>, dim3(blocks_for(total, 64)), dim3(64 * 4), 0, s, b, 0.f, b1, b2, eps, 1.f,
-                       1.f, hyper, advance);
+    hipLaunchKernelGGL(adam_multi_kernel<4>, dim3(blocks), dim3(64 * 4), 0, s, a, 0.f, b1, b2, eps, 1.f, 1.f, hyper,
+                       advance);
   return hipGetLastError();
 }
 
