@@ -216,6 +216,7 @@ struct bude_combined {
   // (field::split_atoms), the groups' bounds, and partial-sum scratch of the
   // chunk scores (second stream) and the sampled poses' scores (third)
   int field_groups = 1;
+  int64_t field_launch_tiles = 1;  // tiles per field launch when field_groups > 1
   DeviceBuffer fgrouped, fgroup_begin, fpart, fpart_lab;
   DeviceBuffer poses;                 // 6 rows of `total`
   DeviceBuffer pred, exact_all;
@@ -439,12 +440,22 @@ int prepare_field(bude_combined *c, const std::vector<Span> &spans, DeviceBuffer
 
 // Interpolated field scores (out[off + j]) of tiles [tile0, tile0 + ntiles)
 // of the spans prepared by prepare_field, from the current cells.
+// Tiles of one field launch when the atoms are split over the XCDs: 16.8M
+// poses, 512 MiB of partial sums at 8 groups (BUDE_MI355X_FIELD_LAUNCH_TILES
+// sets a smaller count, so tests can run many launches on small sets).
+int64_t field_launch_tiles() {
+  const char *env = std::getenv("BUDE_MI355X_FIELD_LAUNCH_TILES");
+  const long long v = env ? std::atoll(env) : 0;
+  return v > 0 && v < (1ll << 18) ? int64_t(v) : int64_t(1) << 18;
+}
+
 // The field scorer's atom groups (field::split_atoms): one per XCD, or one
 // in all when BUDE_MI355X_FIELD_GROUPS=1 (the form without the split, kept
 // to measure the split against).
 int group_field_atoms(bude_combined *c, const std::vector<float> &lig4) {
   const char *env = std::getenv("BUDE_MI355X_FIELD_GROUPS");
   c->field_groups = env && std::strcmp(env, "1") == 0 ? 1 : bude::field::kXcdGroups;
+  c->field_launch_tiles = field_launch_tiles();
   std::vector<float4> atoms(lig4.size() / 4), grouped;
   std::memcpy(atoms.data(), lig4.data(), atoms.size() * sizeof(float4));
   std::vector<int32_t> off(c->lig_off), begin;
@@ -454,13 +465,10 @@ int group_field_atoms(bude_combined *c, const std::vector<float> &lig4) {
   return c->fgroup_begin.upload(begin.data(), begin.size() * sizeof(int32_t));
 }
 
-// Tiles of one field launch when the atoms are split over the XCDs: 16.8M
-// poses, 512 MiB of partial sums at 8 groups.
-constexpr int64_t kFieldLaunchTiles = int64_t(1) << 18;
-
 // Bytes of partial-sum scratch a field launch over `ntiles` tiles needs.
 size_t field_partial_bytes(const bude_combined *c, int64_t ntiles) {
-  return c->field_groups > 1 ? size_t(std::min(ntiles, kFieldLaunchTiles)) * size_t(c->field_groups) * 64 * 4 : 0;
+  return c->field_groups > 1 ? size_t(std::min(ntiles, c->field_launch_tiles)) * size_t(c->field_groups) * 64 * 4
+                             : 0;
 }
 
 // partial: scratch of field_partial_bytes(ntiles), used by nothing else on
@@ -485,7 +493,7 @@ int launch_field(bude_combined *c, const float *poses, size_t stride, float *out
   a.stride = stride;
   a.out = out;
   a.raise_priority = raise_priority;
-  const int64_t step = c->field_groups > 1 ? kFieldLaunchTiles : ntiles;
+  const int64_t step = c->field_groups > 1 ? c->field_launch_tiles : ntiles;
   for (int64_t t = 0; t < ntiles; t += step) {
     a.tile0 = tile0 + t;
     if (int st = bude::hip_check(bude::field::score(a, std::min(step, ntiles - t), stream), "field score")) return st;

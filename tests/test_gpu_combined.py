@@ -72,12 +72,15 @@ def test_field_maps_are_single_atom_plain_energies(gpu_device):
     assert np.array_equal(m.ravel().view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("groups,chunks", [("8", 0), ("8", 3), ("1", 0)])
-def test_field_score_is_trilinear_sum_over_atoms(gpu_device, monkeypatch, groups, chunks):
+@pytest.mark.parametrize("groups,chunks,launch_tiles", [("8", 0, 0), ("8", 3, 0), ("8", 2, 5), ("1", 0, 0)])
+def test_field_score_is_trilinear_sum_over_atoms(gpu_device, monkeypatch, groups, chunks, launch_tiles):
     # groups: the scorer's atoms split by map slot over the 8 XCDs (partial
-    # sums added in a second kernel; with chunks, launches from tile0 > 0),
-    # or one group, the form without the split
+    # sums added in a second kernel; with chunks, launches from tile0 > 0;
+    # launch_tiles: each chunk scored in launches of that many tiles, a last
+    # one shorter), or one group, the form without the split
     monkeypatch.setenv("BUDE_MI355X_FIELD_GROUPS", groups)
+    if launch_tiles:
+        monkeypatch.setenv("BUDE_MI355X_FIELD_LAUNCH_TILES", str(launch_tiles))
     w = small_workload(nposes=4096, nligands=5, nproteins=1)
     cfg = C.default_config("field", grid_spacing=0.5, max_val_rmse=1e9, gate_z=0.0, pipeline_chunks=chunks)
     with C.Combined(w.forcefield, w.ligands, w.poses, cfg, gpu_device) as c:
